@@ -1,0 +1,99 @@
+"""The C-ABI library loads and exports every function include/*.h declares
+(no compute calls: this runs without a GPU)."""
+import ctypes
+import glob
+import os
+import re
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DECL = re.compile(r"^\s*(?:extern\s+)?[A-Za-z_][\w\s\*]*?\b(xkblas_\w+|[a-z]+_)\s*\(", re.M)
+
+
+def declared():
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in DECL.finditer(text):
+            names.add(m.group(1))
+    return sorted(names)
+
+
+def test_headers_declare_the_drop_in_surface():
+    names = declared()
+    for must in ["xkblas_init", "xkblas_finalize", "xkblas_sync", "xkblas_dgemm_async", "xkblas_dsyrk_async",
+                 "xkblas_dtrsm_async", "xkblas_dpotrf_async", "xkblas_memory_coherent_async",
+                 "xkblas_memory_invalidate_caches", "xkblas_dgemm_native_"]:
+        assert must in names
+
+
+def test_library_exports_every_declared_symbol():
+    import xkblas_amd
+    lib = xkblas_amd.lib()
+    missing = [n for n in declared() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_python_binding_covers_every_symbol():
+    import xkblas_amd
+    missing = [n for n in declared() if n not in xkblas_amd.SIGNATURES]
+    assert not missing, missing
+
+
+def test_query_calls_without_device():
+    """Calls that must work before init and without a GPU."""
+    import xkblas_amd
+    L = xkblas_amd.lib()
+    assert L.xkblas_get_ngpus() == 1
+    assert L.xkblas_set_ngpus(2) != 0
+    assert b"gfx950" in L.xkblas_get_configuration()
+    L.xkblas_set_param(512, 0)
+    assert L.xkblas_get_param() == 512
+    assert L.xkblas_auto_tilesize(None, 1, 2048, 2048, 2048) == 512
+    L.xkblas_set_param(0, 0)
+    # the reference GEMM heuristic for one GPU (blas/xkblas.c:2502-2515)
+    assert L.xkblas_auto_tilesize(None, 1, 16384, 16384, 16384) == 8192
+    assert L.xkblas_auto_tilesize(None, 1, 2048, 2048, 2048) == 2048
+
+
+def test_illegal_argument_returns_code_when_asked():
+    """With XKBLAS_ERROR_RETURN=1 the reference's negative codes come back
+    (blas/zgemm.c:131-180); checks run before any device use."""
+    code = r'''
+import ctypes, os, sys
+sys.path.insert(0, %r)
+import xkblas_amd as x
+L = x.lib()
+one = (ctypes.c_double * 1)(1.0)
+buf = (ctypes.c_double * 16)()
+print(L.xkblas_dgemm_async(999, 111, 2, 2, 2, one, buf, 2, buf, 2, one, buf, 2),
+      L.xkblas_dgemm_async(111, 111, 4, 2, 2, one, buf, 2, buf, 2, one, buf, 4),
+      L.xkblas_dgemm_async(111, 111, 2, 2, 2, one, buf, 2, buf, 2, one, buf, 1),
+      L.xkblas_dtrsm_async(141, 999, 111, 131, 2, 2, one, buf, 2, buf, 2),
+      L.xkblas_dsyrk_async(122, 111, -1, 2, one, buf, 2, one, buf, 2),
+      L.xkblas_dpotrf_async(122, 4, buf, 2))
+''' % ROOT
+    env = dict(os.environ, XKBLAS_ERROR_RETURN="1")
+    out = subprocess.run(["python", "-c", code], capture_output=True, text=True, env=env, timeout=120)
+    assert out.returncode == 0, out.stderr
+    assert out.stdout.split() == ["-1", "-8", "-13", "-2", "-3", "-8"]
+
+
+def test_illegal_argument_aborts_by_default():
+    """kaapi_error semantics: print and abort (blas/common.gen:312-316)."""
+    code = r'''
+import ctypes, sys
+sys.path.insert(0, %r)
+import xkblas_amd as x
+L = x.lib()
+one = (ctypes.c_double * 1)(1.0)
+buf = (ctypes.c_double * 16)()
+L.xkblas_dgemm_async(999, 111, 2, 2, 2, one, buf, 2, buf, 2, one, buf, 2)
+''' % ROOT
+    env = {k: v for k, v in os.environ.items() if k != "XKBLAS_ERROR_RETURN"}
+    out = subprocess.run(["python", "-c", code], capture_output=True, text=True, env=env, timeout=120)
+    assert out.returncode != 0
+    assert "illegal value of transA" in out.stderr

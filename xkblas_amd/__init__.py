@@ -1,0 +1,182 @@
+"""Python binding of libxkblas_amd.so (the xkblas.h C API) over ctypes.
+
+The product is the C-ABI library; this module only declares its symbols and
+offers thin helpers that call them on NumPy arrays (Fortran order), the way
+the reference's test drivers call the C API (testing/ztesting_gemm.c:157-162:
+async -> coherent_async -> sync).  Loading fails loudly when the library has
+not been built: there is no fallback path.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libxkblas_amd.so")
+
+# CBLAS enum values (reference blas/xkblas.h:114-119)
+NoTrans, Trans, ConjTrans = 111, 112, 113
+Upper, Lower = 121, 122
+NonUnit, Unit = 131, 132
+Left, Right = 141, 142
+
+DEFAULT_MATH, TENSOR_OP_MATH, SPLIT_BF16_MATH, SPLIT_FP8_MATH = 0, 1, 2, 3
+
+# every symbol include/*.h declares, with (restype, argtypes)
+_i, _d, _f, _p, _sz, _l = ctypes.c_int, ctypes.c_double, ctypes.c_float, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_long
+_u64 = ctypes.c_uint64
+SIGNATURES = {
+    "xkblas_init": (_i, []),
+    "xkblas_finalize": (_i, []),
+    "xkblas_sync": (_i, []),
+    "xkblas_set_param": (None, [_sz, _sz]),
+    "xkblas_get_param": (_sz, []),
+    "xkblas_get_ngpus": (_i, []),
+    "xkblas_set_ngpus": (_i, [_i]),
+    "xkblas_get_devicecount": (_i, []),
+    "xkblas_get_device_count": (_i, [_p]),
+    "xkblas_set_modemath": (None, [_i]),
+    "xkblas_get_configuration": (ctypes.c_char_p, []),
+    "xkblas_load_sym": (None, [_p, ctypes.c_char_p]),
+    "xkblas_elapsedns": (_u64, []),
+    "xkblas_elapsedtime": (_d, []),
+    "xkblas_auto_tilesize": (_sz, [_p, _i, _sz, _sz, _sz]),
+    "xkblas_malloc": (_p, [_sz]),
+    "xkblas_free": (None, [_p, _sz]),
+    "xkblas_memory_coherent_async": (_i, [_i, _i, _sz, _sz, _p, _sz, _sz]),
+    "xkblas_memory_syncall": (_i, []),
+    "xkblas_memory_invalidate": (_i, [_p]),
+    "xkblas_memory_invalidate_caches": (_i, []),
+    "xkblas_memory_free": (_i, []),
+    "xkblas_dgemm_async": (_i, [_i, _i, _i, _i, _i, _p, _p, _i, _p, _i, _p, _p, _i]),
+    "xkblas_dsyrk_async": (_i, [_i, _i, _i, _i, _p, _p, _i, _p, _p, _i]),
+    "xkblas_dtrsm_async": (_i, [_i, _i, _i, _i, _i, _i, _p, _p, _i, _p, _i]),
+    "xkblas_dpotrf_async": (_i, [_i, _i, _p, _i]),
+    "xkblas_dgemm_native": (_i, [_i, _i, _i, _i, _i, _p, _p, _i, _p, _i, _p, _p, _i]),
+    "xkblas_dsyrk_native": (_i, [_i, _i, _i, _i, _p, _p, _i, _p, _p, _i]),
+    "xkblas_dtrsm_native": (_i, [_i, _i, _i, _i, _i, _i, _p, _p, _i, _p, _i]),
+    "xkblas_dpotrf_native": (_i, [_i, _i, _p, _i]),
+    "xkblas_sgemm_async": (_i, [_i, _i, _i, _i, _i, _p, _p, _i, _p, _i, _p, _p, _i]),
+    "xkblas_ssyrk_async": (_i, [_i, _i, _i, _i, _p, _p, _i, _p, _p, _i]),
+    "xkblas_strsm_async": (_i, [_i, _i, _i, _i, _i, _i, _p, _p, _i, _p, _i]),
+    "xkblas_spotrf_async": (_i, [_i, _i, _p, _i]),
+    "xkblas_potrf_info": (_i, [_p]),
+    "xkblas_set_split_count": (None, [_i]),
+    "xkblas_get_split_count": (_i, []),
+    "xkblas_profile_enable": (None, [_i]),
+    "xkblas_profile_reset": (None, []),
+    "xkblas_profile_get": (_i, [ctypes.c_char_p, _p, _p, _p]),
+    "xkblas_set_lookahead": (None, [_i]),
+    "xkblas_cache_bytes": (_l, []),
+    "xkblas_cache_evictions": (_l, []),
+    "xkblas_set_cache_limit": (None, [_sz]),
+    "xkblas_dgemm_native_": (None, [_p] * 13),
+    "xkblas_dsyrk_native_": (None, [_p] * 10),
+    "xkblas_dtrsm_native_": (None, [_p] * 11),
+    "xkblas_dpotrf_native_": (None, [_p] * 5),
+}
+
+_lib = None
+
+
+def lib():
+    """Load libxkblas_amd.so; raises if it was not built (no fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libxkblas_amd.so is not built: run __graft_entry__.build() "
+                               "or python -m xkblas_amd.build")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name, None)
+            if f is None:
+                continue
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _check(a, dtype):
+    if not (a.flags.f_contiguous and a.dtype == dtype):
+        raise ValueError("expected a Fortran-ordered %s array" % np.dtype(dtype).name)
+
+
+def _scalar(v, dtype):
+    return np.array([v], dtype=dtype)
+
+
+def gemm(ta, tb, alpha, A, B, beta, C, sync=True):
+    """C <- alpha op(A) op(B) + beta C through xkblas_?gemm_async (+ coherent + sync)."""
+    dt = C.dtype
+    for X in (A, B, C):
+        _check(X, dt)
+    M, N = C.shape
+    K = A.shape[1] if ta == NoTrans else A.shape[0]
+    fn = lib().xkblas_dgemm_async if dt == np.float64 else lib().xkblas_sgemm_async
+    al, be = _scalar(alpha, dt), _scalar(beta, dt)
+    rc = fn(ta, tb, M, N, K, _ptr(al), _ptr(A), max(1, A.shape[0]), _ptr(B), max(1, B.shape[0]), _ptr(be), _ptr(C),
+            max(1, M))
+    if sync:
+        coherent(C)
+    return rc
+
+
+def syrk(uplo, trans, alpha, A, beta, C, sync=True):
+    dt = C.dtype
+    for X in (A, C):
+        _check(X, dt)
+    N = C.shape[0]
+    K = A.shape[1] if trans == NoTrans else A.shape[0]
+    fn = lib().xkblas_dsyrk_async if dt == np.float64 else lib().xkblas_ssyrk_async
+    al, be = _scalar(alpha, dt), _scalar(beta, dt)
+    rc = fn(uplo, trans, N, K, _ptr(al), _ptr(A), max(1, A.shape[0]), _ptr(be), _ptr(C), max(1, N))
+    if sync:
+        coherent(C, uplo)
+    return rc
+
+
+def trsm(side, uplo, ta, diag, alpha, A, B, sync=True):
+    dt = B.dtype
+    for X in (A, B):
+        _check(X, dt)
+    M, N = B.shape
+    fn = lib().xkblas_dtrsm_async if dt == np.float64 else lib().xkblas_strsm_async
+    al = _scalar(alpha, dt)
+    rc = fn(side, uplo, ta, diag, M, N, _ptr(al), _ptr(A), max(1, A.shape[0]), _ptr(B), max(1, M))
+    if sync:
+        coherent(B)
+    return rc
+
+
+def potrf(uplo, A, sync=True):
+    """Returns the LAPACK info (after sync) or the enqueue code."""
+    dt = A.dtype
+    _check(A, dt)
+    N = A.shape[0]
+    fn = lib().xkblas_dpotrf_async if dt == np.float64 else lib().xkblas_spotrf_async
+    rc = fn(uplo, N, _ptr(A), max(1, N))
+    if rc != 0 or not sync:
+        return rc
+    coherent(A, uplo)
+    return lib().xkblas_potrf_info(_ptr(A))
+
+
+def coherent(X, uplo=0):
+    L = lib()
+    L.xkblas_memory_coherent_async(uplo, 0, X.shape[0], X.shape[1], _ptr(X), max(1, X.shape[0]), X.itemsize)
+    L.xkblas_sync()
+
+
+def invalidate():
+    lib().xkblas_memory_invalidate_caches()
+
+
+def profile(cls):
+    n, ms, fl = ctypes.c_long(), ctypes.c_double(), ctypes.c_double()
+    lib().xkblas_profile_get(cls.encode(), ctypes.byref(n), ctypes.byref(ms), ctypes.byref(fl))
+    return n.value, ms.value, fl.value

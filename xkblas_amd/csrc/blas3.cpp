@@ -1,0 +1,301 @@
+// blas3.cpp — BLAS-3 algorithms on device-resident matrices (host side).
+//
+// GEMM / SYRK (reference blas/zgemm.c:279-367, zsyrk.c:206-317): the
+// reference inserts one task per (m,n,k) tile with beta at k == 0; here one
+// launch covers every output tile and runs the whole k loop in registers, so
+// beta is applied once, exactly as the tile sequence does.  SYRK is the same
+// kernel restricted to the uplo triangle of C (diagonal tiles masked).
+//
+// TRSM (reference blas/ztrsm.c:240-496): the reference alternates a diagonal
+// tile solve with GEMM updates by -1, folding alpha into the first step.  The
+// same sweep is kept, but each diagonal solve is a GEMM with the explicit
+// inverse of the diagonal block (all inverses computed up front by a
+// recursive TRTRI whose leaves are 64 x 64 single-wave kernels), so every
+// step runs on MFMA instead of a latency-bound substitution.
+//
+// POTRF (reference blas/zpotrf.c:89-187, unbuilt there): right-looking
+// blocked Cholesky with block order nb.  Step k factors the diagonal block
+// (recursive: leaves of order 64, inverse built alongside), forms the panel
+// as a GEMM with the inverse, and applies the rank-nb update to the trailing
+// triangle in one launch.  With lookahead the next diagonal block is updated
+// first and factored on a side stream while the bulk update runs.
+#include "blas3.h"
+#include "kernels.h"
+#include "split.h"
+#include <algorithm>
+#include <climits>
+
+namespace xk {
+
+template <typename T>
+void dev_gemm(const Exec& x, int transA, int transB, int m, int n, int k, T alpha, const T* A, long lda,
+              const T* B, long ldb, T beta, T* C, long ldc)
+{
+  if (m <= 0 || n <= 0) return;
+  if (split_enabled<T>(x) && k > 0 && alpha != T(0)) {
+    split_gemm<T>(x, transA, transB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, TRI_FULL);
+    return;
+  }
+  gemm<T>(x.s, transA, transB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+
+template <typename T>
+void dev_syrk(const Exec& x, int uplo, int trans, int n, int k, T alpha, const T* A, long lda, T beta, T* C,
+              long ldc)
+{
+  if (n <= 0) return;
+  const int ta = trans == NoTrans ? NoTrans : Trans;
+  const int tb = trans == NoTrans ? Trans : NoTrans;
+  const int tri = uplo == Lower ? TRI_LOWER : TRI_UPPER;
+  if (split_enabled<T>(x) && k > 0 && alpha != T(0)) {
+    split_gemm<T>(x, ta, tb, n, n, k, alpha, A, lda, A, lda, beta, C, ldc, tri);
+    return;
+  }
+  gemm<T>(x.s, ta, tb, n, n, k, alpha, A, lda, A, lda, beta, C, ldc, tri);
+}
+
+// Split point of a recursive step: a multiple of 64 near n/2.
+static inline int half64(int n)
+{
+  int h = ((n / 2) + 63) / 64 * 64;
+  return h >= n ? n - 64 : h;
+}
+
+template <typename T>
+static void fill_zero(hipStream_t s, int m, int n, T* Y, long ldy)
+{
+  // zero a block via the masked GEMM path: alpha = 0, beta = 0, k = 0
+  gemm<T>(s, NoTrans, NoTrans, m, n, 0, T(0), Y, ldy, Y, ldy, T(0), Y, ldy);
+}
+
+template <typename T>
+void dev_trtri(const Exec& x, hipStream_t s, int uplo, int diag, int n, const T* A, long lda, T* Tinv, long ldinv,
+               int wslot)
+{
+  if (n <= 64) {
+    trtri_leaf<T>(s, uplo, diag, n, A, lda, 0, Tinv, ldinv, 0, 1);
+    return;
+  }
+  const int n1 = half64(n), n2 = n - n1;
+  const T* A11 = A;
+  const T* A22 = A + n1 + (long)n1 * lda;
+  T* I11 = Tinv;
+  T* I22 = Tinv + n1 + (long)n1 * ldinv;
+  dev_trtri<T>(x, s, uplo, diag, n1, A11, lda, I11, ldinv, wslot);
+  dev_trtri<T>(x, s, uplo, diag, n2, A22, lda, I22, ldinv, wslot);
+  T* W = (T*)scratch(wslot, sizeof(T) * (size_t)n1 * n2);
+  if (uplo == Lower) {
+    // I21 = -I22 A21 I11
+    const T* A21 = A + n1;
+    T* I21 = Tinv + n1;
+    gemm<T>(s, NoTrans, NoTrans, n2, n1, n1, T(1), A21, lda, I11, ldinv, T(0), W, n2);
+    gemm<T>(s, NoTrans, NoTrans, n2, n1, n2, T(-1), I22, ldinv, W, n2, T(0), I21, ldinv);
+    fill_zero<T>(s, n1, n2, Tinv + (long)n1 * ldinv, ldinv);
+  } else {
+    // I12 = -I11 A12 I22
+    const T* A12 = A + (long)n1 * lda;
+    T* I12 = Tinv + (long)n1 * ldinv;
+    gemm<T>(s, NoTrans, NoTrans, n1, n2, n2, T(1), A12, lda, I22, ldinv, T(0), W, n1);
+    gemm<T>(s, NoTrans, NoTrans, n1, n2, n1, T(-1), I11, ldinv, W, n1, T(0), I12, ldinv);
+    fill_zero<T>(s, n2, n1, Tinv + n1, ldinv);
+  }
+}
+
+// Recursive Cholesky of one diagonal block with its inverse (Tinv = inv(L)
+// for Lower, inv(U) for Upper).
+template <typename T>
+static void potrf_diag(hipStream_t s, int uplo, int n, T* A, long lda, T* Tinv, long ldinv, int* info, int base,
+                       int wslot)
+{
+  if (n <= 64) {
+    potrf_leaf<T>(s, uplo, n, A, lda, 0, Tinv, ldinv, 0, 1, info, base);
+    return;
+  }
+  const int n1 = half64(n), n2 = n - n1;
+  T* A11 = A;
+  T* A22 = A + n1 + (long)n1 * lda;
+  T* I11 = Tinv;
+  T* I22 = Tinv + n1 + (long)n1 * ldinv;
+  potrf_diag<T>(s, uplo, n1, A11, lda, I11, ldinv, info, base, wslot);
+  T* W = (T*)scratch(wslot, sizeof(T) * (size_t)n1 * n2);
+  if (uplo == Lower) {
+    T* A21 = A + n1;
+    T* I21 = Tinv + n1;
+    copy2d<T>(s, n2, n1, A21, lda, W, n2);
+    gemm<T>(s, NoTrans, Trans, n2, n1, n1, T(1), W, n2, I11, ldinv, T(0), A21, lda);   // L21 = A21 inv(L11)^T
+    gemm<T>(s, NoTrans, Trans, n2, n2, n1, T(-1), A21, lda, A21, lda, T(1), A22, lda, TRI_LOWER);
+    potrf_diag<T>(s, uplo, n2, A22, lda, I22, ldinv, info, base + n1, wslot);
+    gemm<T>(s, NoTrans, NoTrans, n2, n1, n1, T(1), A21, lda, I11, ldinv, T(0), W, n2);  // L21 inv(L11)
+    gemm<T>(s, NoTrans, NoTrans, n2, n1, n2, T(-1), I22, ldinv, W, n2, T(0), I21, ldinv);
+    fill_zero<T>(s, n1, n2, Tinv + (long)n1 * ldinv, ldinv);
+  } else {
+    T* A12 = A + (long)n1 * lda;
+    T* I12 = Tinv + (long)n1 * ldinv;
+    copy2d<T>(s, n1, n2, A12, lda, W, n1);
+    gemm<T>(s, Trans, NoTrans, n1, n2, n1, T(1), I11, ldinv, W, n1, T(0), A12, lda);     // U12 = inv(U11)^T A12
+    gemm<T>(s, Trans, NoTrans, n2, n2, n1, T(-1), A12, lda, A12, lda, T(1), A22, lda, TRI_UPPER);
+    potrf_diag<T>(s, uplo, n2, A22, lda, I22, ldinv, info, base + n1, wslot);
+    gemm<T>(s, NoTrans, NoTrans, n1, n2, n2, T(1), A12, lda, I22, ldinv, T(0), W, n1);   // U12 inv(U22)
+    gemm<T>(s, NoTrans, NoTrans, n1, n2, n1, T(-1), I11, ldinv, W, n1, T(0), I12, ldinv);
+    fill_zero<T>(s, n2, n1, Tinv + n1, ldinv);
+  }
+}
+
+enum { WS_DIAG0 = 0, WS_DIAG1 = 1, WS_PANEL = 2, WS_INV = 3, WS_TRSM = 4, WS_TRSMW = 5 };
+
+template <typename T>
+void dev_potrf(const Exec& x, int uplo, int n, T* A, long lda, int nb, int* info)
+{
+  if (n <= 0) return;
+  nb = std::max(64, std::min(nb, n));
+  const int nt = (n + nb - 1) / nb;
+  // inverses of two consecutive diagonal blocks (ping-pong for lookahead)
+  T* inv = (T*)scratch(WS_INV, sizeof(T) * (size_t)2 * nb * nb);
+  hipEvent_t ev_diag, ev_look;
+  XK_HIP(hipEventCreateWithFlags(&ev_diag, hipEventDisableTiming));
+  XK_HIP(hipEventCreateWithFlags(&ev_look, hipEventDisableTiming));
+  const bool lookahead = x.s_diag != nullptr && nt > 1;
+  hipStream_t sd = lookahead ? x.s_diag : x.s;
+
+  auto blk = [&](int i, int j) { return A + (long)i * nb + (long)j * nb * lda; };
+  auto tsz = [&](int i) { return i == nt - 1 ? n - i * nb : nb; };
+
+  // factor block 0
+  if (lookahead) {
+    XK_HIP(hipEventRecord(ev_look, x.s));
+    XK_HIP(hipStreamWaitEvent(sd, ev_look, 0));
+  }
+  potrf_diag<T>(sd, uplo, tsz(0), blk(0, 0), lda, inv, nb, info, 0, WS_DIAG0);
+  for (int k = 0; k < nt; ++k) {
+    const int kk = tsz(k);
+    T* ik = inv + (size_t)(k & 1) * nb * nb;
+    if (lookahead) {
+      XK_HIP(hipEventRecord(ev_diag, sd));
+      XK_HIP(hipStreamWaitEvent(x.s, ev_diag, 0));
+    }
+    if (k == nt - 1) break;
+    const int rest = n - (k + 1) * nb;
+    T* W = (T*)scratch(WS_PANEL, sizeof(T) * (size_t)rest * kk);
+    if (uplo == Lower) {
+      T* P = blk(k + 1, k);  // rest x kk panel
+      copy2d<T>(x.s, rest, kk, P, lda, W, rest);
+      dev_gemm<T>(x, NoTrans, Trans, rest, kk, kk, T(1), W, rest, ik, nb, T(0), P, lda);
+      // lookahead: update the next diagonal block and factor it on the side
+      const int k1 = tsz(k + 1);
+      dev_syrk<T>(x, Lower, NoTrans, k1, kk, T(-1), P, lda, T(1), blk(k + 1, k + 1), lda);
+      if (lookahead) {
+        XK_HIP(hipEventRecord(ev_look, x.s));
+        XK_HIP(hipStreamWaitEvent(sd, ev_look, 0));
+        potrf_diag<T>(sd, uplo, k1, blk(k + 1, k + 1), lda, inv + (size_t)((k + 1) & 1) * nb * nb, nb, info,
+                      (k + 1) * nb, ((k + 1) & 1) ? WS_DIAG1 : WS_DIAG0);
+      }
+      // bulk: rows/cols beyond block k+1.  Columns of block k+1 below its
+      // diagonal block, then the trailing triangle.
+      const int r2 = rest - k1;
+      if (r2 > 0) {
+        dev_gemm<T>(x, NoTrans, Trans, r2, k1, kk, T(-1), P + k1, lda, P, lda, T(1), blk(k + 2, k + 1), lda);
+        dev_syrk<T>(x, Lower, NoTrans, r2, kk, T(-1), P + k1, lda, T(1), blk(k + 2, k + 2), lda);
+      }
+    } else {
+      T* P = blk(k, k + 1);  // kk x rest panel
+      copy2d<T>(x.s, kk, rest, P, lda, W, kk);
+      dev_gemm<T>(x, Trans, NoTrans, kk, rest, kk, T(1), ik, nb, W, kk, T(0), P, lda);
+      const int k1 = tsz(k + 1);
+      dev_syrk<T>(x, Upper, Trans, k1, kk, T(-1), P, lda, T(1), blk(k + 1, k + 1), lda);
+      if (lookahead) {
+        XK_HIP(hipEventRecord(ev_look, x.s));
+        XK_HIP(hipStreamWaitEvent(sd, ev_look, 0));
+        potrf_diag<T>(sd, uplo, k1, blk(k + 1, k + 1), lda, inv + (size_t)((k + 1) & 1) * nb * nb, nb, info,
+                      (k + 1) * nb, ((k + 1) & 1) ? WS_DIAG1 : WS_DIAG0);
+      }
+      const int r2 = rest - k1;
+      if (r2 > 0) {
+        dev_gemm<T>(x, Trans, NoTrans, k1, r2, kk, T(-1), P, lda, P + (long)k1 * lda, lda, T(1), blk(k + 1, k + 2),
+                    lda);
+        dev_syrk<T>(x, Upper, Trans, r2, kk, T(-1), P + (long)k1 * lda, lda, T(1), blk(k + 2, k + 2), lda);
+      }
+    }
+    if (!lookahead)
+      potrf_diag<T>(x.s, uplo, tsz(k + 1), blk(k + 1, k + 1), lda, inv + (size_t)((k + 1) & 1) * nb * nb, nb,
+                    info, (k + 1) * nb, WS_DIAG0);
+  }
+  XK_HIP(hipEventDestroy(ev_diag));
+  XK_HIP(hipEventDestroy(ev_look));
+}
+
+template <typename T>
+void dev_trsm(const Exec& x, int side, int uplo, int transA, int diag, int m, int n, T alpha, const T* A, long lda,
+              T* B, long ldb, int nb)
+{
+  if (m <= 0 || n <= 0) return;
+  if (alpha == T(0)) {
+    fill_zero<T>(x.s, m, n, B, ldb);
+    return;
+  }
+  const bool tr = transA != NoTrans;
+  const bool lower_eff = (uplo == Lower) != tr;
+  const int na = side == Left ? m : n;
+  nb = std::max(64, std::min(nb, na));
+  const int nblk = (na + nb - 1) / nb;
+  auto bsz = [&](int i) { return i == nblk - 1 ? na - i * nb : nb; };
+  // inverses of all diagonal blocks of A, each nb x nb
+  T* inv = (T*)scratch(WS_TRSM, sizeof(T) * (size_t)nblk * nb * nb);
+  for (int d = 0; d < nblk; ++d)
+    dev_trtri<T>(x, x.s, uplo, diag, bsz(d), A + (long)d * nb * (1 + lda), lda, inv + (size_t)d * nb * nb, nb,
+                 WS_DIAG0);
+  // op(A) block (i,j) pointer and the transpose flag to use with it
+  auto opblk = [&](int i, int j) -> const T* {
+    return tr ? A + (long)j * nb + (long)i * nb * lda : A + (long)i * nb + (long)j * nb * lda;
+  };
+  const int ta = tr ? Trans : NoTrans;
+  if (side == Left) {
+    T* W = (T*)scratch(WS_TRSMW, sizeof(T) * (size_t)nb * n);
+    for (int s = 0; s < nblk; ++s) {
+      const int k = lower_eff ? s : nblk - 1 - s;
+      const int kk = bsz(k);
+      const T la = s == 0 ? alpha : T(1);
+      T* Bk = B + (long)k * nb;
+      copy2d<T>(x.s, kk, n, Bk, ldb, W, kk);
+      gemm<T>(x.s, ta, NoTrans, kk, n, kk, la, inv + (size_t)k * nb * nb, nb, W, kk, T(0), Bk, ldb);
+      if (s == nblk - 1) break;
+      if (lower_eff) {  // rows below block k
+        const int r0 = (k + 1) * nb, rows = m - r0;
+        dev_gemm<T>(x, ta, NoTrans, rows, n, kk, T(-1), opblk(k + 1, k), lda, Bk, ldb, la, B + r0, ldb);
+      } else {  // rows above block k
+        const int rows = k * nb;
+        dev_gemm<T>(x, ta, NoTrans, rows, n, kk, T(-1), opblk(0, k), lda, Bk, ldb, la, B, ldb);
+      }
+    }
+  } else {
+    T* W = (T*)scratch(WS_TRSMW, sizeof(T) * (size_t)m * nb);
+    const bool forward = !lower_eff;  // X T = B with T upper: first column block first
+    for (int s = 0; s < nblk; ++s) {
+      const int k = forward ? s : nblk - 1 - s;
+      const int kk = bsz(k);
+      const T la = s == 0 ? alpha : T(1);
+      T* Bk = B + (long)k * nb * ldb;
+      copy2d<T>(x.s, m, kk, Bk, ldb, W, m);
+      gemm<T>(x.s, NoTrans, ta, m, kk, kk, la, W, m, inv + (size_t)k * nb * nb, nb, T(0), Bk, ldb);
+      if (s == nblk - 1) break;
+      if (forward) {  // columns right of block k
+        const int c0 = (k + 1) * nb, cols = n - c0;
+        dev_gemm<T>(x, NoTrans, ta, m, cols, kk, T(-1), Bk, ldb, opblk(k, k + 1), lda, la, B + (long)c0 * ldb, ldb);
+      } else {  // columns left of block k
+        const int cols = k * nb;
+        dev_gemm<T>(x, NoTrans, ta, m, cols, kk, T(-1), Bk, ldb, opblk(k, 0), lda, la, B, ldb);
+      }
+    }
+  }
+}
+
+#define XK_INST(T)                                                                                             \
+  template void dev_gemm<T>(const Exec&, int, int, int, int, int, T, const T*, long, const T*, long, T, T*, long); \
+  template void dev_syrk<T>(const Exec&, int, int, int, int, T, const T*, long, T, T*, long);                     \
+  template void dev_trsm<T>(const Exec&, int, int, int, int, int, int, T, const T*, long, T*, long, int);        \
+  template void dev_potrf<T>(const Exec&, int, int, T*, long, int, int*);                                         \
+  template void dev_trtri<T>(const Exec&, hipStream_t, int, int, int, const T*, long, T*, long, int);
+XK_INST(double)
+XK_INST(float)
+#undef XK_INST
+
+}  // namespace xk

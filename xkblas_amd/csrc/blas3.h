@@ -1,0 +1,45 @@
+// blas3.h — BLAS-3 algorithms on device-resident column-major matrices.
+// These are the MI355X restatement of the reference tile algorithms
+// (blas/zgemm.c, zsyrk.c, ztrsm.c, zpotrf.c): instead of one task per tile,
+// each step of the algorithm is one launch over all tiles it touches.
+#pragma once
+#include "common.h"
+
+namespace xk {
+
+// Device scratch handed out by the runtime: slot-indexed, grow-only buffers.
+void* scratch(int slot, size_t bytes);
+
+struct Exec {
+  hipStream_t s;      // main compute stream
+  hipStream_t s_diag; // side stream for latency-bound diagonal-block work
+  int mode;           // xkblas_mode_math_t
+  int nsplit;         // CRT moduli for split modes (0 = automatic)
+};
+
+template <typename T>
+void dev_gemm(const Exec& x, int transA, int transB, int m, int n, int k, T alpha, const T* A, long lda,
+              const T* B, long ldb, T beta, T* C, long ldc);
+
+template <typename T>
+void dev_syrk(const Exec& x, int uplo, int trans, int n, int k, T alpha, const T* A, long lda, T beta, T* C,
+              long ldc);
+
+// op(A) X = alpha B (side Left) or X op(A) = alpha B (Right); B is m x n.
+// nb is the diagonal block order (multiple of 64 recommended).
+template <typename T>
+void dev_trsm(const Exec& x, int side, int uplo, int transA, int diag, int m, int n, T alpha, const T* A, long lda,
+              T* B, long ldb, int nb);
+
+// In-place blocked Cholesky with block order nb.  info (device int) receives
+// the LAPACK info as an atomic min over failing orders (INT_MAX = success);
+// the caller initialises it.
+template <typename T>
+void dev_potrf(const Exec& x, int uplo, int n, T* A, long lda, int nb, int* info);
+
+// Inverse of the triangular n x n block A into Tinv (other triangle zeroed).
+template <typename T>
+void dev_trtri(const Exec& x, hipStream_t s, int uplo, int diag, int n, const T* A, long lda, T* Tinv, long ldinv,
+               int wslot);
+
+}  // namespace xk

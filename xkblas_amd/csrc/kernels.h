@@ -1,0 +1,50 @@
+// kernels.h — host-side launchers of the hand-written gfx950 kernels.
+// All pointers are device pointers into column-major buffers; every launch is
+// asynchronous on the given stream.
+#pragma once
+#include "common.h"
+
+namespace xk {
+
+// C <- alpha op(A) op(B) + beta C, op(A) m x k, op(B) k x n.  `tri` restricts
+// the written part of C to one triangle (C's (0,0) lies on the diagonal).
+// `batch` > 1 runs independent problems at element strides sA, sB, sC.
+// beta == 0 never reads C.  FP64 uses v_mfma_f64_16x16x4f64, FP32 uses
+// v_mfma_f32_16x16x4f32; tiles are staged through LDS, double-buffered.
+template <typename T>
+void gemm(hipStream_t s, int transA, int transB, int m, int n, int k, T alpha,
+          const T* A, long lda, const T* B, long ldb, T beta, T* C, long ldc,
+          int tri = TRI_FULL, int batch = 1, long sA = 0, long sB = 0, long sC = 0);
+
+// Forces the macro-tile shape of gemm(): 0 = automatic, 64 or 128.
+void gemm_set_tile(int bm);
+int gemm_get_tile();
+
+// Cholesky of `batch` diagonal blocks of order n <= 64 (lower storage, or
+// upper when uplo == Upper), in place, plus the inverse of each factor
+// written to Linv (lower: inv(L); upper: inv(U)).  info[0] receives
+// base_order + j + 1 for the first non-positive pivot (atomic min, starts at
+// INT_MAX), nothing on success.
+template <typename T>
+void potrf_leaf(hipStream_t s, int uplo, int n, T* A, long lda, long strideA, T* Linv, long ldinv,
+                long strideInv, int batch, int* info, int base_order);
+
+// Inverse of `batch` triangular blocks of order n <= 64 into Tinv (the other
+// triangle of Tinv is zeroed, like the full block).
+template <typename T>
+void trtri_leaf(hipStream_t s, int uplo, int diag, int n, const T* A, long lda, long strideA, T* Tinv,
+                long ldinv, long strideInv, int batch);
+
+// Y <- X (m x n blocks), element-wise copy between column-major buffers.
+template <typename T>
+void copy2d(hipStream_t s, int m, int n, const T* X, long ldx, T* Y, long ldy);
+
+// Zero the strictly upper (uplo == Lower) or strictly lower part of an n x n
+// block.
+template <typename T>
+void zero_triangle(hipStream_t s, int uplo, int n, T* A, long lda);
+
+// Fill an int array with a value (stream-ordered, graph-capturable).
+void fill_int(hipStream_t s, int* p, int n, int v);
+
+}  // namespace xk

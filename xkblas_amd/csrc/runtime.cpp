@@ -1,0 +1,857 @@
+// runtime.cpp — the xkblas.h C API on one MI355X.
+//
+// What replaces the reference runtime (XKaapi DAG + DSM + HIP plugin):
+//   * Tile cache (reference kaapi/kaapi_memory.c: replica bitmaps, LRU,
+//     fetch/writeback).  Each host matrix known to the library (keyed by its
+//     host pointer, as xkblas_find does, blas/xkblas.c:426-455) owns one
+//     column-major mirror in HBM with a padded leading dimension; validity is
+//     tracked per NB x NB tile (device-valid / host-valid bits).  A call
+//     fetches the invalid tiles it reads with pinned-or-pageable
+//     hipMemcpy2DAsync on the H2D stream; outputs become device-only until
+//     xkblas_memory_coherent_async copies the dirty tiles of the requested
+//     triangle back on the D2H stream (blas/xkblas.c:1953-2055).  A
+//     capacity limit (XKBLAS_CACHE_LIMIT) evicts whole least-recently-used
+//     mirrors after writing their dirty tiles back.
+//   * Scheduling (reference kaapi/kaapi_offload_device.c, one device thread
+//     per GPU with event polling): calls are enqueued in order on one compute
+//     stream; copy streams are ordered against it with events; the caller
+//     blocks only in xkblas_sync (blas/xkblas.c:1773-1850).
+//   * Errors: illegal arguments abort like kaapi_error (blas/common.gen:
+//     312-316) unless XKBLAS_ERROR_RETURN=1, HIP errors abort.
+#include "blas3.h"
+#include "kernels.h"
+#include "runtime.h"
+#include "../../include/xkblas.h"
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstring>
+#include <ctime>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+namespace xk {
+
+[[noreturn]] void fatal(const char* what, const char* file, int line, const char* msg)
+{
+  fprintf(stderr, "[xkblas_amd] fatal: %s (%s:%d): %s\n", what, file, line, msg);
+  fflush(stderr);
+  abort();
+}
+
+Runtime& rt()
+{
+  static Runtime r;
+  return r;
+}
+
+static size_t env_size(const char* name, size_t dflt)
+{
+  const char* v = getenv(name);
+  if (!v || !*v) return dflt;
+  return (size_t)strtoull(v, nullptr, 10);
+}
+
+void* scratch(int slot, size_t bytes)
+{
+  Runtime& r = rt();
+  XK_CHECK(slot >= 0 && slot < Runtime::NSCRATCH, "bad scratch slot");
+  if (bytes == 0) bytes = 256;
+  if (r.scratch_size[slot] < bytes) {
+    size_t sz = std::max(bytes + bytes / 4, (size_t)16 << 20);
+    if (r.scratch_ptr[slot]) {
+      // kernels already enqueued may still use the old buffer
+      XK_HIP(hipDeviceSynchronize());
+      XK_HIP(hipFree(r.scratch_ptr[slot]));
+      r.scratch_ptr[slot] = nullptr;
+      r.scratch_size[slot] = 0;
+    }
+    XK_HIP(hipMalloc(&r.scratch_ptr[slot], sz));
+    r.scratch_size[slot] = sz;
+  }
+  return r.scratch_ptr[slot];
+}
+
+// ---------------------------------------------------------------- tile cache
+static long pad_ld(long m, size_t elt)
+{
+  const long q = 256 / (long)elt;  // 256-byte aligned columns
+  return std::max(q, (m + q - 1) / q * q);
+}
+
+void Runtime::evict_for(size_t need, const std::vector<Replica*>& keep)
+{
+  while (cache_used + need > cache_limit) {
+    Replica* victim = nullptr;
+    for (auto& kv : map) {
+      Replica* r = kv.second;
+      if (std::find(keep.begin(), keep.end(), r) != keep.end()) continue;
+      if (!victim || r->lru < victim->lru) victim = r;
+    }
+    if (!victim) {
+      char msg[256];
+      snprintf(msg, sizeof msg, "operands need %zu bytes of HBM beyond the cache limit of %zu bytes", need,
+               cache_limit);
+      fatal("tile cache", __FILE__, __LINE__, msg);
+    }
+    writeback(victim, 0, 0, victim->M, victim->N, 0);
+    XK_HIP(hipStreamSynchronize(s_d2h));
+    drop(victim);
+    stats.evictions++;
+  }
+}
+
+void Runtime::drop(Replica* r)
+{
+  map.erase(r->host);
+  if (r->dev) {
+    XK_HIP(hipDeviceSynchronize());
+    XK_HIP(hipFree(r->dev));
+  }
+  cache_used -= r->bytes;
+  delete r;
+}
+
+Replica* Runtime::replica(const void* host, int M, int N, long ld, int elt, int nb, std::vector<Replica*>& keep)
+{
+  auto it = map.find(host);
+  if (it != map.end()) {
+    Replica* r = it->second;
+    if (r->M == M && r->N == N && r->ld == ld && r->elt == elt) {
+      r->lru = ++clock;
+      keep.push_back(r);
+      return r;
+    }
+    // The reference keeps the stale descriptor (blas/xkblas.c:416-421); here
+    // a pointer reused with new dimensions gets a fresh mirror.
+    writeback(r, 0, 0, r->M, r->N, 0);
+    XK_HIP(hipStreamSynchronize(s_d2h));
+    drop(r);
+  }
+  Replica* r = new Replica();
+  r->host = host;
+  r->M = M; r->N = N; r->ld = ld; r->elt = elt;
+  r->nb = nb;
+  r->mt = (M + nb - 1) / nb;
+  r->nt = (N + nb - 1) / nb;
+  r->ldd = pad_ld(M, elt);
+  r->bytes = (size_t)r->ldd * (size_t)std::max(N, 1) * elt;
+  r->state.assign((size_t)r->mt * r->nt, HOST_VALID);
+  evict_for(r->bytes, keep);
+  XK_HIP(hipMalloc(&r->dev, r->bytes));
+  cache_used += r->bytes;
+  r->lru = ++clock;
+  map[host] = r;
+  keep.push_back(r);
+  return r;
+}
+
+// Fetch the host-valid-only tiles of r intersecting rows [r0,r0+m) x cols
+// [c0,c0+n) to the device (H2D stream).
+void Runtime::fetch(Replica* r, int r0, int c0, int m, int n)
+{
+  if (m <= 0 || n <= 0) return;
+  const int ti0 = r0 / r->nb, ti1 = (r0 + m - 1) / r->nb;
+  const int tj0 = c0 / r->nb, tj1 = (c0 + n - 1) / r->nb;
+  for (int tj = tj0; tj <= tj1; ++tj) {
+    // coalesce vertical runs of invalid tiles into one 2-D copy
+    int ti = ti0;
+    while (ti <= ti1) {
+      if (r->state[tj * r->mt + ti] & DEV_VALID) { ++ti; continue; }
+      int te = ti;
+      while (te + 1 <= ti1 && !(r->state[tj * r->mt + te + 1] & DEV_VALID)) ++te;
+      const int row0 = ti * r->nb, row1 = std::min(r->M, (te + 1) * r->nb);
+      const int col0 = tj * r->nb, col1 = std::min(r->N, (tj + 1) * r->nb);
+      const char* src = (const char*)r->host + ((size_t)row0 + (size_t)col0 * r->ld) * r->elt;
+      char* dst = r->dev + ((size_t)row0 + (size_t)col0 * r->ldd) * r->elt;
+      XK_HIP(hipMemcpy2DAsync(dst, r->ldd * r->elt, src, r->ld * r->elt, (size_t)(row1 - row0) * r->elt,
+                              (size_t)(col1 - col0), hipMemcpyHostToDevice, s_h2d));
+      stats.h2d_bytes += (double)(row1 - row0) * (col1 - col0) * r->elt;
+      for (int t = ti; t <= te; ++t) r->state[tj * r->mt + t] |= DEV_VALID;
+      pending_h2d = true;
+      ti = te + 1;
+    }
+  }
+}
+
+// Mark tiles as written on the device (host copy stale).
+void Runtime::mark_written(Replica* r, int r0, int c0, int m, int n, int uplo)
+{
+  if (m <= 0 || n <= 0) return;
+  const int ti0 = r0 / r->nb, ti1 = (r0 + m - 1) / r->nb;
+  const int tj0 = c0 / r->nb, tj1 = (c0 + n - 1) / r->nb;
+  for (int tj = tj0; tj <= tj1; ++tj)
+    for (int ti = ti0; ti <= ti1; ++ti) {
+      if (uplo == Lower && ti < tj) continue;
+      if (uplo == Upper && ti > tj) continue;
+      r->state[tj * r->mt + ti] = DEV_VALID;
+    }
+}
+
+// Enqueue device-to-host copies of the dirty tiles in the uplo part
+// (0 = whole matrix) of rows [r0,..) x cols [c0,..).
+void Runtime::writeback(Replica* r, int r0, int c0, int m, int n, int uplo)
+{
+  if (m <= 0 || n <= 0) return;
+  XK_HIP(hipEventRecord(ev_comp, s_comp));
+  XK_HIP(hipStreamWaitEvent(s_d2h, ev_comp, 0));
+  const int ti0 = r0 / r->nb, ti1 = (r0 + m - 1) / r->nb;
+  const int tj0 = c0 / r->nb, tj1 = (c0 + n - 1) / r->nb;
+  bool any = false;
+  for (int tj = tj0; tj <= tj1; ++tj) {
+    int ti = ti0;
+    auto wanted = [&](int t) {
+      if (uplo == Lower && t < tj) return false;
+      if (uplo == Upper && t > tj) return false;
+      return !(r->state[tj * r->mt + t] & HOST_VALID);
+    };
+    while (ti <= ti1) {
+      if (!wanted(ti)) { ++ti; continue; }
+      int te = ti;
+      while (te + 1 <= ti1 && wanted(te + 1)) ++te;
+      const int row0 = ti * r->nb, row1 = std::min(r->M, (te + 1) * r->nb);
+      const int col0 = tj * r->nb, col1 = std::min(r->N, (tj + 1) * r->nb);
+      char* dst = (char*)r->host + ((size_t)row0 + (size_t)col0 * r->ld) * r->elt;
+      const char* src = r->dev + ((size_t)row0 + (size_t)col0 * r->ldd) * r->elt;
+      XK_HIP(hipMemcpy2DAsync(dst, r->ld * r->elt, src, r->ldd * r->elt, (size_t)(row1 - row0) * r->elt,
+                              (size_t)(col1 - col0), hipMemcpyDeviceToHost, s_d2h));
+      stats.d2h_bytes += (double)(row1 - row0) * (col1 - col0) * r->elt;
+      for (int t = ti; t <= te; ++t) r->state[tj * r->mt + t] |= HOST_VALID;
+      any = true;
+      ti = te + 1;
+    }
+  }
+  if (any) {
+    XK_HIP(hipEventRecord(ev_d2h, s_d2h));
+    pending_d2h = true;
+  }
+}
+
+// Order the compute stream after the pending copies.
+void Runtime::before_compute()
+{
+  if (pending_h2d) {
+    XK_HIP(hipEventRecord(ev_h2d, s_h2d));
+    XK_HIP(hipStreamWaitEvent(s_comp, ev_h2d, 0));
+    pending_h2d = false;
+  }
+  if (pending_d2h) {
+    // device tiles being copied back must not be overwritten meanwhile
+    XK_HIP(hipStreamWaitEvent(s_comp, ev_d2h, 0));
+    pending_d2h = false;
+  }
+}
+
+Exec Runtime::exec() const
+{
+  Exec x;
+  x.s = s_comp;
+  x.s_diag = lookahead ? s_diag : nullptr;
+  x.mode = mode;
+  x.nsplit = nsplit;
+  return x;
+}
+
+void Runtime::init_device()
+{
+  int ndev = 0;
+  XK_HIP(hipGetDeviceCount(&ndev));
+  XK_CHECK(ndev > 0, "no HIP device visible");
+  XK_HIP(hipSetDevice(0));
+  XK_HIP(hipStreamCreateWithFlags(&s_comp, hipStreamNonBlocking));
+  XK_HIP(hipStreamCreateWithFlags(&s_diag, hipStreamNonBlocking));
+  XK_HIP(hipStreamCreateWithFlags(&s_h2d, hipStreamNonBlocking));
+  XK_HIP(hipStreamCreateWithFlags(&s_d2h, hipStreamNonBlocking));
+  XK_HIP(hipEventCreateWithFlags(&ev_comp, hipEventDisableTiming));
+  XK_HIP(hipEventCreateWithFlags(&ev_h2d, hipEventDisableTiming));
+  XK_HIP(hipEventCreateWithFlags(&ev_d2h, hipEventDisableTiming));
+  XK_HIP(hipMalloc(&d_info, sizeof(int) * INFO_SLOTS));
+  XK_HIP(hipHostMalloc(&h_info, sizeof(int) * INFO_SLOTS, hipHostMallocDefault));
+  for (int i = 0; i < INFO_SLOTS; ++i) h_info[i] = INT_MAX;
+  size_t freeb = 0, total = 0;
+  XK_HIP(hipMemGetInfo(&freeb, &total));
+  // reference default: 98 % of free memory minus a reserve
+  // (kaapi/kaapi_impl.c:208, kaapi/kaapi_plugin_hip.c:2365-2366)
+  const char* lim = getenv("XKBLAS_CACHE_LIMIT");
+  size_t reserve = (size_t)4 << 30;
+  size_t dflt = freeb > reserve ? (size_t)((double)(freeb - reserve) * 0.9) : freeb / 2;
+  if (lim && *lim) {
+    double v = atof(lim);
+    cache_limit = v <= 1.0 ? (size_t)(v * (double)freeb) : (size_t)v;
+  } else {
+    cache_limit = dflt;
+  }
+  device_ready = true;
+}
+
+void Runtime::sync_all()
+{
+  if (!device_ready) return;
+  XK_HIP(hipStreamSynchronize(s_h2d));
+  XK_HIP(hipStreamSynchronize(s_comp));
+  XK_HIP(hipStreamSynchronize(s_diag));
+  XK_HIP(hipStreamSynchronize(s_d2h));
+  prof_collect();
+}
+
+// ------------------------------------------------------------- profiling
+void Runtime::prof_begin(const char* cls, double flops)
+{
+  if (!profiling) return;
+  ProfEntry e;
+  e.cls = cls;
+  e.flops = flops;
+  XK_HIP(hipEventCreate(&e.b));
+  XK_HIP(hipEventCreate(&e.e));
+  XK_HIP(hipEventRecord(e.b, s_comp));
+  prof_open.push_back(e);
+}
+
+void Runtime::prof_end()
+{
+  if (!profiling || prof_open.empty()) return;
+  ProfEntry e = prof_open.back();
+  prof_open.pop_back();
+  XK_HIP(hipEventRecord(e.e, s_comp));
+  prof_pending.push_back(e);
+}
+
+void Runtime::prof_collect()
+{
+  for (auto& e : prof_pending) {
+    float ms = 0.0f;
+    XK_HIP(hipEventSynchronize(e.e));
+    XK_HIP(hipEventElapsedTime(&ms, e.b, e.e));
+    ProfStat& st = prof[e.cls];
+    st.count += 1;
+    st.ms += ms;
+    st.flops += e.flops;
+    XK_HIP(hipEventDestroy(e.b));
+    XK_HIP(hipEventDestroy(e.e));
+  }
+  prof_pending.clear();
+}
+
+}  // namespace xk
+
+using namespace xk;
+
+// ======================================================================
+// C API
+// ======================================================================
+static std::mutex g_mu;
+typedef std::lock_guard<std::mutex> Lock;
+
+static bool err_return() { return getenv("XKBLAS_ERROR_RETURN") && atoi(getenv("XKBLAS_ERROR_RETURN")) != 0; }
+
+// kaapi_error semantics (blas/common.gen:312-316): print, then abort unless
+// XKBLAS_ERROR_RETURN asks for the code.
+static int arg_error(const char* fn, const char* msg, int code)
+{
+  fprintf(stderr, "[xkblas_amd] %s: %s\n", fn, msg);
+  fflush(stderr);
+  if (!err_return()) abort();
+  return code;
+}
+
+static void ensure_init()
+{
+  Runtime& r = rt();
+  if (!r.device_ready) {
+    r.init_device();
+    const char* ts = getenv("XKBLAS_TILE_SIZE");
+    if (ts && *ts) r.nb = (size_t)atol(ts);
+    const char* dm = getenv("XKBLAS_DEFAULT_MATH");
+    if (dm && *dm) r.mode = atoi(dm);
+    const char* ns = getenv("XKBLAS_SPLIT_COUNT");
+    if (ns && *ns) r.nsplit = atoi(ns);
+    const char* la = getenv("XKBLAS_LOOKAHEAD");
+    if (la && *la) r.lookahead = atoi(la) != 0;
+    const char* pr = getenv("XKBLAS_PROFILE");
+    if (pr && *pr) r.profiling = atoi(pr) != 0;
+  }
+}
+
+extern "C" {
+
+int xkblas_init(void)
+{
+  Lock l(g_mu);
+  ensure_init();
+  rt().refcount++;
+  return 0;
+}
+
+int xkblas_finalize(void)
+{
+  Lock l(g_mu);
+  Runtime& r = rt();
+  if (r.refcount > 0) r.refcount--;
+  if (r.refcount == 0 && r.device_ready) {
+    r.sync_all();
+    if (getenv("XKBLAS_VERBOSE") && atoi(getenv("XKBLAS_VERBOSE")) > 0) {
+      fprintf(stderr, "[xkblas_amd] h2d %.3f GB  d2h %.3f GB  evictions %ld  mirrors %zu (%.3f GB)\n",
+              r.stats.h2d_bytes / 1e9, r.stats.d2h_bytes / 1e9, r.stats.evictions, r.map.size(),
+              r.cache_used / 1e9);
+      for (auto& kv : r.prof)
+        fprintf(stderr, "[xkblas_amd]   %-14s launches %6ld  time %10.3f ms  %8.2f TFLOP/s\n", kv.first.c_str(),
+                kv.second.count, kv.second.ms, kv.second.ms > 0 ? kv.second.flops / kv.second.ms / 1e9 : 0.0);
+    }
+  }
+  return 0;
+}
+
+int xkblas_sync(void)
+{
+  Lock l(g_mu);
+  ensure_init();
+  Runtime& r = rt();
+  r.sync_all();
+  // publish POTRF info values copied at the end of each factorization
+  return 0;
+}
+
+void xkblas_set_param(size_t nb, size_t p)
+{
+  (void)p;
+  Lock l(g_mu);
+  rt().nb = nb;
+}
+
+size_t xkblas_get_param(void)
+{
+  Lock l(g_mu);
+  return rt().nb;
+}
+
+int xkblas_get_ngpus(void) { return 1; }
+int xkblas_set_ngpus(int ngpus) { return ngpus == 1 ? 0 : -1; }
+
+int xkblas_get_devicecount(void)
+{
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n > 0 ? 1 : 0;
+}
+
+int xkblas_get_device_count(int* count)
+{
+  if (!count) return 22; /* EINVAL */
+  *count = xkblas_get_devicecount();
+  return 0;
+}
+
+void xkblas_set_modemath(xkblas_mode_math_t mode)
+{
+  Lock l(g_mu);
+  rt().mode = (int)mode;
+}
+
+const char* xkblas_get_configuration(void)
+{
+  return "xkblas_amd: gfx950 MFMA (FP64/FP32), HBM tile cache, split-BF16/FP8 CRT modes, 1 GPU";
+}
+
+uint64_t xkblas_elapsedns(void)
+{
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return (uint64_t)ts.tv_sec * 1000000000ull + (uint64_t)ts.tv_nsec;
+}
+
+double xkblas_elapsedtime(void) { return (double)xkblas_elapsedns() * 1e-9; }
+
+// Tile-size choice (blas/xkblas.c:2239-2530): the set NB wins; otherwise the
+// reference's per-kernel heuristic for one GPU.
+size_t xkblas_auto_tilesize(void* ctx, xkblas_kernel_t kernel, size_t M, size_t N, size_t K)
+{
+  (void)ctx;
+  (void)K;
+  size_t nb = rt().nb;
+  if (nb) return nb;
+  if (N > M) M = N;
+  switch (kernel) {
+    case KERN_SYRK:
+    case KERN_HERK: {
+      size_t fact = 2;
+      nb = M / fact;
+      if (nb >= 2048) nb &= ~(size_t)2047;
+      if (nb <= 1024) nb = M;
+      if (nb > 4096) nb = M / (2 * fact);
+      if (nb < 512) nb = 512;
+      break;
+    }
+    case KERN_TRSM:
+    case KERN_TRMM:
+      nb = M / 2;
+      nb = (nb + 63) & ~(size_t)63;
+      if (nb < 2048) nb = 2048;
+      break;
+    case KERN_GEMM:
+    case KERN_GEMMT:
+    default:
+      nb = M;
+      if (M > 16384 && nb < 2048) nb = 2048;
+      if (nb > 4096) nb = M / 2;
+      if (nb < 512) nb = 512;
+      nb = (nb + 63) & ~(size_t)63;
+      break;
+  }
+  return nb;
+}
+
+void* xkblas_malloc(size_t sz)
+{
+  void* p = nullptr;
+  if (hipHostMalloc(&p, sz, hipHostMallocDefault) != hipSuccess) return nullptr;
+  return p;
+}
+
+void xkblas_free(void* ptr, size_t sz)
+{
+  (void)sz;
+  if (ptr) (void)hipHostFree(ptr);
+}
+
+int xkblas_memory_coherent_async(int uplo, int memflag, size_t M, size_t N, void* A, size_t ld, size_t elsize)
+{
+  (void)ld;
+  (void)elsize;
+  Lock l(g_mu);
+  ensure_init();
+  Runtime& r = rt();
+  auto it = r.map.find(A);
+  if (it == r.map.end()) return 0;  // unknown matrix: nothing to do (reference returns 0)
+  Replica* rep = it->second;
+  if (uplo != 0 && uplo != Lower && uplo != Upper) {
+    fprintf(stderr, "[xkblas_amd] xkblas_memory_coherent_async: invalid argument uplo\n");
+    abort();
+  }
+  r.writeback(rep, 0, 0, (int)std::min<size_t>(M, rep->M), (int)std::min<size_t>(N, rep->N), uplo);
+  if (memflag == 1 && uplo != 0) {
+    // strict mode: device copies of the other triangle become useless
+    for (int tj = 0; tj < rep->nt; ++tj)
+      for (int ti = 0; ti < rep->mt; ++ti) {
+        if ((uplo == Lower && ti < tj) || (uplo == Upper && ti > tj))
+          rep->state[tj * rep->mt + ti] &= (uint8_t)~DEV_VALID;
+      }
+  }
+  return 0;
+}
+
+int xkblas_memory_syncall(void)
+{
+  {
+    Lock l(g_mu);
+    ensure_init();
+    Runtime& r = rt();
+    std::vector<Replica*> all;
+    for (auto& kv : r.map) all.push_back(kv.second);
+    for (Replica* rep : all) r.writeback(rep, 0, 0, rep->M, rep->N, 0);
+  }
+  return xkblas_sync();
+}
+
+int xkblas_memory_invalidate(const void* A)
+{
+  Lock l(g_mu);
+  Runtime& r = rt();
+  auto it = r.map.find(A);
+  if (it == r.map.end()) return 0;
+  r.sync_all();
+  r.drop(it->second);
+  return 0;
+}
+
+int xkblas_memory_invalidate_caches(void)
+{
+  Lock l(g_mu);
+  Runtime& r = rt();
+  if (!r.device_ready) return 0;
+  r.sync_all();
+  std::vector<Replica*> all;
+  for (auto& kv : r.map) all.push_back(kv.second);
+  for (Replica* rep : all) r.drop(rep);
+  return 0;
+}
+
+int xkblas_memory_free(void) { return xkblas_memory_invalidate_caches(); }
+
+void xkblas_set_split_count(int nsplit)
+{
+  Lock l(g_mu);
+  rt().nsplit = nsplit;
+}
+
+int xkblas_get_split_count(void)
+{
+  Lock l(g_mu);
+  return rt().nsplit;
+}
+
+int xkblas_potrf_info(const void* A)
+{
+  Lock l(g_mu);
+  Runtime& r = rt();
+  auto it = r.map.find(A);
+  if (it == r.map.end() || it->second->info_slot < 0) return -1;
+  const int v = r.h_info[it->second->info_slot];
+  return v == INT_MAX ? 0 : v;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- compute
+namespace {
+
+template <typename T>
+int gemm_async(const char* fn, int transA, int transB, int M, int N, int K, const T* alpha, const T* A, int LDA,
+               const T* B, int LDB, const T* beta, T* C, int LDC)
+{
+  // argument checks and codes of blas/zgemm.c:131-180
+  if (transA < NoTrans || transA > ConjTrans) return arg_error(fn, "illegal value of transA", -1);
+  if (transB < NoTrans || transB > ConjTrans) return arg_error(fn, "illegal value of transB", -2);
+  const int Am = transA == NoTrans ? M : K, An = transA == NoTrans ? K : M;
+  const int Bm = transB == NoTrans ? K : N, Bn = transB == NoTrans ? N : K;
+  if (M < 0) return arg_error(fn, "illegal value of M", -3);
+  if (N < 0) return arg_error(fn, "illegal value of N", -4);
+  if (K < 0) return arg_error(fn, "illegal value of K", -5);
+  if (LDA < std::max(1, Am)) return arg_error(fn, "illegal value of LDA", -8);
+  if (LDB < std::max(1, Bm)) return arg_error(fn, "illegal value of LDB", -10);
+  if (LDC < std::max(1, M)) return arg_error(fn, "illegal value of LDC", -13);
+  if (M == 0 || N == 0 || ((*alpha == T(0) || K == 0) && *beta == T(1))) return 0;  // quick return
+
+  Lock l(g_mu);
+  ensure_init();
+  Runtime& r = rt();
+  const int nb = (int)xkblas_auto_tilesize(nullptr, KERN_GEMM, M, N, K);
+  std::vector<Replica*> keep;
+  const bool use_ab = K > 0 && *alpha != T(0);
+  Replica* rA = use_ab ? r.replica(A, Am, An, LDA, sizeof(T), nb, keep) : nullptr;
+  Replica* rB = use_ab ? r.replica(B, Bm, Bn, LDB, sizeof(T), nb, keep) : nullptr;
+  Replica* rC = r.replica(C, M, N, LDC, sizeof(T), nb, keep);
+  if (rA) r.fetch(rA, 0, 0, Am, An);
+  if (rB) r.fetch(rB, 0, 0, Bm, Bn);
+  // C is write-only when beta == 0 (reference access mode W, ztask_gemm.c:133-134)
+  if (*beta != T(0)) r.fetch(rC, 0, 0, M, N);
+  r.before_compute();
+  Exec x = r.exec();
+  const double flops = 2.0 * M * N * (double)K;
+  r.prof_begin(sizeof(T) == 8 ? "gemm_f64" : "gemm_f32", flops);
+  dev_gemm<T>(x, transA, transB, M, N, use_ab ? K : 0, use_ab ? *alpha : T(0), rA ? (const T*)rA->dev : (const T*)rC->dev,
+              rA ? rA->ldd : rC->ldd, rB ? (const T*)rB->dev : (const T*)rC->dev, rB ? rB->ldd : rC->ldd, *beta,
+              (T*)rC->dev, rC->ldd);
+  r.prof_end();
+  r.mark_written(rC, 0, 0, M, N, 0);
+  return 0;
+}
+
+template <typename T>
+int syrk_async(const char* fn, int uplo, int trans, int N, int K, const T* alpha, const T* A, int LDA,
+               const T* beta, T* C, int LDC)
+{
+  // checks of blas/zsyrk.c:111-160 (same order and codes)
+  if (uplo != Upper && uplo != Lower) return arg_error(fn, "illegal value of uplo", -1);
+  if (trans != NoTrans && trans != Trans && trans != ConjTrans) return arg_error(fn, "illegal value of trans", -2);
+  const int Am = trans == NoTrans ? N : K, An = trans == NoTrans ? K : N;
+  if (N < 0) return arg_error(fn, "illegal value of N", -3);
+  if (K < 0) return arg_error(fn, "illegal value of K", -4);
+  if (LDA < std::max(1, Am)) return arg_error(fn, "illegal value of LDA", -7);
+  if (LDC < std::max(1, N)) return arg_error(fn, "illegal value of LDC", -10);
+  if (N == 0 || ((*alpha == T(0) || K == 0) && *beta == T(1))) return 0;
+
+  Lock l(g_mu);
+  ensure_init();
+  Runtime& r = rt();
+  const int nb = (int)xkblas_auto_tilesize(nullptr, KERN_SYRK, N, N, K);
+  std::vector<Replica*> keep;
+  const bool use_a = K > 0 && *alpha != T(0);
+  Replica* rA = use_a ? r.replica(A, Am, An, LDA, sizeof(T), nb, keep) : nullptr;
+  Replica* rC = r.replica(C, N, N, LDC, sizeof(T), nb, keep);
+  if (rA) r.fetch(rA, 0, 0, Am, An);
+  r.fetch(rC, 0, 0, N, N);  // the other triangle must stay intact on the device
+  r.before_compute();
+  Exec x = r.exec();
+  r.prof_begin(sizeof(T) == 8 ? "syrk_f64" : "syrk_f32", (double)K * N * (N + 1.0));
+  dev_syrk<T>(x, uplo, trans, N, use_a ? K : 0, use_a ? *alpha : T(0), rA ? (const T*)rA->dev : (const T*)rC->dev,
+              rA ? rA->ldd : rC->ldd, *beta, (T*)rC->dev, rC->ldd);
+  r.prof_end();
+  r.mark_written(rC, 0, 0, N, N, uplo);
+  return 0;
+}
+
+template <typename T>
+int trsm_async(const char* fn, int side, int uplo, int transA, int diag, int N, int NRHS, const T* alpha, const T* A,
+               int LDA, T* B, int LDB)
+{
+  // checks of blas/ztrsm.c:117-167; N = rows of B, NRHS = columns
+  const int NA = side == Left ? N : NRHS;
+  if (side != Left && side != Right) return arg_error(fn, "illegal value of side", -1);
+  if (uplo != Upper && uplo != Lower) return arg_error(fn, "illegal value of uplo", -2);
+  if (transA < NoTrans || transA > ConjTrans) return arg_error(fn, "illegal value of transA", -3);
+  if (diag != Unit && diag != NonUnit) return arg_error(fn, "illegal value of diag", -4);
+  if (N < 0) return arg_error(fn, "illegal value of N", -5);
+  if (NRHS < 0) return arg_error(fn, "illegal value of NRHS", -6);
+  if (LDA < std::max(1, NA)) return arg_error(fn, "illegal value of LDA", -8);
+  if (LDB < std::max(1, N)) return arg_error(fn, "illegal value of LDB", -10);
+  if (std::min(N, NRHS) == 0) return 0;
+
+  Lock l(g_mu);
+  ensure_init();
+  Runtime& r = rt();
+  const int nb = (int)xkblas_auto_tilesize(nullptr, KERN_TRSM, N, NRHS, NA);
+  std::vector<Replica*> keep;
+  Replica* rA = r.replica(A, NA, NA, LDA, sizeof(T), nb, keep);
+  Replica* rB = r.replica(B, N, NRHS, LDB, sizeof(T), nb, keep);
+  r.fetch(rA, 0, 0, NA, NA);
+  r.fetch(rB, 0, 0, N, NRHS);
+  r.before_compute();
+  Exec x = r.exec();
+  const double flops = side == Left ? (double)N * N * NRHS : (double)N * NRHS * NRHS;
+  r.prof_begin(sizeof(T) == 8 ? "trsm_f64" : "trsm_f32", flops);
+  // inner diagonal block order: the tile size, capped for the inverse GEMMs
+  const int dnb = std::min(nb, 512);
+  dev_trsm<T>(x, side, uplo, transA, diag, N, NRHS, *alpha, (const T*)rA->dev, rA->ldd, (T*)rB->dev, rB->ldd, dnb);
+  r.prof_end();
+  r.mark_written(rB, 0, 0, N, NRHS, 0);
+  return 0;
+}
+
+template <typename T>
+int potrf_async(const char* fn, int uplo, int N, T* A, int LDA)
+{
+  // checks of blas/zpotrf.c:57-68
+  if (uplo != Upper && uplo != Lower) return arg_error(fn, "illegal value of uplo", -2);
+  if (N < 0) return arg_error(fn, "illegal value of N", -5);
+  if (LDA < std::max(1, N)) return arg_error(fn, "illegal value of LDA", -8);
+  Lock l(g_mu);
+  ensure_init();
+  Runtime& r = rt();
+  std::vector<Replica*> keep;
+  if (N == 0) return 0;
+  const int nb = (int)xkblas_auto_tilesize(nullptr, KERN_POTRF, N, N, N);
+  Replica* rA = r.replica(A, N, N, LDA, sizeof(T), nb, keep);
+  if (rA->info_slot < 0) rA->info_slot = r.next_info++ % Runtime::INFO_SLOTS;
+  r.fetch(rA, 0, 0, N, N);
+  r.before_compute();
+  Exec x = r.exec();
+  int* dinfo = r.d_info + rA->info_slot;
+  fill_int(x.s, dinfo, 1, INT_MAX);
+  r.prof_begin(sizeof(T) == 8 ? "potrf_f64" : "potrf_f32", (double)N * N * N / 3.0 + (double)N * N / 2.0 + N / 6.0);
+  dev_potrf<T>(x, uplo, N, (T*)rA->dev, rA->ldd, nb, dinfo);
+  r.prof_end();
+  XK_HIP(hipMemcpyAsync(r.h_info + rA->info_slot, dinfo, sizeof(int), hipMemcpyDeviceToHost, x.s));
+  r.mark_written(rA, 0, 0, N, N, uplo);
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int xkblas_dgemm_async(int transA, int transB, int M, int N, int K, const double* alpha, const double* A, int LDA,
+                       const double* B, int LDB, const double* beta, double* C, int LDC)
+{
+  return gemm_async<double>("xkblas_dgemm_async", transA, transB, M, N, K, alpha, A, LDA, B, LDB, beta, C, LDC);
+}
+
+int xkblas_sgemm_async(int transA, int transB, int M, int N, int K, const float* alpha, const float* A, int LDA,
+                       const float* B, int LDB, const float* beta, float* C, int LDC)
+{
+  return gemm_async<float>("xkblas_sgemm_async", transA, transB, M, N, K, alpha, A, LDA, B, LDB, beta, C, LDC);
+}
+
+int xkblas_dsyrk_async(int uplo, int trans, int N, int K, const double* alpha, const double* A, int LDA,
+                       const double* beta, double* C, int LDC)
+{
+  return syrk_async<double>("xkblas_dsyrk_async", uplo, trans, N, K, alpha, A, LDA, beta, C, LDC);
+}
+
+int xkblas_ssyrk_async(int uplo, int trans, int N, int K, const float* alpha, const float* A, int LDA,
+                       const float* beta, float* C, int LDC)
+{
+  return syrk_async<float>("xkblas_ssyrk_async", uplo, trans, N, K, alpha, A, LDA, beta, C, LDC);
+}
+
+int xkblas_dtrsm_async(int side, int uplo, int transA, int diag, int N, int NRHS, const double* alpha,
+                       const double* A, int LDA, double* B, int LDB)
+{
+  return trsm_async<double>("xkblas_dtrsm_async", side, uplo, transA, diag, N, NRHS, alpha, A, LDA, B, LDB);
+}
+
+int xkblas_strsm_async(int side, int uplo, int transA, int diag, int N, int NRHS, const float* alpha,
+                       const float* A, int LDA, float* B, int LDB)
+{
+  return trsm_async<float>("xkblas_strsm_async", side, uplo, transA, diag, N, NRHS, alpha, A, LDA, B, LDB);
+}
+
+int xkblas_dpotrf_async(int uplo, int N, double* A, int LDA)
+{
+  return potrf_async<double>("xkblas_dpotrf_async", uplo, N, A, LDA);
+}
+
+int xkblas_spotrf_async(int uplo, int N, float* A, int LDA)
+{
+  return potrf_async<float>("xkblas_spotrf_async", uplo, N, A, LDA);
+}
+
+// ---- profiling extension
+void xkblas_profile_enable(int on)
+{
+  Lock l(g_mu);
+  rt().profiling = on != 0;
+}
+
+void xkblas_profile_reset(void)
+{
+  Lock l(g_mu);
+  rt().sync_all();
+  rt().prof.clear();
+}
+
+int xkblas_profile_get(const char* cls, long* launches, double* total_ms, double* flops)
+{
+  Lock l(g_mu);
+  Runtime& r = rt();
+  auto it = r.prof.find(cls);
+  if (it == r.prof.end()) {
+    if (launches) *launches = 0;
+    if (total_ms) *total_ms = 0;
+    if (flops) *flops = 0;
+    return -1;
+  }
+  if (launches) *launches = it->second.count;
+  if (total_ms) *total_ms = it->second.ms;
+  if (flops) *flops = it->second.flops;
+  return 0;
+}
+
+void xkblas_set_lookahead(int on)
+{
+  Lock l(g_mu);
+  rt().lookahead = on != 0;
+}
+
+long xkblas_cache_bytes(void)
+{
+  Lock l(g_mu);
+  return (long)rt().cache_used;
+}
+
+long xkblas_cache_evictions(void)
+{
+  Lock l(g_mu);
+  return rt().stats.evictions;
+}
+
+void xkblas_set_cache_limit(size_t bytes)
+{
+  Lock l(g_mu);
+  ensure_init();
+  rt().cache_limit = bytes;
+}
+
+}  // extern "C"

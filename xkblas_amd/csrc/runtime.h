@@ -1,0 +1,92 @@
+// runtime.h — state of the single-GPU runtime behind xkblas.h.
+#pragma once
+#include "blas3.h"
+#include <map>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+namespace xk {
+
+enum : uint8_t { DEV_VALID = 1, HOST_VALID = 2 };
+
+// Device mirror of one host matrix (the reference's matrix descriptor plus
+// the DSM replica state of its tiles, blas/common.gen:167-187,
+// kaapi/kaapi_memory.h:151-163).
+struct Replica {
+  const void* host = nullptr;
+  int M = 0, N = 0;
+  long ld = 0;
+  int elt = 8;
+  int nb = 512, mt = 0, nt = 0;
+  char* dev = nullptr;
+  long ldd = 0;
+  size_t bytes = 0;
+  std::vector<uint8_t> state;  // per tile (column-major tile order)
+  uint64_t lru = 0;
+  int info_slot = -1;
+};
+
+struct ProfEntry {
+  const char* cls;
+  double flops;
+  hipEvent_t b, e;
+};
+struct ProfStat {
+  long count = 0;
+  double ms = 0.0, flops = 0.0;
+};
+
+struct Runtime {
+  static constexpr int NSCRATCH = 16;
+  static constexpr int INFO_SLOTS = 1024;
+
+  bool device_ready = false;
+  int refcount = 0;
+  size_t nb = 0;  // 0 = heuristic per call
+  int mode = 0;
+  int nsplit = 0;
+  bool lookahead = true;
+  bool profiling = false;
+
+  hipStream_t s_comp = nullptr, s_diag = nullptr, s_h2d = nullptr, s_d2h = nullptr;
+  hipEvent_t ev_comp = nullptr, ev_h2d = nullptr, ev_d2h = nullptr;
+  bool pending_h2d = false, pending_d2h = false;
+
+  std::unordered_map<const void*, Replica*> map;
+  size_t cache_used = 0, cache_limit = 0;
+  uint64_t clock = 0;
+
+  void* scratch_ptr[NSCRATCH] = {};
+  size_t scratch_size[NSCRATCH] = {};
+
+  int* d_info = nullptr;
+  int* h_info = nullptr;
+  int next_info = 0;
+
+  struct {
+    double h2d_bytes = 0, d2h_bytes = 0;
+    long evictions = 0;
+  } stats;
+
+  std::vector<ProfEntry> prof_open, prof_pending;
+  std::map<std::string, ProfStat> prof;
+
+  void init_device();
+  void sync_all();
+  Replica* replica(const void* host, int M, int N, long ld, int elt, int nb, std::vector<Replica*>& keep);
+  void fetch(Replica* r, int r0, int c0, int m, int n);
+  void mark_written(Replica* r, int r0, int c0, int m, int n, int uplo);
+  void writeback(Replica* r, int r0, int c0, int m, int n, int uplo);
+  void evict_for(size_t need, const std::vector<Replica*>& keep);
+  void drop(Replica* r);
+  void before_compute();
+  Exec exec() const;
+  void prof_begin(const char* cls, double flops);
+  void prof_end();
+  void prof_collect();
+};
+
+Runtime& rt();
+
+}  // namespace xk

@@ -1,0 +1,17 @@
+// split.h — FP64-accurate products from BF16 / FP8 residue matrices.
+#pragma once
+#include "blas3.h"
+
+namespace xk {
+
+// True when x.mode asks for a split mode and T is double.
+template <typename T>
+bool split_enabled(const Exec& x);
+
+// C <- alpha op(A) op(B) + beta C (tri: written triangle) computed from CRT
+// residues of row/column-scaled integer images of op(A), op(B).
+template <typename T>
+void split_gemm(const Exec& x, int transA, int transB, int m, int n, int k, T alpha, const T* A, long lda,
+                const T* B, long ldb, T beta, T* C, long ldc, int tri);
+
+}  // namespace xk
