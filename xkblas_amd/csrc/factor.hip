@@ -2,13 +2,13 @@
 // Cholesky + inverse of blocks of order <= 64, triangular inverse, and small
 // helpers (2-D copy, triangle zeroing, int fill).
 //
-// A leaf is one wave (64 lanes) per block.  Lane i keeps row i of the block
-// in registers; the padded rows/columns beyond n are the identity, so every
-// block is handled as a 64 x 64 problem.  Step j broadcasts the pivot with
-// v_readlane, scales column j, publishes it through LDS and applies the rank-1
-// update to the lane's own row.  The inverse is then a forward substitution
-// per column (lane j owns column j of inv(L)), reading L from LDS with
-// broadcast reads.
+// A leaf is one wave (64 lanes) per block, held in LDS (column stride 65 so
+// row and column walks are conflict-free); the padded rows/columns beyond n
+// are the identity, so every block is handled as a 64 x 64 problem.  Step j
+// scales column j by the pivot and lane i applies the rank-1 update to row i
+// (the column values are broadcast reads).  The inverse is then a forward
+// substitution per column (lane j owns column j of inv(L)), launched as a
+// second kernel on the factor.
 #include "common.h"
 #include "kernels.h"
 #include <climits>
@@ -73,47 +73,48 @@ __global__ __launch_bounds__(64) void potrf_leaf_kernel(int uplo, int n, T* A, l
                                                         long ldinv, long sInv, int* info, int base)
 {
   __shared__ T Ls[LEAF * LLD];
-  __shared__ T col[2][LEAF];
   const int lane = threadIdx.x;
   A += (long)blockIdx.x * sA;
   Tinv += (long)blockIdx.x * sInv;
+#pragma unroll 4
+  for (int j = 0; j < LEAF; ++j) Ls[lane + j * LLD] = lower_view(A, lda, uplo, n, lane, j);
+  __syncthreads();
 
-  T a[LEAF];
-#pragma unroll
-  for (int j = 0; j < LEAF; ++j) a[j] = lower_view(A, lda, uplo, n, lane, j);
-
+  // right-looking column sweep; lane i owns row i of the trailing update
   bool failed = false;
-#pragma unroll
   for (int j = 0; j < LEAF; ++j) {
-    T d = bcast(a[j], j);
+    const T d = Ls[j + j * LLD];
     if (!(d > T(0)) && !failed && j < n) {
       failed = true;
       if (lane == 0) atomicMin(info, base + j + 1);
     }
-    d = sqrt(d);
-    const T inv = T(1) / d;
-    T l = (lane > j) ? a[j] * inv : (lane == j ? d : T(0));
-    a[j] = l;
-    col[j & 1][lane] = l;
+    const T s = sqrt(d);
+    const T v = Ls[lane + j * LLD];
     __syncthreads();
-#pragma unroll
-    for (int k = j + 1; k < LEAF; ++k) {
-      const T lk = col[j & 1][k];
-      if (lane >= k) a[k] -= l * lk;
+    const T l = lane > j ? v / s : (lane == j ? s : T(0));
+    if (lane >= j) Ls[lane + j * LLD] = l;
+    __syncthreads();
+    if (lane > j) {
+      int k = j + 1;
+      for (; k + 3 <= lane; k += 4) {
+        const T c0 = Ls[k + j * LLD], c1 = Ls[k + 1 + j * LLD], c2 = Ls[k + 2 + j * LLD], c3 = Ls[k + 3 + j * LLD];
+        Ls[lane + k * LLD] -= l * c0;
+        Ls[lane + (k + 1) * LLD] -= l * c1;
+        Ls[lane + (k + 2) * LLD] -= l * c2;
+        Ls[lane + (k + 3) * LLD] -= l * c3;
+      }
+      for (; k <= lane; ++k) Ls[lane + k * LLD] -= l * Ls[k + j * LLD];
     }
+    __syncthreads();
   }
-  // write the factor back (only the stored triangle) and stage it in LDS
-#pragma unroll
+  // factor back to the stored triangle
+#pragma unroll 4
   for (int j = 0; j < LEAF; ++j) {
-    Ls[lane + j * LLD] = a[j];
     if (lane < n && j < n && j <= lane) {
-      if (uplo == Lower) A[lane + (long)j * lda] = a[j];
-      else A[j + (long)lane * lda] = a[j];
+      const T x = Ls[lane + j * LLD];
+      if (uplo == Lower) A[lane + (long)j * lda] = x;
+      else A[j + (long)lane * lda] = x;
     }
-  }
-  __syncthreads();
-  if (Tinv) {
-    lower_inverse_to(Ls, lane, false, Tinv, ldinv, n, uplo);
   }
 }
 
@@ -140,6 +141,9 @@ void potrf_leaf(hipStream_t s, int uplo, int n, T* A, long lda, long sA, T* Tinv
   hipLaunchKernelGGL((potrf_leaf_kernel<T>), dim3(batch), dim3(64), 0, s, uplo, n, A, lda, sA, Tinv, ldinv, sInv,
                      info, base);
   XK_HIP(hipGetLastError());
+  // inverse of the fresh factor (a separate launch keeps both kernels
+  // within the register file)
+  if (Tinv) trtri_leaf<T>(s, uplo, NonUnit, n, A, lda, sA, Tinv, ldinv, sInv, batch);
 }
 
 template <typename T>
