@@ -42,6 +42,7 @@ def lib():
         L.oracle_tile_dpotrf.argtypes = [i, i, p, i]
         L.oracle_dgemm_dd.argtypes = [i, i, i, i, i, d, p, i, p, i, d, p, i, p, p]
         L.oracle_plgsy.argtypes = [d, i, p, i, ctypes.c_ulonglong]
+        L.oracle_dd_entries.argtypes = [i, i, i, d, p, i, p, i, d, p, i, i, p, p, p, p]
         L.oracle_hostblas_open.argtypes = [ctypes.c_char_p]
         L.oracle_host_dgemm.argtypes = [i, i, i, i, i, d, p, i, p, i, d, p, i]
         L.oracle_host_dsyrk.argtypes = [i, i, i, i, d, p, i, d, p, i]
@@ -99,6 +100,19 @@ def dgemm_dd(ta, tb, alpha, A, B, beta, C):
     lo = np.zeros((M, N), order="F")
     lib().oracle_dgemm_dd(ta, tb, M, N, K, alpha, _p(A), A.shape[0], _p(B), B.shape[0], beta, _p(C), C.shape[0],
                           _p(hi), _p(lo))
+    return hi, lo
+
+
+def dd_entries(ta, tb, alpha, A, B, beta, C0, rows, cols):
+    """Double-double (hi, lo) of entries (rows[e], cols[e]) of alpha op(A) op(B) + beta C0."""
+    K = A.shape[1] if ta == NoTrans else A.shape[0]
+    rows = np.ascontiguousarray(rows, dtype=np.int32)
+    cols = np.ascontiguousarray(cols, dtype=np.int32)
+    hi = np.zeros(len(rows))
+    lo = np.zeros(len(rows))
+    lib().oracle_dd_entries(ta, tb, K, alpha, _p(A), A.shape[0], _p(B), B.shape[0], beta, _p(C0), C0.shape[0],
+                            len(rows), rows.ctypes.data_as(ctypes.c_void_p), cols.ctypes.data_as(ctypes.c_void_p),
+                            hi.ctypes.data_as(ctypes.c_void_p), lo.ctypes.data_as(ctypes.c_void_p))
     return hi, lo
 
 

@@ -528,3 +528,31 @@ int oracle_host_dpotrf(int uplo, int n, double* A, int lda)
   p_dpotrf(&u, &n, A, &lda, &info);
   return info;
 }
+
+/* Double-double value of selected entries (rows[e], cols[e]) of
+ * alpha op(A) op(B) + beta C0, for accuracy statements on large products
+ * where the full double-double GEMM would be too slow. */
+void oracle_dd_entries(int transA, int transB, int k, double alpha, const double* A, int lda, const double* B,
+                       int ldb, double beta, const double* C0, int ldc, int ne, const int* rows, const int* cols,
+                       double* hi_out, double* lo_out)
+{
+  for (int e = 0; e < ne; ++e) {
+    const int i = rows[e], j = cols[e];
+    double hi = 0.0, lo = 0.0;
+    for (int l = 0; l < k; ++l) {
+      double a = opval(A, lda, transA, i, l), b = opval(B, ldb, transB, l, j);
+      double p = a * b, pe = fma(a, b, -p);
+      dd_add(&hi, &lo, p, pe);
+    }
+    double ph = hi * alpha, pe = fma(hi, alpha, -ph) + lo * alpha;
+    double rh = 0.0, rl = 0.0;
+    dd_add(&rh, &rl, ph, pe);
+    if (beta != 0.0) {
+      double c = C0[IDX(i, j, ldc)];
+      double qh = c * beta, qe = fma(c, beta, -qh);
+      dd_add(&rh, &rl, qh, qe);
+    }
+    hi_out[e] = rh;
+    lo_out[e] = rl;
+  }
+}
