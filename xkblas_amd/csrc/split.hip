@@ -6,12 +6,12 @@
 //     t is chosen so that k * 2^(2t) < P/2, P = p_1 ... p_N (pairwise coprime
 //     moduli <= 256), which makes the integer product C' = A' B' recoverable
 //     from its residues;
-//  2. split: residues A' mod p_l, B' mod p_l in [-128, 127] are exact BF16
+//  2. split: residues A' mod p_l, B' mod p_l in [-127, 127] are exact BF16
 //     values; all N residue matrices are written in one pass (k-contiguous
 //     rows, padded to the GEMM tile);
 //  3. multiply: for each modulus one BF16 MFMA GEMM (v_mfma_f32_32x32x16_bf16)
 //     accumulates in FP32.  Products are <= 2^14 in magnitude, so FP32 sums
-//     stay exact; every 512 k the accumulator is reduced mod p (exactly, by
+//     stay exact; every 1024 k the accumulator is reduced mod p (exactly, by
 //     an FMA) before it could pass 2^24.  The output is the residue of
 //     C' mod p_l as int8;
 //  4. recombine: C' = sum_l r_l W_l mod P, W_l = (P/p_l) * ((P/p_l)^-1 mod
@@ -30,7 +30,9 @@
 
 namespace xk {
 
-static const int kModuli[15] = {256, 255, 253, 251, 247, 241, 239, 233, 229, 227, 223, 217, 211, 199, 197};
+// Pairwise coprime odd moduli <= 255: residues fit [-127, 127], so a
+// product is < 2^14 and 1024 of them plus a folded remainder stay below 2^24.
+static const int kModuli[15] = {255, 253, 251, 247, 241, 239, 233, 229, 227, 223, 217, 211, 199, 197, 193};
 constexpr int MAXMOD = 15;
 constexpr double LIMB = 2199023255552.0;  // 2^41
 
@@ -176,11 +178,11 @@ struct SplitParams {
 __device__ __forceinline__ double sym_mod(double x, double p, double invp)
 {
   // x integer, |x| < 2^53: x - p * rint(x / p) is exact (FMA of integers);
-  // then fold into [-floor(p/2), ceil(p/2) - 1]
+  // then fold into [-(p-1)/2, (p-1)/2] (p odd)
   double r = fma(-p, rint(x * invp), x);
-  const double hi = floor((p - 1.0) * 0.5);
+  const double hi = (p - 1.0) * 0.5;
   if (r > hi) r -= p;
-  if (r < -hi - (p == 256.0 ? 1.0 : 0.0)) r += p;
+  if (r < -hi) r += p;
   return r;
 }
 
@@ -237,7 +239,7 @@ struct ModGemmParams {
 
 constexpr int MG_BK = 64, MG_LK = MG_BK + 8;
 constexpr int MG_PAD = 256;          // row padding of the residue matrices
-constexpr int MG_RED = 512 / MG_BK;  // k-steps between modular reductions
+constexpr int MG_RED = 1024 / MG_BK;  // k-steps between modular reductions
 
 __device__ __forceinline__ float fold(float v, float p, float ip)
 {
@@ -351,8 +353,7 @@ __global__ __launch_bounds__(256) void mod_gemm_kernel(ModGemmParams prm)
   }
 
   // residues in [-128, 127] as int8; D = (B A^T) block: lane -> m, (lane, r) -> n
-  const float hi = floorf((p - 1.0f) * 0.5f);
-  const float lo = -hi - (p == 256.0f ? 1.0f : 0.0f);
+  const float hi = (p - 1.0f) * 0.5f, lo = -hi;
   int8_t* C = prm.C + (long)l * prm.sC;
 #pragma unroll
   for (int i = 0; i < TI; ++i) {
@@ -366,6 +367,166 @@ __global__ __launch_bounds__(256) void mod_gemm_kernel(ModGemmParams prm)
         if (v > hi) v -= p;
         if (v < lo) v += p;
         C[gn * prm.ldc + gm] = (int8_t)(int)v;
+      }
+    }
+  }
+}
+
+// ---- pipelined variant: 256 x 256 tile, 4-stage LDS ring filled by
+// global_load_lds (16 bytes per lane straight into LDS, no VGPR staging).
+// Each 1 KB LDS block holds 16 rows of one stage (BK = 32 bf16 = 64 bytes per
+// row); the 16-byte chunk c of row r sits at chunk c ^ ((r >> 2) & 3), which
+// makes the MFMA operand reads (16 rows x one chunk per 16 lanes) hit 16
+// distinct 16-byte bank groups.  One barrier per stage: after it, the stage
+// read in the previous iteration is free and receives the loads of the stage
+// STAGES-1 ahead.
+constexpr int DM_BM = 256, DM_BN = 256, DM_BK = 32, DM_STAGES = 4;
+constexpr int DM_FOLD = 1024 / DM_BK;  // stages between modular reductions
+
+typedef __attribute__((address_space(3))) void lds_void;
+
+__device__ __forceinline__ void wait_vmcnt_le(int n)
+{
+  // s_waitcnt with only vmcnt constrained (expcnt, lgkmcnt at maximum)
+  if (n >= 16) __builtin_amdgcn_s_waitcnt((16 & 0xf) | (0x7 << 4) | (0xf << 8) | ((16 >> 4) << 14));
+  else if (n >= 8) __builtin_amdgcn_s_waitcnt((8 & 0xf) | (0x7 << 4) | (0xf << 8));
+  else __builtin_amdgcn_s_waitcnt((0x7 << 4) | (0xf << 8));
+}
+
+__device__ __forceinline__ void wait_lgkm0()
+{
+  __builtin_amdgcn_s_waitcnt((0xf) | (0x7 << 4) | (0x0 << 8) | (0x3 << 14));
+}
+
+__global__ __launch_bounds__(256) void mod_gemm_dma_kernel(ModGemmParams prm)
+{
+  // [stage][A rows 256 x 32 | B rows 256 x 32]
+  __shared__ __attribute__((aligned(1024))) __bf16 S[DM_STAGES][(DM_BM + DM_BN) * DM_BK];
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+  const int l = blockIdx.z;
+  const float p = prm.p[l], ip = prm.invp[l];
+
+  int tm, tn;
+  {
+    int pid = blockIdx.x;
+    const int total = gridDim.x;
+    if ((total & 7) == 0) pid = (pid & 7) * (total >> 3) + (pid >> 3);
+    if (prm.tri == TRI_FULL) {
+      constexpr int GM = 4;
+      const int group = GM * prm.tiles_n;
+      const int g = pid / group;
+      const int first = g * GM;
+      const int gm = min(prm.tiles_m - first, GM);
+      const int r = pid - g * group;
+      tm = first + r % gm;
+      tn = r / gm;
+    } else {
+      int i = (int)((sqrtf(8.0f * (float)pid + 1.0f) - 1.0f) * 0.5f);
+      while ((i + 1) * (i + 2) / 2 <= pid) ++i;
+      while (i * (i + 1) / 2 > pid) --i;
+      const int j = pid - i * (i + 1) / 2;
+      if (prm.tri == TRI_LOWER) { tm = i; tn = j; }
+      else { tm = j; tn = i; }
+    }
+  }
+  const long m0 = (long)tm * DM_BM, n0 = (long)tn * DM_BN;
+  const __bf16* A = prm.A + (long)l * prm.sA + m0 * prm.lk;
+  const __bf16* B = prm.B + (long)l * prm.sB + n0 * prm.lk;
+
+  // this lane's source for LDS block b = wave*4 + i: row 16b + lane/4 and the
+  // chunk stored at physical position lane%4 (the swizzle term depends only
+  // on lane/4 because 16b is a multiple of 16)
+  const int srow = lane >> 2;
+  const int schunk = (lane & 3) ^ ((srow >> 2) & 3);
+  const __bf16* gA = A + (long)(wave * 64 + srow) * prm.lk + schunk * 8;
+  const __bf16* gB = B + (long)(wave * 64 + srow) * prm.lk + schunk * 8;
+  const long step16 = 16 * prm.lk;
+  auto issue = [&](int stage, int k0) {
+    __bf16* base = S[stage] + wave * 2048;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      __builtin_amdgcn_global_load_lds((const void*)(gA + i * step16 + k0), (lds_void*)(base + i * 512), 16, 0, 0);
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      __builtin_amdgcn_global_load_lds((const void*)(gB + i * step16 + k0),
+                                       (lds_void*)(base + DM_BM * DM_BK + i * 512), 16, 0, 0);
+  };
+
+  f32x16 acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
+
+  const int nk = prm.kpad / DM_BK;
+  for (int s = 0; s < DM_STAGES - 1 && s < nk; ++s) issue(s, s * DM_BK);
+
+  // operand reads: row (wm*128 + 32i + lane%32), logical chunk 2ks + lane/32;
+  // the swizzle term depends on lane%32 only, so rows of different i are at
+  // constant offsets
+  const int fr = lane & 31, fh = lane >> 5;
+  const int sw = (fr >> 2) & 3;
+  const int rdA0 = (wm * 128 + fr) * DM_BK + ((0 + fh) ^ sw) * 8;
+  const int rdA1 = (wm * 128 + fr) * DM_BK + ((2 + fh) ^ sw) * 8;
+  const int rdB0 = DM_BM * DM_BK + (wn * 128 + fr) * DM_BK + ((0 + fh) ^ sw) * 8;
+  const int rdB1 = DM_BM * DM_BK + (wn * 128 + fr) * DM_BK + ((2 + fh) ^ sw) * 8;
+
+  // k loop as chunks of DM_FOLD stages; the accumulators are folded mod p
+  // between chunks, outside the MFMA loop
+  for (int c0 = 0; c0 < nk; c0 += DM_FOLD) {
+    const int c1 = min(nk, c0 + DM_FOLD);
+    for (int kt = c0; kt < c1; ++kt) {
+      // stage kt landed (loads of later stages may still be in flight)
+      const int ahead = min(nk - 1, kt + DM_STAGES - 2) - kt;  // stages issued after kt
+      wait_vmcnt_le(ahead * 8);
+      __builtin_amdgcn_s_barrier();
+      if (kt + DM_STAGES - 1 < nk) issue((kt + DM_STAGES - 1) % DM_STAGES, (kt + DM_STAGES - 1) * DM_BK);
+      const __bf16* st = S[kt % DM_STAGES];
+#pragma unroll
+      for (int ks = 0; ks < 2; ++ks) {
+        const __bf16* pa = st + (ks ? rdA1 : rdA0);
+        const __bf16* pb = st + (ks ? rdB1 : rdB0);
+        bf16x8 a[4], b[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) a[i] = *reinterpret_cast<const bf16x8*>(pa + i * 32 * DM_BK);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) b[j] = *reinterpret_cast<const bf16x8*>(pb + j * 32 * DM_BK);
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+          for (int j = 0; j < 4; ++j)
+            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(b[j], a[i], acc[i][j], 0, 0, 0);
+      }
+      // this wave's reads of stage kt are complete before the next barrier
+      wait_lgkm0();
+    }
+    if (c1 < nk) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+          for (int r = 0; r < 16; ++r) acc[i][j][r] = fold(acc[i][j][r], p, ip);
+    }
+  }
+
+  const float hi = (p - 1.0f) * 0.5f;
+  int8_t* C = prm.C + (long)l * prm.sC + (n0 + wn * 128 + 4 * fh) * prm.ldc + m0 + wm * 128 + fr;
+  const long ldc = prm.ldc;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        float v = fold(acc[i][j][r], p, ip);
+        if (v > hi) v -= p;
+        if (v < -hi) v += p;
+        C[(long)(j * 32 + 8 * (r >> 2) + (r & 3)) * ldc + i * 32] = (int8_t)(int)v;
       }
     }
   }
@@ -442,6 +603,9 @@ static inline long rup(long v, long q) { return (v + q - 1) / q * q; }
 
 enum { WS_SA = 8, WS_SB = 9, WS_SC = 10, WS_SE = 11 };
 
+// 0: pipelined 256 x 256 DMA kernel, 1: register-staged 256 x 128 kernel
+static int g_mod_gemm_variant = getenv("XKBLAS_SPLIT_KERNEL") ? atoi(getenv("XKBLAS_SPLIT_KERNEL")) : 0;
+
 template <>
 void split_gemm<double>(const Exec& x, int transA, int transB, int m, int n, int k, double alpha, const double* A,
                         long lda, const double* B, long ldb, double beta, double* C, long ldc, int tri)
@@ -452,6 +616,7 @@ void split_gemm<double>(const Exec& x, int transA, int transB, int m, int n, int
   const int tb = side_bits(M, k);
   XK_CHECK(tb >= 8, "split count too small for this k");
   const long mpad = rup(m, MG_PAD), npad = rup(n, MG_PAD), kpad = rup(k, MG_BK);
+  static_assert(MG_PAD % DM_BM == 0 && MG_BK % DM_BK == 0, "padding must cover the tiles");
   const bool same = (A == B) && (lda == ldb) && (transA != transB) && m == n;  // SYRK: op(B) = op(A)^T
 
   char* ws_e = (char*)scratch(WS_SE, sizeof(unsigned long long) * (mpad + npad) + sizeof(int) * (mpad + npad));
@@ -503,15 +668,21 @@ void split_gemm<double>(const Exec& x, int transA, int transB, int m, int n, int
   for (int l = 0; l < nmod; ++l) { g.p[l] = M.p[l]; g.invp[l] = M.invp[l]; }
   g.tri = tri;
   R.prof_begin("split_mod_gemm", 2.0 * nmod * (double)m * n * k * (tri == TRI_FULL ? 1.0 : 0.5));
-  if (tri == TRI_FULL) {
-    g.tiles_m = (int)(mpad / 256); g.tiles_n = (int)(npad / 128);
-    hipLaunchKernelGGL((mod_gemm_kernel<256, 128>), dim3((unsigned)((long)g.tiles_m * g.tiles_n), 1, nmod),
-                       dim3(256), 0, s, g);
+  g.tiles_m = (int)(mpad / DM_BM);
+  g.tiles_n = (int)(npad / DM_BN);
+  const long tiles = tri == TRI_FULL ? (long)g.tiles_m * g.tiles_n : (long)g.tiles_m * (g.tiles_m + 1) / 2;
+  if (g_mod_gemm_variant == 1) {
+    g.tiles_n = (int)(npad / 128);
+    if (tri == TRI_FULL)
+      hipLaunchKernelGGL((mod_gemm_kernel<256, 128>), dim3((unsigned)((long)g.tiles_m * g.tiles_n), 1, nmod),
+                         dim3(256), 0, s, g);
+    else {
+      g.tiles_m = g.tiles_n = (int)(mpad / 128);
+      hipLaunchKernelGGL((mod_gemm_kernel<128, 128>),
+                         dim3((unsigned)((long)g.tiles_m * (g.tiles_m + 1) / 2), 1, nmod), dim3(256), 0, s, g);
+    }
   } else {
-    // triangular tiles need square tiles
-    g.tiles_m = g.tiles_n = (int)(mpad / 128);
-    hipLaunchKernelGGL((mod_gemm_kernel<128, 128>),
-                       dim3((unsigned)((long)g.tiles_m * (g.tiles_m + 1) / 2), 1, nmod), dim3(256), 0, s, g);
+    hipLaunchKernelGGL(mod_gemm_dma_kernel, dim3((unsigned)tiles, 1, nmod), dim3(256), 0, s, g);
   }
   XK_HIP(hipGetLastError());
   R.prof_end();
