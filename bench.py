@@ -120,6 +120,17 @@ def cpu_baseline(budget_s=12.0):
                       "the reference's dlopen host-BLAS path" % (os.path.basename(path), len(times))}
 
 
+def copy_probe():
+    """A plain copy kernel of known size: 1 GiB read + 1 GiB written."""
+    import torch
+    a = torch.empty(1 << 27, dtype=torch.float64, device="cuda").fill_(1.0)
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    print("copy probe: %d bytes" % (a.numel() * 8), file=sys.stderr)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -130,7 +141,13 @@ def main():
     ap.add_argument("--potrf-n", type=int, default=16384)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-potrf", action="store_true")
+    ap.add_argument("--only-split", action="store_true", help="profile helper: one split-BF16 DGEMM, no JSON")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="profile helper: one 1 GiB device-to-device copy (calibrates HBM byte counters)")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe()
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
@@ -147,6 +164,11 @@ def main():
     print("bench: generating inputs n=%d" % n, file=sys.stderr, flush=True)
     A, B, C = make_inputs(n, 2026)
     flops = 2.0 * n**3
+
+    if args.only_split:
+        time_dgemm(xk, A, B, C, 1, 0, 2, args.nb)
+        L.xkblas_finalize()
+        return
 
     # plain FP64 MFMA
     print("bench: fp64 dgemm", file=sys.stderr, flush=True)
