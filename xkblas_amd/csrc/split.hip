@@ -380,15 +380,19 @@ __global__ __launch_bounds__(256) void mod_gemm_kernel(ModGemmParams prm)
 // distinct 16-byte bank groups.  One barrier per stage: after it, the stage
 // read in the previous iteration is free and receives the loads of the stage
 // STAGES-1 ahead.
-constexpr int DM_BM = 256, DM_BN = 256, DM_BK = 32, DM_STAGES = 4;
-constexpr int DM_FOLD = 1024 / DM_BK;  // stages between modular reductions
+#ifndef XK_DM_STAGES
+#define XK_DM_STAGES 4
+#endif
+constexpr int DM_BM = 256, DM_BN = 256, DM_BK = 32, DM_STAGES = XK_DM_STAGES;
 
 typedef __attribute__((address_space(3))) void lds_void;
 
 __device__ __forceinline__ void wait_vmcnt_le(int n)
 {
-  // s_waitcnt with only vmcnt constrained (expcnt, lgkmcnt at maximum)
-  if (n >= 16) __builtin_amdgcn_s_waitcnt((16 & 0xf) | (0x7 << 4) | (0xf << 8) | ((16 >> 4) << 14));
+  // s_waitcnt with only vmcnt constrained (expcnt, lgkmcnt at maximum);
+  // vmcnt is 6 bits: low 4 in [3:0], high 2 in [15:14]
+  if (n >= 24) __builtin_amdgcn_s_waitcnt((24 & 0xf) | (0x7 << 4) | (0xf << 8) | ((24 >> 4) << 14));
+  else if (n >= 16) __builtin_amdgcn_s_waitcnt((16 & 0xf) | (0x7 << 4) | (0xf << 8) | ((16 >> 4) << 14));
   else if (n >= 8) __builtin_amdgcn_s_waitcnt((8 & 0xf) | (0x7 << 4) | (0xf << 8));
   else __builtin_amdgcn_s_waitcnt((0x7 << 4) | (0xf << 8));
 }
@@ -462,7 +466,7 @@ __global__ __launch_bounds__(256) void mod_gemm_dma_kernel(ModGemmParams prm)
 #pragma unroll
       for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
 
-  const int nk = prm.kpad / DM_BK;
+  const int nk = prm.kpad / DM_BK;  // a multiple of 16
   for (int s = 0; s < DM_STAGES - 1 && s < nk; ++s) issue(s, s * DM_BK);
 
   // operand reads: row (wm*128 + 32i + lane%32), logical chunk 2ks + lane/32;
@@ -475,60 +479,114 @@ __global__ __launch_bounds__(256) void mod_gemm_dma_kernel(ModGemmParams prm)
   const int rdB0 = DM_BM * DM_BK + (wn * 128 + fr) * DM_BK + ((0 + fh) ^ sw) * 8;
   const int rdB1 = DM_BM * DM_BK + (wn * 128 + fr) * DM_BK + ((2 + fh) ^ sw) * 8;
 
-  // k loop as chunks of DM_FOLD stages; the accumulators are folded mod p
-  // between chunks, outside the MFMA loop
-  for (int c0 = 0; c0 < nk; c0 += DM_FOLD) {
-    const int c1 = min(nk, c0 + DM_FOLD);
-    for (int kt = c0; kt < c1; ++kt) {
-      // stage kt landed (loads of later stages may still be in flight)
-      const int ahead = min(nk - 1, kt + DM_STAGES - 2) - kt;  // stages issued after kt
-      wait_vmcnt_le(ahead * 8);
-      __builtin_amdgcn_s_barrier();
-      if (kt + DM_STAGES - 1 < nk) issue((kt + DM_STAGES - 1) % DM_STAGES, (kt + DM_STAGES - 1) * DM_BK);
+  // Software pipeline over 16-deep halves of each stage: the operands of
+  // half 1 are read while half 0 multiplies; after the mid-stage barrier
+  // (next stage landed, previous stage free for new loads) the operands of
+  // the next stage's half 0 are read while half 1 multiplies.
+  bf16x8 a0[4], b0[4], a1[4], b1[4];
+  wait_vmcnt_le(8 * (min(nk, DM_STAGES - 1) - 1));
+  __builtin_amdgcn_s_barrier();
+#pragma unroll
+  for (int i = 0; i < 4; ++i) a0[i] = *reinterpret_cast<const bf16x8*>(&S[0][0] + rdA0 + i * 32 * DM_BK);
+#pragma unroll
+  for (int j = 0; j < 4; ++j) b0[j] = *reinterpret_cast<const bf16x8*>(&S[0][0] + rdB0 + j * 32 * DM_BK);
+
+  // The k loop runs in groups of 16 stages (kpad is a multiple of 512).
+  // Stage u of a group folds accumulator block u mod p right after that
+  // block's half-1 MFMA (issued first in that half), so every block is
+  // reduced once per 512 k and the VALU work of the fold overlaps the
+  // remaining MFMAs instead of stalling the wave in one long block.
+  for (int kt0 = 0; kt0 < nk; kt0 += 16) {
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      const int kt = kt0 + u;
       const __bf16* st = S[kt % DM_STAGES];
-#pragma unroll
-      for (int ks = 0; ks < 2; ++ks) {
-        const __bf16* pa = st + (ks ? rdA1 : rdA0);
-        const __bf16* pb = st + (ks ? rdB1 : rdB0);
-        bf16x8 a[4], b[4];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) a[i] = *reinterpret_cast<const bf16x8*>(pa + i * 32 * DM_BK);
-#pragma unroll
-        for (int j = 0; j < 4; ++j) b[j] = *reinterpret_cast<const bf16x8*>(pb + j * 32 * DM_BK);
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-#pragma unroll
-          for (int j = 0; j < 4; ++j)
-            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(b[j], a[i], acc[i][j], 0, 0, 0);
-      }
-      // this wave's reads of stage kt are complete before the next barrier
+      // half-0 operands were requested a full half-stage ago
       wait_lgkm0();
-    }
-    if (c1 < nk) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) a1[i] = *reinterpret_cast<const bf16x8*>(st + rdA1 + i * 32 * DM_BK);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) b1[j] = *reinterpret_cast<const bf16x8*>(st + rdB1 + j * 32 * DM_BK);
 #pragma unroll
       for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
+        for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0[i], b0[j], acc[i][j], 0, 0, 0);
+      // interleave: one operand read after every two MFMAs
 #pragma unroll
-          for (int r = 0; r < 16; ++r) acc[i][j][r] = fold(acc[i][j][r], p, ip);
+      for (int g = 0; g < 8; ++g) {
+        __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+        __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);
+      }
+      // this wave's reads of stage kt are done (and, before the barrier, of
+      // stage kt-1, which the loads issued below overwrite)
+      wait_lgkm0();
+      if (kt + 1 < nk) {
+        // stage kt+1 landed: only the loads of later, already issued stages
+        // may be pending (8 per stage, completed in issue order)
+        wait_vmcnt_le(8 * (min(nk - 1, kt + DM_STAGES - 2) - (kt + 1)));
+        __builtin_amdgcn_s_barrier();
+        if (kt + DM_STAGES - 1 < nk) issue((kt + DM_STAGES - 1) % DM_STAGES, (kt + DM_STAGES - 1) * DM_BK);
+      }
+      // (on the last stage these reads fetch unused data from an idle buffer;
+      // keeping them unconditional lets them interleave with the MFMAs)
+      const __bf16* sn = S[(kt + 1) % DM_STAGES];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) a0[i] = *reinterpret_cast<const bf16x8*>(sn + rdA0 + i * 32 * DM_BK);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) b0[j] = *reinterpret_cast<const bf16x8*>(sn + rdB0 + j * 32 * DM_BK);
+#pragma unroll
+      for (int t = 0; t < 16; ++t) {
+        const int blk = (u + t) & 15, i = blk >> 2, j = blk & 3;
+        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1[i], b1[j], acc[i][j], 0, 0, 0);
+      }
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[u >> 2][u & 3][r] = fold(acc[u >> 2][u & 3][r], p, ip);
+#pragma unroll
+      for (int g = 0; g < 8; ++g) {
+        __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+        __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);
+      }
     }
   }
 
+  // Epilogue: residues in [-(p-1)/2, (p-1)/2] as int8.  The accumulator block
+  // (i, j) holds D = A B^T with lane -> n = 32j + lane%32 and (lane, r) ->
+  // m = 32i + 8(r/4) + 4(lane/32) + r%4, so r%4 = 0..3 are 4 consecutive rows:
+  // one packed 32-bit word.  Words go to an LDS tile (column stride 272
+  // bytes), then every column leaves as 16 aligned 16-byte stores.
   const float hi = (p - 1.0f) * 0.5f;
-  int8_t* C = prm.C + (long)l * prm.sC + (n0 + wn * 128 + 4 * fh) * prm.ldc + m0 + wm * 128 + fr;
-  const long ldc = prm.ldc;
+  wait_vmcnt_le(0);
+  __syncthreads();
+  constexpr int CS = 272;
+  char* T = reinterpret_cast<char*>(&S[0][0]);
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
 #pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        float v = fold(acc[i][j][r], p, ip);
-        if (v > hi) v -= p;
-        if (v < -hi) v += p;
-        C[(long)(j * 32 + 8 * (r >> 2) + (r & 3)) * ldc + i * 32] = (int8_t)(int)v;
+      for (int q = 0; q < 4; ++q) {
+        unsigned word = 0;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+          float v = fold(acc[i][j][4 * q + b], p, ip);
+          if (v > hi) v -= p;
+          if (v < -hi) v += p;
+          word |= ((unsigned)(int)v & 0xffu) << (8 * b);
+        }
+        const int ml = wm * 128 + i * 32 + 8 * q + 4 * fh;
+        const int nl = wn * 128 + j * 32 + fr;
+        *reinterpret_cast<unsigned*>(T + nl * CS + ml) = word;
       }
     }
+  }
+  __syncthreads();
+  int8_t* C = prm.C + (long)l * prm.sC + n0 * prm.ldc + m0;
+  const int chunk = tid & 15;
+#pragma unroll 4
+  for (int s2 = 0; s2 < 16; ++s2) {
+    const int col = (tid >> 4) + 16 * s2;
+    const uint4 v = *reinterpret_cast<const uint4*>(T + col * CS + chunk * 16);
+    *reinterpret_cast<uint4*>(C + (long)col * prm.ldc + chunk * 16) = v;
   }
 }
 
@@ -615,7 +673,7 @@ void split_gemm<double>(const Exec& x, int transA, int transB, int m, int n, int
   const Moduli M = make_moduli(nmod);
   const int tb = side_bits(M, k);
   XK_CHECK(tb >= 8, "split count too small for this k");
-  const long mpad = rup(m, MG_PAD), npad = rup(n, MG_PAD), kpad = rup(k, MG_BK);
+  const long mpad = rup(m, MG_PAD), npad = rup(n, MG_PAD), kpad = rup(k, 512);
   static_assert(MG_PAD % DM_BM == 0 && MG_BK % DM_BK == 0, "padding must cover the tiles");
   const bool same = (A == B) && (lda == ldb) && (transA != transB) && m == n;  // SYRK: op(B) = op(A)^T
 
