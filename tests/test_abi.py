@@ -12,9 +12,14 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 DECL = re.compile(r"^\s*(?:extern\s+)?[A-Za-z_][\w\s\*]*?\b(xkblas_\w+|[a-z]+_)\s*\(", re.M)
 
 
-def declared():
+WRAPPER_HEADER = "xkblas_blas_f77.h"
+
+
+def declared(wrapper=False):
     names = set()
     for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        if (os.path.basename(h) == WRAPPER_HEADER) != wrapper:
+            continue
         text = open(h).read()
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
         for m in DECL.finditer(text):
@@ -34,6 +39,17 @@ def test_library_exports_every_declared_symbol():
     import xkblas_amd
     lib = xkblas_amd.lib()
     missing = [n for n in declared() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_wrapper_exports_the_f77_drop_in():
+    """libxkblas_amd_wrapper.so exports dgemm_ ... without touching the GPU
+    when loaded (the device is initialised on the first GPU-bound call)."""
+    import xkblas_amd
+    lib = ctypes.CDLL(os.path.join(ROOT, "xkblas_amd", "libxkblas_amd_wrapper.so"))
+    names = declared(wrapper=True)
+    assert set(names) >= {"dgemm_", "dsyrk_", "dtrsm_", "dpotrf_"}
+    missing = [n for n in names if not hasattr(lib, n)]
     assert not missing, missing
 
 

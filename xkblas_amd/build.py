@@ -13,6 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libxkblas_amd.so")
+WRAPPER = os.path.join(HERE, "libxkblas_amd_wrapper.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 # -packed-fp32-ops: keep hipcc from emitting packed FP32 instructions.
@@ -47,6 +48,15 @@ def build(force=True, jobs=8):
     if r.returncode != 0:
         raise RuntimeError("link failed:\n%s\n%s" % (" ".join(cmd), r.stderr[-6000:]))
     os.replace(tmp, LIB)
+    # the f77 drop-in, a separate library linked against the main one
+    wobj = _compile("wrapper.cpp", force)
+    tmp = WRAPPER + ".tmp"
+    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, wobj, "-L" + HERE, "-lxkblas_amd",
+           "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("wrapper link failed:\n%s\n%s" % (" ".join(cmd), r.stderr[-6000:]))
+    os.replace(tmp, WRAPPER)
     return LIB
 
 

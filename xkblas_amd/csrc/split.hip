@@ -207,22 +207,27 @@ __global__ __launch_bounds__(256) void split_kernel(SplitParams sp)
     T[i][k] = v;
   }
   __syncthreads();
-  const int i = t >> 2, kc = (t & 3) * 16;
-  double x[16];
+  // thread -> rows (t/8, t/8 + 32), 8 consecutive k: per modulus two 16-byte
+  // stores, and 8 lanes cover one 128-byte row segment
+  const int r0 = t >> 3, kc = (t & 7) * 8;
+  double x0[8], x1[8];
 #pragma unroll
-  for (int v = 0; v < 16; ++v) x[v] = T[i][kc + v];
-  __bf16* out = sp.R + (long)(i0 + i) * sp.rld + k0 + kc;
+  for (int v = 0; v < 8; ++v) {
+    x0[v] = T[r0][kc + v];
+    x1[v] = T[r0 + 32][kc + v];
+  }
+  __bf16* out0 = sp.R + (long)(i0 + r0) * sp.rld + k0 + kc;
+  __bf16* out1 = out0 + 32L * sp.rld;
   for (int l = 0; l < sp.nmod; ++l) {
     const double p = sp.pd[l], ip = sp.invpd[l];
-    bf16x8 lo, hi;
+    bf16x8 v0, v1;
 #pragma unroll
     for (int v = 0; v < 8; ++v) {
-      lo[v] = (__bf16)(float)sym_mod(x[v], p, ip);
-      hi[v] = (__bf16)(float)sym_mod(x[v + 8], p, ip);
+      v0[v] = (__bf16)(float)sym_mod(x0[v], p, ip);
+      v1[v] = (__bf16)(float)sym_mod(x1[v], p, ip);
     }
-    bf16x8* o = reinterpret_cast<bf16x8*>(out + (long)l * sp.rstride);
-    o[0] = lo;
-    o[1] = hi;
+    *reinterpret_cast<bf16x8*>(out0 + (long)l * sp.rstride) = v0;
+    *reinterpret_cast<bf16x8*>(out1 + (long)l * sp.rstride) = v1;
   }
 }
 
@@ -611,30 +616,18 @@ __device__ __forceinline__ void two_sum(double a, double b, double& s, double& e
   e = (a - (s - z)) + (b - z);
 }
 
-// C = alpha * 2^-(sA_i + sB_j) * CRT(r_1..r_N) + beta * C.
-__global__ __launch_bounds__(256) void reconstruct_kernel(RecParams rp)
+// C = alpha * 2^-(sA_i + sB_j) * CRT(r_1..r_N) + beta * C; each thread
+// recombines 4 consecutive rows of one column (one 32-bit load per modulus).
+__device__ __forceinline__ double crt_one(const RecParams& rp, double S0, double S1, double S2)
 {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y;
-  if (i >= rp.m || j >= rp.n) return;
-  if (rp.tri == TRI_LOWER && i < j) return;
-  if (rp.tri == TRI_UPPER && i > j) return;
-  double S0 = 0.0, S1 = 0.0, S2 = 0.0;
-  const int8_t* r = rp.R + (long)j * rp.ldr + i;
-  for (int l = 0; l < rp.nmod; ++l) {
-    const double v = (double)r[(long)l * rp.sR];
-    S0 = fma(v, rp.W[l][0], S0);
-    S1 = fma(v, rp.W[l][1], S1);
-    S2 = fma(v, rp.W[l][2], S2);
-  }
   const double B1 = LIMB, B2 = LIMB * LIMB;
   double q = rint((S0 * B2 + S1 * B1 + S2) * rp.invP);
-  double T0, T1, T2, approx;
+  double T0 = 0.0, T1 = 0.0, T2 = 0.0;
   for (int it = 0; it < 3; ++it) {
     T0 = fma(-q, rp.P[0], S0);
     T1 = fma(-q, rp.P[1], S1);
     T2 = fma(-q, rp.P[2], S2);
-    approx = T0 * B2 + T1 * B1 + T2;
+    const double approx = T0 * B2 + T1 * B1 + T2;
     if (approx > rp.halfP) q += 1.0;
     else if (approx < -rp.halfP) q -= 1.0;
     else break;
@@ -642,12 +635,42 @@ __global__ __launch_bounds__(256) void reconstruct_kernel(RecParams rp)
   double hi, lo;
   two_sum(T0 * B2, T1 * B1, hi, lo);
   lo += T2;
-  const double x = hi + lo;
-  const double res = ldexp(x, -(rp.shA[i] + rp.shB[j]));
-  double* c = rp.C + i + (long)j * rp.ldc;
-  double out = rp.alpha * res;
-  if (rp.beta != 0.0) out = fma(rp.beta, *c, out);
-  *c = out;
+  return hi + lo;
+}
+
+__global__ __launch_bounds__(256) void reconstruct_kernel(RecParams rp)
+{
+  const int i0 = 4 * (blockIdx.x * blockDim.x + threadIdx.x);
+  const int j = blockIdx.y;
+  if (i0 >= rp.m || j >= rp.n) return;
+  if (rp.tri == TRI_LOWER && i0 + 3 < j) return;
+  if (rp.tri == TRI_UPPER && i0 > j) return;
+  double S[4][3] = {};
+  const int8_t* r = rp.R + (long)j * rp.ldr + i0;
+  for (int l = 0; l < rp.nmod; ++l) {
+    const unsigned w = *reinterpret_cast<const unsigned*>(r + (long)l * rp.sR);
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const double v = (double)(int8_t)(w >> (8 * e));
+      S[e][0] = fma(v, rp.W[l][0], S[e][0]);
+      S[e][1] = fma(v, rp.W[l][1], S[e][1]);
+      S[e][2] = fma(v, rp.W[l][2], S[e][2]);
+    }
+  }
+  const int sb = rp.shB[j];
+  double* c = rp.C + (long)j * rp.ldc;
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    const int i = i0 + e;
+    if (i >= rp.m) break;
+    if (rp.tri == TRI_LOWER && i < j) continue;
+    if (rp.tri == TRI_UPPER && i > j) continue;
+    const double x = crt_one(rp, S[e][0], S[e][1], S[e][2]);
+    const double res = ldexp(x, -(rp.shA[i] + sb));
+    double out = rp.alpha * res;
+    if (rp.beta != 0.0) out = fma(rp.beta, c[i], out);
+    c[i] = out;
+  }
 }
 
 // ---------------------------------------------------------------- host side
@@ -764,7 +787,7 @@ void split_gemm<double>(const Exec& x, int transA, int transB, int m, int n, int
     r2.C = C + (long)j0 * ldc;
     // the triangle test uses global indices: shift i-j by j0 via tri only when j0 == 0
     XK_CHECK(tri == TRI_FULL || j0 == 0, "triangular split beyond 65535 columns");
-    hipLaunchKernelGGL(reconstruct_kernel, dim3((m + 255) / 256, nn), dim3(256), 0, s, r2);
+    hipLaunchKernelGGL(reconstruct_kernel, dim3((m + 1023) / 1024, nn), dim3(256), 0, s, r2);
   }
   XK_HIP(hipGetLastError());
   R.prof_end();

@@ -1,0 +1,170 @@
+// wrapper.cpp — libxkblas_amd_wrapper.so: Fortran-ABI drop-in dgemm_,
+// dsyrk_, dtrsm_, dpotrf_ (and s-variants of the first three) for
+// LD_PRELOAD or link-time substitution of a host BLAS.
+//
+// Restates the reference's unbuilt wrapper (blas/libxkblas_wrapper.c:88-187,
+// blas/zlibxkblas_wrapper.c:46-76): quick return, then a CPU/GPU choice on
+// the arithmetic intensity flops/bytes of the call (blas/flops.h FLOPS_* /
+// DATA_*) against XKBLAS_THRESHOLD (default 1600, per kernel
+// XKBLAS_THRESHOLD_DGEMM, ...).  Below it the host BLAS of XKBLAS_BLASLIB
+// runs the call; above it the GPU path runs synchronously (async + coherent +
+// sync + invalidate).  dpotrf_ has no counterpart in the reference; it
+// follows LAPACK's interface.
+#include "../../include/xkblas.h"
+#include "../../include/xkblas_f77.h"
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+
+namespace {
+
+enum Kern { K_DGEMM, K_SGEMM, K_DSYRK, K_SSYRK, K_DTRSM, K_STRSM, K_DPOTRF, K_LAST };
+const char* kKernName[K_LAST] = {"DGEMM", "SGEMM", "DSYRK", "SSYRK", "DTRSM", "STRSM", "DPOTRF"};
+double g_threshold[K_LAST];
+bool g_gpu_ready = false;
+
+// GPU side initialised on the first call that goes to the GPU, so that a
+// process whose calls all stay on the host never touches the device (the
+// reference initialises in the library constructor).
+void gpu_ready()
+{
+  if (g_gpu_ready) return;
+  if (xkblas_init() != 0) {
+    fprintf(stderr, "libxkblas_amd_wrapper: cannot initialize\n");
+    abort();
+  }
+  g_gpu_ready = true;
+}
+
+__attribute__((constructor)) void wrapper_init()
+{
+  long th = 1600;
+  if (const char* v = getenv("XKBLAS_THRESHOLD")) th = atol(v) < 0 ? 0 : atol(v);
+  for (int i = 0; i < K_LAST; ++i) {
+    char name[64];
+    snprintf(name, sizeof name, "XKBLAS_THRESHOLD_%s", kKernName[i]);
+    const char* v = getenv(name);
+    g_threshold[i] = v ? (atol(v) < 0 ? 0.0 : (double)atol(v)) : (double)th;
+  }
+}
+
+__attribute__((destructor)) void wrapper_fini()
+{
+  if (g_gpu_ready) xkblas_finalize();
+}
+
+int trans_of(const char* t)
+{
+  switch (t[0]) {
+    case 'n': case 'N': return CblasNoTrans;
+    case 't': case 'T': return CblasTrans;
+    case 'c': case 'C': return CblasConjTrans;
+    default: abort();
+  }
+}
+int uplo_of(const char* u) { return (u[0] == 'l' || u[0] == 'L') ? CblasLower : CblasUpper; }
+int side_of(const char* s) { return (s[0] == 'l' || s[0] == 'L') ? CblasLeft : CblasRight; }
+int diag_of(const char* d) { return (d[0] == 'u' || d[0] == 'U') ? CblasUnit : CblasNonUnit; }
+
+// arithmetic intensity of the reference's flop/byte model (blas/flops.h)
+double ai_gemm(double m, double n, double k, double elt) { return 2.0 * m * n * k / (elt * (m * n + m * k + k * n)); }
+double ai_syrk(double n, double k, double elt) { return k * n * (n + 1.0) / (elt * (0.5 * n * n + n * k)); }
+double ai_trsm(bool left, double m, double n, double elt)
+{
+  const double f = left ? n * m * (m + 1.0) : m * n * (n + 1.0);
+  const double d = left ? (0.5 * m * m + 2.0 * m * n) : (0.5 * n * n + 2.0 * n * m);
+  return f / (elt * d);
+}
+
+template <typename T>
+void coherent_sync(int uplo, int m, int n, T* C, int ldc)
+{
+  xkblas_memory_coherent_async(uplo, 0, m, n, C, ldc, sizeof(T));
+  xkblas_sync();
+  xkblas_memory_invalidate_caches();
+}
+
+}  // namespace
+
+extern "C" {
+
+void dgemm_(const char* transa, const char* transb, const int* m, const int* n, const int* k, const double* alpha,
+            const double* A, const int* lda, const double* B, const int* ldb, const double* beta, double* C,
+            const int* ldc)
+{
+  if (*m == 0 || *n == 0 || ((*alpha == 0.0 || *k == 0) && *beta == 1.0)) return;
+  if (ai_gemm(*m, *n, *k, 8.0) < g_threshold[K_DGEMM]) {
+    xkblas_dgemm_native_(transa, transb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+    return;
+  }
+  gpu_ready();
+  xkblas_dgemm_async(trans_of(transa), trans_of(transb), *m, *n, *k, alpha, A, *lda, B, *ldb, beta, C, *ldc);
+  coherent_sync<double>(0, *m, *n, C, *ldc);
+}
+
+void sgemm_(const char* transa, const char* transb, const int* m, const int* n, const int* k, const float* alpha,
+            const float* A, const int* lda, const float* B, const int* ldb, const float* beta, float* C,
+            const int* ldc)
+{
+  if (*m == 0 || *n == 0 || ((*alpha == 0.0f || *k == 0) && *beta == 1.0f)) return;
+  if (ai_gemm(*m, *n, *k, 4.0) < g_threshold[K_SGEMM]) {
+    typedef void (*f_t)(const char*, const char*, const int*, const int*, const int*, const float*, const float*,
+                        const int*, const float*, const int*, const float*, float*, const int*);
+    static f_t f = nullptr;
+    if (!f) xkblas_load_sym((void**)&f, "sgemm_");
+    f(transa, transb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+    return;
+  }
+  gpu_ready();
+  xkblas_sgemm_async(trans_of(transa), trans_of(transb), *m, *n, *k, alpha, A, *lda, B, *ldb, beta, C, *ldc);
+  coherent_sync<float>(0, *m, *n, C, *ldc);
+}
+
+void dsyrk_(const char* uplo, const char* trans, const int* n, const int* k, const double* alpha, const double* A,
+            const int* lda, const double* beta, double* C, const int* ldc)
+{
+  if (*n == 0 || ((*alpha == 0.0 || *k == 0) && *beta == 1.0)) return;
+  if (ai_syrk(*n, *k, 8.0) < g_threshold[K_DSYRK]) {
+    xkblas_dsyrk_native_(uplo, trans, n, k, alpha, A, lda, beta, C, ldc);
+    return;
+  }
+  gpu_ready();
+  xkblas_dsyrk_async(uplo_of(uplo), trans_of(trans), *n, *k, alpha, A, *lda, beta, C, *ldc);
+  coherent_sync<double>(uplo_of(uplo), *n, *n, C, *ldc);
+}
+
+void dtrsm_(const char* side, const char* uplo, const char* transa, const char* diag, const int* m, const int* n,
+            const double* alpha, const double* A, const int* lda, double* B, const int* ldb)
+{
+  if (*m == 0 || *n == 0) return;
+  if (ai_trsm(side_of(side) == CblasLeft, *m, *n, 8.0) < g_threshold[K_DTRSM]) {
+    xkblas_dtrsm_native_(side, uplo, transa, diag, m, n, alpha, A, lda, B, ldb);
+    return;
+  }
+  gpu_ready();
+  xkblas_dtrsm_async(side_of(side), uplo_of(uplo), trans_of(transa), diag_of(diag), *m, *n, alpha, A, *lda, B,
+                     *ldb);
+  coherent_sync<double>(0, *m, *n, B, *ldb);
+}
+
+void dpotrf_(const char* uplo, const int* n, double* A, const int* lda, int* info)
+{
+  *info = 0;
+  if (*n == 0) return;
+  const double ai = (*n / 3.0) / (8.0 * 0.5);  // (n^3/3) / (8 n^2 / 2)
+  if (ai < g_threshold[K_DPOTRF]) {
+    xkblas_dpotrf_native_(uplo, n, A, lda, info);
+    return;
+  }
+  gpu_ready();
+  if (xkblas_dpotrf_async(uplo_of(uplo), *n, A, *lda) != 0) {
+    *info = -1;
+    return;
+  }
+  xkblas_memory_coherent_async(uplo_of(uplo), 0, *n, *n, A, *lda, sizeof(double));
+  xkblas_sync();
+  *info = xkblas_potrf_info(A);
+  xkblas_memory_invalidate_caches();
+}
+
+}  // extern "C"
