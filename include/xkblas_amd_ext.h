@@ -30,6 +30,42 @@ long xkblas_cache_bytes(void);
 long xkblas_cache_evictions(void);
 void xkblas_set_cache_limit(size_t bytes);
 
+/* ---- batched uneven problems in one persistent launch -------------------
+ * Problems live in device memory (xkblas_device_malloc or any HIP allocation).
+ * GEMM: C <- alpha op(A) op(B) + beta C (op = transA/transB, C m x n, k).
+ * TRSM: Left, Lower, NoTrans, Non-unit: L X = alpha B with L = A (m x m,
+ *       lda) and B = C (m x n, ldc) overwritten by X.
+ * dtype 0 = FP64, 1 = FP32 (mixed in one batch).  A plan holds the work list
+ * (64 x 64 GEMM tiles and 64-column TRSM panels, longest first) on the
+ * device; xkblas_batch_run enqueues one counter fill and one persistent
+ * kernel on the library's compute stream (xkblas_sync waits for it);
+ * xkblas_batch_use_graph(plan, 1) captures that pair into a hipGraph that
+ * later runs replay. */
+#define XKBLAS_BATCH_GEMM 0
+#define XKBLAS_BATCH_TRSM 1
+typedef struct xkblas_batch_problem {
+  int kind, dtype, transA, transB, m, n, k;
+  double alpha, beta;
+  const void* A;
+  long lda;
+  const void* B;
+  long ldb;
+  void* C;
+  long ldc;
+} xkblas_batch_problem_t;
+
+void* xkblas_batch_create(const xkblas_batch_problem_t* problems, int count);
+int   xkblas_batch_run(void* plan);
+int   xkblas_batch_use_graph(void* plan, int on);
+int   xkblas_batch_info(void* plan, int* nitems, int* grid, double* flops, double* bytes);
+void  xkblas_batch_destroy(void* plan);
+
+/* device memory helpers for batched operands */
+void* xkblas_device_malloc(size_t bytes);
+void  xkblas_device_free(void* ptr);
+int   xkblas_memcpy_htod(void* dst, const void* src, size_t bytes);
+int   xkblas_memcpy_dtoh(void* dst, const void* src, size_t bytes);
+
 #ifdef __cplusplus
 }
 #endif

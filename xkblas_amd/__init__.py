@@ -70,6 +70,15 @@ SIGNATURES = {
     "xkblas_cache_bytes": (_l, []),
     "xkblas_cache_evictions": (_l, []),
     "xkblas_set_cache_limit": (None, [_sz]),
+    "xkblas_batch_create": (_p, [_p, _i]),
+    "xkblas_batch_run": (_i, [_p]),
+    "xkblas_batch_use_graph": (_i, [_p, _i]),
+    "xkblas_batch_info": (_i, [_p, _p, _p, _p, _p]),
+    "xkblas_batch_destroy": (None, [_p]),
+    "xkblas_device_malloc": (_p, [_sz]),
+    "xkblas_device_free": (None, [_p]),
+    "xkblas_memcpy_htod": (_i, [_p, _p, _sz]),
+    "xkblas_memcpy_dtoh": (_i, [_p, _p, _sz]),
     "xkblas_dgemm_native_": (None, [_p] * 13),
     "xkblas_dsyrk_native_": (None, [_p] * 10),
     "xkblas_dtrsm_native_": (None, [_p] * 11),
@@ -174,6 +183,47 @@ def coherent(X, uplo=0):
 
 def invalidate():
     lib().xkblas_memory_invalidate_caches()
+
+
+BATCH_GEMM, BATCH_TRSM = 0, 1
+
+
+class BatchProblem(ctypes.Structure):
+    """xkblas_batch_problem_t (include/xkblas_amd_ext.h)."""
+    _fields_ = [("kind", ctypes.c_int), ("dtype", ctypes.c_int), ("transA", ctypes.c_int), ("transB", ctypes.c_int),
+                ("m", ctypes.c_int), ("n", ctypes.c_int), ("k", ctypes.c_int),
+                ("alpha", ctypes.c_double), ("beta", ctypes.c_double),
+                ("A", ctypes.c_void_p), ("lda", ctypes.c_long),
+                ("B", ctypes.c_void_p), ("ldb", ctypes.c_long),
+                ("C", ctypes.c_void_p), ("ldc", ctypes.c_long)]
+
+
+class DeviceArray:
+    """A Fortran-ordered device copy of a NumPy array (xkblas_device_malloc)."""
+
+    def __init__(self, host):
+        self.shape, self.dtype = host.shape, host.dtype
+        self.nbytes = host.nbytes
+        self.ptr = lib().xkblas_device_malloc(max(self.nbytes, 1))
+        if not self.ptr:
+            raise MemoryError("xkblas_device_malloc failed")
+        self.upload(host)
+
+    def upload(self, host):
+        h = np.asfortranarray(host, dtype=self.dtype)
+        if lib().xkblas_memcpy_htod(self.ptr, h.ctypes.data_as(ctypes.c_void_p), self.nbytes) != 0:
+            raise RuntimeError("host-to-device copy failed")
+
+    def download(self):
+        h = np.empty(self.shape, dtype=self.dtype, order="F")
+        if lib().xkblas_memcpy_dtoh(h.ctypes.data_as(ctypes.c_void_p), self.ptr, self.nbytes) != 0:
+            raise RuntimeError("device-to-host copy failed")
+        return h
+
+    def free(self):
+        if self.ptr:
+            lib().xkblas_device_free(self.ptr)
+            self.ptr = None
 
 
 def profile(cls):

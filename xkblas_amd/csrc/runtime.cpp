@@ -342,7 +342,12 @@ using namespace xk;
 // ======================================================================
 // C API
 // ======================================================================
-static std::mutex g_mu;
+std::mutex& xk::api_mutex()
+{
+  static std::mutex m;
+  return m;
+}
+#define g_mu xk::api_mutex()
 typedef std::lock_guard<std::mutex> Lock;
 
 static bool err_return() { return getenv("XKBLAS_ERROR_RETURN") && atoi(getenv("XKBLAS_ERROR_RETURN")) != 0; }

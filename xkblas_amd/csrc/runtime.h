@@ -2,6 +2,7 @@
 #pragma once
 #include "blas3.h"
 #include <map>
+#include <mutex>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -88,5 +89,8 @@ struct Runtime {
 };
 
 Runtime& rt();
+
+// serialises every C API entry point
+std::mutex& api_mutex();
 
 }  // namespace xk
