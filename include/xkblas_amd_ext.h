@@ -60,6 +60,19 @@ int   xkblas_batch_use_graph(void* plan, int on);
 int   xkblas_batch_info(void* plan, int* nitems, int* grid, double* flops, double* bytes);
 void  xkblas_batch_destroy(void* plan);
 
+/* ---- tile-cost model: 3-layer MLP (in -> 64 -> 64 -> 1, ReLU, FP32) ------
+ * trained on the GPU by the library's own forward / backward / Adam kernels.
+ * Parameters are one FP32 vector [W1 (in x 64), b1, W2 (64 x 64), b2, w3, b3];
+ * samples are row-major (n x in), in <= 16. */
+void* xkblas_mlp_create(int in_features, unsigned seed);
+long  xkblas_mlp_num_params(void* mlp);
+int   xkblas_mlp_set_params(void* mlp, const float* params);
+int   xkblas_mlp_get_params(void* mlp, float* params);
+int   xkblas_mlp_forward(void* mlp, const float* X, int n, float* Y);
+int   xkblas_mlp_gradient(void* mlp, const float* X, const float* T, int n, float* grad, float* loss);
+float xkblas_mlp_train(void* mlp, const float* X, const float* T, int n, int epochs, int batch, float lr);
+void  xkblas_mlp_destroy(void* mlp);
+
 /* device memory helpers for batched operands */
 void* xkblas_device_malloc(size_t bytes);
 void  xkblas_device_free(void* ptr);

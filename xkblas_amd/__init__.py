@@ -79,6 +79,14 @@ SIGNATURES = {
     "xkblas_device_free": (None, [_p]),
     "xkblas_memcpy_htod": (_i, [_p, _p, _sz]),
     "xkblas_memcpy_dtoh": (_i, [_p, _p, _sz]),
+    "xkblas_mlp_create": (_p, [_i, ctypes.c_uint]),
+    "xkblas_mlp_num_params": (_l, [_p]),
+    "xkblas_mlp_set_params": (_i, [_p, _p]),
+    "xkblas_mlp_get_params": (_i, [_p, _p]),
+    "xkblas_mlp_forward": (_i, [_p, _p, _i, _p]),
+    "xkblas_mlp_gradient": (_i, [_p, _p, _p, _i, _p, _p]),
+    "xkblas_mlp_train": (_f, [_p, _p, _p, _i, _i, _i, _f]),
+    "xkblas_mlp_destroy": (None, [_p]),
     "xkblas_dgemm_native_": (None, [_p] * 13),
     "xkblas_dsyrk_native_": (None, [_p] * 10),
     "xkblas_dtrsm_native_": (None, [_p] * 11),
