@@ -73,6 +73,18 @@ int   xkblas_mlp_gradient(void* mlp, const float* X, const float* T, int n, floa
 float xkblas_mlp_train(void* mlp, const float* X, const float* T, int n, int epochs, int batch, float lr);
 void  xkblas_mlp_destroy(void* mlp);
 
+/* ---- autotuner timing service and the knobs the cost model sets ---------
+ * op: 0 GEMM (C m x n, inner k), 1 SYRK (lower, C n x n, inner k),
+ * 2 TRSM (Left, Lower, NoTrans, Non-unit, B m x n), 3 POTRF (lower, order m).
+ * Operands are device-resident and filled by a kernel; returns the best
+ * kernel time over `iters` runs in ms. */
+double xkblas_tune_time(int op, int dtype, int mode, int transA, int transB, int m, int n, int k, int tile, int nb,
+                        int split_count, int split_kernel, int iters);
+void xkblas_tune_release(void);
+void xkblas_set_gemm_tile(int tile);        /* 0 = automatic, 64, 128 */
+void xkblas_set_split_kernel(int variant);  /* BF16 residue GEMM: 0 = 256x256 pipelined, 1 = 256x128 */
+int  xkblas_split_side_bits(int k, int nsplit);
+
 /* device memory helpers for batched operands */
 void* xkblas_device_malloc(size_t bytes);
 void  xkblas_device_free(void* ptr);

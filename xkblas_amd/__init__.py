@@ -87,6 +87,11 @@ SIGNATURES = {
     "xkblas_mlp_gradient": (_i, [_p, _p, _p, _i, _p, _p]),
     "xkblas_mlp_train": (_f, [_p, _p, _p, _i, _i, _i, _f]),
     "xkblas_mlp_destroy": (None, [_p]),
+    "xkblas_tune_time": (_d, [_i] * 13),
+    "xkblas_tune_release": (None, []),
+    "xkblas_set_gemm_tile": (None, [_i]),
+    "xkblas_set_split_kernel": (None, [_i]),
+    "xkblas_split_side_bits": (_i, [_i, _i]),
     "xkblas_dgemm_native_": (None, [_p] * 13),
     "xkblas_dsyrk_native_": (None, [_p] * 10),
     "xkblas_dtrsm_native_": (None, [_p] * 11),

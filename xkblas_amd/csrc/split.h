@@ -14,4 +14,14 @@ template <typename T>
 void split_gemm(const Exec& x, int transA, int transB, int m, int n, int k, T alpha, const T* A, long lda,
                 const T* B, long ldb, T beta, T* C, long ldc, int tri);
 
+// Number of CRT moduli for inner dimension k (requested >= 2 wins; else the
+// smallest count giving >= 48 bits per side), and the resulting bits.
+int split_choose_count(int k, int requested);
+int split_side_bits(int k, int nsplit);
+
+// BF16 residue GEMM kernel: 0 = pipelined 256 x 256 (direct-to-LDS loads),
+// 1 = register-staged 256 x 128.
+void split_set_kernel(int v);
+int split_get_kernel();
+
 }  // namespace xk

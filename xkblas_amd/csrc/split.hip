@@ -686,6 +686,15 @@ enum { WS_SA = 8, WS_SB = 9, WS_SC = 10, WS_SE = 11 };
 
 // 0: pipelined 256 x 256 DMA kernel, 1: register-staged 256 x 128 kernel
 static int g_mod_gemm_variant = getenv("XKBLAS_SPLIT_KERNEL") ? atoi(getenv("XKBLAS_SPLIT_KERNEL")) : 0;
+void split_set_kernel(int v) { g_mod_gemm_variant = v; }
+int split_get_kernel() { return g_mod_gemm_variant; }
+
+// bits per side t for k and a split count (0 = automatic count)
+int split_side_bits(int k, int nsplit)
+{
+  const int n = split_choose_count(k, nsplit);
+  return side_bits(make_moduli(n), k);
+}
 
 template <>
 void split_gemm<double>(const Exec& x, int transA, int transB, int m, int n, int k, double alpha, const double* A,
