@@ -1,0 +1,279 @@
+"""Autotuner and learned tile-cost model.
+
+The library exposes a timing service (``xkblas_tune_time``: one call of a
+given shape and configuration on device-resident operands, HIP-event kernel
+time) and the knobs the model chooses between: the GEMM macro tile, the block
+order NB of TRSM/POTRF, the split count and the BF16 residue-GEMM variant.
+This module
+
+* collects timings of GEMM / SYRK / TRSM / POTRF calls with shapes drawn
+  log-uniformly from [64, 32768] (``collect``);
+* trains the 3-layer MLP of the library (in -> 64 -> 64 -> 1, FP32; forward,
+  backward and Adam run as the library's own HIP kernels) to predict
+  log(kernel time) from shape and configuration (``CostModel.fit``);
+* picks the configuration with the lowest predicted time
+  (``CostModel.pick``), falling back to exhaustive measurement when a shape
+  lies outside the range the model was trained on (``choose``);
+* judges the picks against exhaustive autotuning and against the method
+  without the model, the reference's tile-size rule
+  (``xkblas_auto_tilesize``, blas/xkblas.c:2437-2521) with the library's
+  default kernels (``evaluate``).
+"""
+import ctypes
+import math
+import time
+
+import numpy as np
+
+import xkblas_amd as xk
+
+OPS = {"gemm": 0, "syrk": 1, "trsm": 2, "potrf": 3}
+NFEAT = 13
+LOG_LO, LOG_HI = 6.0, 15.0  # shapes in [64, 32768]
+# xkblas_kernel_t of the reference rule used as the method without the
+# model (POTRF has no reference rule; it gets the TRSM one)
+KERN = {"gemm": 1, "syrk": 6, "trsm": 4, "potrf": 4}
+
+
+def split_counts(k):
+    """Split counts that keep >= 48 bits per side for inner dimension k
+    (the accuracy contract of the split mode), plus one more."""
+    L = xk.lib()
+    ok = [c for c in range(8, 16) if L.xkblas_split_side_bits(max(k, 2), c) >= 48]
+    return ok[:2] if ok else [15]
+
+
+def candidates(op, dtype, mode, m, n, k):
+    """Configurations the model chooses between for one call."""
+    if op in ("gemm", "syrk"):
+        if mode == 2 and dtype == 0:
+            return [{"tile": 0, "nb": 0, "split": c, "kernel": kv} for c in split_counts(k) for kv in (0, 1)]
+        return [{"tile": t, "nb": 0, "split": 0, "kernel": 0} for t in (64, 128)]
+    dim = m
+    nbs = [b for b in (64, 128, 256, 512, 1024, 2048) if b <= max(64, dim)]
+    return [{"tile": 0, "nb": b, "split": 0, "kernel": 0} for b in nbs]
+
+
+def heuristic(op, dtype, mode, m, n, k):
+    """The method without the model: the reference's tile-size rule for one
+    GPU and the library's default kernels."""
+    L = xk.lib()
+    old = L.xkblas_get_param()
+    L.xkblas_set_param(0, 0)
+    nb = int(L.xkblas_auto_tilesize(None, KERN[op], m, n if op != "potrf" else m, k))
+    L.xkblas_set_param(old, 0)
+    if op in ("gemm", "syrk"):
+        if mode == 2 and dtype == 0:
+            return {"tile": 0, "nb": 0, "split": 0, "kernel": 0}
+        return {"tile": 0, "nb": 0, "split": 0, "kernel": 0}
+    return {"tile": 0, "nb": min(nb, max(64, m)), "split": 0, "kernel": 0}
+
+
+def features(op, dtype, mode, m, n, k, cfg):
+    f = np.zeros(NFEAT, dtype=np.float32)
+    lg = lambda v: (math.log2(max(v, 1)) - LOG_LO) / (LOG_HI - LOG_LO)
+    f[0], f[1], f[2] = lg(m), lg(n), lg(k)
+    f[3 + OPS[op]] = 1.0
+    f[7] = float(dtype)
+    f[8] = 1.0 if mode == 2 else 0.0
+    f[9] = {0: 0.5, 64: 0.0, 128: 1.0}[cfg["tile"]]
+    f[10] = (math.log2(cfg["nb"]) - 6.0) / 5.0 if cfg["nb"] else 0.0
+    f[11] = cfg["split"] / 15.0
+    f[12] = float(cfg["kernel"])
+    return f
+
+
+def measure(op, dtype, mode, m, n, k, cfg, iters=1):
+    """Kernel time (ms) of one call through the library's timing service."""
+    ms = xk.lib().xkblas_tune_time(OPS[op], dtype, mode, 111, 111, m, n, k, cfg["tile"], cfg["nb"], cfg["split"],
+                                   cfg["kernel"], iters)
+    if ms < 0:
+        raise ValueError("bad tuning request %r" % ((op, dtype, mode, m, n, k, cfg),))
+    return ms
+
+
+def work(op, m, n, k):
+    if op == "gemm":
+        return 2.0 * m * n * k
+    if op == "syrk":
+        return float(n) * n * k
+    if op == "trsm":
+        return float(m) * m * n
+    return m ** 3 / 3.0
+
+
+def draw(r, max_flop=2e13, ops=("gemm", "syrk", "trsm", "potrf")):
+    """One tuning problem: op, dtype, mode and a log-uniform shape."""
+    while True:
+        op = ops[int(r.integers(len(ops)))]
+        dtype = int(r.random() < 0.25)
+        mode = 2 if (dtype == 0 and op in ("gemm", "syrk") and r.random() < 0.35) else 0
+        m, n, k = (int(round(2 ** r.uniform(LOG_LO, LOG_HI))) for _ in range(3))
+        if op == "potrf":
+            n = k = m
+        if op == "trsm":
+            k = m
+        f = work(op, m, n, k) * (15 if mode == 2 else 1)
+        if f <= max_flop:
+            return op, dtype, mode, m, n, k
+
+
+def collect(n_samples, seed=0, time_budget=None, max_flop=2e13, progress=None):
+    """Timings of random (problem, configuration) pairs: X (n x 13), y = log ms."""
+    r = np.random.default_rng(seed)
+    X, y, meta = [], [], []
+    t0 = time.time()
+    while len(y) < n_samples:
+        if time_budget is not None and time.time() - t0 > time_budget:
+            break
+        op, dtype, mode, m, n, k = draw(r, max_flop)
+        cands = candidates(op, dtype, mode, m, n, k)
+        cfg = cands[int(r.integers(len(cands)))]
+        ms = measure(op, dtype, mode, m, n, k, cfg)
+        X.append(features(op, dtype, mode, m, n, k, cfg))
+        y.append(math.log(max(ms, 1e-4)))
+        meta.append((OPS[op], dtype, mode, m, n, k, cfg["tile"], cfg["nb"], cfg["split"], cfg["kernel"]))
+        if progress and len(y) % progress == 0:
+            print("tuner: %d timings (%.0f s)" % (len(y), time.time() - t0), flush=True)
+    return np.array(X, dtype=np.float32), np.array(y, dtype=np.float32), np.array(meta, dtype=np.int64)
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class CostModel:
+    """The library's MLP wrapped for the tuner."""
+
+    def __init__(self, seed=1):
+        self.L = xk.lib()
+        self.h = self.L.xkblas_mlp_create(NFEAT, seed)
+        if not self.h:
+            raise RuntimeError("xkblas_mlp_create failed")
+        self.mu = 0.0
+        self.sd = 1.0
+
+    def fit(self, X, y, epochs=400, batch=256, lr=2e-3, seed=0):
+        r = np.random.default_rng(seed)
+        self.mu, self.sd = float(y.mean()), float(y.std() or 1.0)
+        t = ((y - self.mu) / self.sd).astype(np.float32)
+        idx = r.permutation(len(y))
+        Xs, ts = np.ascontiguousarray(X[idx]), np.ascontiguousarray(t[idx])
+        loss = self.L.xkblas_mlp_train(self.h, _p(Xs), _p(ts), len(ts), epochs, batch, lr)
+        return float(loss)
+
+    def predict(self, X):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        out = np.zeros(len(X), dtype=np.float32)
+        if len(X):
+            self.L.xkblas_mlp_forward(self.h, _p(X), len(X), _p(out))
+        return out * self.sd + self.mu
+
+    def pick(self, op, dtype, mode, m, n, k):
+        cands = candidates(op, dtype, mode, m, n, k)
+        pred = self.predict(np.stack([features(op, dtype, mode, m, n, k, c) for c in cands]))
+        return cands[int(np.argmin(pred))]
+
+    def params(self):
+        p = np.zeros(self.L.xkblas_mlp_num_params(self.h), dtype=np.float32)
+        self.L.xkblas_mlp_get_params(self.h, _p(p))
+        return p
+
+    def load(self, params, mu, sd):
+        params = np.ascontiguousarray(params, dtype=np.float32)
+        self.L.xkblas_mlp_set_params(self.h, _p(params))
+        self.mu, self.sd = float(mu), float(sd)
+
+    def close(self):
+        if self.h:
+            self.L.xkblas_mlp_destroy(self.h)
+            self.h = None
+
+
+def in_range(m, n, k):
+    return all(64 <= v <= 32768 for v in (m, n, k))
+
+
+def choose(model, op, dtype, mode, m, n, k):
+    """Model pick inside the trained range, exhaustive autotuning outside it."""
+    if model is not None and in_range(m, n, k):
+        return model.pick(op, dtype, mode, m, n, k), "model"
+    cands = candidates(op, dtype, mode, m, n, k)
+    times = [measure(op, dtype, mode, m, n, k, c, iters=2) for c in cands]
+    return cands[int(np.argmin(times))], "exhaustive"
+
+
+def evaluate(model, problems, iters=2):
+    """Judge: time of the model's pick, the exhaustive best and the heuristic
+    for each problem.  Returns per-problem rows and the geometric means of
+    heuristic/model (gain) and model/exhaustive (regret)."""
+    rows = []
+    for (op, dtype, mode, m, n, k) in problems:
+        cands = candidates(op, dtype, mode, m, n, k)
+        times = {i: measure(op, dtype, mode, m, n, k, c, iters) for i, c in enumerate(cands)}
+        best = min(times.values())
+        pick = model.pick(op, dtype, mode, m, n, k)
+        t_model = times[cands.index(pick)]
+        h = heuristic(op, dtype, mode, m, n, k)
+        t_heur = times[cands.index(h)] if h in cands else measure(op, dtype, mode, m, n, k, h, iters)
+        rows.append({"op": op, "dtype": dtype, "mode": mode, "m": m, "n": n, "k": k, "model_ms": t_model,
+                     "best_ms": best, "heuristic_ms": t_heur, "pick": pick, "heuristic": h})
+    gain = float(np.exp(np.mean([math.log(r["heuristic_ms"] / r["model_ms"]) for r in rows])))
+    regret = float(np.exp(np.mean([math.log(r["model_ms"] / r["best_ms"]) for r in rows])))
+    return rows, gain, regret
+
+
+def main(argv=None):
+    """Offline job: collect timings, train, judge, save the model.
+
+    python -m xkblas_amd.tuner --samples 20000 --out DIR [--budget SECONDS]
+    """
+    import argparse
+    import json
+    import os
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--samples", type=int, default=20000)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--budget", type=float, default=None, help="seconds of collection")
+    ap.add_argument("--data", default=None, help="append to / train from this .npz")
+    ap.add_argument("--out", default="gpurun_out/tuner")
+    ap.add_argument("--collect-only", action="store_true")
+    ap.add_argument("--eval", type=int, default=24, help="held-out problems to judge")
+    args = ap.parse_args(argv)
+    os.makedirs(args.out, exist_ok=True)
+    L = xk.lib()
+    L.xkblas_init()
+    X = y = meta = None
+    if args.data and os.path.exists(args.data):
+        d = np.load(args.data)
+        X, y, meta = d["X"], d["y"], d["meta"]
+    need = args.samples - (0 if y is None else len(y))
+    if need > 0:
+        Xn, yn, mn = collect(need, seed=args.seed + (0 if y is None else len(y)), time_budget=args.budget,
+                             progress=500)
+        X = Xn if X is None else np.concatenate([X, Xn])
+        y = yn if y is None else np.concatenate([y, yn])
+        meta = mn if meta is None else np.concatenate([meta, mn])
+        np.savez_compressed(args.data or os.path.join(args.out, "timings.npz"), X=X, y=y, meta=meta)
+    print("tuner: %d timings" % len(y), flush=True)
+    if args.collect_only or len(y) < args.samples:
+        return
+    model = CostModel()
+    t0 = time.time()
+    loss = model.fit(X, y)
+    t_fit = time.time() - t0
+    r = np.random.default_rng(12345)
+    probs = [draw(r, max_flop=5e12) for _ in range(args.eval)]
+    rows, gain, regret = evaluate(model, probs)
+    res = {"timings": int(len(y)), "train_loss_std_units": loss, "fit_seconds": t_fit, "gain_vs_heuristic": gain,
+           "regret_vs_exhaustive": regret, "rows": rows, "mu": model.mu, "sd": model.sd}
+    np.save(os.path.join(args.out, "cost_model_params.npy"), model.params())
+    with open(os.path.join(args.out, "tuner_result.json"), "w") as f:
+        json.dump(res, f, indent=1, default=str)
+    print("tuner: loss %.4f  gain vs heuristic %.3fx  regret vs exhaustive %.3fx" % (loss, gain, regret), flush=True)
+    model.close()
+    L.xkblas_tune_release()
+
+
+if __name__ == "__main__":
+    main()
