@@ -5,11 +5,12 @@ FP64 MFMA.  A step is one DGEMM C <- 1.5 A B - 0.5 C with M = N = K = 16384,
 rows of A and columns of B scaled by 2^u, u ~ U[-8, 8] (base entries
 U[-0.5, 0.5], seeded), called through xkblas_dgemm_async with A, B, C already
 resident in the HBM tile cache.  `value` is the FP64-accurate split-BF16
-rate when that mode is built, else the plain FP64-MFMA rate; both are
-reported, with the relative error of the split result on sampled entries
-against a double-double oracle.  A DPOTRF (N = 16384, A = G G^T + N I) is
-timed beside it.  The CPU baseline is the host BLAS (dgemm_ via dlopen, the
-reference's native path) on a bounded 4096^3 sample.
+rate; the plain FP64-MFMA rate and the split-FP8 rate are reported beside it,
+each with the relative error on sampled entries against a double-double
+oracle.  DPOTRF of A = G G^T + N I is timed at N = 16384 (FP64 MFMA) and at
+N = 32768 in FP64 MFMA and split-BF16 modes (sampled residuals, and the
+difference between the two factors).  The CPU baseline is the host BLAS
+(dgemm_ via dlopen, the reference's native path) on a bounded 4096^3 sample.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 16384]
 """
@@ -27,6 +28,7 @@ sys.path.insert(0, ROOT)
 
 FP64_PEAK = 78.6  # TFLOP/s, MI355X dense FP64 MFMA (AMD datasheet)
 BF16_PEAK = 2500.0
+FP8_PEAK = 5000.0
 
 
 def make_inputs(n, seed):
@@ -141,6 +143,9 @@ def main():
     ap.add_argument("--potrf-n", type=int, default=16384)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-potrf", action="store_true")
+    ap.add_argument("--no-fp8", action="store_true")
+    ap.add_argument("--potrf-split-n", type=int, default=32768,
+                    help="DPOTRF size for the split-BF16 vs FP64 comparison (0: skip)")
     ap.add_argument("--only-split", action="store_true", help="profile helper: one split-BF16 DGEMM, no JSON")
     ap.add_argument("--copy-probe", action="store_true",
                     help="profile helper: one 1 GiB device-to-device copy (calibrates HBM byte counters)")
@@ -180,32 +185,55 @@ def main():
     cfg = L.xkblas_get_configuration().decode()
     errs64 = accuracy(xk, A, B, 0, args.nb)
 
-    # split-BF16 mode (FP64-accurate)
-    print("bench: split-bf16 dgemm", file=sys.stderr, flush=True)
-    errs = accuracy(xk, A, B, 2, args.nb)
-    t_split = time_dgemm(xk, A, B, C, args.steps, args.warmup, 2, args.nb)
-    cnt_s, ms_s, _ = kernel_stats(xk, "gemm_f64")
-    c_mg, ms_mg, fl_mg = kernel_stats(xk, "split_mod_gemm")
-    c_pr, ms_pr, _ = kernel_stats(xk, "split_prep")
-    c_cr, ms_cr, _ = kernel_stats(xk, "split_crt")
-    L.xkblas_set_modemath(0)
-    split = {"tflops": round(flops / t_split / 1e12, 3), "ms_per_step": round(t_split * 1e3, 3),
-             "speedup_vs_fp64_mfma": round(t_fp64 / t_split, 3),
-             "moduli": L.xkblas_get_split_count() or "auto",
-             "bf16_gemm_tflops": round(fl_mg / ms_mg / 1e9, 1), "bf16_gemm_ms": round(ms_mg / max(c_mg, 1), 3),
-             "prep_ms": round(ms_pr / max(c_pr, 1), 3), "crt_ms": round(ms_cr / max(c_cr, 1), 3),
-             "rel_err": errs, "fp64_mfma_rel_err": errs64}
+    # split modes (FP64-accurate): BF16 (the headline) and FP8 residue products
+    def split_run(mode, cls, label, peak):
+        print("bench: split-%s dgemm" % label, file=sys.stderr, flush=True)
+        errs = accuracy(xk, A, B, mode, args.nb)
+        t = time_dgemm(xk, A, B, C, args.steps, args.warmup, mode, args.nb)
+        c_mg, ms_mg, fl_mg = kernel_stats(xk, cls)
+        c_pr, ms_pr, _ = kernel_stats(xk, "split_prep")
+        c_cr, ms_cr, _ = kernel_stats(xk, "split_crt")
+        L.xkblas_set_modemath(0)
+        nm, npl = ctypes.c_int(), ctypes.c_int()
+        bits = L.xkblas_split_info(mode, n, L.xkblas_get_split_count(), ctypes.byref(nm), ctypes.byref(npl))
+        return t, fl_mg / ms_mg / 1e9, ms_mg / max(c_mg, 1), {
+            "tflops": round(flops / t / 1e12, 3), "ms_per_step": round(t * 1e3, 3),
+            "speedup_vs_fp64_mfma": round(t_fp64 / t, 3),
+            "moduli": nm.value, "residue_gemms": npl.value, "bits_per_side": bits,
+            label + "_gemm_tflops": round(fl_mg / ms_mg / 1e9, 1),
+            label + "_gemm_frac_of_peak": round(fl_mg / ms_mg / 1e9 / peak, 4),
+            label + "_gemm_ms": round(ms_mg / max(c_mg, 1), 3),
+            "prep_ms": round(ms_pr / max(c_pr, 1), 3), "crt_ms": round(ms_cr / max(c_cr, 1), 3),
+            "rel_err": errs}
+
+    t_split, bf16_achieved, bf16_ms, split = split_run(2, "split_mod_gemm", "bf16", BF16_PEAK)
+    split["fp64_mfma_rel_err"] = errs64
+    split_fp8 = None
+    if not args.no_fp8:
+        split_fp8 = split_run(3, "split_mod_gemm_fp8", "fp8", FP8_PEAK)[3]
 
     potrf = None
     if not args.no_potrf:
+        del A, B, C
         print("bench: dpotrf", file=sys.stderr, flush=True)
-        potrf = bench_potrf(xk, args.potrf_n, 1024)
+        S, syrk_rate = make_spd(xk, args.potrf_n, 1024)
+        potrf = {"fp64_mfma": bench_potrf(xk, S, 1024, 0, syrk_rate)[0]}
+        del S
+        if args.potrf_split_n:
+            print("bench: dpotrf n=%d fp64 and split-bf16" % args.potrf_split_n, file=sys.stderr, flush=True)
+            S, syrk_rate = make_spd(xk, args.potrf_split_n, 1024)
+            p64, F64 = bench_potrf(xk, S, 1024, 0, syrk_rate)
+            psp, Fsp = bench_potrf(xk, S, 1024, 2)
+            psp["max_rel_diff_vs_fp64_factor"] = tril_rel_diff(Fsp, F64)
+            potrf["large_fp64_mfma"] = p64
+            potrf["large_split_bf16"] = psp
+            potrf["large_split_speedup"] = round(p64["kernel_ms"] / psp["kernel_ms"], 3)
+            del S, F64, Fsp
 
     xk.invalidate()
     cpu = None if args.no_cpu else cpu_baseline()
 
     value = split["tflops"]
-    bf16_achieved = fl_mg / ms_mg / 1e9
     out = {
         "metric": "FP64-accurate DGEMM/DPOTRF TFLOP/s, split-BF16 vs plain FP64 MFMA, 1 GPU",
         "value": round(value, 3),
@@ -227,11 +255,13 @@ def main():
         "fp64_mfma_kernel": {"achieved": round(flops / k_fp64 / 1e12, 3), "peak": FP64_PEAK,
                              "frac": round(flops / k_fp64 / 1e12 / FP64_PEAK, 4)},
         "split_bf16": split,
+        "split_fp8": split_fp8,
         "dpotrf": potrf,
         "roofline": {"bound": "mfma", "achieved": round(bf16_achieved, 1), "peak": BF16_PEAK, "unit": "TFLOP/s",
                      "frac": round(bf16_achieved / BF16_PEAK, 4), "traffic": None,
-                     "kernel": "mod_gemm_kernel<256,128> (BF16 residue GEMM, all moduli in one launch)",
-                     "kernel_ms": round(ms_mg / max(c_mg, 1), 3),
+                     "kernel": "mod_gemm_dma_kernel (BF16 residue GEMM, 256x256 tiles, all moduli in one "
+                               "launch)",
+                     "kernel_ms": round(bf16_ms, 3),
                      "fp64_equivalent_frac_of_fp64_peak": round(flops / t_split / 1e12 / FP64_PEAK, 4)},
         "cpu_baseline": cpu,
         "library": cfg,
@@ -240,11 +270,9 @@ def main():
     print(json.dumps(out))
 
 
-def bench_potrf(xk, n, nb):
-    """DPOTRF(lower) of A = G G^T + n I, G ~ U[-1,1]; A formed on the GPU by
-    DSYRK (the DSYRK + DPOTRF workload).  Each factorization starts from a
-    device copy of A refreshed by a host-to-device copy outside the timed
-    kernel window; the rate uses HIP-event kernel time."""
+def make_spd(xk, n, nb):
+    """A = G G^T + n I (lower triangle), G ~ U[-1,1]; G G^T formed on the GPU
+    by DSYRK (the DSYRK + DPOTRF workload).  Returns A and the DSYRK rate."""
     L = xk.lib()
     r = np.random.default_rng(7)
     G = np.asfortranarray(r.uniform(-1.0, 1.0, (n, n)))
@@ -258,8 +286,31 @@ def bench_potrf(xk, n, nb):
     xk.coherent(S, 122)
     S[np.diag_indices(n)] += n
     xk.invalidate()
-    del G
-    best = None
+    return S, fl_syrk / ms_syrk / 1e9
+
+
+def tril_rel_diff(X, Y):
+    """max |X - Y| / max |Y| over the lower triangles, in column blocks."""
+    n = X.shape[0]
+    d = m = 0.0
+    for j0 in range(0, n, 2048):
+        Xb, Yb = np.tril(X[:, j0:j0 + 2048], -j0), np.tril(Y[:, j0:j0 + 2048], -j0)
+        d = max(d, float(np.abs(Xb - Yb).max()))
+        m = max(m, float(np.abs(Yb).max()))
+    return d / m
+
+
+def bench_potrf(xk, S, nb, mode, dsyrk_tflops=None, samples=96):
+    """DPOTRF(lower) of S in math mode `mode`.  Each factorization starts from
+    a fresh host copy (its host-to-device copy is outside the timed kernel
+    window); the rate uses HIP-event kernel time.  The result's error is the
+    sampled residual max |S_ij - (L L^T)_ij| / (||L_i|| ||L_j||) over random
+    lower entries.  Returns (stats, factor)."""
+    L = xk.lib()
+    n = S.shape[0]
+    L.xkblas_set_modemath(mode)
+    L.xkblas_set_param(nb, 0)
+    best, F = None, None
     for _ in range(2):
         F = S.copy(order="F")
         L.xkblas_profile_reset()
@@ -267,12 +318,27 @@ def bench_potrf(xk, n, nb):
         assert rc == 0
         L.xkblas_sync()
         cnt, ms, fl = xk.profile("potrf_f64")
+        xk.coherent(F, 122)
         info = L.xkblas_potrf_info(F.ctypes.data_as(ctypes.c_void_p))
         xk.invalidate()
         best = ms if best is None else min(best, ms)
+    L.xkblas_set_modemath(0)
+    r = np.random.default_rng(5)
+    i = r.integers(0, n, samples)
+    j = r.integers(0, n, samples)
+    i, j = np.maximum(i, j), np.minimum(i, j)
+    res = 0.0
+    for a, b in zip(i, j):
+        la, lb = F[a, :a + 1], F[b, :b + 1]
+        dot = la[:b + 1] @ lb
+        res = max(res, abs(S[a, b] - dot) / (np.linalg.norm(la) * np.linalg.norm(lb)))
     fl = n**3 / 3.0 + n**2 / 2.0 + n / 6.0
-    return {"n": n, "tile": nb, "info": info, "fp64_tflops": round(fl / best / 1e9, 3), "kernel_ms": round(best, 3),
-            "dsyrk_tflops": round(fl_syrk / ms_syrk / 1e9, 3)}
+    out = {"n": n, "tile": nb, "mode": {0: "fp64_mfma", 2: "split_bf16", 3: "split_fp8"}[mode], "info": info,
+           "tflops": round(fl / best / 1e9, 3), "kernel_ms": round(best, 3),
+           "sampled_residual": float(res), "residual_samples": samples}
+    if dsyrk_tflops is not None:
+        out["dsyrk_tflops"] = round(dsyrk_tflops, 3)
+    return out, F
 
 
 if __name__ == "__main__":

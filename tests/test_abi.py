@@ -75,6 +75,39 @@ def test_query_calls_without_device():
     assert L.xkblas_auto_tilesize(None, 1, 2048, 2048, 2048) == 2048
 
 
+def test_split_plans():
+    """Moduli and residue-GEMM counts of the split modes (host arithmetic).
+    The bits per side t satisfy k 2^(2t) < P / 2 for P the product of the
+    moduli, recomputed here from the moduli sets."""
+    import ctypes
+    import math
+    import xkblas_amd
+    L = xkblas_amd.lib()
+    bf16 = [255, 253, 251, 247, 241, 239, 233, 229, 227, 223, 217, 211, 199, 197, 193]
+    fp8 = [32, 31, 29, 27, 25, 23, 19, 17, 13, 11, 7, 251, 241, 239, 233, 229, 227, 223, 211, 199]
+    for mods in (bf16, fp8):  # pairwise coprime, product below 2^123
+        assert all(math.gcd(a, b) == 1 for i, a in enumerate(mods) for b in mods[i + 1:])
+        assert math.log2(math.prod(mods)) < 123
+    assert all(m <= 33 for m in fp8[:11]) and all(m <= 255 for m in fp8)
+    nm, npl = ctypes.c_int(), ctypes.c_int()
+    for mode, mods, direct in ((2, bf16, 15), (3, fp8, 11)):
+        for k in (64, 1000, 16384, 32768):
+            t = L.xkblas_split_info(mode, k, 0, ctypes.byref(nm), ctypes.byref(npl))
+            n = nm.value
+            assert npl.value == min(n, direct) + 3 * max(0, n - direct)
+            bits = lambda c: math.log2(math.prod(mods[:c]))
+            want = lambda c: min(52, math.floor((bits(c) - 1 - math.ceil(math.log2(k))) / 2))
+            assert t == want(n) >= 48
+            assert n == 8 or want(n - 1) < 48  # smallest count reaching 48 bits
+    assert L.xkblas_split_info(3, 16384, 0, ctypes.byref(nm), ctypes.byref(npl)) == 51
+    assert (nm.value, npl.value) == (20, 38)
+    assert L.xkblas_split_info(2, 16384, 0, ctypes.byref(nm), ctypes.byref(npl)) == 51
+    assert (nm.value, npl.value) == (15, 15)
+    assert L.xkblas_split_info(3, 512, 12, ctypes.byref(nm), ctypes.byref(npl)) >= 8
+    assert (nm.value, npl.value) == (12, 14)
+    assert L.xkblas_split_info(0, 512, 0, None, None) == -1
+
+
 def test_illegal_argument_returns_code_when_asked():
     """With XKBLAS_ERROR_RETURN=1 the reference's negative codes come back
     (blas/zgemm.c:131-180); checks run before any device use."""

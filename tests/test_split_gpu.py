@@ -1,5 +1,6 @@
-"""Split-BF16 mode (XKBLAS_SPLIT_BF16_MATH): FP64-accurate products from BF16
-residue GEMMs with exact FP32 accumulation and CRT recombination.
+"""Split modes (XKBLAS_SPLIT_BF16_MATH, XKBLAS_SPLIT_FP8_MATH): FP64-accurate
+products from BF16 or FP8 residue GEMMs with exact FP32 accumulation and CRT
+recombination.  Every test runs in both modes.
 
 Bars:
   * integer inputs that fit the scaled width: the split product is exact, so
@@ -16,13 +17,14 @@ from helpers import rng, uniform, small_int, spd
 
 pytestmark = pytest.mark.gpu
 
-SPLIT = 2
+SPLIT_BF16, SPLIT_FP8 = 2, 3
 
 
-@pytest.fixture
-def split_mode(xk):
+@pytest.fixture(params=[SPLIT_BF16, SPLIT_FP8], ids=["bf16", "fp8"])
+def split_mode(xk, request):
     L = xk.lib()
-    L.xkblas_set_modemath(SPLIT)
+    L.xkblas_set_modemath(request.param)
+    L.mode = request.param
     yield L
     L.xkblas_set_modemath(0)
     L.xkblas_set_split_count(0)
@@ -81,7 +83,7 @@ def test_split_count_controls_accuracy(xk, orc, split_mode):
     Z = np.zeros((M, N), order="F")
     hi, lo = orc.dgemm_dd(111, 111, 1.0, A, B, 0.0, Z)
     errs = []
-    for ns in (9, 12, 15):
+    for ns in ((9, 12, 15) if split_mode.mode == SPLIT_BF16 else (12, 15, 20)):
         split_mode.xkblas_set_split_count(ns)
         C = np.zeros((M, N), order="F")
         xk.gemm(111, 111, 1.0, A, B, 0.0, C)
@@ -131,3 +133,21 @@ def test_split_trsm(xk, orc, split_mode):
     split_mode.xkblas_set_param(128, 0)
     xk.trsm(141, 122, 111, 131, 0.8, A, X)
     assert orc.trsm_residual(A, 122, 131, Xref, X) <= 10.0
+
+
+def test_split_long_inner_dimension(xk, orc, split_mode):
+    """k beyond one FP8 launch's exact range (32768) is summed over chunks."""
+    r = rng(31)
+    M, N, K = 96, 80, 40000
+    A = scaled(r, M, K, rows=True)
+    B = scaled(r, K, N, rows=False)
+    C0 = uniform(r, M, N)
+    C = C0.copy(order="F")
+    xk.gemm(111, 111, -1.25, A, B, 0.5, C)
+    hi, lo = orc.dgemm_dd(111, 111, -1.25, A, B, 0.5, C0)
+    assert np.linalg.norm(C - hi - lo) / np.linalg.norm(hi) <= 1e-14
+    lim = 2**12
+    Ai, Bi = small_int(r, M, K, -lim, lim), small_int(r, K, N, -lim, lim)
+    Ci = np.zeros((M, N), order="F")
+    xk.gemm(111, 111, 1.0, Ai, Bi, 0.0, Ci)
+    assert np.array_equal(Ci, (Ai.astype(np.int64) @ Bi.astype(np.int64)).astype(np.float64))

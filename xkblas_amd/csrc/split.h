@@ -1,4 +1,4 @@
-// split.h — FP64-accurate products from BF16 / FP8 residue matrices.
+// split.h — FP64-accurate products from BF16 / FP8 residue planes.
 #pragma once
 #include "blas3.h"
 
@@ -15,9 +15,12 @@ void split_gemm(const Exec& x, int transA, int transB, int m, int n, int k, T al
                 const T* B, long ldb, T beta, T* C, long ldc, int tri);
 
 // Number of CRT moduli for inner dimension k (requested >= 2 wins; else the
-// smallest count giving >= 48 bits per side), and the resulting bits.
-int split_choose_count(int k, int requested);
-int split_side_bits(int k, int nsplit);
+// smallest count giving >= 48 bits per side), the resulting bits per side,
+// and the number of residue GEMMs (planes) it takes; fp8 selects the FP8
+// moduli set (single planes for moduli <= 33, Karatsuba triples above).
+int split_choose_count(int k, int requested, bool fp8);
+int split_side_bits(int k, int nsplit, bool fp8);
+int split_plane_count(int k, int requested, bool fp8);
 
 // BF16 residue GEMM kernel: 0 = pipelined 256 x 256 (direct-to-LDS loads),
 // 1 = register-staged 256 x 128.

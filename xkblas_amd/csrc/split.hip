@@ -1,4 +1,4 @@
-// split.hip — FP64-accurate GEMM from BF16 residue products (gfx950).
+// split.hip — FP64-accurate GEMM from BF16 or FP8 residue products (gfx950).
 //
 // Method (CRT emulation of an FP64 product, Ozaki-scheme-II style):
 //  1. scale: every row i of op(A) by 2^sA_i and column j of op(B) by 2^sB_j,
@@ -6,19 +6,27 @@
 //     t is chosen so that k * 2^(2t) < P/2, P = p_1 ... p_N (pairwise coprime
 //     moduli <= 256), which makes the integer product C' = A' B' recoverable
 //     from its residues;
-//  2. split: residues A' mod p_l, B' mod p_l in [-127, 127] are exact BF16
-//     values; all N residue matrices are written in one pass (k-contiguous
-//     rows, padded to the GEMM tile);
-//  3. multiply: for each modulus one BF16 MFMA GEMM (v_mfma_f32_32x32x16_bf16)
-//     accumulates in FP32.  Products are <= 2^14 in magnitude, so FP32 sums
-//     stay exact; every 1024 k the accumulator is reduced mod p (exactly, by
-//     an FMA) before it could pass 2^24.  The output is the residue of
-//     C' mod p_l as int8;
-//  4. recombine: C' = sum_l r_l W_l mod P, W_l = (P/p_l) * ((P/p_l)^-1 mod
-//     p_l), with W_l held in three 41-bit limbs so every partial sum is an
-//     exact FP64 integer; the quotient by P is removed exactly per limb and
-//     the signed result is rounded once to FP64, scaled by 2^-(sA_i+sB_j),
-//     then alpha C' + beta C is formed in FP64.
+//  2. split: the residues A' mod p_l, B' mod p_l are written as "planes" of
+//     k-contiguous rows, padded to the GEMM tile, all in one pass:
+//       BF16: one plane per modulus, residues in [-127, 127] are exact BF16;
+//       FP8 (E4M3 holds the integers -16..16 exactly): a modulus <= 33 is one
+//       plane of its residue; a larger modulus p <= 255 is three planes of
+//       base-16 digits r = 16 h + l (h, l in [-8, 8]) and their sum s = h + l,
+//       so that  r_a r_b = 256 h_a h_b + 16 (s_a s_b - h_a h_b - l_a l_b)
+//       + l_a l_b  (Karatsuba) needs three FP8 products instead of four;
+//  3. multiply: one MFMA GEMM per plane (v_mfma_f32_32x32x16_bf16, or
+//     v_mfma_f32_32x32x64_f8f6f4 for FP8) accumulates in FP32.  Products are
+//     <= 2^14 (BF16) or <= 2^8 (FP8) in magnitude, so FP32 sums stay exact;
+//     the BF16 accumulator is reduced mod p (exactly, by an FMA) at least
+//     every 1024 k, before it could pass 2^24; FP8 products run over at most
+//     32768 k per call (longer inner dimensions are summed in FP64 over
+//     chunks).  The output is the plane's sum mod p as int8;
+//  4. recombine: the digit planes of a Karatsuba modulus are first combined
+//     into its residue (exact in FP32), then C' = sum_l r_l W_l mod P,
+//     W_l = (P/p_l) * ((P/p_l)^-1 mod p_l), with W_l held in three 41-bit limbs
+//     so every partial sum is an exact FP64 integer; the quotient by P is
+//     removed exactly per limb and the signed result is rounded once to FP64,
+//     scaled by 2^-(sA_i+sB_j), then alpha C' + beta C is formed in FP64.
 // Every step is exact except the initial rounding of the scaled inputs to t
 // bits and the final rounding to FP64.  N (the split count) is the only
 // accuracy knob: more moduli, larger t.
@@ -30,19 +38,27 @@
 
 namespace xk {
 
-// Pairwise coprime odd moduli <= 255: residues fit [-127, 127], so a
+// BF16: pairwise coprime odd moduli <= 255, residues fit [-127, 127], so a
 // product is < 2^14 and 1024 of them plus a folded remainder stay below 2^24.
-static const int kModuli[15] = {255, 253, 251, 247, 241, 239, 233, 229, 227, 223, 217, 211, 199, 197, 193};
-constexpr int MAXMOD = 15;
+static const int kModuliBF16[15] = {255, 253, 251, 247, 241, 239, 233, 229, 227, 223, 217, 211, 199, 197, 193};
+// FP8: the first FP8_DIRECT moduli (<= 33, residues in [-16, 16]) are single
+// planes; the primes after them are Karatsuba digit triples.  Together they
+// are pairwise coprime and P < 2^123 (the top limb of W stays below 2^41).
+constexpr int FP8_DIRECT = 11;
+static const int kModuliFP8[20] = {32,  31,  29,  27,  25,  23,  19,  17,  13,  11,
+                                   7,   251, 241, 239, 233, 229, 227, 223, 211, 199};
+constexpr int MAXMOD = 20;
+constexpr int MAXPLANE = FP8_DIRECT + 3 * (MAXMOD - FP8_DIRECT);
 constexpr double LIMB = 2199023255552.0;  // 2^41
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 struct Moduli {
-  int n;
-  float p[MAXMOD], invp[MAXMOD];
+  int n, nplane, fp8;
+  unsigned kara;  // bit l: modulus l is a Karatsuba digit triple
   double pd[MAXMOD], invpd[MAXMOD];
+  float plane_p[MAXPLANE], plane_ip[MAXPLANE];  // modulus of every plane
   double W[MAXMOD][3];  // W_l in limbs (most significant first)
   double P[3];
   double Pd, invP, halfP;
@@ -73,20 +89,29 @@ static void limbs(u128 v, double out[3])
   out[0] = (double)(unsigned long long)v;
 }
 
-static Moduli make_moduli(int n)
+static Moduli make_moduli(int n, bool fp8)
 {
+  const int* table = fp8 ? kModuliFP8 : kModuliBF16;
   Moduli M;
   M.n = n;
+  M.fp8 = fp8;
+  M.kara = 0;
+  M.nplane = 0;
   u128 P = 1;
   M.log2P = 0;
   for (int l = 0; l < n; ++l) {
-    P *= (u128)kModuli[l];
-    M.log2P += std::log2((double)kModuli[l]);
+    P *= (u128)table[l];
+    M.log2P += std::log2((double)table[l]);
   }
   for (int l = 0; l < n; ++l) {
-    const long long p = kModuli[l];
-    M.p[l] = (float)p;
-    M.invp[l] = 1.0f / (float)p;
+    const long long p = table[l];
+    const bool kara = fp8 && l >= FP8_DIRECT;
+    if (kara) M.kara |= 1u << l;
+    for (int q = 0; q < (kara ? 3 : 1); ++q) {
+      M.plane_p[M.nplane] = (float)p;
+      M.plane_ip[M.nplane] = 1.0f / (float)p;
+      ++M.nplane;
+    }
     M.pd[l] = (double)p;
     M.invpd[l] = 1.0 / (double)p;
     const u128 Ml = P / (u128)p;
@@ -110,15 +135,21 @@ static int side_bits(const Moduli& M, int k)
   return std::min(t, 52);
 }
 
-int split_choose_count(int k, int requested)
+int split_choose_count(int k, int requested, bool fp8)
 {
-  if (requested >= 2) return std::min(requested, MAXMOD);
+  const int maxn = fp8 ? MAXMOD : 15;
+  if (requested >= 2) return std::min(requested, maxn);
   // smallest count giving >= 48 bits per side (error ~ 2^-49 of the row max)
-  for (int n = 8; n <= MAXMOD; ++n) {
-    Moduli M = make_moduli(n);
+  for (int n = 8; n <= maxn; ++n) {
+    Moduli M = make_moduli(n, fp8);
     if (side_bits(M, k) >= 48) return n;
   }
-  return MAXMOD;
+  return maxn;
+}
+
+int split_plane_count(int k, int requested, bool fp8)
+{
+  return make_moduli(split_choose_count(k, requested, fp8), fp8).nplane;
 }
 
 // ---------------------------------------------------------------- kernels
@@ -169,16 +200,17 @@ struct SplitParams {
   const double* X;
   long ld;
   int rows, kdim, row_contig, nmod;
+  unsigned kara;
   const int* shift;
-  __bf16* R;
-  long rld, rstride;  // row length (kpad), per-modulus stride
+  void* R;            // planes [nplane][rows padded][rld]
+  long rld, rstride;  // row length (kpad), per-plane stride (elements)
   double pd[MAXMOD], invpd[MAXMOD];
 };
 
 __device__ __forceinline__ double sym_mod(double x, double p, double invp)
 {
   // x integer, |x| < 2^53: x - p * rint(x / p) is exact (FMA of integers);
-  // then fold into [-(p-1)/2, (p-1)/2] (p odd)
+  // then fold into [-(p-1)/2, (p-1)/2] (for even p: [-p/2, p/2 - 1])
   double r = fma(-p, rint(x * invp), x);
   const double hi = (p - 1.0) * 0.5;
   if (r > hi) r -= p;
@@ -186,8 +218,35 @@ __device__ __forceinline__ double sym_mod(double x, double p, double invp)
   return r;
 }
 
-// Residues of the scaled, rounded rows: R[l][i][k] (bf16, k contiguous);
-// the padded region (rows beyond `rows`, k beyond kdim) is written as 0.
+// four integers |v| <= 16 as packed E4M3 bytes (exact)
+__device__ __forceinline__ unsigned pack_fp8x4(float a, float b, float c, float d)
+{
+  int w = __builtin_amdgcn_cvt_pk_fp8_f32(a, b, 0, false);
+  w = __builtin_amdgcn_cvt_pk_fp8_f32(c, d, w, true);
+  return (unsigned)w;
+}
+
+// stores 8 k-consecutive plane values of one row
+__device__ __forceinline__ void store8(__bf16* out, const float* v)
+{
+  bf16x8 o;
+#pragma unroll
+  for (int e = 0; e < 8; ++e) o[e] = (__bf16)v[e];
+  *reinterpret_cast<bf16x8*>(out) = o;
+}
+
+__device__ __forceinline__ void store8(uint8_t* out, const float* v)
+{
+  uint2 o;
+  o.x = pack_fp8x4(v[0], v[1], v[2], v[3]);
+  o.y = pack_fp8x4(v[4], v[5], v[6], v[7]);
+  *reinterpret_cast<uint2*>(out) = o;
+}
+
+// Residue planes of the scaled, rounded rows: R[q][i][k] (k contiguous, Elem
+// = __bf16 or E4M3 bytes); the padded region (rows beyond `rows`, k beyond
+// kdim) is written as 0.
+template <typename Elem>
 __global__ __launch_bounds__(256) void split_kernel(SplitParams sp)
 {
   __shared__ double T[64][65];
@@ -207,8 +266,8 @@ __global__ __launch_bounds__(256) void split_kernel(SplitParams sp)
     T[i][k] = v;
   }
   __syncthreads();
-  // thread -> rows (t/8, t/8 + 32), 8 consecutive k: per modulus two 16-byte
-  // stores, and 8 lanes cover one 128-byte row segment
+  // thread -> rows (t/8, t/8 + 32), 8 consecutive k: per plane two stores,
+  // and 8 lanes cover one row segment of 64 k
   const int r0 = t >> 3, kc = (t & 7) * 8;
   double x0[8], x1[8];
 #pragma unroll
@@ -216,30 +275,53 @@ __global__ __launch_bounds__(256) void split_kernel(SplitParams sp)
     x0[v] = T[r0][kc + v];
     x1[v] = T[r0 + 32][kc + v];
   }
-  __bf16* out0 = sp.R + (long)(i0 + r0) * sp.rld + k0 + kc;
-  __bf16* out1 = out0 + 32L * sp.rld;
+  Elem* out0 = static_cast<Elem*>(sp.R) + (long)(i0 + r0) * sp.rld + k0 + kc;
+  Elem* out1 = out0 + 32L * sp.rld;
+  int q = 0;
   for (int l = 0; l < sp.nmod; ++l) {
     const double p = sp.pd[l], ip = sp.invpd[l];
-    bf16x8 v0, v1;
+    float v0[8], v1[8];
 #pragma unroll
     for (int v = 0; v < 8; ++v) {
-      v0[v] = (__bf16)(float)sym_mod(x0[v], p, ip);
-      v1[v] = (__bf16)(float)sym_mod(x1[v], p, ip);
+      v0[v] = (float)sym_mod(x0[v], p, ip);
+      v1[v] = (float)sym_mod(x1[v], p, ip);
     }
-    *reinterpret_cast<bf16x8*>(out0 + (long)l * sp.rstride) = v0;
-    *reinterpret_cast<bf16x8*>(out1 + (long)l * sp.rstride) = v1;
+    if (!((sp.kara >> l) & 1u)) {
+      store8(out0 + (long)q * sp.rstride, v0);
+      store8(out1 + (long)q * sp.rstride, v1);
+      ++q;
+      continue;
+    }
+    // r = 16 h + l with h = rint(r / 16): h, l in [-8, 8], h + l in [-16, 16]
+    float h0[8], h1[8], s0[8], s1[8];
+#pragma unroll
+    for (int v = 0; v < 8; ++v) {
+      h0[v] = rintf(v0[v] * 0.0625f);
+      h1[v] = rintf(v1[v] * 0.0625f);
+      v0[v] = fmaf(-16.0f, h0[v], v0[v]);
+      v1[v] = fmaf(-16.0f, h1[v], v1[v]);
+      s0[v] = h0[v] + v0[v];
+      s1[v] = h1[v] + v1[v];
+    }
+    store8(out0 + (long)q * sp.rstride, h0);
+    store8(out1 + (long)q * sp.rstride, h1);
+    store8(out0 + (long)(q + 1) * sp.rstride, v0);
+    store8(out1 + (long)(q + 1) * sp.rstride, v1);
+    store8(out0 + (long)(q + 2) * sp.rstride, s0);
+    store8(out1 + (long)(q + 2) * sp.rstride, s1);
+    q += 3;
   }
 }
 
 struct ModGemmParams {
-  const __bf16* A;  // [nmod][mpad][kpad]
-  const __bf16* B;  // [nmod][npad][kpad]
-  int8_t* C;        // [nmod][npad][mpad] (column-major residues)
-  long lk;          // kpad
+  const void* A;  // [nplane][mpad][kpad] (__bf16 or E4M3 bytes)
+  const void* B;  // [nplane][npad][kpad]
+  int8_t* C;      // [nplane][npad][mpad] (column-major residues)
+  long lk;        // kpad
   long sA, sB, sC;
-  long ldc;         // mpad
+  long ldc;       // mpad
   int kpad, tiles_m, tiles_n, tri;
-  float p[MAXMOD], invp[MAXMOD];
+  float p[MAXPLANE], invp[MAXPLANE];
 };
 
 constexpr int MG_BK = 64, MG_LK = MG_BK + 8;
@@ -291,8 +373,8 @@ __global__ __launch_bounds__(256) void mod_gemm_kernel(ModGemmParams prm)
     }
   }
   const long m0 = (long)tm * MG_BM, n0 = (long)tn * MG_BN;
-  const __bf16* A = prm.A + (long)l * prm.sA + m0 * prm.lk;
-  const __bf16* B = prm.B + (long)l * prm.sB + n0 * prm.lk;
+  const __bf16* A = static_cast<const __bf16*>(prm.A) + (long)l * prm.sA + m0 * prm.lk;
+  const __bf16* B = static_cast<const __bf16*>(prm.B) + (long)l * prm.sB + n0 * prm.lk;
 
   // global -> LDS: 8 lanes per 128-byte row segment, 32 rows per pass
   const int lrow = tid >> 3, lchunk = (tid & 7) * 8;
@@ -407,6 +489,79 @@ __device__ __forceinline__ void wait_lgkm0()
   __builtin_amdgcn_s_waitcnt((0xf) | (0x7 << 4) | (0x0 << 8) | (0x3 << 14));
 }
 
+// Output tile of this workgroup: consecutive workgroup ids go round-robin
+// over the 8 XCDs, so the remap gives every XCD a contiguous range of ids;
+// full products walk groups of 4 tile rows (their A panels share the XCD's
+// L2), triangles enumerate the lower (or upper) tiles.
+__device__ __forceinline__ void dm_tile(const ModGemmParams& prm, int& tm, int& tn)
+{
+  int pid = blockIdx.x;
+  const int total = gridDim.x;
+  if ((total & 7) == 0) pid = (pid & 7) * (total >> 3) + (pid >> 3);
+  if (prm.tri == TRI_FULL) {
+    constexpr int GM = 4;
+    const int group = GM * prm.tiles_n;
+    const int g = pid / group;
+    const int first = g * GM;
+    const int gm = min(prm.tiles_m - first, GM);
+    const int r = pid - g * group;
+    tm = first + r % gm;
+    tn = r / gm;
+  } else {
+    int i = (int)((sqrtf(8.0f * (float)pid + 1.0f) - 1.0f) * 0.5f);
+    while ((i + 1) * (i + 2) / 2 <= pid) ++i;
+    while (i * (i + 1) / 2 > pid) --i;
+    const int j = pid - i * (i + 1) / 2;
+    if (prm.tri == TRI_LOWER) { tm = i; tn = j; }
+    else { tm = j; tn = i; }
+  }
+}
+
+// Epilogue of the 256 x 256 kernels: residues in [-(p-1)/2, (p-1)/2] as
+// int8.  The accumulator block (i, j) holds D = A B^T with lane -> n =
+// 32j + lane%32 and (lane, r) -> m = 32i + 8(r/4) + 4(lane/32) + r%4, so
+// r%4 = 0..3 are 4 consecutive rows: one packed 32-bit word.  Words go to an
+// LDS tile T (column stride 272 bytes, 69632 bytes), then every column
+// leaves as 16 aligned 16-byte stores.  The caller has drained its loads and
+// LDS reads; T may alias the operand stages.
+__device__ __forceinline__ void dm_store_residues(const f32x16 (&acc)[4][4], float p, float ip, char* T,
+                                                  int8_t* C, long ldc)
+{
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+  const int fr = lane & 31, fh = lane >> 5;
+  const float hi = (p - 1.0f) * 0.5f;
+  constexpr int CS = 272;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        unsigned word = 0;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+          float v = fold(acc[i][j][4 * q + b], p, ip);
+          if (v > hi) v -= p;
+          if (v < -hi) v += p;
+          word |= ((unsigned)(int)v & 0xffu) << (8 * b);
+        }
+        const int ml = wm * 128 + i * 32 + 8 * q + 4 * fh;
+        const int nl = wn * 128 + j * 32 + fr;
+        *reinterpret_cast<unsigned*>(T + nl * CS + ml) = word;
+      }
+    }
+  }
+  __syncthreads();
+  const int chunk = tid & 15;
+#pragma unroll 4
+  for (int s2 = 0; s2 < 16; ++s2) {
+    const int col = (tid >> 4) + 16 * s2;
+    const uint4 v = *reinterpret_cast<const uint4*>(T + col * CS + chunk * 16);
+    *reinterpret_cast<uint4*>(C + (long)col * ldc + chunk * 16) = v;
+  }
+}
+
 __global__ __launch_bounds__(256) void mod_gemm_dma_kernel(ModGemmParams prm)
 {
   // [stage][A rows 256 x 32 | B rows 256 x 32]
@@ -418,31 +573,10 @@ __global__ __launch_bounds__(256) void mod_gemm_dma_kernel(ModGemmParams prm)
   const float p = prm.p[l], ip = prm.invp[l];
 
   int tm, tn;
-  {
-    int pid = blockIdx.x;
-    const int total = gridDim.x;
-    if ((total & 7) == 0) pid = (pid & 7) * (total >> 3) + (pid >> 3);
-    if (prm.tri == TRI_FULL) {
-      constexpr int GM = 4;
-      const int group = GM * prm.tiles_n;
-      const int g = pid / group;
-      const int first = g * GM;
-      const int gm = min(prm.tiles_m - first, GM);
-      const int r = pid - g * group;
-      tm = first + r % gm;
-      tn = r / gm;
-    } else {
-      int i = (int)((sqrtf(8.0f * (float)pid + 1.0f) - 1.0f) * 0.5f);
-      while ((i + 1) * (i + 2) / 2 <= pid) ++i;
-      while (i * (i + 1) / 2 > pid) --i;
-      const int j = pid - i * (i + 1) / 2;
-      if (prm.tri == TRI_LOWER) { tm = i; tn = j; }
-      else { tm = j; tn = i; }
-    }
-  }
+  dm_tile(prm, tm, tn);
   const long m0 = (long)tm * DM_BM, n0 = (long)tn * DM_BN;
-  const __bf16* A = prm.A + (long)l * prm.sA + m0 * prm.lk;
-  const __bf16* B = prm.B + (long)l * prm.sB + n0 * prm.lk;
+  const __bf16* A = static_cast<const __bf16*>(prm.A) + (long)l * prm.sA + m0 * prm.lk;
+  const __bf16* B = static_cast<const __bf16*>(prm.B) + (long)l * prm.sB + n0 * prm.lk;
 
   // this lane's source for LDS block b = wave*4 + i: row 16b + lane/4 and the
   // chunk stored at physical position lane%4 (the swizzle term depends only
@@ -554,56 +688,186 @@ __global__ __launch_bounds__(256) void mod_gemm_dma_kernel(ModGemmParams prm)
     }
   }
 
-  // Epilogue: residues in [-(p-1)/2, (p-1)/2] as int8.  The accumulator block
-  // (i, j) holds D = A B^T with lane -> n = 32j + lane%32 and (lane, r) ->
-  // m = 32i + 8(r/4) + 4(lane/32) + r%4, so r%4 = 0..3 are 4 consecutive rows:
-  // one packed 32-bit word.  Words go to an LDS tile (column stride 272
-  // bytes), then every column leaves as 16 aligned 16-byte stores.
-  const float hi = (p - 1.0f) * 0.5f;
   wait_vmcnt_le(0);
   __syncthreads();
-  constexpr int CS = 272;
-  char* T = reinterpret_cast<char*>(&S[0][0]);
+  dm_store_residues(acc, p, ip, reinterpret_cast<char*>(&S[0][0]), prm.C + (long)l * prm.sC + n0 * prm.ldc + m0,
+                    prm.ldc);
+}
+
+// ---- FP8 variant: the same 256 x 256 tile and 4-stage ring of 64-byte row
+// slices (one stage = 64 k of E4M3 bytes), one v_mfma_f32_32x32x64_f8f6f4 per
+// 32 x 32 block and stage.  A lane's 32-byte operand is row lane%32, logical
+// chunks 2 (lane/32) and 2 (lane/32) + 1 of the stage: byte v is k =
+// 32 (lane/32) + v for A and B alike, which is all the product needs.
+// Products are <= 2^8, so FP32 sums of up to F8_KMAX of them are exact
+// (< 2^24) and the kernel needs no modular fold: the host splits longer
+// inner dimensions.
+typedef int i32x8 __attribute__((ext_vector_type(8)));
+constexpr int F8_BK = 64;        // bytes (= k) per row and stage
+constexpr int F8_KMAX = 32768;   // kpad limit of one launch
+
+// (the reads are typed like those of the BF16 kernel: the compiler then
+// leaves the waits for the direct-to-LDS loads to the kernel's own
+// s_waitcnt / s_barrier sequence instead of draining all loads in flight)
+__device__ __forceinline__ i32x8 lds_read32(const uint8_t* base, int off0, int off1)
+{
+  const bf16x8 lo8 = *reinterpret_cast<const bf16x8*>(base + off0);
+  const bf16x8 hi8 = *reinterpret_cast<const bf16x8*>(base + off1);
+  const uint4 lo = __builtin_bit_cast(uint4, lo8);
+  const uint4 hi = __builtin_bit_cast(uint4, hi8);
+  i32x8 v;
+  v[0] = (int)lo.x; v[1] = (int)lo.y; v[2] = (int)lo.z; v[3] = (int)lo.w;
+  v[4] = (int)hi.x; v[5] = (int)hi.y; v[6] = (int)hi.z; v[7] = (int)hi.w;
+  return v;
+}
+
+__device__ __forceinline__ f32x16 mfma_fp8(const i32x8& a, const i32x8& b, const f32x16& c)
+{
+  // format codes 0 (E4M3) for both operands, no block scales
+  return __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a, b, c, 0, 0, 0, 0, 0, 0);
+}
+
+__global__ __launch_bounds__(256) void mod_gemm_fp8_kernel(ModGemmParams prm)
+{
+  // [stage][A rows 256 x 64 B | B rows 256 x 64 B]
+  __shared__ __attribute__((aligned(1024))) uint8_t S[DM_STAGES][(DM_BM + DM_BN) * F8_BK];
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+  const int l = blockIdx.z;
+  const float p = prm.p[l], ip = prm.invp[l];
+
+  int tm, tn;
+  dm_tile(prm, tm, tn);
+  const long m0 = (long)tm * DM_BM, n0 = (long)tn * DM_BN;
+  const uint8_t* A = static_cast<const uint8_t*>(prm.A) + (long)l * prm.sA + m0 * prm.lk;
+  const uint8_t* B = static_cast<const uint8_t*>(prm.B) + (long)l * prm.sB + n0 * prm.lk;
+
+  // direct-to-LDS loads: LDS block b = wave*4 + i holds rows 16b..16b+15 of a
+  // stage (1 KB); lane writes physical chunk lane%4 of row 16b + lane/4 and
+  // fetches the logical chunk stored there (chunk c of row r sits at c ^ ((r >> 2) & 3))
+  const int srow = lane >> 2;
+  const int schunk = (lane & 3) ^ ((srow >> 2) & 3);
+  const uint8_t* gA = A + (long)(wave * 64 + srow) * prm.lk + schunk * 16;
+  const uint8_t* gB = B + (long)(wave * 64 + srow) * prm.lk + schunk * 16;
+  const long step16 = 16 * prm.lk;
+  auto issue = [&](int stage, int k0) {
+    uint8_t* base = S[stage] + wave * 4096;
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
+    for (int i = 0; i < 4; ++i)
+      __builtin_amdgcn_global_load_lds((const void*)(gA + i * step16 + k0), (lds_void*)(base + i * 1024), 16, 0, 0);
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
+    for (int i = 0; i < 4; ++i)
+      __builtin_amdgcn_global_load_lds((const void*)(gB + i * step16 + k0),
+                                       (lds_void*)(base + DM_BM * F8_BK + i * 1024), 16, 0, 0);
+  };
+
+  f32x16 acc[4][4];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        unsigned word = 0;
+  for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int b = 0; b < 4; ++b) {
-          float v = fold(acc[i][j][4 * q + b], p, ip);
-          if (v > hi) v -= p;
-          if (v < -hi) v += p;
-          word |= ((unsigned)(int)v & 0xffu) << (8 * b);
-        }
-        const int ml = wm * 128 + i * 32 + 8 * q + 4 * fh;
-        const int nl = wn * 128 + j * 32 + fr;
-        *reinterpret_cast<unsigned*>(T + nl * CS + ml) = word;
-      }
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
+
+  const int nk = prm.kpad / F8_BK;  // even (kpad is a multiple of 512)
+  for (int s = 0; s < DM_STAGES - 1 && s < nk; ++s) issue(s, s * F8_BK);
+
+  // operand reads: row wm*128 + 32i + lane%32 (A) / wn*128 + 32j + lane%32
+  // (B), logical chunks 2fh, 2fh + 1; the swizzle term depends on lane%32 only
+  const int fr = lane & 31, fh = lane >> 5;
+  const int sw = (fr >> 2) & 3;
+  const int rdA0 = (wm * 128 + fr) * F8_BK + ((2 * fh) ^ sw) * 16;
+  const int rdA1 = (wm * 128 + fr) * F8_BK + ((2 * fh + 1) ^ sw) * 16;
+  const int rdB0 = DM_BM * F8_BK + (wn * 128 + fr) * F8_BK + ((2 * fh) ^ sw) * 16;
+  const int rdB1 = DM_BM * F8_BK + (wn * 128 + fr) * F8_BK + ((2 * fh + 1) ^ sw) * 16;
+  auto readA = [&](int i, const uint8_t* st) {
+    return lds_read32(st, rdA0 + i * 32 * F8_BK, rdA1 + i * 32 * F8_BK);
+  };
+  auto readB = [&](int j, const uint8_t* st) {
+    return lds_read32(st, rdB0 + j * 32 * F8_BK, rdB1 + j * 32 * F8_BK);
+  };
+
+  // Registers: a[4] and two B sets (b of this stage, bn of the next).  On
+  // entry to stage kt, a[0], a[1] and b hold stage kt (read during stage
+  // kt-1).  a[2], a[3] are read under the MFMAs of rows i = 0, 1; then the
+  // barrier (stage kt+1 landed everywhere, stage kt-1 free) and the loads of
+  // stage kt+3; then a[0], a[1] and bn of stage kt+1 are read under the MFMAs
+  // of rows i = 2, 3.
+  i32x8 a[4], b0[4], b1[4];
+  // u = kt % DM_STAGES is a compile-time constant at every call, so the
+  // compiler can tell the LDS reads of a stage from the direct-to-LDS loads
+  // in flight into the others (and need not wait for those loads)
+  auto stage = [&](int kt, int u, i32x8 (&b)[4], i32x8 (&bn)[4]) {
+    const uint8_t* st = S[u];
+    wait_lgkm0();
+    a[2] = readA(2, st);
+    a[3] = readA(3, st);
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[i][j] = mfma_fp8(a[i], b[j], acc[i][j]);
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+      __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+      __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);
     }
+    wait_lgkm0();
+    if (kt + 1 < nk) {
+      wait_vmcnt_le(8 * (min(nk - 1, kt + DM_STAGES - 2) - (kt + 1)));
+      __builtin_amdgcn_s_barrier();
+      if (kt + DM_STAGES - 1 < nk) issue((u + DM_STAGES - 1) % DM_STAGES, (kt + DM_STAGES - 1) * F8_BK);
+    }
+    // (after the last stage these reads fetch unused data from an idle buffer)
+    const uint8_t* sn = S[(u + 1) % DM_STAGES];
+#pragma unroll
+    for (int i = 2; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[i][j] = mfma_fp8(a[i], b[j], acc[i][j]);
+    a[0] = readA(0, sn);
+    a[1] = readA(1, sn);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) bn[j] = readB(j, sn);
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+      __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+      __builtin_amdgcn_sched_group_barrier(0x100, 3, 0);
+    }
+#pragma unroll
+    for (int g = 0; g < 4; ++g) __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+  };
+
+  wait_vmcnt_le(8 * (min(nk, DM_STAGES - 1) - 1));
+  __builtin_amdgcn_s_barrier();
+  a[0] = readA(0, S[0]);
+  a[1] = readA(1, S[0]);
+#pragma unroll
+  for (int j = 0; j < 4; ++j) b0[j] = readB(j, S[0]);
+  static_assert(DM_STAGES == 4, "the k loop is unrolled by the ring depth");
+  for (int kt = 0; kt < nk; kt += 4) {  // nk is a multiple of 8
+    stage(kt, 0, b0, b1);
+    stage(kt + 1, 1, b1, b0);
+    stage(kt + 2, 2, b0, b1);
+    stage(kt + 3, 3, b1, b0);
   }
+
+  wait_vmcnt_le(0);
   __syncthreads();
-  int8_t* C = prm.C + (long)l * prm.sC + n0 * prm.ldc + m0;
-  const int chunk = tid & 15;
-#pragma unroll 4
-  for (int s2 = 0; s2 < 16; ++s2) {
-    const int col = (tid >> 4) + 16 * s2;
-    const uint4 v = *reinterpret_cast<const uint4*>(T + col * CS + chunk * 16);
-    *reinterpret_cast<uint4*>(C + (long)col * prm.ldc + chunk * 16) = v;
-  }
+  dm_store_residues(acc, p, ip, reinterpret_cast<char*>(&S[0][0]), prm.C + (long)l * prm.sC + n0 * prm.ldc + m0,
+                    prm.ldc);
 }
 
 struct RecParams {
   const int8_t* R;
   long ldr, sR;
   int nmod, m, n, tri;
+  unsigned kara;  // bit l: modulus l is a Karatsuba digit triple (3 planes)
   const int* shA;
   const int* shB;
   double* C;
   long ldc;
   double alpha, beta;
+  float p[MAXMOD], invp[MAXMOD];
   double W[MAXMOD][3];
   double P[3];
   double invP, halfP;
@@ -647,14 +911,35 @@ __global__ __launch_bounds__(256) void reconstruct_kernel(RecParams rp)
   if (rp.tri == TRI_UPPER && i0 > j) return;
   double S[4][3] = {};
   const int8_t* r = rp.R + (long)j * rp.ldr + i0;
+  int q = 0;  // plane of modulus l
   for (int l = 0; l < rp.nmod; ++l) {
-    const unsigned w = *reinterpret_cast<const unsigned*>(r + (long)l * rp.sR);
+    const unsigned w = *reinterpret_cast<const unsigned*>(r + (long)q * rp.sR);
+    double v[4];
+    if (!((rp.kara >> l) & 1u)) {
+#pragma unroll
+      for (int e = 0; e < 4; ++e) v[e] = (double)(int8_t)(w >> (8 * e));
+      q += 1;
+    } else {
+      // planes h_a h_b, l_a l_b, s_a s_b (each mod p):
+      // r = 256 hh + 16 (ss - hh - ll) + ll, |r| < 2^16 before the reduction
+      const unsigned wl = *reinterpret_cast<const unsigned*>(r + (long)(q + 1) * rp.sR);
+      const unsigned ws = *reinterpret_cast<const unsigned*>(r + (long)(q + 2) * rp.sR);
+      const float p = rp.p[l], ip = rp.invp[l];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const float hh = (float)(int8_t)(w >> (8 * e));
+        const float ll = (float)(int8_t)(wl >> (8 * e));
+        const float ss = (float)(int8_t)(ws >> (8 * e));
+        const float x = fmaf(256.0f, hh, fmaf(16.0f, ss - hh - ll, ll));
+        v[e] = (double)fmaf(-p, rintf(x * ip), x);
+      }
+      q += 3;
+    }
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
-      const double v = (double)(int8_t)(w >> (8 * e));
-      S[e][0] = fma(v, rp.W[l][0], S[e][0]);
-      S[e][1] = fma(v, rp.W[l][1], S[e][1]);
-      S[e][2] = fma(v, rp.W[l][2], S[e][2]);
+      S[e][0] = fma(v[e], rp.W[l][0], S[e][0]);
+      S[e][1] = fma(v[e], rp.W[l][1], S[e][1]);
+      S[e][2] = fma(v[e], rp.W[l][2], S[e][2]);
     }
   }
   const int sb = rp.shB[j];
@@ -690,33 +975,58 @@ void split_set_kernel(int v) { g_mod_gemm_variant = v; }
 int split_get_kernel() { return g_mod_gemm_variant; }
 
 // bits per side t for k and a split count (0 = automatic count)
-int split_side_bits(int k, int nsplit)
+int split_side_bits(int k, int nsplit, bool fp8)
 {
-  const int n = split_choose_count(k, nsplit);
-  return side_bits(make_moduli(n), k);
+  const int n = split_choose_count(k, nsplit, fp8);
+  return side_bits(make_moduli(n, fp8), k);
 }
+
+static void split_gemm_chunk(const Exec& x, int transA, int transB, int m, int n, int k, double alpha,
+                             const double* A, long lda, const double* B, long ldb, double beta, double* C, long ldc,
+                             int tri);
 
 template <>
 void split_gemm<double>(const Exec& x, int transA, int transB, int m, int n, int k, double alpha, const double* A,
                         long lda, const double* B, long ldb, double beta, double* C, long ldc, int tri)
 {
-  XK_CHECK(x.mode == 2, "split-FP8 mode is not available in this build");
-  const int nmod = split_choose_count(k, x.nsplit);
-  const Moduli M = make_moduli(nmod);
+  if (x.mode != 3 || k <= F8_KMAX) {
+    split_gemm_chunk(x, transA, transB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, tri);
+    return;
+  }
+  // FP8: C = alpha sum_c op(A)[:, c] op(B)[c, :] + beta C over k-chunks c of
+  // at most F8_KMAX (each chunk exact up to its final FP64 rounding)
+  for (int k0 = 0; k0 < k; k0 += F8_KMAX) {
+    const int kc = std::min(F8_KMAX, k - k0);
+    const double* Ac = transA == NoTrans ? A + (long)k0 * lda : A + k0;
+    const double* Bc = transB == NoTrans ? B + k0 : B + (long)k0 * ldb;
+    split_gemm_chunk(x, transA, transB, m, n, kc, alpha, Ac, lda, Bc, ldb, k0 == 0 ? beta : 1.0, C, ldc, tri);
+  }
+}
+
+static void split_gemm_chunk(const Exec& x, int transA, int transB, int m, int n, int k, double alpha,
+                             const double* A, long lda, const double* B, long ldb, double beta, double* C, long ldc,
+                             int tri)
+{
+  const bool fp8 = x.mode == 3;
+  const int nmod = split_choose_count(k, x.nsplit, fp8);
+  const Moduli M = make_moduli(nmod, fp8);
+  const int np = M.nplane;
   const int tb = side_bits(M, k);
   XK_CHECK(tb >= 8, "split count too small for this k");
   const long mpad = rup(m, MG_PAD), npad = rup(n, MG_PAD), kpad = rup(k, 512);
   static_assert(MG_PAD % DM_BM == 0 && MG_BK % DM_BK == 0, "padding must cover the tiles");
+  XK_CHECK(!fp8 || kpad <= F8_KMAX, "FP8 residue product beyond its exact inner dimension");
   const bool same = (A == B) && (lda == ldb) && (transA != transB) && m == n;  // SYRK: op(B) = op(A)^T
+  const size_t esz = fp8 ? 1 : sizeof(__bf16);
 
   char* ws_e = (char*)scratch(WS_SE, sizeof(unsigned long long) * (mpad + npad) + sizeof(int) * (mpad + npad));
   unsigned long long* bitsA = (unsigned long long*)ws_e;
   unsigned long long* bitsB = bitsA + mpad;
   int* shA = (int*)(bitsB + npad);
   int* shB = shA + mpad;
-  __bf16* RA = (__bf16*)scratch(WS_SA, sizeof(__bf16) * (size_t)nmod * mpad * kpad);
-  __bf16* RB = same ? RA : (__bf16*)scratch(WS_SB, sizeof(__bf16) * (size_t)nmod * npad * kpad);
-  int8_t* RC = (int8_t*)scratch(WS_SC, (size_t)nmod * mpad * npad);
+  void* RA = scratch(WS_SA, esz * (size_t)np * mpad * kpad);
+  void* RB = same ? RA : scratch(WS_SB, esz * (size_t)np * npad * kpad);
+  int8_t* RC = (int8_t*)scratch(WS_SC, (size_t)np * mpad * npad);
   hipStream_t s = x.s;
 
   // 1-2: exponents of the rows of op(A) and columns of op(B)
@@ -734,45 +1044,54 @@ void split_gemm<double>(const Exec& x, int transA, int transB, int m, int n, int
   }
   XK_HIP(hipGetLastError());
 
-  // 3: residue matrices
+  // 3: residue planes
   SplitParams sp;
   sp.nmod = nmod;
+  sp.kara = M.kara;
   for (int l = 0; l < nmod; ++l) { sp.pd[l] = M.pd[l]; sp.invpd[l] = M.invpd[l]; }
   sp.X = A; sp.ld = lda; sp.rows = m; sp.kdim = k; sp.row_contig = rcA; sp.shift = shA;
   sp.R = RA; sp.rld = kpad; sp.rstride = mpad * kpad;
-  hipLaunchKernelGGL(split_kernel, dim3((int)(kpad / 64), (int)(mpad / 64)), dim3(256), 0, s, sp);
+  auto split = [&](long rows_pad) {
+    const dim3 grid((unsigned)(kpad / 64), (unsigned)(rows_pad / 64));
+    if (fp8) hipLaunchKernelGGL(split_kernel<uint8_t>, grid, dim3(256), 0, s, sp);
+    else hipLaunchKernelGGL(split_kernel<__bf16>, grid, dim3(256), 0, s, sp);
+  };
+  split(mpad);
   if (!same) {
     sp.X = B; sp.ld = ldb; sp.rows = n; sp.row_contig = rcB; sp.shift = shB;
     sp.R = RB; sp.rstride = npad * kpad;
-    hipLaunchKernelGGL(split_kernel, dim3((int)(kpad / 64), (int)(npad / 64)), dim3(256), 0, s, sp);
+    split(npad);
   }
   XK_HIP(hipGetLastError());
 
   R.prof_end();
 
-  // 4: one BF16 GEMM per modulus
+  // 4: one MFMA GEMM per plane
   ModGemmParams g;
   g.A = RA; g.B = RB; g.C = RC;
   g.lk = kpad; g.sA = mpad * kpad; g.sB = same ? mpad * kpad : npad * kpad; g.sC = mpad * npad;
   g.ldc = mpad; g.kpad = (int)kpad;
-  for (int l = 0; l < nmod; ++l) { g.p[l] = M.p[l]; g.invp[l] = M.invp[l]; }
+  for (int q = 0; q < np; ++q) { g.p[q] = M.plane_p[q]; g.invp[q] = M.plane_ip[q]; }
   g.tri = tri;
-  R.prof_begin("split_mod_gemm", 2.0 * nmod * (double)m * n * k * (tri == TRI_FULL ? 1.0 : 0.5));
+  R.prof_begin(fp8 ? "split_mod_gemm_fp8" : "split_mod_gemm",
+               2.0 * np * (double)m * n * k * (tri == TRI_FULL ? 1.0 : 0.5));
   g.tiles_m = (int)(mpad / DM_BM);
   g.tiles_n = (int)(npad / DM_BN);
   const long tiles = tri == TRI_FULL ? (long)g.tiles_m * g.tiles_n : (long)g.tiles_m * (g.tiles_m + 1) / 2;
-  if (g_mod_gemm_variant == 1) {
+  if (fp8) {
+    hipLaunchKernelGGL(mod_gemm_fp8_kernel, dim3((unsigned)tiles, 1, np), dim3(256), 0, s, g);
+  } else if (g_mod_gemm_variant == 1) {
     g.tiles_n = (int)(npad / 128);
     if (tri == TRI_FULL)
-      hipLaunchKernelGGL((mod_gemm_kernel<256, 128>), dim3((unsigned)((long)g.tiles_m * g.tiles_n), 1, nmod),
+      hipLaunchKernelGGL((mod_gemm_kernel<256, 128>), dim3((unsigned)((long)g.tiles_m * g.tiles_n), 1, np),
                          dim3(256), 0, s, g);
     else {
       g.tiles_m = g.tiles_n = (int)(mpad / 128);
       hipLaunchKernelGGL((mod_gemm_kernel<128, 128>),
-                         dim3((unsigned)((long)g.tiles_m * (g.tiles_m + 1) / 2), 1, nmod), dim3(256), 0, s, g);
+                         dim3((unsigned)((long)g.tiles_m * (g.tiles_m + 1) / 2), 1, np), dim3(256), 0, s, g);
     }
   } else {
-    hipLaunchKernelGGL(mod_gemm_dma_kernel, dim3((unsigned)tiles, 1, nmod), dim3(256), 0, s, g);
+    hipLaunchKernelGGL(mod_gemm_dma_kernel, dim3((unsigned)tiles, 1, np), dim3(256), 0, s, g);
   }
   XK_HIP(hipGetLastError());
   R.prof_end();
@@ -781,9 +1100,13 @@ void split_gemm<double>(const Exec& x, int transA, int transB, int m, int n, int
   R.prof_begin("split_crt", 0.0);
   RecParams rp;
   rp.R = RC; rp.ldr = mpad; rp.sR = mpad * npad; rp.nmod = nmod; rp.m = m; rp.n = n; rp.tri = tri;
+  rp.kara = M.kara;
   rp.shA = shA; rp.shB = same ? shA : shB; rp.C = C; rp.ldc = ldc; rp.alpha = alpha; rp.beta = beta;
-  for (int l = 0; l < nmod; ++l)
+  for (int l = 0; l < nmod; ++l) {
+    rp.p[l] = (float)M.pd[l];
+    rp.invp[l] = 1.0f / (float)M.pd[l];
     for (int q = 0; q < 3; ++q) rp.W[l][q] = M.W[l][q];
+  }
   for (int q = 0; q < 3; ++q) rp.P[q] = M.P[q];
   rp.invP = M.invP; rp.halfP = M.halfP;
   XK_CHECK(n <= 65535 * 64, "reconstruct grid");

@@ -167,7 +167,16 @@ void xkblas_set_split_kernel(int variant)
   split_set_kernel(variant == 1 ? 1 : 0);
 }
 
-int xkblas_split_side_bits(int k, int nsplit) { return split_side_bits(k, nsplit); }
+int xkblas_split_side_bits(int k, int nsplit) { return split_side_bits(k, nsplit, false); }
+
+int xkblas_split_info(int mode, int k, int nsplit, int* nmod, int* nplane)
+{
+  if (mode != XKBLAS_SPLIT_BF16_MATH && mode != XKBLAS_SPLIT_FP8_MATH) return -1;
+  const bool fp8 = mode == XKBLAS_SPLIT_FP8_MATH;
+  if (nmod) *nmod = split_choose_count(k, nsplit, fp8);
+  if (nplane) *nplane = split_plane_count(k, nsplit, fp8);
+  return split_side_bits(k, nsplit, fp8);
+}
 
 void xkblas_tune_release(void)
 {

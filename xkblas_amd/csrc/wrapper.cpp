@@ -26,6 +26,8 @@ bool g_gpu_ready = false;
 // GPU side initialised on the first call that goes to the GPU, so that a
 // process whose calls all stay on the host never touches the device (the
 // reference initialises in the library constructor).
+void wrapper_exit() { xkblas_finalize(); }
+
 void gpu_ready()
 {
   if (g_gpu_ready) return;
@@ -33,6 +35,9 @@ void gpu_ready()
     fprintf(stderr, "libxkblas_amd_wrapper: cannot initialize\n");
     abort();
   }
+  // registered after the HIP runtime's own exit handlers, so it runs before
+  // them (a library destructor would run after the runtime is gone)
+  atexit(wrapper_exit);
   g_gpu_ready = true;
 }
 
@@ -46,11 +51,6 @@ __attribute__((constructor)) void wrapper_init()
     const char* v = getenv(name);
     g_threshold[i] = v ? (atol(v) < 0 ? 0.0 : (double)atol(v)) : (double)th;
   }
-}
-
-__attribute__((destructor)) void wrapper_fini()
-{
-  if (g_gpu_ready) xkblas_finalize();
 }
 
 int trans_of(const char* t)

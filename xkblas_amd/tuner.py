@@ -28,26 +28,31 @@ import numpy as np
 import xkblas_amd as xk
 
 OPS = {"gemm": 0, "syrk": 1, "trsm": 2, "potrf": 3}
-NFEAT = 13
+NFEAT = 14
 LOG_LO, LOG_HI = 6.0, 15.0  # shapes in [64, 32768]
 # xkblas_kernel_t of the reference rule used as the method without the
 # model (POTRF has no reference rule; it gets the TRSM one)
 KERN = {"gemm": 1, "syrk": 6, "trsm": 4, "potrf": 4}
 
 
-def split_counts(k):
+SPLIT_MODES = (2, 3)  # XKBLAS_SPLIT_BF16_MATH, XKBLAS_SPLIT_FP8_MATH
+MAXCOUNT = {2: 15, 3: 20}
+
+
+def split_counts(mode, k):
     """Split counts that keep >= 48 bits per side for inner dimension k
-    (the accuracy contract of the split mode), plus one more."""
+    (the accuracy contract of the split modes): the smallest and one more."""
     L = xk.lib()
-    ok = [c for c in range(8, 16) if L.xkblas_split_side_bits(max(k, 2), c) >= 48]
-    return ok[:2] if ok else [15]
+    ok = [c for c in range(8, MAXCOUNT[mode] + 1) if L.xkblas_split_info(mode, max(k, 2), c, None, None) >= 48]
+    return ok[:2] if ok else [MAXCOUNT[mode]]
 
 
 def candidates(op, dtype, mode, m, n, k):
     """Configurations the model chooses between for one call."""
     if op in ("gemm", "syrk"):
-        if mode == 2 and dtype == 0:
-            return [{"tile": 0, "nb": 0, "split": c, "kernel": kv} for c in split_counts(k) for kv in (0, 1)]
+        if mode in SPLIT_MODES and dtype == 0:
+            kernels = (0, 1) if mode == 2 else (0,)
+            return [{"tile": 0, "nb": 0, "split": c, "kernel": kv} for c in split_counts(mode, k) for kv in kernels]
         return [{"tile": t, "nb": 0, "split": 0, "kernel": 0} for t in (64, 128)]
     dim = m
     nbs = [b for b in (64, 128, 256, 512, 1024, 2048) if b <= max(64, dim)]
@@ -63,8 +68,6 @@ def heuristic(op, dtype, mode, m, n, k):
     nb = int(L.xkblas_auto_tilesize(None, KERN[op], m, n if op != "potrf" else m, k))
     L.xkblas_set_param(old, 0)
     if op in ("gemm", "syrk"):
-        if mode == 2 and dtype == 0:
-            return {"tile": 0, "nb": 0, "split": 0, "kernel": 0}
         return {"tile": 0, "nb": 0, "split": 0, "kernel": 0}
     return {"tile": 0, "nb": min(nb, max(64, m)), "split": 0, "kernel": 0}
 
@@ -78,8 +81,9 @@ def features(op, dtype, mode, m, n, k, cfg):
     f[8] = 1.0 if mode == 2 else 0.0
     f[9] = {0: 0.5, 64: 0.0, 128: 1.0}[cfg["tile"]]
     f[10] = (math.log2(cfg["nb"]) - 6.0) / 5.0 if cfg["nb"] else 0.0
-    f[11] = cfg["split"] / 15.0
+    f[11] = cfg["split"] / 20.0
     f[12] = float(cfg["kernel"])
+    f[13] = 1.0 if mode == 3 else 0.0
     return f
 
 
@@ -107,19 +111,21 @@ def draw(r, max_flop=2e13, ops=("gemm", "syrk", "trsm", "potrf")):
     while True:
         op = ops[int(r.integers(len(ops)))]
         dtype = int(r.random() < 0.25)
-        mode = 2 if (dtype == 0 and op in ("gemm", "syrk") and r.random() < 0.35) else 0
+        mode = 0
+        if dtype == 0 and op in ("gemm", "syrk") and r.random() < 0.4:
+            mode = 2 if r.random() < 0.6 else 3
         m, n, k = (int(round(2 ** r.uniform(LOG_LO, LOG_HI))) for _ in range(3))
         if op == "potrf":
             n = k = m
         if op == "trsm":
             k = m
-        f = work(op, m, n, k) * (15 if mode == 2 else 1)
+        f = work(op, m, n, k) * {0: 1, 2: 15, 3: 19}[mode]
         if f <= max_flop:
             return op, dtype, mode, m, n, k
 
 
 def collect(n_samples, seed=0, time_budget=None, max_flop=2e13, progress=None):
-    """Timings of random (problem, configuration) pairs: X (n x 13), y = log ms."""
+    """Timings of random (problem, configuration) pairs: X (n x NFEAT), y = log ms."""
     r = np.random.default_rng(seed)
     X, y, meta = [], [], []
     t0 = time.time()
