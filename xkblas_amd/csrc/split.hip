@@ -562,6 +562,7 @@ __device__ __forceinline__ void dm_store_residues(const f32x16 (&acc)[4][4], flo
   }
 }
 
+template <bool FOLD_NEXT>
 __global__ __launch_bounds__(256) void mod_gemm_dma_kernel(ModGemmParams prm)
 {
   // [stage][A rows 256 x 32 | B rows 256 x 32]
@@ -631,11 +632,14 @@ __global__ __launch_bounds__(256) void mod_gemm_dma_kernel(ModGemmParams prm)
   for (int j = 0; j < 4; ++j) b0[j] = *reinterpret_cast<const bf16x8*>(&S[0][0] + rdB0 + j * 32 * DM_BK);
 
   // The k loop runs in groups of 16 stages (kpad is a multiple of 512).
-  // Stage u of a group folds accumulator block u mod p right after that
-  // block's half-1 MFMA (issued first in that half), so every block is
-  // reduced once per 512 k and the VALU work of the fold overlaps the
-  // remaining MFMAs instead of stalling the wave in one long block.
-  for (int kt0 = 0; kt0 < nk; kt0 += 16) {
+  // Folds are spread over the stages, one accumulator block per stage, so
+  // every block is reduced mod p once per 512 k.  FOLD_NEXT = false: stage u
+  // folds block u right after its half-1 MFMA (issued first in that half).
+  // FOLD_NEXT = true: stage u folds block f = u - 1 (whose last MFMA was
+  // issued first in half 1 of the previous stage) during its half 0, whose
+  // MFMAs start after block f and end with it, so the fold's VALU work has 15
+  // independent MFMAs to hide under and no result to wait for.
+  auto group = [&](int kt0) {
 #pragma unroll
     for (int u = 0; u < 16; ++u) {
       const int kt = kt0 + u;
@@ -646,15 +650,34 @@ __global__ __launch_bounds__(256) void mod_gemm_dma_kernel(ModGemmParams prm)
       for (int i = 0; i < 4; ++i) a1[i] = *reinterpret_cast<const bf16x8*>(st + rdA1 + i * 32 * DM_BK);
 #pragma unroll
       for (int j = 0; j < 4; ++j) b1[j] = *reinterpret_cast<const bf16x8*>(st + rdB1 + j * 32 * DM_BK);
+      if constexpr (FOLD_NEXT) {
+        const int f = (u + 15) & 15;
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
+        for (int r = 0; r < 16; ++r) acc[f >> 2][f & 3][r] = fold(acc[f >> 2][f & 3][r], p, ip);
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0[i], b0[j], acc[i][j], 0, 0, 0);
-      // interleave: one operand read after every two MFMAs
+        for (int t = 0; t < 16; ++t) {
+          const int blk = (f + 1 + t) & 15, i = blk >> 2, j = blk & 3;
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0[i], b0[j], acc[i][j], 0, 0, 0);
+        }
+        // interleave: one operand read and the fold's VALU work between MFMAs
 #pragma unroll
-      for (int g = 0; g < 8; ++g) {
-        __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
-        __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);
+        for (int g = 0; g < 8; ++g) {
+          __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+          __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);
+          __builtin_amdgcn_sched_group_barrier(0x002, 10, 0);
+        }
+      } else {
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+          for (int j = 0; j < 4; ++j)
+            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0[i], b0[j], acc[i][j], 0, 0, 0);
+        // interleave: one operand read after every two MFMAs
+#pragma unroll
+        for (int g = 0; g < 8; ++g) {
+          __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+          __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);
+        }
       }
       // this wave's reads of stage kt are done (and, before the barrier, of
       // stage kt-1, which the loads issued below overwrite)
@@ -678,15 +701,18 @@ __global__ __launch_bounds__(256) void mod_gemm_dma_kernel(ModGemmParams prm)
         const int blk = (u + t) & 15, i = blk >> 2, j = blk & 3;
         acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1[i], b1[j], acc[i][j], 0, 0, 0);
       }
+      if constexpr (!FOLD_NEXT) {
 #pragma unroll
-      for (int r = 0; r < 16; ++r) acc[u >> 2][u & 3][r] = fold(acc[u >> 2][u & 3][r], p, ip);
+        for (int r = 0; r < 16; ++r) acc[u >> 2][u & 3][r] = fold(acc[u >> 2][u & 3][r], p, ip);
+      }
 #pragma unroll
       for (int g = 0; g < 8; ++g) {
         __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
         __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);
       }
     }
-  }
+  };
+  for (int kt0 = 0; kt0 < nk; kt0 += 16) group(kt0);
 
   wait_vmcnt_le(0);
   __syncthreads();
@@ -704,6 +730,7 @@ __global__ __launch_bounds__(256) void mod_gemm_dma_kernel(ModGemmParams prm)
 // inner dimensions.
 typedef int i32x8 __attribute__((ext_vector_type(8)));
 constexpr int F8_BK = 64;        // bytes (= k) per row and stage
+constexpr int F8_STAGES = 4;     // LDS ring depth
 constexpr int F8_KMAX = 32768;   // kpad limit of one launch
 
 // (the reads are typed like those of the BF16 kernel: the compiler then
@@ -730,7 +757,7 @@ __device__ __forceinline__ f32x16 mfma_fp8(const i32x8& a, const i32x8& b, const
 __global__ __launch_bounds__(256) void mod_gemm_fp8_kernel(ModGemmParams prm)
 {
   // [stage][A rows 256 x 64 B | B rows 256 x 64 B]
-  __shared__ __attribute__((aligned(1024))) uint8_t S[DM_STAGES][(DM_BM + DM_BN) * F8_BK];
+  __shared__ __attribute__((aligned(1024))) uint8_t S[F8_STAGES][(DM_BM + DM_BN) * F8_BK];
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave & 1, wn = wave >> 1;
@@ -771,7 +798,7 @@ __global__ __launch_bounds__(256) void mod_gemm_fp8_kernel(ModGemmParams prm)
       for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
 
   const int nk = prm.kpad / F8_BK;  // even (kpad is a multiple of 512)
-  for (int s = 0; s < DM_STAGES - 1 && s < nk; ++s) issue(s, s * F8_BK);
+  for (int s = 0; s < F8_STAGES - 1 && s < nk; ++s) issue(s, s * F8_BK);
 
   // operand reads: row wm*128 + 32i + lane%32 (A) / wn*128 + 32j + lane%32
   // (B), logical chunks 2fh, 2fh + 1; the swizzle term depends on lane%32 only
@@ -795,7 +822,7 @@ __global__ __launch_bounds__(256) void mod_gemm_fp8_kernel(ModGemmParams prm)
   // stage kt+3; then a[0], a[1] and bn of stage kt+1 are read under the MFMAs
   // of rows i = 2, 3.
   i32x8 a[4], b0[4], b1[4];
-  // u = kt % DM_STAGES is a compile-time constant at every call, so the
+  // u = kt % F8_STAGES is a compile-time constant at every call, so the
   // compiler can tell the LDS reads of a stage from the direct-to-LDS loads
   // in flight into the others (and need not wait for those loads)
   auto stage = [&](int kt, int u, i32x8 (&b)[4], i32x8 (&bn)[4]) {
@@ -814,12 +841,12 @@ __global__ __launch_bounds__(256) void mod_gemm_fp8_kernel(ModGemmParams prm)
     }
     wait_lgkm0();
     if (kt + 1 < nk) {
-      wait_vmcnt_le(8 * (min(nk - 1, kt + DM_STAGES - 2) - (kt + 1)));
+      wait_vmcnt_le(8 * (min(nk - 1, kt + F8_STAGES - 2) - (kt + 1)));
       __builtin_amdgcn_s_barrier();
-      if (kt + DM_STAGES - 1 < nk) issue((u + DM_STAGES - 1) % DM_STAGES, (kt + DM_STAGES - 1) * F8_BK);
+      if (kt + F8_STAGES - 1 < nk) issue((u + F8_STAGES - 1) % F8_STAGES, (kt + F8_STAGES - 1) * F8_BK);
     }
     // (after the last stage these reads fetch unused data from an idle buffer)
-    const uint8_t* sn = S[(u + 1) % DM_STAGES];
+    const uint8_t* sn = S[(u + 1) % F8_STAGES];
 #pragma unroll
     for (int i = 2; i < 4; ++i)
 #pragma unroll
@@ -837,13 +864,13 @@ __global__ __launch_bounds__(256) void mod_gemm_fp8_kernel(ModGemmParams prm)
     for (int g = 0; g < 4; ++g) __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
   };
 
-  wait_vmcnt_le(8 * (min(nk, DM_STAGES - 1) - 1));
+  wait_vmcnt_le(8 * (min(nk, F8_STAGES - 1) - 1));
   __builtin_amdgcn_s_barrier();
   a[0] = readA(0, S[0]);
   a[1] = readA(1, S[0]);
 #pragma unroll
   for (int j = 0; j < 4; ++j) b0[j] = readB(j, S[0]);
-  static_assert(DM_STAGES == 4, "the k loop is unrolled by the ring depth");
+  static_assert(F8_STAGES == 4, "the k loop is unrolled by the ring depth");
   for (int kt = 0; kt < nk; kt += 4) {  // nk is a multiple of 8
     stage(kt, 0, b0, b1);
     stage(kt + 1, 1, b1, b0);
@@ -969,7 +996,8 @@ static inline long rup(long v, long q) { return (v + q - 1) / q * q; }
 
 enum { WS_SA = 8, WS_SB = 9, WS_SC = 10, WS_SE = 11 };
 
-// 0: pipelined 256 x 256 DMA kernel, 1: register-staged 256 x 128 kernel
+// 0: pipelined 256 x 256 DMA kernel (folds under the next stage's MFMAs),
+// 1: register-staged 256 x 128 kernel, 2: as 0 with folds after each stage
 static int g_mod_gemm_variant = getenv("XKBLAS_SPLIT_KERNEL") ? atoi(getenv("XKBLAS_SPLIT_KERNEL")) : 0;
 void split_set_kernel(int v) { g_mod_gemm_variant = v; }
 int split_get_kernel() { return g_mod_gemm_variant; }
@@ -1090,8 +1118,10 @@ static void split_gemm_chunk(const Exec& x, int transA, int transB, int m, int n
       hipLaunchKernelGGL((mod_gemm_kernel<128, 128>),
                          dim3((unsigned)((long)g.tiles_m * (g.tiles_m + 1) / 2), 1, np), dim3(256), 0, s, g);
     }
+  } else if (g_mod_gemm_variant == 2) {
+    hipLaunchKernelGGL(mod_gemm_dma_kernel<false>, dim3((unsigned)tiles, 1, np), dim3(256), 0, s, g);
   } else {
-    hipLaunchKernelGGL(mod_gemm_dma_kernel, dim3((unsigned)tiles, 1, np), dim3(256), 0, s, g);
+    hipLaunchKernelGGL(mod_gemm_dma_kernel<true>, dim3((unsigned)tiles, 1, np), dim3(256), 0, s, g);
   }
   XK_HIP(hipGetLastError());
   R.prof_end();
