@@ -66,10 +66,25 @@ def time_dgemm(xk, A, B, C, steps, warmup, mode, nb, alpha=1.5, beta=-0.5):
     return (t1 - t0) / steps
 
 
-def accuracy(xk, A, B, mode, nb, samples=96):
+def host_reference(A, B):
+    """C = 1.5 A B - 0.5 C0 in FP64 by the host BLAS (the reference's CPU
+    path) on the full matrices, for the full normwise comparison; None when
+    no host BLAS is found."""
+    import oracle
+    if oracle.hostblas() is None:
+        return None
+    n = A.shape[0]
+    r = np.random.default_rng(11)
+    C = np.asfortranarray(r.uniform(-0.5, 0.5, (n, n)))
+    oracle.host_dgemm(111, 111, 1.5, A, B, -0.5, C)
+    return C
+
+
+def accuracy(xk, A, B, mode, nb, samples=96, full_ref=None):
     """Relative error of one C <- 1.5 A B - 0.5 C0 against the double-double
     oracle on sampled entries: normwise over the samples, max elementwise,
-    and max componentwise relative to (|alpha||A||B| + |beta||C0|)."""
+    and max componentwise relative to (|alpha||A||B| + |beta||C0|); with
+    full_ref (the host BLAS result) also the full normwise difference."""
     import oracle
     n = A.shape[0]
     r = np.random.default_rng(11)
@@ -86,10 +101,35 @@ def accuracy(xk, A, B, mode, nb, samples=96):
     got = C[rows, cols]
     err = np.abs((got - hi) - lo)
     absprod = np.array([1.5 * np.abs(A[i, :]) @ np.abs(B[:, j]) + 0.5 * abs(C0[i, j]) for i, j in zip(rows, cols)])
-    return {"normwise_sampled": float(np.linalg.norm(err) / np.linalg.norm(hi)),
-            "max_elementwise": float(np.max(err / np.abs(hi))),
-            "max_componentwise": float(np.max(err / absprod)), "samples": int(samples),
-            "oracle": "double-double dot products"}
+    out = {"normwise_sampled": float(np.linalg.norm(err) / np.linalg.norm(hi)),
+           "max_elementwise": float(np.max(err / np.abs(hi))),
+           "median_elementwise": float(np.median(err / np.abs(hi))),
+           "max_componentwise": float(np.max(err / absprod)), "samples": int(samples),
+           "oracle": "double-double dot products"}
+    if full_ref is not None:
+        out["normwise_full_vs_host_fp64"] = float(np.linalg.norm(C - full_ref) / np.linalg.norm(full_ref))
+    return out
+
+
+def host_to_host(xk, A, B, C, mode, nb):
+    """One call from host buffers: invalidated caches, async + coherent +
+    sync (the reference tests' timed sequence), so the host-to-device copies
+    of A, B, C and the copy back of C are inside the time."""
+    L = xk.lib()
+    xk.invalidate()
+    L.xkblas_set_modemath(mode)
+    L.xkblas_set_param(nb, 0)
+    n = A.shape[0]
+    al, be = np.array([1.5]), np.array([-0.5])
+    p = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    L.xkblas_sync()
+    t0 = time.perf_counter()
+    L.xkblas_dgemm_async(111, 111, n, n, n, p(al), p(A), n, p(B), n, p(be), p(C), n)
+    L.xkblas_memory_coherent_async(0, 0, n, n, p(C), n, 8)
+    L.xkblas_sync()
+    t = time.perf_counter() - t0
+    L.xkblas_set_modemath(0)
+    return t
 
 
 def kernel_stats(xk, cls):
@@ -144,6 +184,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-potrf", action="store_true")
     ap.add_argument("--no-fp8", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-full-check", action="store_true", help="skip the full host-BLAS normwise comparison")
     ap.add_argument("--potrf-split-n", type=int, default=32768,
                     help="DPOTRF size for the split-BF16 vs FP64 comparison (0: skip)")
     ap.add_argument("--only-split", action="store_true", help="profile helper: one split-BF16 DGEMM, no JSON")
@@ -183,13 +225,18 @@ def main():
     fp64_tflops = flops / t_fp64 / 1e12
 
     cfg = L.xkblas_get_configuration().decode()
-    errs64 = accuracy(xk, A, B, 0, args.nb)
+    full_ref = None
+    if not args.no_full_check:
+        print("bench: host-BLAS reference product (full normwise check)", file=sys.stderr, flush=True)
+        full_ref = host_reference(A, B)
+    errs64 = accuracy(xk, A, B, 0, args.nb, full_ref=full_ref)
 
     # split modes (FP64-accurate): BF16 (the headline) and FP8 residue products
     def split_run(mode, cls, label, peak):
         print("bench: split-%s dgemm" % label, file=sys.stderr, flush=True)
-        errs = accuracy(xk, A, B, mode, args.nb)
+        errs = accuracy(xk, A, B, mode, args.nb, full_ref=full_ref)
         t = time_dgemm(xk, A, B, C, args.steps, args.warmup, mode, args.nb)
+        h2h = host_to_host(xk, A, B, C, mode, args.nb)
         c_mg, ms_mg, fl_mg = kernel_stats(xk, cls)
         c_pr, ms_pr, _ = kernel_stats(xk, "split_prep")
         c_cr, ms_cr, _ = kernel_stats(xk, "split_crt")
@@ -204,6 +251,7 @@ def main():
             label + "_gemm_frac_of_peak": round(fl_mg / ms_mg / 1e9 / peak, 4),
             label + "_gemm_ms": round(ms_mg / max(c_mg, 1), 3),
             "prep_ms": round(ms_pr / max(c_pr, 1), 3), "crt_ms": round(ms_cr / max(c_cr, 1), 3),
+            "host_to_host_ms": round(h2h * 1e3, 3), "host_to_host_tflops": round(flops / h2h / 1e12, 3),
             "rel_err": errs}
 
     t_split, bf16_achieved, bf16_ms, split = split_run(2, "split_mod_gemm", "bf16", BF16_PEAK)
@@ -211,6 +259,11 @@ def main():
     split_fp8 = None
     if not args.no_fp8:
         split_fp8 = split_run(3, "split_mod_gemm_fp8", "fp8", FP8_PEAK)[3]
+
+    batched = None
+    if not args.no_batched:
+        print("bench: batched uneven gemm/trsm", file=sys.stderr, flush=True)
+        batched = bench_batched(xk)
 
     potrf = None
     if not args.no_potrf:
@@ -257,6 +310,7 @@ def main():
         "split_bf16": split,
         "split_fp8": split_fp8,
         "dpotrf": potrf,
+        "batched": batched,
         "roofline": {"bound": "mfma", "achieved": round(bf16_achieved, 1), "peak": BF16_PEAK, "unit": "TFLOP/s",
                      "frac": round(bf16_achieved / BF16_PEAK, 4), "traffic": None,
                      "kernel": "mod_gemm_dma_kernel (BF16 residue GEMM, 256x256 tiles, all moduli in one "
@@ -268,6 +322,107 @@ def main():
     }
     L.xkblas_finalize()
     print(json.dumps(out))
+
+
+def bench_batched(xk, seed=2027, reps=5):
+    """Batched uneven work: 4096 GEMMs (m, n, k log-uniform in [16, 512]) and
+    1024 lower-triangular TRSMs (n log-uniform in [32, 1024], 64 right-hand
+    sides, diagonally dominant L), all FP64 or all FP32, in one persistent
+    launch replayed from a hipGraph.  The shapes are the committed list
+    tests/golden/batched_c4_shapes.json; entries are drawn from `seed`.
+    Operands live in one device pool.  The
+    baseline runs the same problems as one launch each.  A few problems are
+    checked against NumPy."""
+    L = xk.lib()
+    out = {"workload": "4096 gemm m,n,k~logU[16,512] + 1024 trsm lower n~logU[32,1024] nrhs 64"}
+    with open(os.path.join(ROOT, "tests", "golden", "batched_c4_shapes.json")) as f:
+        shapes = json.load(f)
+    g = np.array(shapes["gemm_mnk"], dtype=int)
+    t = np.array(shapes["trsm_n"], dtype=int)
+    for dt, name, code in ((np.float64, "fp64", 0), (np.float32, "fp32", 1)):
+        r = np.random.default_rng(seed)
+        es = np.dtype(dt).itemsize
+        sizes = [m * k + k * n + m * n for m, n, k in g] + [n * n + n * 64 for n in t]
+        offs = np.concatenate([[0], np.cumsum([(x + 63) // 64 * 64 for x in sizes])])
+        pool = L.xkblas_device_malloc(int(offs[-1]) * es)
+        assert pool
+        host = np.empty(int(offs[-1]), dtype=dt)
+        probs, flops, checks = [], 0.0, []
+        for i, (m, n, k) in enumerate(g):
+            o = int(offs[i])
+            A = r.uniform(-0.5, 0.5, m * k).astype(dt)
+            B = r.uniform(-0.5, 0.5, k * n).astype(dt)
+            C = r.uniform(-0.5, 0.5, m * n).astype(dt)
+            host[o:o + m * k], host[o + m * k:o + m * k + k * n] = A, B
+            host[o + m * k + k * n:o + m * k + k * n + m * n] = C
+            base = pool + o * es
+            probs.append(xk.BatchProblem(xk.BATCH_GEMM, code, 111, 111, int(m), int(n), int(k), 1.5, -0.5,
+                                         base, int(m), base + m * k * es, int(k), base + (m * k + k * n) * es, int(m)))
+            flops += 2.0 * m * n * k
+            if i % 1024 == 0:
+                ref = 1.5 * A.reshape(k, m).T.astype(np.float64) @ B.reshape(n, k).T.astype(np.float64) \
+                    - 0.5 * C.reshape(n, m).T.astype(np.float64)
+                checks.append((o + m * k + k * n, m, n, ref))
+        for j, n in enumerate(t):
+            o = int(offs[4096 + j])
+            Lm = np.tril(r.uniform(-1, 1, (n, n)))
+            np.fill_diagonal(Lm, 1.0 + np.abs(Lm).sum(axis=1))
+            Bm = r.uniform(-1, 1, (n, 64))
+            host[o:o + n * n] = Lm.T.reshape(-1).astype(dt)  # column-major
+            host[o + n * n:o + n * n + n * 64] = Bm.T.reshape(-1).astype(dt)
+            base = pool + o * es
+            probs.append(xk.BatchProblem(xk.BATCH_TRSM, code, 111, 111, int(n), 64, int(n), 1.0, 0.0,
+                                         base, int(n), None, 0, base + n * n * es, int(n)))
+            flops += float(n) * n * 64
+            if j % 256 == 0:
+                checks.append((o + n * n, n, 64, np.linalg.solve(Lm, Bm)))
+        arr = (xk.BatchProblem * len(probs))(*probs)
+        p = ctypes.c_void_p
+
+        def upload():
+            assert L.xkblas_memcpy_htod(pool, host.ctypes.data_as(p), host.nbytes) == 0
+
+        upload()
+        plan = L.xkblas_batch_create(ctypes.cast(arr, p), len(arr))
+        assert plan
+        assert L.xkblas_batch_use_graph(plan, 1) == 0
+        L.xkblas_batch_run(plan)
+        L.xkblas_sync()
+        ms = []
+        for _ in range(reps):
+            upload()
+            L.xkblas_sync()
+            L.xkblas_profile_reset()
+            L.xkblas_batch_run(plan)
+            L.xkblas_sync()
+            ms.append(xk.profile("batch")[1])
+        res = np.empty(int(offs[-1]), dtype=dt)
+        assert L.xkblas_memcpy_dtoh(res.ctypes.data_as(p), pool, res.nbytes) == 0
+        err = 0.0
+        for o, m, n, ref in checks:
+            got = res[o:o + m * n].reshape(n, m).T.astype(np.float64)
+            err = max(err, float(np.abs(got - ref).max() / max(1.0, np.abs(ref).max())))
+        L.xkblas_batch_use_graph(plan, 0)
+        L.xkblas_batch_destroy(plan)
+        # baseline: one launch per problem
+        upload()
+        base_addr, psz = ctypes.addressof(arr), ctypes.sizeof(xk.BatchProblem)
+        plans = [L.xkblas_batch_create(p(base_addr + i * psz), 1) for i in range(len(arr))]
+        L.xkblas_sync()
+        t0 = time.perf_counter()
+        for pl in plans:
+            L.xkblas_batch_run(pl)
+        L.xkblas_sync()
+        t_each = time.perf_counter() - t0
+        for pl in plans:
+            L.xkblas_batch_destroy(pl)
+        L.xkblas_device_free(pool)
+        best = min(ms)
+        out[name] = {"ms": round(best, 3), "gflops": round(flops / best / 1e6, 1), "problems": len(arr),
+                     "one_launch_per_problem_ms": round(t_each * 1e3, 3),
+                     "speedup_vs_one_launch_per_problem": round(t_each * 1e3 / best, 2),
+                     "max_rel_err_sampled": err}
+    return out
 
 
 def make_spd(xk, n, nb):

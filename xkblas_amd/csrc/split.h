@@ -22,9 +22,9 @@ int split_choose_count(int k, int requested, bool fp8);
 int split_side_bits(int k, int nsplit, bool fp8);
 int split_plane_count(int k, int requested, bool fp8);
 
-// BF16 residue GEMM kernel: 0 = pipelined 256 x 256 (direct-to-LDS loads,
-// a modular fold per accumulator every 1024 k), 1 = register-staged
-// 256 x 128, 2 = as 0 with a fold every 512 k.
+// BF16 residue GEMM kernel (256 x 256 tiles, direct-to-LDS loads): 0 = the
+// modular folds run under the next stage's MFMAs, 1 = right after each
+// stage's MFMAs.
 void split_set_kernel(int v);
 int split_get_kernel();
 

@@ -202,8 +202,9 @@ struct SplitParams {
   int rows, kdim, row_contig, nmod;
   unsigned kara;
   const int* shift;
-  void* R;            // planes [nplane][rows padded][rld]
-  long rld, rstride;  // row length (kpad), per-plane stride (elements)
+  void* R;            // planes, each [kpad / kb][rpad][kb]
+  long rpad, rstride; // padded row count, per-plane stride (elements)
+  int kb;             // k-block width (elements): 64 bytes of BF16, 64 of FP8
   double pd[MAXMOD], invpd[MAXMOD];
 };
 
@@ -243,9 +244,10 @@ __device__ __forceinline__ void store8(uint8_t* out, const float* v)
   *reinterpret_cast<uint2*>(out) = o;
 }
 
-// Residue planes of the scaled, rounded rows: R[q][i][k] (k contiguous, Elem
-// = __bf16 or E4M3 bytes); the padded region (rows beyond `rows`, k beyond
-// kdim) is written as 0.
+// Residue planes of the scaled, rounded rows (Elem = __bf16 or E4M3 bytes)
+// in k-blocks: plane q holds R[k / kb][i][k % kb], so the rows of one GEMM
+// stage (one k-block) form one contiguous run; the padded region (rows
+// beyond `rows`, k beyond kdim) is written as 0.
 template <typename Elem>
 __global__ __launch_bounds__(256) void split_kernel(SplitParams sp)
 {
@@ -275,8 +277,9 @@ __global__ __launch_bounds__(256) void split_kernel(SplitParams sp)
     x0[v] = T[r0][kc + v];
     x1[v] = T[r0 + 32][kc + v];
   }
-  Elem* out0 = static_cast<Elem*>(sp.R) + (long)(i0 + r0) * sp.rld + k0 + kc;
-  Elem* out1 = out0 + 32L * sp.rld;
+  const int gk = k0 + kc;  // 8 consecutive k inside one k-block
+  Elem* out0 = static_cast<Elem*>(sp.R) + ((long)(gk / sp.kb) * sp.rpad + i0 + r0) * sp.kb + gk % sp.kb;
+  Elem* out1 = out0 + 32L * sp.kb;
   int q = 0;
   for (int l = 0; l < sp.nmod; ++l) {
     const double p = sp.pd[l], ip = sp.invpd[l];
@@ -317,146 +320,18 @@ struct ModGemmParams {
   const void* A;  // [nplane][mpad][kpad] (__bf16 or E4M3 bytes)
   const void* B;  // [nplane][npad][kpad]
   int8_t* C;      // [nplane][npad][mpad] (column-major residues)
-  long lk;        // kpad
+  long blkA, blkB;  // elements per k-block of an A / B plane (rows padded x 64 bytes)
   long sA, sB, sC;
   long ldc;       // mpad
   int kpad, tiles_m, tiles_n, tri;
   float p[MAXPLANE], invp[MAXPLANE];
 };
 
-constexpr int MG_BK = 64, MG_LK = MG_BK + 8;
-constexpr int MG_PAD = 256;          // row padding of the residue matrices
-constexpr int MG_RED = 1024 / MG_BK;  // k-steps between modular reductions
+constexpr int MG_PAD = 256;  // row padding of the residue planes
 
 __device__ __forceinline__ float fold(float v, float p, float ip)
 {
   return fmaf(-p, rintf(v * ip), v);
-}
-
-// One BF16 GEMM per modulus (blockIdx.z): C_l = (A_l B_l^T) mod p_l.
-// Workgroup tile MG_BM x MG_BN, 4 waves in 2 x 2, wave tile (BM/2) x (BN/2)
-// as 32 x 32 MFMA blocks.
-template <int MG_BM, int MG_BN>
-__global__ __launch_bounds__(256) void mod_gemm_kernel(ModGemmParams prm)
-{
-  constexpr int WM = MG_BM / 2, WN = MG_BN / 2, TI = WM / 32, TJ = WN / 32;
-  constexpr int PA = MG_BM / 32, PB = MG_BN / 32;  // 16-byte loads per lane per slice
-  __shared__ __attribute__((aligned(16))) __bf16 As[2][MG_BM * MG_LK];
-  __shared__ __attribute__((aligned(16))) __bf16 Bs[2][MG_BN * MG_LK];
-
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm = wave & 1, wn = wave >> 1;
-  const int l = blockIdx.z;
-  const float p = prm.p[l], ip = prm.invp[l];
-
-  int tm, tn;
-  {
-    int pid = blockIdx.x;
-    const int total = gridDim.x;
-    if (prm.tri == TRI_FULL) {
-      if ((total & 7) == 0) pid = (pid & 7) * (total >> 3) + (pid >> 3);
-      constexpr int GM = 4;
-      const int group = GM * prm.tiles_n;
-      const int g = pid / group;
-      const int first = g * GM;
-      const int gm = min(prm.tiles_m - first, GM);
-      const int r = pid - g * group;
-      tm = first + r % gm;
-      tn = r / gm;
-    } else {
-      int i = (int)((sqrtf(8.0f * (float)pid + 1.0f) - 1.0f) * 0.5f);
-      while ((i + 1) * (i + 2) / 2 <= pid) ++i;
-      while (i * (i + 1) / 2 > pid) --i;
-      const int j = pid - i * (i + 1) / 2;
-      if (prm.tri == TRI_LOWER) { tm = i; tn = j; }
-      else { tm = j; tn = i; }
-    }
-  }
-  const long m0 = (long)tm * MG_BM, n0 = (long)tn * MG_BN;
-  const __bf16* A = static_cast<const __bf16*>(prm.A) + (long)l * prm.sA + m0 * prm.lk;
-  const __bf16* B = static_cast<const __bf16*>(prm.B) + (long)l * prm.sB + n0 * prm.lk;
-
-  // global -> LDS: 8 lanes per 128-byte row segment, 32 rows per pass
-  const int lrow = tid >> 3, lchunk = (tid & 7) * 8;
-  uint4 ra[PA], rb[PB];
-  auto gload = [&](int k0) {
-#pragma unroll
-    for (int s = 0; s < PA; ++s)
-      ra[s] = *reinterpret_cast<const uint4*>(A + (long)(lrow + 32 * s) * prm.lk + k0 + lchunk);
-#pragma unroll
-    for (int s = 0; s < PB; ++s)
-      rb[s] = *reinterpret_cast<const uint4*>(B + (long)(lrow + 32 * s) * prm.lk + k0 + lchunk);
-  };
-  auto lstore = [&](int buf) {
-#pragma unroll
-    for (int s = 0; s < PA; ++s) *reinterpret_cast<uint4*>(&As[buf][(lrow + 32 * s) * MG_LK + lchunk]) = ra[s];
-#pragma unroll
-    for (int s = 0; s < PB; ++s) *reinterpret_cast<uint4*>(&Bs[buf][(lrow + 32 * s) * MG_LK + lchunk]) = rb[s];
-  };
-
-  f32x16 acc[TI][TJ];
-#pragma unroll
-  for (int i = 0; i < TI; ++i)
-#pragma unroll
-    for (int j = 0; j < TJ; ++j)
-#pragma unroll
-      for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
-
-  const int nk = prm.kpad / MG_BK;
-  const int fr = lane & 31, fk = (lane >> 5) * 8;
-  gload(0);
-  lstore(0);
-  __syncthreads();
-  int buf = 0;
-  for (int kt = 0; kt < nk; ++kt) {
-    const bool more = kt + 1 < nk;
-    if (more) gload((kt + 1) * MG_BK);
-#pragma unroll
-    for (int ks = 0; ks < MG_BK / 16; ++ks) {
-      bf16x8 a[TI], b[TJ];
-#pragma unroll
-      for (int i = 0; i < TI; ++i)
-        a[i] = *reinterpret_cast<const bf16x8*>(&As[buf][(wm * WM + i * 32 + fr) * MG_LK + ks * 16 + fk]);
-#pragma unroll
-      for (int j = 0; j < TJ; ++j)
-        b[j] = *reinterpret_cast<const bf16x8*>(&Bs[buf][(wn * WN + j * 32 + fr) * MG_LK + ks * 16 + fk]);
-#pragma unroll
-      for (int i = 0; i < TI; ++i)
-#pragma unroll
-        for (int j = 0; j < TJ; ++j)
-          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(b[j], a[i], acc[i][j], 0, 0, 0);
-    }
-    if ((kt + 1) % MG_RED == 0) {
-#pragma unroll
-      for (int i = 0; i < TI; ++i)
-#pragma unroll
-        for (int j = 0; j < TJ; ++j)
-#pragma unroll
-          for (int r = 0; r < 16; ++r) acc[i][j][r] = fold(acc[i][j][r], p, ip);
-    }
-    if (more) lstore(buf ^ 1);
-    __syncthreads();
-    buf ^= 1;
-  }
-
-  // residues in [-128, 127] as int8; D = (B A^T) block: lane -> m, (lane, r) -> n
-  const float hi = (p - 1.0f) * 0.5f, lo = -hi;
-  int8_t* C = prm.C + (long)l * prm.sC;
-#pragma unroll
-  for (int i = 0; i < TI; ++i) {
-    const long gm = m0 + wm * WM + i * 32 + fr;
-#pragma unroll
-    for (int j = 0; j < TJ; ++j) {
-#pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        const long gn = n0 + wn * WN + j * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-        float v = fold(acc[i][j][r], p, ip);
-        if (v > hi) v -= p;
-        if (v < lo) v += p;
-        C[gn * prm.ldc + gm] = (int8_t)(int)v;
-      }
-    }
-  }
 }
 
 // ---- pipelined variant: 256 x 256 tile, 4-stage LDS ring filled by
@@ -576,25 +451,29 @@ __global__ __launch_bounds__(256) void mod_gemm_dma_kernel(ModGemmParams prm)
   int tm, tn;
   dm_tile(prm, tm, tn);
   const long m0 = (long)tm * DM_BM, n0 = (long)tn * DM_BN;
-  const __bf16* A = static_cast<const __bf16*>(prm.A) + (long)l * prm.sA + m0 * prm.lk;
-  const __bf16* B = static_cast<const __bf16*>(prm.B) + (long)l * prm.sB + n0 * prm.lk;
+  // k-blocked planes: the 256 rows of this tile in k-block kt are one
+  // contiguous 16 KB run at kt * blk + m0 * 32
+  const __bf16* A = static_cast<const __bf16*>(prm.A) + (long)l * prm.sA + m0 * DM_BK;
+  const __bf16* B = static_cast<const __bf16*>(prm.B) + (long)l * prm.sB + n0 * DM_BK;
 
   // this lane's source for LDS block b = wave*4 + i: row 16b + lane/4 and the
   // chunk stored at physical position lane%4 (the swizzle term depends only
-  // on lane/4 because 16b is a multiple of 16)
+  // on lane/4 because 16b is a multiple of 16); one load instruction covers
+  // 1 KB of contiguous memory
   const int srow = lane >> 2;
   const int schunk = (lane & 3) ^ ((srow >> 2) & 3);
-  const __bf16* gA = A + (long)(wave * 64 + srow) * prm.lk + schunk * 8;
-  const __bf16* gB = B + (long)(wave * 64 + srow) * prm.lk + schunk * 8;
-  const long step16 = 16 * prm.lk;
-  auto issue = [&](int stage, int k0) {
+  const __bf16* gA = A + (wave * 64 + srow) * DM_BK + schunk * 8;
+  const __bf16* gB = B + (wave * 64 + srow) * DM_BK + schunk * 8;
+  auto issue = [&](int stage, int kt) {
     __bf16* base = S[stage] + wave * 2048;
+    const __bf16* a = gA + (long)kt * prm.blkA;
+    const __bf16* b = gB + (long)kt * prm.blkB;
 #pragma unroll
     for (int i = 0; i < 4; ++i)
-      __builtin_amdgcn_global_load_lds((const void*)(gA + i * step16 + k0), (lds_void*)(base + i * 512), 16, 0, 0);
+      __builtin_amdgcn_global_load_lds((const void*)(a + i * 16 * DM_BK), (lds_void*)(base + i * 512), 16, 0, 0);
 #pragma unroll
     for (int i = 0; i < 4; ++i)
-      __builtin_amdgcn_global_load_lds((const void*)(gB + i * step16 + k0),
+      __builtin_amdgcn_global_load_lds((const void*)(b + i * 16 * DM_BK),
                                        (lds_void*)(base + DM_BM * DM_BK + i * 512), 16, 0, 0);
   };
 
@@ -607,7 +486,7 @@ __global__ __launch_bounds__(256) void mod_gemm_dma_kernel(ModGemmParams prm)
       for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
 
   const int nk = prm.kpad / DM_BK;  // a multiple of 16
-  for (int s = 0; s < DM_STAGES - 1 && s < nk; ++s) issue(s, s * DM_BK);
+  for (int s = 0; s < DM_STAGES - 1 && s < nk; ++s) issue(s, s);
 
   // operand reads: row (wm*128 + 32i + lane%32), logical chunk 2ks + lane/32;
   // the swizzle term depends on lane%32 only, so rows of different i are at
@@ -687,7 +566,7 @@ __global__ __launch_bounds__(256) void mod_gemm_dma_kernel(ModGemmParams prm)
         // may be pending (8 per stage, completed in issue order)
         wait_vmcnt_le(8 * (min(nk - 1, kt + DM_STAGES - 2) - (kt + 1)));
         __builtin_amdgcn_s_barrier();
-        if (kt + DM_STAGES - 1 < nk) issue((kt + DM_STAGES - 1) % DM_STAGES, (kt + DM_STAGES - 1) * DM_BK);
+        if (kt + DM_STAGES - 1 < nk) issue((kt + DM_STAGES - 1) % DM_STAGES, kt + DM_STAGES - 1);
       }
       // (on the last stage these reads fetch unused data from an idle buffer;
       // keeping them unconditional lets them interleave with the MFMAs)
@@ -767,25 +646,26 @@ __global__ __launch_bounds__(256) void mod_gemm_fp8_kernel(ModGemmParams prm)
   int tm, tn;
   dm_tile(prm, tm, tn);
   const long m0 = (long)tm * DM_BM, n0 = (long)tn * DM_BN;
-  const uint8_t* A = static_cast<const uint8_t*>(prm.A) + (long)l * prm.sA + m0 * prm.lk;
-  const uint8_t* B = static_cast<const uint8_t*>(prm.B) + (long)l * prm.sB + n0 * prm.lk;
+  const uint8_t* A = static_cast<const uint8_t*>(prm.A) + (long)l * prm.sA + m0 * F8_BK;
+  const uint8_t* B = static_cast<const uint8_t*>(prm.B) + (long)l * prm.sB + n0 * F8_BK;
 
   // direct-to-LDS loads: LDS block b = wave*4 + i holds rows 16b..16b+15 of a
   // stage (1 KB); lane writes physical chunk lane%4 of row 16b + lane/4 and
   // fetches the logical chunk stored there (chunk c of row r sits at c ^ ((r >> 2) & 3))
   const int srow = lane >> 2;
   const int schunk = (lane & 3) ^ ((srow >> 2) & 3);
-  const uint8_t* gA = A + (long)(wave * 64 + srow) * prm.lk + schunk * 16;
-  const uint8_t* gB = B + (long)(wave * 64 + srow) * prm.lk + schunk * 16;
-  const long step16 = 16 * prm.lk;
-  auto issue = [&](int stage, int k0) {
+  const uint8_t* gA = A + (wave * 64 + srow) * F8_BK + schunk * 16;
+  const uint8_t* gB = B + (wave * 64 + srow) * F8_BK + schunk * 16;
+  auto issue = [&](int stage, int kt) {
     uint8_t* base = S[stage] + wave * 4096;
+    const uint8_t* a = gA + (long)kt * prm.blkA;
+    const uint8_t* b = gB + (long)kt * prm.blkB;
 #pragma unroll
     for (int i = 0; i < 4; ++i)
-      __builtin_amdgcn_global_load_lds((const void*)(gA + i * step16 + k0), (lds_void*)(base + i * 1024), 16, 0, 0);
+      __builtin_amdgcn_global_load_lds((const void*)(a + i * 16 * F8_BK), (lds_void*)(base + i * 1024), 16, 0, 0);
 #pragma unroll
     for (int i = 0; i < 4; ++i)
-      __builtin_amdgcn_global_load_lds((const void*)(gB + i * step16 + k0),
+      __builtin_amdgcn_global_load_lds((const void*)(b + i * 16 * F8_BK),
                                        (lds_void*)(base + DM_BM * F8_BK + i * 1024), 16, 0, 0);
   };
 
@@ -798,7 +678,7 @@ __global__ __launch_bounds__(256) void mod_gemm_fp8_kernel(ModGemmParams prm)
       for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
 
   const int nk = prm.kpad / F8_BK;  // even (kpad is a multiple of 512)
-  for (int s = 0; s < F8_STAGES - 1 && s < nk; ++s) issue(s, s * F8_BK);
+  for (int s = 0; s < F8_STAGES - 1 && s < nk; ++s) issue(s, s);
 
   // operand reads: row wm*128 + 32i + lane%32 (A) / wn*128 + 32j + lane%32
   // (B), logical chunks 2fh, 2fh + 1; the swizzle term depends on lane%32 only
@@ -843,7 +723,7 @@ __global__ __launch_bounds__(256) void mod_gemm_fp8_kernel(ModGemmParams prm)
     if (kt + 1 < nk) {
       wait_vmcnt_le(8 * (min(nk - 1, kt + F8_STAGES - 2) - (kt + 1)));
       __builtin_amdgcn_s_barrier();
-      if (kt + F8_STAGES - 1 < nk) issue((u + F8_STAGES - 1) % F8_STAGES, (kt + F8_STAGES - 1) * F8_BK);
+      if (kt + F8_STAGES - 1 < nk) issue((u + F8_STAGES - 1) % F8_STAGES, kt + F8_STAGES - 1);
     }
     // (after the last stage these reads fetch unused data from an idle buffer)
     const uint8_t* sn = S[(u + 1) % F8_STAGES];
@@ -996,8 +876,8 @@ static inline long rup(long v, long q) { return (v + q - 1) / q * q; }
 
 enum { WS_SA = 8, WS_SB = 9, WS_SC = 10, WS_SE = 11 };
 
-// 0: pipelined 256 x 256 DMA kernel (folds under the next stage's MFMAs),
-// 1: register-staged 256 x 128 kernel, 2: as 0 with folds after each stage
+// BF16 residue GEMM: 0 folds under the next stage's MFMAs, 1 folds right
+// after each stage's MFMAs
 static int g_mod_gemm_variant = getenv("XKBLAS_SPLIT_KERNEL") ? atoi(getenv("XKBLAS_SPLIT_KERNEL")) : 0;
 void split_set_kernel(int v) { g_mod_gemm_variant = v; }
 int split_get_kernel() { return g_mod_gemm_variant; }
@@ -1042,7 +922,7 @@ static void split_gemm_chunk(const Exec& x, int transA, int transB, int m, int n
   const int tb = side_bits(M, k);
   XK_CHECK(tb >= 8, "split count too small for this k");
   const long mpad = rup(m, MG_PAD), npad = rup(n, MG_PAD), kpad = rup(k, 512);
-  static_assert(MG_PAD % DM_BM == 0 && MG_BK % DM_BK == 0, "padding must cover the tiles");
+  static_assert(MG_PAD % DM_BM == 0 && MG_PAD % DM_BN == 0, "padding must cover the tiles");
   XK_CHECK(!fp8 || kpad <= F8_KMAX, "FP8 residue product beyond its exact inner dimension");
   const bool same = (A == B) && (lda == ldb) && (transA != transB) && m == n;  // SYRK: op(B) = op(A)^T
   const size_t esz = fp8 ? 1 : sizeof(__bf16);
@@ -1078,7 +958,8 @@ static void split_gemm_chunk(const Exec& x, int transA, int transB, int m, int n
   sp.kara = M.kara;
   for (int l = 0; l < nmod; ++l) { sp.pd[l] = M.pd[l]; sp.invpd[l] = M.invpd[l]; }
   sp.X = A; sp.ld = lda; sp.rows = m; sp.kdim = k; sp.row_contig = rcA; sp.shift = shA;
-  sp.R = RA; sp.rld = kpad; sp.rstride = mpad * kpad;
+  sp.kb = fp8 ? F8_BK : DM_BK;  // 64 bytes per row and k-block
+  sp.R = RA; sp.rpad = mpad; sp.rstride = mpad * kpad;
   auto split = [&](long rows_pad) {
     const dim3 grid((unsigned)(kpad / 64), (unsigned)(rows_pad / 64));
     if (fp8) hipLaunchKernelGGL(split_kernel<uint8_t>, grid, dim3(256), 0, s, sp);
@@ -1087,7 +968,7 @@ static void split_gemm_chunk(const Exec& x, int transA, int transB, int m, int n
   split(mpad);
   if (!same) {
     sp.X = B; sp.ld = ldb; sp.rows = n; sp.row_contig = rcB; sp.shift = shB;
-    sp.R = RB; sp.rstride = npad * kpad;
+    sp.R = RB; sp.rpad = npad; sp.rstride = npad * kpad;
     split(npad);
   }
   XK_HIP(hipGetLastError());
@@ -1097,7 +978,8 @@ static void split_gemm_chunk(const Exec& x, int transA, int transB, int m, int n
   // 4: one MFMA GEMM per plane
   ModGemmParams g;
   g.A = RA; g.B = RB; g.C = RC;
-  g.lk = kpad; g.sA = mpad * kpad; g.sB = same ? mpad * kpad : npad * kpad; g.sC = mpad * npad;
+  g.blkA = mpad * sp.kb; g.blkB = (same ? mpad : npad) * sp.kb;
+  g.sA = mpad * kpad; g.sB = same ? mpad * kpad : npad * kpad; g.sC = mpad * npad;
   g.ldc = mpad; g.kpad = (int)kpad;
   for (int q = 0; q < np; ++q) { g.p[q] = M.plane_p[q]; g.invp[q] = M.plane_ip[q]; }
   g.tri = tri;
@@ -1106,23 +988,10 @@ static void split_gemm_chunk(const Exec& x, int transA, int transB, int m, int n
   g.tiles_m = (int)(mpad / DM_BM);
   g.tiles_n = (int)(npad / DM_BN);
   const long tiles = tri == TRI_FULL ? (long)g.tiles_m * g.tiles_n : (long)g.tiles_m * (g.tiles_m + 1) / 2;
-  if (fp8) {
-    hipLaunchKernelGGL(mod_gemm_fp8_kernel, dim3((unsigned)tiles, 1, np), dim3(256), 0, s, g);
-  } else if (g_mod_gemm_variant == 1) {
-    g.tiles_n = (int)(npad / 128);
-    if (tri == TRI_FULL)
-      hipLaunchKernelGGL((mod_gemm_kernel<256, 128>), dim3((unsigned)((long)g.tiles_m * g.tiles_n), 1, np),
-                         dim3(256), 0, s, g);
-    else {
-      g.tiles_m = g.tiles_n = (int)(mpad / 128);
-      hipLaunchKernelGGL((mod_gemm_kernel<128, 128>),
-                         dim3((unsigned)((long)g.tiles_m * (g.tiles_m + 1) / 2), 1, np), dim3(256), 0, s, g);
-    }
-  } else if (g_mod_gemm_variant == 2) {
+  if (fp8) hipLaunchKernelGGL(mod_gemm_fp8_kernel, dim3((unsigned)tiles, 1, np), dim3(256), 0, s, g);
+  else if (g_mod_gemm_variant == 1)
     hipLaunchKernelGGL(mod_gemm_dma_kernel<false>, dim3((unsigned)tiles, 1, np), dim3(256), 0, s, g);
-  } else {
-    hipLaunchKernelGGL(mod_gemm_dma_kernel<true>, dim3((unsigned)tiles, 1, np), dim3(256), 0, s, g);
-  }
+  else hipLaunchKernelGGL(mod_gemm_dma_kernel<true>, dim3((unsigned)tiles, 1, np), dim3(256), 0, s, g);
   XK_HIP(hipGetLastError());
   R.prof_end();
 

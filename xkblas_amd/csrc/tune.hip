@@ -164,7 +164,7 @@ void xkblas_set_gemm_tile(int tile)
 void xkblas_set_split_kernel(int variant)
 {
   std::lock_guard<std::mutex> lock(api_mutex());
-  split_set_kernel(variant >= 0 && variant <= 2 ? variant : 0);
+  split_set_kernel(variant == 1 ? 1 : 0);
 }
 
 int xkblas_split_side_bits(int k, int nsplit) { return split_side_bits(k, nsplit, false); }

@@ -112,15 +112,14 @@ def draw(r, max_flop=2e13, ops=("gemm", "syrk", "trsm", "potrf")):
         op = ops[int(r.integers(len(ops)))]
         dtype = int(r.random() < 0.25)
         mode = 0
-        if dtype == 0 and op in ("gemm", "syrk") and r.random() < 0.4:
+        if dtype == 0 and r.random() < 0.4:
             mode = 2 if r.random() < 0.6 else 3
         m, n, k = (int(round(2 ** r.uniform(LOG_LO, LOG_HI))) for _ in range(3))
         if op == "potrf":
             n = k = m
         if op == "trsm":
             k = m
-        f = work(op, m, n, k) * {0: 1, 2: 15, 3: 19}[mode]
-        if f <= max_flop:
+        if work(op, m, n, k) <= max_flop:
             return op, dtype, mode, m, n, k
 
 
@@ -210,23 +209,43 @@ def choose(model, op, dtype, mode, m, n, k):
 
 
 def evaluate(model, problems, iters=2):
-    """Judge: time of the model's pick, the exhaustive best and the heuristic
-    for each problem.  Returns per-problem rows and the geometric means of
-    heuristic/model (gain) and model/exhaustive (regret)."""
+    """Judge: time of the pick (the model inside its trained range, else the
+    exhaustive fallback), the exhaustive best and the heuristic for each
+    problem.  Returns per-problem rows and the geometric means of
+    heuristic/pick (gain) and pick/exhaustive (regret)."""
     rows = []
     for (op, dtype, mode, m, n, k) in problems:
         cands = candidates(op, dtype, mode, m, n, k)
         times = {i: measure(op, dtype, mode, m, n, k, c, iters) for i, c in enumerate(cands)}
         best = min(times.values())
-        pick = model.pick(op, dtype, mode, m, n, k)
+        pick, how = choose(model, op, dtype, mode, m, n, k)
         t_model = times[cands.index(pick)]
         h = heuristic(op, dtype, mode, m, n, k)
         t_heur = times[cands.index(h)] if h in cands else measure(op, dtype, mode, m, n, k, h, iters)
         rows.append({"op": op, "dtype": dtype, "mode": mode, "m": m, "n": n, "k": k, "model_ms": t_model,
-                     "best_ms": best, "heuristic_ms": t_heur, "pick": pick, "heuristic": h})
+                     "best_ms": best, "heuristic_ms": t_heur, "pick": pick, "picked_by": how, "heuristic": h})
     gain = float(np.exp(np.mean([math.log(r["heuristic_ms"] / r["model_ms"]) for r in rows])))
     regret = float(np.exp(np.mean([math.log(r["model_ms"] / r["best_ms"]) for r in rows])))
     return rows, gain, regret
+
+
+def workload_problems(seed=4):
+    """The problems the picks are judged on: the split-precision workload
+    (DGEMM 16384^3 in split-BF16 and split-FP8 modes, DPOTRF 32768 in
+    split-BF16 mode and 16384 in FP64) and a sample of the batched workload's
+    shapes (GEMM m, n, k log-uniform in [16, 512]; TRSM n log-uniform in
+    [32, 1024] with 64 right-hand sides), FP64 and FP32."""
+    probs = [("gemm", 0, 2, 16384, 16384, 16384), ("gemm", 0, 3, 16384, 16384, 16384),
+             ("potrf", 0, 2, 32768, 32768, 32768), ("potrf", 0, 0, 16384, 16384, 16384)]
+    r = np.random.default_rng(seed)
+    for i in range(12):
+        dtype = i % 2
+        m, n, k = (int(round(2 ** r.uniform(4, 9))) for _ in range(3))
+        probs.append(("gemm", dtype, 0, m, n, k))
+    for i in range(6):
+        n = int(round(2 ** r.uniform(5, 10)))
+        probs.append(("trsm", i % 2, 0, n, 64, n))
+    return probs
 
 
 def main(argv=None):
@@ -241,7 +260,7 @@ def main(argv=None):
     ap.add_argument("--samples", type=int, default=20000)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--budget", type=float, default=None, help="seconds of collection")
-    ap.add_argument("--data", default=None, help="append to / train from this .npz")
+    ap.add_argument("--data", default=None, help="start from the timings in this .npz")
     ap.add_argument("--out", default="gpurun_out/tuner")
     ap.add_argument("--collect-only", action="store_true")
     ap.add_argument("--eval", type=int, default=24, help="held-out problems to judge")
@@ -253,15 +272,19 @@ def main(argv=None):
     if args.data and os.path.exists(args.data):
         d = np.load(args.data)
         X, y, meta = d["X"], d["y"], d["meta"]
-    need = args.samples - (0 if y is None else len(y))
-    if need > 0:
-        Xn, yn, mn = collect(need, seed=args.seed + (0 if y is None else len(y)), time_budget=args.budget,
-                             progress=500)
+    t_start = time.time()
+    save_to = os.path.join(args.out, "timings.npz")
+    while y is None or len(y) < args.samples:
+        left = None if args.budget is None else args.budget - (time.time() - t_start)
+        if left is not None and left <= 0:
+            break
+        have = 0 if y is None else len(y)
+        Xn, yn, mn = collect(min(1000, args.samples - have), seed=args.seed + have, time_budget=left)
         X = Xn if X is None else np.concatenate([X, Xn])
         y = yn if y is None else np.concatenate([y, yn])
         meta = mn if meta is None else np.concatenate([meta, mn])
-        np.savez_compressed(args.data or os.path.join(args.out, "timings.npz"), X=X, y=y, meta=meta)
-    print("tuner: %d timings" % len(y), flush=True)
+        np.savez_compressed(save_to, X=X, y=y, meta=meta)  # checkpoint after every chunk
+        print("tuner: %d timings (%.0f s)" % (len(y), time.time() - t_start), flush=True)
     if args.collect_only or len(y) < args.samples:
         return
     model = CostModel()
@@ -271,12 +294,16 @@ def main(argv=None):
     r = np.random.default_rng(12345)
     probs = [draw(r, max_flop=5e12) for _ in range(args.eval)]
     rows, gain, regret = evaluate(model, probs)
-    res = {"timings": int(len(y)), "train_loss_std_units": loss, "fit_seconds": t_fit, "gain_vs_heuristic": gain,
-           "regret_vs_exhaustive": regret, "rows": rows, "mu": model.mu, "sd": model.sd}
+    wrows, wgain, wregret = evaluate(model, workload_problems())
+    res = {"timings": int(len(y)), "train_loss_std_units": loss, "fit_seconds": t_fit,
+           "held_out": {"gain_vs_heuristic": gain, "regret_vs_exhaustive": regret, "rows": rows},
+           "workloads": {"gain_vs_heuristic": wgain, "regret_vs_exhaustive": wregret, "rows": wrows},
+           "mu": model.mu, "sd": model.sd}
     np.save(os.path.join(args.out, "cost_model_params.npy"), model.params())
     with open(os.path.join(args.out, "tuner_result.json"), "w") as f:
         json.dump(res, f, indent=1, default=str)
-    print("tuner: loss %.4f  gain vs heuristic %.3fx  regret vs exhaustive %.3fx" % (loss, gain, regret), flush=True)
+    print("tuner: loss %.4f  held-out gain %.3fx regret %.3fx  workloads gain %.3fx regret %.3fx"
+          % (loss, gain, regret, wgain, wregret), flush=True)
     model.close()
     L.xkblas_tune_release()
 
