@@ -355,24 +355,26 @@ def bench_batched(xk, seed=2027, reps=5):
             C = r.uniform(-0.5, 0.5, m * n).astype(dt)
             host[o:o + m * k], host[o + m * k:o + m * k + k * n] = A, B
             host[o + m * k + k * n:o + m * k + k * n + m * n] = C
-            base = pool + o * es
-            probs.append(xk.BatchProblem(xk.BATCH_GEMM, code, 111, 111, int(m), int(n), int(k), 1.5, -0.5,
-                                         base, int(m), base + m * k * es, int(k), base + (m * k + k * n) * es, int(m)))
+            m, n, k = int(m), int(n), int(k)
+            base = int(pool) + o * es
+            probs.append(xk.BatchProblem(xk.BATCH_GEMM, code, 111, 111, m, n, k, 1.5, -0.5,
+                                         base, m, base + m * k * es, k, base + (m * k + k * n) * es, m))
             flops += 2.0 * m * n * k
             if i % 1024 == 0:
                 ref = 1.5 * A.reshape(k, m).T.astype(np.float64) @ B.reshape(n, k).T.astype(np.float64) \
                     - 0.5 * C.reshape(n, m).T.astype(np.float64)
                 checks.append((o + m * k + k * n, m, n, ref))
         for j, n in enumerate(t):
+            n = int(n)
             o = int(offs[4096 + j])
             Lm = np.tril(r.uniform(-1, 1, (n, n)))
             np.fill_diagonal(Lm, 1.0 + np.abs(Lm).sum(axis=1))
             Bm = r.uniform(-1, 1, (n, 64))
             host[o:o + n * n] = Lm.T.reshape(-1).astype(dt)  # column-major
             host[o + n * n:o + n * n + n * 64] = Bm.T.reshape(-1).astype(dt)
-            base = pool + o * es
-            probs.append(xk.BatchProblem(xk.BATCH_TRSM, code, 111, 111, int(n), 64, int(n), 1.0, 0.0,
-                                         base, int(n), None, 0, base + n * n * es, int(n)))
+            base = int(pool) + o * es
+            probs.append(xk.BatchProblem(xk.BATCH_TRSM, code, 111, 111, n, 64, n, 1.0, 0.0,
+                                         base, n, None, 0, base + n * n * es, n))
             flops += float(n) * n * 64
             if j % 256 == 0:
                 checks.append((o + n * n, n, 64, np.linalg.solve(Lm, Bm)))
