@@ -29,6 +29,10 @@ sys.path.insert(0, ROOT)
 FP64_PEAK = 78.6  # TFLOP/s, MI355X dense FP64 MFMA (AMD datasheet)
 BF16_PEAK = 2500.0
 FP8_PEAK = 5000.0
+# HBM bytes per launch of the BF16 residue GEMM at 16384^3 (15 planes):
+# rocprofv3 FETCH_SIZE x 2 + WRITE_SIZE x 1 (factors from a 1 GiB device copy),
+# profiles/split_dgemm16384_hbm_traffic.md
+BF16_GEMM_TRAFFIC_16384 = 261.4e9 + 4.03e9
 
 
 def make_inputs(n, seed):
@@ -188,7 +192,8 @@ def main():
     ap.add_argument("--no-full-check", action="store_true", help="skip the full host-BLAS normwise comparison")
     ap.add_argument("--potrf-split-n", type=int, default=32768,
                     help="DPOTRF size for the split-BF16 vs FP64 comparison (0: skip)")
-    ap.add_argument("--only-split", action="store_true", help="profile helper: one split-BF16 DGEMM, no JSON")
+    ap.add_argument("--only-split", type=int, default=0, metavar="MODE",
+                    help="profile helper: two split DGEMMs in math mode MODE (2 BF16, 3 FP8), no JSON")
     ap.add_argument("--copy-probe", action="store_true",
                     help="profile helper: one 1 GiB device-to-device copy (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -213,7 +218,7 @@ def main():
     flops = 2.0 * n**3
 
     if args.only_split:
-        time_dgemm(xk, A, B, C, 1, 0, 2, args.nb)
+        time_dgemm(xk, A, B, C, 1, 1, args.only_split, args.nb)
         L.xkblas_finalize()
         return
 
@@ -312,7 +317,10 @@ def main():
         "dpotrf": potrf,
         "batched": batched,
         "roofline": {"bound": "mfma", "achieved": round(bf16_achieved, 1), "peak": BF16_PEAK, "unit": "TFLOP/s",
-                     "frac": round(bf16_achieved / BF16_PEAK, 4), "traffic": None,
+                     "frac": round(bf16_achieved / BF16_PEAK, 4),
+                     "traffic": BF16_GEMM_TRAFFIC_16384 if n == 16384 else None,
+                     # A and B planes read once (2 bytes each), int8 residues written
+                     "traffic_algorithmic": 15 * (2 * 2 * n * n + n * n),
                      "kernel": "mod_gemm_dma_kernel (BF16 residue GEMM, 256x256 tiles, all moduli in one "
                                "launch)",
                      "kernel_ms": round(bf16_ms, 3),
