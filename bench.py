@@ -278,10 +278,23 @@ def main():
         potrf = {"fp64_mfma": bench_potrf(xk, S, 1024, 0, syrk_rate)[0]}
         del S
         if args.potrf_split_n:
-            print("bench: dpotrf n=%d fp64 and split-bf16" % args.potrf_split_n, file=sys.stderr, flush=True)
-            S, syrk_rate = make_spd(xk, args.potrf_split_n, 1024)
-            p64, F64 = bench_potrf(xk, S, 1024, 0, syrk_rate)
-            psp, Fsp = bench_potrf(xk, S, 1024, 2)
+            # block order NB of both modes picked by the trained cost model
+            nbig = args.potrf_split_n
+            nb64 = nbsp = 1024
+            picked = "default"
+            from xkblas_amd import tuner
+            model = tuner.load_trained()
+            if model is not None:
+                nb64 = tuner.choose(model, "potrf", 0, 0, nbig, nbig, nbig)[0]["nb"]
+                nbsp = tuner.choose(model, "potrf", 0, 2, nbig, nbig, nbig)[0]["nb"]
+                picked = "cost model (xkblas_amd/tuning)"
+                model.close()
+            print("bench: dpotrf n=%d fp64 (nb %d) and split-bf16 (nb %d)" % (nbig, nb64, nbsp), file=sys.stderr,
+                  flush=True)
+            S, syrk_rate = make_spd(xk, nbig, 1024)
+            p64, F64 = bench_potrf(xk, S, nb64, 0, syrk_rate)
+            psp, Fsp = bench_potrf(xk, S, nbsp, 2)
+            p64["tile_picked_by"] = psp["tile_picked_by"] = picked
             psp["max_rel_diff_vs_fp64_factor"] = tril_rel_diff(Fsp, F64)
             potrf["large_fp64_mfma"] = p64
             potrf["large_split_bf16"] = psp

@@ -35,3 +35,19 @@ def test_cost_model_learns_and_picks(xk):
     rows, gain, regret = tuner.evaluate(model, probs)
     assert regret < 1.5, rows
     model.close()
+
+
+def test_shipped_model_gains_over_the_rule(xk):
+    """The model trained on 20000 timings (xkblas_amd/tuning) picks faster
+    configurations than the reference's tile-size rule with default kernels,
+    and close to exhaustive search, on problems it was not trained on."""
+    from xkblas_amd import tuner
+    model = tuner.load_trained()
+    assert model is not None
+    probs = [("potrf", 0, 0, 8192, 8192, 8192), ("trsm", 0, 0, 3031, 451, 3031), ("gemm", 0, 0, 559, 7713, 4335),
+             ("trsm", 1, 0, 1007, 4096, 1007), ("potrf", 0, 2, 4096, 4096, 4096), ("syrk", 0, 0, 2730, 9000, 2048)]
+    rows, gain, regret = tuner.evaluate(model, probs)
+    model.close()
+    assert all(r["picked_by"] == "model" for r in rows)
+    assert gain >= 1.05, rows
+    assert regret <= 1.10, rows

@@ -21,6 +21,7 @@ This module
 """
 import ctypes
 import math
+import os
 import time
 
 import numpy as np
@@ -195,6 +196,25 @@ class CostModel:
             self.h = None
 
 
+TRAINED = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning")
+
+
+def load_trained(path=TRAINED):
+    """The model trained by `main` and shipped in xkblas_amd/tuning
+    (cost_model_params.npy + cost_model.json), or None if absent."""
+    import json
+    pf, jf = os.path.join(path, "cost_model_params.npy"), os.path.join(path, "cost_model.json")
+    if not (os.path.exists(pf) and os.path.exists(jf)):
+        return None
+    with open(jf) as f:
+        meta = json.load(f)
+    if meta.get("features") != NFEAT:
+        return None
+    model = CostModel()
+    model.load(np.load(pf), meta["mu"], meta["sd"])
+    return model
+
+
 def in_range(m, n, k):
     return all(64 <= v <= 32768 for v in (m, n, k))
 
@@ -255,7 +275,6 @@ def main(argv=None):
     """
     import argparse
     import json
-    import os
     ap = argparse.ArgumentParser()
     ap.add_argument("--samples", type=int, default=20000)
     ap.add_argument("--seed", type=int, default=0)
