@@ -29,10 +29,10 @@ sys.path.insert(0, ROOT)
 FP64_PEAK = 78.6  # TFLOP/s, MI355X dense FP64 MFMA (AMD datasheet)
 BF16_PEAK = 2500.0
 FP8_PEAK = 5000.0
-# HBM bytes per launch of the BF16 residue GEMM at 16384^3 (15 planes):
+# HBM bytes per launch of the BF16 residue GEMM at 16384^3 (13 planes):
 # rocprofv3 FETCH_SIZE x 2 + WRITE_SIZE x 1 (factors from a 1 GiB device copy),
-# profiles/split_dgemm16384_hbm_traffic.md
-BF16_GEMM_TRAFFIC_16384 = 261.4e9 + 4.03e9
+# profiles/split_dgemm16384_hbm_traffic.md; None until measured for this kernel
+BF16_GEMM_TRAFFIC_16384 = None
 
 
 def make_inputs(n, seed):
@@ -332,8 +332,8 @@ def main():
         "roofline": {"bound": "mfma", "achieved": round(bf16_achieved, 1), "peak": BF16_PEAK, "unit": "TFLOP/s",
                      "frac": round(bf16_achieved / BF16_PEAK, 4),
                      "traffic": BF16_GEMM_TRAFFIC_16384 if n == 16384 else None,
-                     # A and B planes read once (2 bytes each), int8 residues written
-                     "traffic_algorithmic": 15 * (2 * 2 * n * n + n * n),
+                     # A and B planes read once (2 bytes each), int16 residues written
+                     "traffic_algorithmic": split["residue_gemms"] * (2 * 2 * n * n + 2 * n * n),
                      "kernel": "mod_gemm_dma_kernel (BF16 residue GEMM, 256x256 tiles, all moduli in one "
                                "launch)",
                      "kernel_ms": round(bf16_ms, 3),

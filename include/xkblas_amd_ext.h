@@ -82,8 +82,8 @@ double xkblas_tune_time(int op, int dtype, int mode, int transA, int transB, int
                         int split_count, int split_kernel, int iters);
 void xkblas_tune_release(void);
 void xkblas_set_gemm_tile(int tile);        /* 0 = automatic, 64, 128 */
-void xkblas_set_split_kernel(int variant);  /* BF16 residue GEMM: 0 = modular folds under the next
-                                             stage's MFMAs, 1 = folds after each stage */
+void xkblas_set_split_kernel(int variant);  /* BF16 residue GEMM: 0 = 8 waves of 128x64,
+                                             1 = 4 waves of 128x128 */
 int  xkblas_split_side_bits(int k, int nsplit);  /* split-BF16 bits per side t */
 /* split mode (XKBLAS_SPLIT_BF16_MATH / XKBLAS_SPLIT_FP8_MATH), inner dimension
  * k and requested count (0 = automatic): returns the bits per side t, stores

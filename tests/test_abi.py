@@ -83,14 +83,17 @@ def test_split_plans():
     import math
     import xkblas_amd
     L = xkblas_amd.lib()
-    bf16 = [255, 253, 251, 247, 241, 239, 233, 229, 227, 223, 217, 211, 199, 197, 193]
+    bf16 = [481, 479, 477, 475, 473, 469, 467, 463, 461, 457, 449, 443, 439]
     fp8 = [32, 31, 29, 27, 25, 23, 19, 17, 13, 11, 7, 251, 241, 239, 233, 229, 227, 223, 211, 199]
     for mods in (bf16, fp8):  # pairwise coprime, product below 2^123
         assert all(math.gcd(a, b) == 1 for i, a in enumerate(mods) for b in mods[i + 1:])
         assert math.log2(math.prod(mods)) < 123
     assert all(m <= 33 for m in fp8[:11]) and all(m <= 255 for m in fp8)
     nm, npl = ctypes.c_int(), ctypes.c_int()
-    for mode, mods, direct in ((2, bf16, 15), (3, fp8, 11)):
+    # residues within [-240, 240]: exact BF16 values, and 272 products (the
+    # longest run between two folds) plus a remainder stay below 2^24
+    assert max(bf16) <= 481 and 272 * 240**2 + 240 < 2**24
+    for mode, mods, direct in ((2, bf16, 13), (3, fp8, 11)):
         for k in (64, 1000, 16384, 32768):
             t = L.xkblas_split_info(mode, k, 0, ctypes.byref(nm), ctypes.byref(npl))
             n = nm.value
@@ -101,8 +104,8 @@ def test_split_plans():
             assert n == 8 or want(n - 1) < 48  # smallest count reaching 48 bits
     assert L.xkblas_split_info(3, 16384, 0, ctypes.byref(nm), ctypes.byref(npl)) == 51
     assert (nm.value, npl.value) == (20, 38)
-    assert L.xkblas_split_info(2, 16384, 0, ctypes.byref(nm), ctypes.byref(npl)) == 51
-    assert (nm.value, npl.value) == (15, 15)
+    assert L.xkblas_split_info(2, 16384, 0, ctypes.byref(nm), ctypes.byref(npl)) == 50
+    assert (nm.value, npl.value) == (13, 13)
     assert L.xkblas_split_info(3, 512, 12, ctypes.byref(nm), ctypes.byref(npl)) >= 8
     assert (nm.value, npl.value) == (12, 14)
     assert L.xkblas_split_info(0, 512, 0, None, None) == -1

@@ -83,7 +83,7 @@ def test_split_count_controls_accuracy(xk, orc, split_mode):
     Z = np.zeros((M, N), order="F")
     hi, lo = orc.dgemm_dd(111, 111, 1.0, A, B, 0.0, Z)
     errs = []
-    for ns in ((9, 12, 15) if split_mode.mode == SPLIT_BF16 else (12, 15, 20)):
+    for ns in ((9, 11, 13) if split_mode.mode == SPLIT_BF16 else (12, 15, 20)):
         split_mode.xkblas_set_split_count(ns)
         C = np.zeros((M, N), order="F")
         xk.gemm(111, 111, 1.0, A, B, 0.0, C)

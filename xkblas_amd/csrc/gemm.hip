@@ -290,8 +290,11 @@ void gemm(hipStream_t s, int transA, int transB, int m, int n, int k, T alpha, c
   const bool ta = transA != NoTrans, tb = transB != NoTrans;
   int bm = g_force_tile;
   if (bm == 0) {
+    // FP64: 64 x 64 tiles at every size (measured on MI355X: 55.8 vs 47.1
+    // TFLOP/s at 16384^3, more resident waves per CU); FP32: 128 x 128 once
+    // there are enough tiles to fill the chip
     long t128 = (long)((m + 127) / 128) * ((n + 127) / 128) * batch;
-    bm = (m <= 64 || n <= 64 || t128 < 160) ? 64 : 128;
+    bm = (sizeof(T) == 8 || m <= 64 || n <= 64 || t128 < 160) ? 64 : 128;
   }
   if (bm == 128) launch_shape<T, 128, 16>(s, ta, tb, p, batch);
   else launch_shape<T, 64, 16>(s, ta, tb, p, batch);

@@ -22,9 +22,8 @@ int split_choose_count(int k, int requested, bool fp8);
 int split_side_bits(int k, int nsplit, bool fp8);
 int split_plane_count(int k, int requested, bool fp8);
 
-// BF16 residue GEMM kernel (256 x 256 tiles, direct-to-LDS loads): 0 = the
-// modular folds run under the next stage's MFMAs, 1 = right after each
-// stage's MFMAs.
+// BF16 residue GEMM kernel (256 x 256 tiles, direct-to-LDS loads): 0 = 8
+// waves of 128 x 64 (two per SIMD), 1 = 4 waves of 128 x 128.
 void split_set_kernel(int v);
 int split_get_kernel();
 

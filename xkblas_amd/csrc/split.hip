@@ -8,7 +8,8 @@
 //     from its residues;
 //  2. split: the residues A' mod p_l, B' mod p_l are written as "planes" of
 //     k-contiguous rows, padded to the GEMM tile, all in one pass:
-//       BF16: one plane per modulus, residues in [-127, 127] are exact BF16;
+//       BF16: one plane per modulus <= 481, residues in [-240, 240] are exact
+//       BF16 values (8 significant bits);
 //       FP8 (E4M3 holds the integers -16..16 exactly): a modulus <= 33 is one
 //       plane of its residue; a larger modulus p <= 255 is three planes of
 //       base-16 digits r = 16 h + l (h, l in [-8, 8]) and their sum s = h + l,
@@ -16,11 +17,11 @@
 //       + l_a l_b  (Karatsuba) needs three FP8 products instead of four;
 //  3. multiply: one MFMA GEMM per plane (v_mfma_f32_32x32x16_bf16, or
 //     v_mfma_f32_32x32x64_f8f6f4 for FP8) accumulates in FP32.  Products are
-//     <= 2^14 (BF16) or <= 2^8 (FP8) in magnitude, so FP32 sums stay exact;
+//     < 2^16 (BF16) or <= 2^8 (FP8) in magnitude, so FP32 sums stay exact;
 //     the BF16 accumulator is reduced mod p (exactly, by an FMA) at least
-//     every 1024 k, before it could pass 2^24; FP8 products run over at most
-//     32768 k per call (longer inner dimensions are summed in FP64 over
-//     chunks).  The output is the plane's sum mod p as int8;
+//     every 272 k, before it could pass 2^24; FP8 products run over at most 32768 k per
+//     call (longer inner dimensions are summed in FP64 over chunks).  The
+//     output is the plane's sum mod p as int16 (BF16) or int8 (FP8);
 //  4. recombine: the digit planes of a Karatsuba modulus are first combined
 //     into its residue (exact in FP32), then C' = sum_l r_l W_l mod P,
 //     W_l = (P/p_l) * ((P/p_l)^-1 mod p_l), with W_l held in three 41-bit limbs
@@ -38,9 +39,12 @@
 
 namespace xk {
 
-// BF16: pairwise coprime odd moduli <= 255, residues fit [-127, 127], so a
-// product is < 2^14 and 1024 of them plus a folded remainder stay below 2^24.
-static const int kModuliBF16[15] = {255, 253, 251, 247, 241, 239, 233, 229, 227, 223, 217, 211, 199, 197, 193};
+// BF16: pairwise coprime odd moduli <= 481 (prime factors 13.37, 479,
+// 3^2.53, 5^2.19, 11.43, 7.67, 467, 463, 461, 457, 449, 443, 439), residues
+// fit [-240, 240] (exact BF16 values: 8 significant bits), so a product is
+// <= 57600 and 272 of them plus a folded remainder stay below 2^24.
+constexpr int BF16_MAXMOD = 13;
+static const int kModuliBF16[BF16_MAXMOD] = {481, 479, 477, 475, 473, 469, 467, 463, 461, 457, 449, 443, 439};
 // FP8: the first FP8_DIRECT moduli (<= 33, residues in [-16, 16]) are single
 // planes; the primes after them are Karatsuba digit triples.  Together they
 // are pairwise coprime and P < 2^123 (the top limb of W stays below 2^41).
@@ -137,7 +141,7 @@ static int side_bits(const Moduli& M, int k)
 
 int split_choose_count(int k, int requested, bool fp8)
 {
-  const int maxn = fp8 ? MAXMOD : 15;
+  const int maxn = fp8 ? MAXMOD : BF16_MAXMOD;
   if (requested >= 2) return std::min(requested, maxn);
   // smallest count giving >= 48 bits per side (error ~ 2^-49 of the row max)
   for (int n = 8; n <= maxn; ++n) {
@@ -319,7 +323,7 @@ __global__ __launch_bounds__(256) void split_kernel(SplitParams sp)
 struct ModGemmParams {
   const void* A;  // [nplane][mpad][kpad] (__bf16 or E4M3 bytes)
   const void* B;  // [nplane][npad][kpad]
-  int8_t* C;      // [nplane][npad][mpad] (column-major residues)
+  void* C;        // [nplane][npad][mpad] column-major residues (int16 BF16, int8 FP8)
   long blkA, blkB;  // elements per k-block of an A / B plane (rows padded x 64 bytes)
   long sA, sB, sC;
   long ldc;       // mpad
@@ -393,51 +397,72 @@ __device__ __forceinline__ void dm_tile(const ModGemmParams& prm, int& tm, int& 
 }
 
 // Epilogue of the 256 x 256 kernels: residues in [-(p-1)/2, (p-1)/2] as
-// int8.  The accumulator block (i, j) holds D = A B^T with lane -> n =
-// 32j + lane%32 and (lane, r) -> m = 32i + 8(r/4) + 4(lane/32) + r%4, so
-// r%4 = 0..3 are 4 consecutive rows: one packed 32-bit word.  Words go to an
-// LDS tile T (column stride 272 bytes, 69632 bytes), then every column
-// leaves as 16 aligned 16-byte stores.  The caller has drained its loads and
-// LDS reads; T may alias the operand stages.
-__device__ __forceinline__ void dm_store_residues(const f32x16 (&acc)[4][4], float p, float ip, char* T,
-                                                  int8_t* C, long ldc)
+// Res (int8 or int16).  The accumulator block (i, j) holds D = A B^T with
+// lane -> n = 32j + lane%32 and (lane, r) -> m = 32i + 8(r/4) + 4(lane/32) +
+// r%4, so r%4 = 0..3 are 4 consecutive rows: one packed 4-element word.
+// Words go to an LDS tile T (column stride 256 elements + 16 bytes), then
+// every column leaves as aligned 16-byte stores.  int8: the whole 256 x 256
+// tile at once (69632 bytes); int16: one half of the columns (the waves with
+// wn = 0, then wn = 1; 67584 bytes) at a time.  The caller has drained its
+// loads and LDS reads; T may alias the operand stages.
+template <typename Res>
+__device__ __forceinline__ void dm_store_residues(const f32x16 (&acc)[4][4], float p, float ip, char* T, Res* C,
+                                                  long ldc)
 {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave & 1, wn = wave >> 1;
   const int fr = lane & 31, fh = lane >> 5;
   const float hi = (p - 1.0f) * 0.5f;
-  constexpr int CS = 272;
+  constexpr int ES = sizeof(Res);
+  constexpr int CS = 256 * ES + 16;      // bytes per staged column
+  constexpr int HALVES = ES == 1 ? 1 : 2;  // column halves staged one after the other
+  constexpr int CHUNKS = 256 * ES / 16;    // 16-byte chunks per column
+  constexpr int COLS = 256 / HALVES;
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
+  for (int h = 0; h < HALVES; ++h) {
+    if (HALVES == 1 || wn == h) {
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
+      for (int i = 0; i < 4; ++i) {
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        unsigned word = 0;
+        for (int j = 0; j < 4; ++j) {
 #pragma unroll
-        for (int b = 0; b < 4; ++b) {
-          float v = fold(acc[i][j][4 * q + b], p, ip);
-          if (v > hi) v -= p;
-          if (v < -hi) v += p;
-          word |= ((unsigned)(int)v & 0xffu) << (8 * b);
+          for (int q = 0; q < 4; ++q) {
+            Res w[4];
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+              float v = fold(acc[i][j][4 * q + b], p, ip);
+              if (v > hi) v -= p;
+              if (v < -hi) v += p;
+              w[b] = (Res)(int)v;
+            }
+            const int ml = wm * 128 + i * 32 + 8 * q + 4 * fh;
+            const int nl = (HALVES == 1 ? wn * 128 : 0) + j * 32 + fr;
+            if constexpr (ES == 1) {
+              const unsigned word = (unsigned)(uint8_t)w[0] | ((unsigned)(uint8_t)w[1] << 8) |
+                                    ((unsigned)(uint8_t)w[2] << 16) | ((unsigned)(uint8_t)w[3] << 24);
+              *reinterpret_cast<unsigned*>(T + nl * CS + ml) = word;
+            } else {
+              uint2 word;
+              word.x = (unsigned)(uint16_t)w[0] | ((unsigned)(uint16_t)w[1] << 16);
+              word.y = (unsigned)(uint16_t)w[2] | ((unsigned)(uint16_t)w[3] << 16);
+              *reinterpret_cast<uint2*>(T + nl * CS + ml * 2) = word;
+            }
+          }
         }
-        const int ml = wm * 128 + i * 32 + 8 * q + 4 * fh;
-        const int nl = wn * 128 + j * 32 + fr;
-        *reinterpret_cast<unsigned*>(T + nl * CS + ml) = word;
       }
     }
-  }
-  __syncthreads();
-  const int chunk = tid & 15;
+    __syncthreads();
+    // COLS columns x CHUNKS chunks, 256 threads
 #pragma unroll 4
-  for (int s2 = 0; s2 < 16; ++s2) {
-    const int col = (tid >> 4) + 16 * s2;
-    const uint4 v = *reinterpret_cast<const uint4*>(T + col * CS + chunk * 16);
-    *reinterpret_cast<uint4*>(C + (long)col * ldc + chunk * 16) = v;
+    for (int c = tid; c < COLS * CHUNKS; c += 256) {
+      const int col = c / CHUNKS, chunk = c % CHUNKS;
+      const uint4 v = *reinterpret_cast<const uint4*>(T + col * CS + chunk * 16);
+      *reinterpret_cast<uint4*>(reinterpret_cast<char*>(C + (long)(col + h * COLS) * ldc) + chunk * 16) = v;
+    }
+    if (h + 1 < HALVES) __syncthreads();
   }
 }
 
-template <bool FOLD_NEXT>
 __global__ __launch_bounds__(256) void mod_gemm_dma_kernel(ModGemmParams prm)
 {
   // [stage][A rows 256 x 32 | B rows 256 x 32]
@@ -511,13 +536,13 @@ __global__ __launch_bounds__(256) void mod_gemm_dma_kernel(ModGemmParams prm)
   for (int j = 0; j < 4; ++j) b0[j] = *reinterpret_cast<const bf16x8*>(&S[0][0] + rdB0 + j * 32 * DM_BK);
 
   // The k loop runs in groups of 16 stages (kpad is a multiple of 512).
-  // Folds are spread over the stages, one accumulator block per stage, so
-  // every block is reduced mod p once per 512 k.  FOLD_NEXT = false: stage u
-  // folds block u right after its half-1 MFMA (issued first in that half).
-  // FOLD_NEXT = true: stage u folds block f = u - 1 (whose last MFMA was
-  // issued first in half 1 of the previous stage) during its half 0, whose
-  // MFMAs start after block f and end with it, so the fold's VALU work has 15
-  // independent MFMAs to hide under and no result to wait for.
+  // Folds are spread over the stages, two accumulator blocks per stage:
+  // stage u folds block f = u - 1 during half 0 and block f + 8 during
+  // half 1; each half issues its 16 MFMAs starting after the folded block
+  // and ending with it, so the fold's VALU work hides under 15 independent MFMAs and waits
+  // for no result (the folded block's previous MFMA was issued 8 or 9
+  // MFMAs earlier).  A block is then folded at most 8.5 stages = 272 k
+  // apart: 272 products <= 240^2 plus a folded remainder stay below 2^24.
   auto group = [&](int kt0) {
 #pragma unroll
     for (int u = 0; u < 16; ++u) {
@@ -529,34 +554,22 @@ __global__ __launch_bounds__(256) void mod_gemm_dma_kernel(ModGemmParams prm)
       for (int i = 0; i < 4; ++i) a1[i] = *reinterpret_cast<const bf16x8*>(st + rdA1 + i * 32 * DM_BK);
 #pragma unroll
       for (int j = 0; j < 4; ++j) b1[j] = *reinterpret_cast<const bf16x8*>(st + rdB1 + j * 32 * DM_BK);
-      if constexpr (FOLD_NEXT) {
-        const int f = (u + 15) & 15;
+      // blocks folded in this stage: f in half 0, f + 8 in half 1; each half
+      // issues its MFMAs starting after its folded block and ending with it
+      const int f = (u + 15) & 15, f8 = (u + 7) & 15;
 #pragma unroll
-        for (int r = 0; r < 16; ++r) acc[f >> 2][f & 3][r] = fold(acc[f >> 2][f & 3][r], p, ip);
+      for (int r = 0; r < 16; ++r) acc[f >> 2][f & 3][r] = fold(acc[f >> 2][f & 3][r], p, ip);
 #pragma unroll
-        for (int t = 0; t < 16; ++t) {
-          const int blk = (f + 1 + t) & 15, i = blk >> 2, j = blk & 3;
-          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0[i], b0[j], acc[i][j], 0, 0, 0);
-        }
-        // interleave: one operand read and the fold's VALU work between MFMAs
+      for (int t = 0; t < 16; ++t) {
+        const int blk = (f + 1 + t) & 15, i = blk >> 2, j = blk & 3;
+        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0[i], b0[j], acc[i][j], 0, 0, 0);
+      }
+      // interleave: one operand read and the fold's VALU work between MFMAs
 #pragma unroll
-        for (int g = 0; g < 8; ++g) {
-          __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
-          __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);
-          __builtin_amdgcn_sched_group_barrier(0x002, 10, 0);
-        }
-      } else {
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-#pragma unroll
-          for (int j = 0; j < 4; ++j)
-            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0[i], b0[j], acc[i][j], 0, 0, 0);
-        // interleave: one operand read after every two MFMAs
-#pragma unroll
-        for (int g = 0; g < 8; ++g) {
-          __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
-          __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);
-        }
+      for (int g = 0; g < 8; ++g) {
+        __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+        __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);
+        __builtin_amdgcn_sched_group_barrier(0x002, 10, 0);
       }
       // this wave's reads of stage kt are done (and, before the barrier, of
       // stage kt-1, which the loads issued below overwrite)
@@ -576,18 +589,17 @@ __global__ __launch_bounds__(256) void mod_gemm_dma_kernel(ModGemmParams prm)
 #pragma unroll
       for (int j = 0; j < 4; ++j) b0[j] = *reinterpret_cast<const bf16x8*>(sn + rdB0 + j * 32 * DM_BK);
 #pragma unroll
-      for (int t = 0; t < 16; ++t) {
-        const int blk = (u + t) & 15, i = blk >> 2, j = blk & 3;
-        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1[i], b1[j], acc[i][j], 0, 0, 0);
-      }
-      if constexpr (!FOLD_NEXT) {
+      for (int r = 0; r < 16; ++r) acc[f8 >> 2][f8 & 3][r] = fold(acc[f8 >> 2][f8 & 3][r], p, ip);
 #pragma unroll
-        for (int r = 0; r < 16; ++r) acc[u >> 2][u & 3][r] = fold(acc[u >> 2][u & 3][r], p, ip);
+      for (int t = 0; t < 16; ++t) {
+        const int blk = (f8 + 1 + t) & 15, i = blk >> 2, j = blk & 3;
+        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1[i], b1[j], acc[i][j], 0, 0, 0);
       }
 #pragma unroll
       for (int g = 0; g < 8; ++g) {
         __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
         __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);
+        __builtin_amdgcn_sched_group_barrier(0x002, 10, 0);
       }
     }
   };
@@ -595,8 +607,188 @@ __global__ __launch_bounds__(256) void mod_gemm_dma_kernel(ModGemmParams prm)
 
   wait_vmcnt_le(0);
   __syncthreads();
-  dm_store_residues(acc, p, ip, reinterpret_cast<char*>(&S[0][0]), prm.C + (long)l * prm.sC + n0 * prm.ldc + m0,
-                    prm.ldc);
+  dm_store_residues(acc, p, ip, reinterpret_cast<char*>(&S[0][0]),
+                    static_cast<int16_t*>(prm.C) + (long)l * prm.sC + n0 * prm.ldc + m0, prm.ldc);
+}
+
+// ---- 8-wave variant: the same 256 x 256 tile, LDS ring and k-blocked
+// planes, but 8 waves (2 per SIMD) of 128 x 64 each (4 x 2 blocks of 32 x 32,
+// 128 accumulators per lane), so that one wave's VALU work (the modular
+// folds) and waits overlap the other wave's MFMAs.  Each wave loads 32 rows
+// of A and 32 of B per stage (4 direct-to-LDS loads).  Folds: one block per
+// stage, right after the half-1 MFMAs, whose order starts with that block,
+// so every block is reduced mod p once per 8 stages (256 k).
+template <typename Res>
+__device__ __forceinline__ void w8_store_residues(const f32x16 (&acc)[4][2], float p, float ip, char* T, Res* C,
+                                                  long ldc)
+{
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+  const int fr = lane & 31, fh = lane >> 5;
+  const float hi = (p - 1.0f) * 0.5f;
+  constexpr int ES = sizeof(Res);
+  constexpr int CS = 256 * ES + 16;
+  constexpr int HALVES = ES == 1 ? 1 : 2;
+  constexpr int CHUNKS = 256 * ES / 16;
+  constexpr int COLS = 256 / HALVES;
+#pragma unroll
+  for (int h = 0; h < HALVES; ++h) {
+    if (HALVES == 1 || (wn >> 1) == h) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            Res w[4];
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+              float v = fold(acc[i][j][4 * q + b], p, ip);
+              if (v > hi) v -= p;
+              if (v < -hi) v += p;
+              w[b] = (Res)(int)v;
+            }
+            const int ml = wm * 128 + i * 32 + 8 * q + 4 * fh;
+            const int nl = (HALVES == 1 ? wn * 64 : (wn & 1) * 64) + j * 32 + fr;
+            if constexpr (ES == 1) {
+              const unsigned word = (unsigned)(uint8_t)w[0] | ((unsigned)(uint8_t)w[1] << 8) |
+                                    ((unsigned)(uint8_t)w[2] << 16) | ((unsigned)(uint8_t)w[3] << 24);
+              *reinterpret_cast<unsigned*>(T + nl * CS + ml) = word;
+            } else {
+              uint2 word;
+              word.x = (unsigned)(uint16_t)w[0] | ((unsigned)(uint16_t)w[1] << 16);
+              word.y = (unsigned)(uint16_t)w[2] | ((unsigned)(uint16_t)w[3] << 16);
+              *reinterpret_cast<uint2*>(T + nl * CS + ml * 2) = word;
+            }
+          }
+        }
+      }
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int c = tid; c < COLS * CHUNKS; c += 512) {
+      const int col = c / CHUNKS, chunk = c % CHUNKS;
+      const uint4 v = *reinterpret_cast<const uint4*>(T + col * CS + chunk * 16);
+      *reinterpret_cast<uint4*>(reinterpret_cast<char*>(C + (long)(col + h * COLS) * ldc) + chunk * 16) = v;
+    }
+    if (h + 1 < HALVES) __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(512) void mod_gemm_w8_kernel(ModGemmParams prm)
+{
+  __shared__ __attribute__((aligned(1024))) __bf16 S[DM_STAGES][(DM_BM + DM_BN) * DM_BK];
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave & 1, wn = wave >> 1;  // wn = 0..3
+  const int l = blockIdx.z;
+  const float p = prm.p[l], ip = prm.invp[l];
+
+  int tm, tn;
+  dm_tile(prm, tm, tn);
+  const long m0 = (long)tm * DM_BM, n0 = (long)tn * DM_BN;
+  const __bf16* A = static_cast<const __bf16*>(prm.A) + (long)l * prm.sA + m0 * DM_BK;
+  const __bf16* B = static_cast<const __bf16*>(prm.B) + (long)l * prm.sB + n0 * DM_BK;
+
+  // LDS block b = 2 wave + i (16 rows of a stage, 1 KB): lane writes physical
+  // chunk lane%4 of row 16b + lane/4 and fetches the logical chunk stored there
+  const int srow = lane >> 2;
+  const int schunk = (lane & 3) ^ ((srow >> 2) & 3);
+  const __bf16* gA = A + (wave * 32 + srow) * DM_BK + schunk * 8;
+  const __bf16* gB = B + (wave * 32 + srow) * DM_BK + schunk * 8;
+  auto issue = [&](int stage, int kt) {
+    __bf16* base = S[stage] + wave * 1024;
+    const __bf16* a = gA + (long)kt * prm.blkA;
+    const __bf16* b = gB + (long)kt * prm.blkB;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+      __builtin_amdgcn_global_load_lds((const void*)(a + i * 16 * DM_BK), (lds_void*)(base + i * 512), 16, 0, 0);
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+      __builtin_amdgcn_global_load_lds((const void*)(b + i * 16 * DM_BK),
+                                       (lds_void*)(base + DM_BM * DM_BK + i * 512), 16, 0, 0);
+  };
+
+  f32x16 acc[4][2];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
+
+  const int nk = prm.kpad / DM_BK;  // a multiple of 16
+  for (int s = 0; s < DM_STAGES - 1 && s < nk; ++s) issue(s, s);
+
+  const int fr = lane & 31, fh = lane >> 5;
+  const int sw = (fr >> 2) & 3;
+  const int rdA0 = (wm * 128 + fr) * DM_BK + ((0 + fh) ^ sw) * 8;
+  const int rdA1 = (wm * 128 + fr) * DM_BK + ((2 + fh) ^ sw) * 8;
+  const int rdB0 = DM_BM * DM_BK + (wn * 64 + fr) * DM_BK + ((0 + fh) ^ sw) * 8;
+  const int rdB1 = DM_BM * DM_BK + (wn * 64 + fr) * DM_BK + ((2 + fh) ^ sw) * 8;
+
+  bf16x8 a0[4], b0[2], a1[4], b1[2];
+  wait_vmcnt_le(4 * (min(nk, DM_STAGES - 1) - 1));
+  __builtin_amdgcn_s_barrier();
+#pragma unroll
+  for (int i = 0; i < 4; ++i) a0[i] = *reinterpret_cast<const bf16x8*>(&S[0][0] + rdA0 + i * 32 * DM_BK);
+#pragma unroll
+  for (int j = 0; j < 2; ++j) b0[j] = *reinterpret_cast<const bf16x8*>(&S[0][0] + rdB0 + j * 32 * DM_BK);
+
+  // unrolled by 8 stages (the fold period; twice the ring depth, so the
+  // stage buffers are compile-time constants): a 16-stage unroll makes the
+  // compiler hoist enough addresses to spill
+  for (int kt0 = 0; kt0 < nk; kt0 += 8) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int kt = kt0 + u;
+      const __bf16* st = S[kt % DM_STAGES];
+      wait_lgkm0();
+#pragma unroll
+      for (int i = 0; i < 4; ++i) a1[i] = *reinterpret_cast<const bf16x8*>(st + rdA1 + i * 32 * DM_BK);
+#pragma unroll
+      for (int j = 0; j < 2; ++j) b1[j] = *reinterpret_cast<const bf16x8*>(st + rdB1 + j * 32 * DM_BK);
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0[i], b0[j], acc[i][j], 0, 0, 0);
+#pragma unroll
+      for (int g = 0; g < 6; ++g) {
+        __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+      }
+      wait_lgkm0();
+      if (kt + 1 < nk) {
+        wait_vmcnt_le(4 * (min(nk - 1, kt + DM_STAGES - 2) - (kt + 1)));
+        __builtin_amdgcn_s_barrier();
+        if (kt + DM_STAGES - 1 < nk) issue((kt + DM_STAGES - 1) % DM_STAGES, kt + DM_STAGES - 1);
+      }
+      const __bf16* sn = S[(kt + 1) % DM_STAGES];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) a0[i] = *reinterpret_cast<const bf16x8*>(sn + rdA0 + i * 32 * DM_BK);
+#pragma unroll
+      for (int j = 0; j < 2; ++j) b0[j] = *reinterpret_cast<const bf16x8*>(sn + rdB0 + j * 32 * DM_BK);
+      // half 1 starts with the block folded after it
+      const int fb = u & 7;
+#pragma unroll
+      for (int t = 0; t < 8; ++t) {
+        const int blk = (fb + t) & 7, i = blk >> 1, j = blk & 1;
+        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1[i], b1[j], acc[i][j], 0, 0, 0);
+      }
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[fb >> 1][fb & 1][r] = fold(acc[fb >> 1][fb & 1][r], p, ip);
+#pragma unroll
+      for (int g = 0; g < 6; ++g) {
+        __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+      }
+    }
+  }
+
+  wait_vmcnt_le(0);
+  __syncthreads();
+  w8_store_residues(acc, p, ip, reinterpret_cast<char*>(&S[0][0]),
+                    static_cast<int16_t*>(prm.C) + (long)l * prm.sC + n0 * prm.ldc + m0, prm.ldc);
 }
 
 // ---- FP8 variant: the same 256 x 256 tile and 4-stage ring of 64-byte row
@@ -760,12 +952,12 @@ __global__ __launch_bounds__(256) void mod_gemm_fp8_kernel(ModGemmParams prm)
 
   wait_vmcnt_le(0);
   __syncthreads();
-  dm_store_residues(acc, p, ip, reinterpret_cast<char*>(&S[0][0]), prm.C + (long)l * prm.sC + n0 * prm.ldc + m0,
-                    prm.ldc);
+  dm_store_residues(acc, p, ip, reinterpret_cast<char*>(&S[0][0]),
+                    static_cast<int8_t*>(prm.C) + (long)l * prm.sC + n0 * prm.ldc + m0, prm.ldc);
 }
 
 struct RecParams {
-  const int8_t* R;
+  const void* R;  // residue planes (int16 BF16, int8 FP8)
   long ldr, sR;
   int nmod, m, n, tri;
   unsigned kara;  // bit l: modulus l is a Karatsuba digit triple (3 planes)
@@ -809,6 +1001,24 @@ __device__ __forceinline__ double crt_one(const RecParams& rp, double S0, double
   return hi + lo;
 }
 
+// residues of 4 consecutive rows of one plane column
+__device__ __forceinline__ void load4(const int8_t* r, double (&v)[4])
+{
+  const unsigned w = *reinterpret_cast<const unsigned*>(r);
+#pragma unroll
+  for (int e = 0; e < 4; ++e) v[e] = (double)(int8_t)(w >> (8 * e));
+}
+
+__device__ __forceinline__ void load4(const int16_t* r, double (&v)[4])
+{
+  const uint2 w = *reinterpret_cast<const uint2*>(r);
+  v[0] = (double)(int16_t)(w.x & 0xffffu);
+  v[1] = (double)(int16_t)(w.x >> 16);
+  v[2] = (double)(int16_t)(w.y & 0xffffu);
+  v[3] = (double)(int16_t)(w.y >> 16);
+}
+
+template <typename Res>
 __global__ __launch_bounds__(256) void reconstruct_kernel(RecParams rp)
 {
   const int i0 = 4 * (blockIdx.x * blockDim.x + threadIdx.x);
@@ -817,26 +1027,23 @@ __global__ __launch_bounds__(256) void reconstruct_kernel(RecParams rp)
   if (rp.tri == TRI_LOWER && i0 + 3 < j) return;
   if (rp.tri == TRI_UPPER && i0 > j) return;
   double S[4][3] = {};
-  const int8_t* r = rp.R + (long)j * rp.ldr + i0;
+  const Res* r = static_cast<const Res*>(rp.R) + (long)j * rp.ldr + i0;
   int q = 0;  // plane of modulus l
   for (int l = 0; l < rp.nmod; ++l) {
-    const unsigned w = *reinterpret_cast<const unsigned*>(r + (long)q * rp.sR);
     double v[4];
+    load4(r + (long)q * rp.sR, v);
     if (!((rp.kara >> l) & 1u)) {
-#pragma unroll
-      for (int e = 0; e < 4; ++e) v[e] = (double)(int8_t)(w >> (8 * e));
       q += 1;
     } else {
       // planes h_a h_b, l_a l_b, s_a s_b (each mod p):
       // r = 256 hh + 16 (ss - hh - ll) + ll, |r| < 2^16 before the reduction
-      const unsigned wl = *reinterpret_cast<const unsigned*>(r + (long)(q + 1) * rp.sR);
-      const unsigned ws = *reinterpret_cast<const unsigned*>(r + (long)(q + 2) * rp.sR);
+      double vl[4], vs[4];
+      load4(r + (long)(q + 1) * rp.sR, vl);
+      load4(r + (long)(q + 2) * rp.sR, vs);
       const float p = rp.p[l], ip = rp.invp[l];
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
-        const float hh = (float)(int8_t)(w >> (8 * e));
-        const float ll = (float)(int8_t)(wl >> (8 * e));
-        const float ss = (float)(int8_t)(ws >> (8 * e));
+        const float hh = (float)v[e], ll = (float)vl[e], ss = (float)vs[e];
         const float x = fmaf(256.0f, hh, fmaf(16.0f, ss - hh - ll, ll));
         v[e] = (double)fmaf(-p, rintf(x * ip), x);
       }
@@ -876,8 +1083,7 @@ static inline long rup(long v, long q) { return (v + q - 1) / q * q; }
 
 enum { WS_SA = 8, WS_SB = 9, WS_SC = 10, WS_SE = 11 };
 
-// BF16 residue GEMM: 0 folds under the next stage's MFMAs, 1 folds right
-// after each stage's MFMAs
+// BF16 residue GEMM: 0 = 8 waves of 128 x 64, 1 = 4 waves of 128 x 128
 static int g_mod_gemm_variant = getenv("XKBLAS_SPLIT_KERNEL") ? atoi(getenv("XKBLAS_SPLIT_KERNEL")) : 0;
 void split_set_kernel(int v) { g_mod_gemm_variant = v; }
 int split_get_kernel() { return g_mod_gemm_variant; }
@@ -934,7 +1140,8 @@ static void split_gemm_chunk(const Exec& x, int transA, int transB, int m, int n
   int* shB = shA + mpad;
   void* RA = scratch(WS_SA, esz * (size_t)np * mpad * kpad);
   void* RB = same ? RA : scratch(WS_SB, esz * (size_t)np * npad * kpad);
-  int8_t* RC = (int8_t*)scratch(WS_SC, (size_t)np * mpad * npad);
+  const size_t rsz = fp8 ? sizeof(int8_t) : sizeof(int16_t);  // residue element
+  void* RC = scratch(WS_SC, rsz * (size_t)np * mpad * npad);
   hipStream_t s = x.s;
 
   // 1-2: exponents of the rows of op(A) and columns of op(B)
@@ -990,8 +1197,8 @@ static void split_gemm_chunk(const Exec& x, int transA, int transB, int m, int n
   const long tiles = tri == TRI_FULL ? (long)g.tiles_m * g.tiles_n : (long)g.tiles_m * (g.tiles_m + 1) / 2;
   if (fp8) hipLaunchKernelGGL(mod_gemm_fp8_kernel, dim3((unsigned)tiles, 1, np), dim3(256), 0, s, g);
   else if (g_mod_gemm_variant == 1)
-    hipLaunchKernelGGL(mod_gemm_dma_kernel<false>, dim3((unsigned)tiles, 1, np), dim3(256), 0, s, g);
-  else hipLaunchKernelGGL(mod_gemm_dma_kernel<true>, dim3((unsigned)tiles, 1, np), dim3(256), 0, s, g);
+    hipLaunchKernelGGL(mod_gemm_dma_kernel, dim3((unsigned)tiles, 1, np), dim3(256), 0, s, g);
+  else hipLaunchKernelGGL(mod_gemm_w8_kernel, dim3((unsigned)tiles, 1, np), dim3(512), 0, s, g);
   XK_HIP(hipGetLastError());
   R.prof_end();
 
@@ -1013,12 +1220,13 @@ static void split_gemm_chunk(const Exec& x, int transA, int transB, int m, int n
     RecParams r2 = rp;
     const int nn = std::min(65535, n - j0);
     r2.n = nn;
-    r2.R = RC + (long)j0 * mpad;
+    r2.R = static_cast<char*>(RC) + rsz * (size_t)j0 * mpad;
     r2.shB = rp.shB + j0;
     r2.C = C + (long)j0 * ldc;
     // the triangle test uses global indices: shift i-j by j0 via tri only when j0 == 0
     XK_CHECK(tri == TRI_FULL || j0 == 0, "triangular split beyond 65535 columns");
-    hipLaunchKernelGGL(reconstruct_kernel, dim3((m + 1023) / 1024, nn), dim3(256), 0, s, r2);
+    if (fp8) hipLaunchKernelGGL(reconstruct_kernel<int8_t>, dim3((m + 1023) / 1024, nn), dim3(256), 0, s, r2);
+    else hipLaunchKernelGGL(reconstruct_kernel<int16_t>, dim3((m + 1023) / 1024, nn), dim3(256), 0, s, r2);
   }
   XK_HIP(hipGetLastError());
   R.prof_end();

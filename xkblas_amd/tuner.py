@@ -37,7 +37,7 @@ KERN = {"gemm": 1, "syrk": 6, "trsm": 4, "potrf": 4}
 
 
 SPLIT_MODES = (2, 3)  # XKBLAS_SPLIT_BF16_MATH, XKBLAS_SPLIT_FP8_MATH
-MAXCOUNT = {2: 15, 3: 20}
+MAXCOUNT = {2: 13, 3: 20}
 
 
 def split_counts(mode, k):
