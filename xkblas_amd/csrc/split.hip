@@ -158,29 +158,37 @@ int split_plane_count(int k, int requested, bool fp8)
 
 // ---------------------------------------------------------------- kernels
 // Row maxima of |X| for X viewed as rows x kdim (row_contig: X[i + k*ld],
-// else X[k + i*ld]); bits of non-negative doubles order like integers.
+// else X[k + i*ld]); bits of non-negative doubles order like integers, so
+// the maxima combine with integer atomicMax.
+//  row_contig: a block of 256 threads owns 256 consecutive rows and a chunk
+//  of AM_K k; thread t keeps the running max of row i0 + t (consecutive
+//  lanes read consecutive addresses);
+//  else: each wave owns one row and a chunk of AM_K k (64 lanes x 8 bytes
+//  contiguous per load), reduced across the wave at the end.
+constexpr int AM_K = 256;
 __global__ __launch_bounds__(256) void amax_kernel(const double* __restrict__ X, long ld, int rows, int kdim,
                                                    int row_contig, unsigned long long* __restrict__ bits)
 {
-  __shared__ double T[64][65];
   const int t = threadIdx.x;
-  const int i0 = blockIdx.y * 64, k0 = blockIdx.x * 64;
-#pragma unroll 4
-  for (int s = 0; s < 16; ++s) {
-    int i, k;
-    if (row_contig) { i = t & 63; k = (t >> 6) + 4 * s; }
-    else { k = t & 63; i = (t >> 6) + 4 * s; }
-    const int gi = i0 + i, gk = k0 + k;
-    double v = 0.0;
-    if (gi < rows && gk < kdim) v = row_contig ? X[gi + (long)gk * ld] : X[gk + (long)gi * ld];
-    T[i][k] = fabs(v);
+  const int k0 = blockIdx.x * AM_K, k1 = min(kdim, k0 + AM_K);
+  double m = 0.0;
+  int row;
+  if (row_contig) {
+    row = blockIdx.y * 256 + t;
+    if (row >= rows) return;
+    const double* x = X + row + (long)k0 * ld;
+#pragma unroll 8
+    for (int k = k0; k < k1; ++k, x += ld) m = fmax(m, fabs(*x));
+  } else {
+    row = blockIdx.y * 4 + (t >> 6);
+    if (row >= rows) return;
+    const double* x = X + (long)row * ld;
+    for (int k = k0 + (t & 63); k < k1; k += 64) m = fmax(m, fabs(x[k]));
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
+    if ((t & 63) != 0) return;
   }
-  __syncthreads();
-  if (t < 64 && i0 + t < rows) {
-    double m = 0.0;
-    for (int k = 0; k < 64; ++k) m = fmax(m, T[t][k]);
-    if (m > 0.0) atomicMax(bits + i0 + t, (unsigned long long)__double_as_longlong(m));
-  }
+  if (m > 0.0) atomicMax(bits + row, (unsigned long long)__double_as_longlong(m));
 }
 
 __global__ void fill_u64_kernel(unsigned long long* p, int n)
@@ -1151,10 +1159,15 @@ static void split_gemm_chunk(const Exec& x, int transA, int transB, int m, int n
   R.prof_begin("split_prep", 0.0);
   hipLaunchKernelGGL(fill_u64_kernel, dim3((int)((mpad + npad + 255) / 256)), dim3(256), 0, s, bitsA,
                      (int)(mpad + npad));
-  hipLaunchKernelGGL(amax_kernel, dim3((k + 63) / 64, (m + 63) / 64), dim3(256), 0, s, A, lda, m, k, rcA, bitsA);
+  auto amax = [&](const double* X, long ld, int rows, int rc, unsigned long long* out) {
+    XK_CHECK((rows + 3) / 4 <= 65535, "row-max grid");
+    const dim3 grid((unsigned)((k + AM_K - 1) / AM_K), (unsigned)(rc ? (rows + 255) / 256 : (rows + 3) / 4));
+    hipLaunchKernelGGL(amax_kernel, grid, dim3(256), 0, s, X, ld, rows, k, rc, out);
+  };
+  amax(A, lda, m, rcA, bitsA);
   hipLaunchKernelGGL(shift_kernel, dim3((m + 255) / 256), dim3(256), 0, s, bitsA, m, tb, shA);
   if (!same) {
-    hipLaunchKernelGGL(amax_kernel, dim3((k + 63) / 64, (n + 63) / 64), dim3(256), 0, s, B, ldb, n, k, rcB, bitsB);
+    amax(B, ldb, n, rcB, bitsB);
     hipLaunchKernelGGL(shift_kernel, dim3((n + 255) / 256), dim3(256), 0, s, bitsB, n, tb, shB);
   }
   XK_HIP(hipGetLastError());
