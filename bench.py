@@ -334,7 +334,7 @@ def main():
                      "traffic": BF16_GEMM_TRAFFIC_16384 if n == 16384 else None,
                      # A and B planes read once (2 bytes each), int16 residues written
                      "traffic_algorithmic": split["residue_gemms"] * (2 * 2 * n * n + 2 * n * n),
-                     "kernel": "mod_gemm_dma_kernel (BF16 residue GEMM, 256x256 tiles, all moduli in one "
+                     "kernel": "mod_gemm_w8_kernel (BF16 residue GEMM, 256x256 tiles, 8 waves, all moduli in one "
                                "launch)",
                      "kernel_ms": round(bf16_ms, 3),
                      "fp64_equivalent_frac_of_fp64_peak": round(flops / t_split / 1e12 / FP64_PEAK, 4)},
