@@ -2,13 +2,12 @@
 // Cholesky + inverse of blocks of order <= 64, triangular inverse, and small
 // helpers (2-D copy, triangle zeroing, int fill).
 //
-// A leaf is one wave (64 lanes) per block, held in LDS (column stride 65 so
-// row and column walks are conflict-free); the padded rows/columns beyond n
-// are the identity, so every block is handled as a 64 x 64 problem.  Step j
-// scales column j by the pivot and lane i applies the rank-1 update to row i
-// (the column values are broadcast reads).  The inverse is then a forward
-// substitution per column (lane j owns column j of inv(L)), launched as a
-// second kernel on the factor.
+// A leaf is one wave (64 lanes) per block; the padded rows/columns beyond n
+// are the identity, so every block is handled as a 64 x 64 problem.  The
+// Cholesky leaf keeps the block in registers (one row per lane, cross-lane
+// reads for the broadcasts) and writes the factor and its inverse in one
+// launch; the triangular-inverse leaf holds the block in LDS (column stride
+// 65) and solves one column of inv(L) per lane.
 #include "common.h"
 #include "kernels.h"
 #include <climits>
@@ -68,52 +67,63 @@ __device__ __forceinline__ void lower_inverse_to(const T* Ls, int lane, bool uni
   }
 }
 
+// Cholesky factor and its inverse for one block of order <= 64, in
+// registers: lane i holds row i of the lower-triangular view (x) and later
+// column i of inv(L) (y); the values of other rows come from cross-lane
+// reads (v_readlane), so there is no LDS traffic and no barrier.  Step j
+// takes the pivot from lane j, scales column j and applies the rank-1 update
+// x[k] -= L[i][j] L[k][j] for k > j (entries above the diagonal become
+// garbage and are cleared before they are read).  The inverse is a forward
+// substitution, column `lane` per lane.
 template <typename T>
 __global__ __launch_bounds__(64) void potrf_leaf_kernel(int uplo, int n, T* A, long lda, long sA, T* Tinv,
                                                         long ldinv, long sInv, int* info, int base)
 {
-  __shared__ T Ls[LEAF * LLD];
   const int lane = threadIdx.x;
   A += (long)blockIdx.x * sA;
   Tinv += (long)blockIdx.x * sInv;
-#pragma unroll 4
-  for (int j = 0; j < LEAF; ++j) Ls[lane + j * LLD] = lower_view(A, lda, uplo, n, lane, j);
-  __syncthreads();
+  T x[LEAF];
+#pragma unroll
+  for (int j = 0; j < LEAF; ++j) x[j] = lower_view(A, lda, uplo, n, lane, j);
 
-  // right-looking column sweep; lane i owns row i of the trailing update
   bool failed = false;
+#pragma unroll
   for (int j = 0; j < LEAF; ++j) {
-    const T d = Ls[j + j * LLD];
+    const T d = bcast(x[j], j);
     if (!(d > T(0)) && !failed && j < n) {
       failed = true;
       if (lane == 0) atomicMin(info, base + j + 1);
     }
     const T s = sqrt(d);
-    const T v = Ls[lane + j * LLD];
-    __syncthreads();
-    const T l = lane > j ? v / s : (lane == j ? s : T(0));
-    if (lane >= j) Ls[lane + j * LLD] = l;
-    __syncthreads();
-    if (lane > j) {
-      int k = j + 1;
-      for (; k + 3 <= lane; k += 4) {
-        const T c0 = Ls[k + j * LLD], c1 = Ls[k + 1 + j * LLD], c2 = Ls[k + 2 + j * LLD], c3 = Ls[k + 3 + j * LLD];
-        Ls[lane + k * LLD] -= l * c0;
-        Ls[lane + (k + 1) * LLD] -= l * c1;
-        Ls[lane + (k + 2) * LLD] -= l * c2;
-        Ls[lane + (k + 3) * LLD] -= l * c3;
-      }
-      for (; k <= lane; ++k) Ls[lane + k * LLD] -= l * Ls[k + j * LLD];
-    }
-    __syncthreads();
+    x[j] = lane > j ? x[j] / s : (lane == j ? s : T(0));
+#pragma unroll
+    for (int k = j + 1; k < LEAF; ++k) x[k] = fma(-x[j], bcast(x[j], k), x[k]);
   }
   // factor back to the stored triangle
-#pragma unroll 4
+#pragma unroll
   for (int j = 0; j < LEAF; ++j) {
     if (lane < n && j < n && j <= lane) {
-      const T x = Ls[lane + j * LLD];
-      if (uplo == Lower) A[lane + (long)j * lda] = x;
-      else A[j + (long)lane * lda] = x;
+      if (uplo == Lower) A[lane + (long)j * lda] = x[j];
+      else A[j + (long)lane * lda] = x[j];
+    }
+  }
+  if (!Tinv) return;
+  // inv(L): lane j computes column j, y[i] = inv(L)[i][j]
+  T y[LEAF];
+#pragma unroll
+  for (int i = 0; i < LEAF; ++i) {
+    T acc = (i == lane) ? T(1) : T(0);
+#pragma unroll
+    for (int k = 0; k < i; ++k) acc = fma(-bcast(x[k], i), y[k], acc);
+    y[i] = i < lane ? T(0) : acc / bcast(x[i], i);
+  }
+  if (lane < n) {
+#pragma unroll
+    for (int i = 0; i < LEAF; ++i) {
+      if (i < n) {
+        if (uplo == Lower) Tinv[i + (long)lane * ldinv] = y[i];
+        else Tinv[lane + (long)i * ldinv] = y[i];
+      }
     }
   }
 }
@@ -141,9 +151,6 @@ void potrf_leaf(hipStream_t s, int uplo, int n, T* A, long lda, long sA, T* Tinv
   hipLaunchKernelGGL((potrf_leaf_kernel<T>), dim3(batch), dim3(64), 0, s, uplo, n, A, lda, sA, Tinv, ldinv, sInv,
                      info, base);
   XK_HIP(hipGetLastError());
-  // inverse of the fresh factor (a separate launch keeps both kernels
-  // within the register file)
-  if (Tinv) trtri_leaf<T>(s, uplo, NonUnit, n, A, lda, sA, Tinv, ldinv, sInv, batch);
 }
 
 template <typename T>
