@@ -964,6 +964,114 @@ __global__ __launch_bounds__(256) void mod_gemm_fp8_kernel(ModGemmParams prm)
                     static_cast<int8_t*>(prm.C) + (long)l * prm.sC + n0 * prm.ldc + m0, prm.ldc);
 }
 
+// ---- FP8, 8 waves (2 per SIMD) of 128 x 64: the structure of
+// mod_gemm_fp8_kernel with the wave layout of mod_gemm_w8_kernel.
+__global__ __launch_bounds__(512) void mod_gemm_fp8_w8_kernel(ModGemmParams prm)
+{
+  __shared__ __attribute__((aligned(1024))) uint8_t S[F8_STAGES][(DM_BM + DM_BN) * F8_BK];
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave & 1, wn = wave >> 1;  // wn = 0..3
+  const int l = blockIdx.z;
+  const float p = prm.p[l], ip = prm.invp[l];
+
+  int tm, tn;
+  dm_tile(prm, tm, tn);
+  const long m0 = (long)tm * DM_BM, n0 = (long)tn * DM_BN;
+  const uint8_t* A = static_cast<const uint8_t*>(prm.A) + (long)l * prm.sA + m0 * F8_BK;
+  const uint8_t* B = static_cast<const uint8_t*>(prm.B) + (long)l * prm.sB + n0 * F8_BK;
+
+  const int srow = lane >> 2;
+  const int schunk = (lane & 3) ^ ((srow >> 2) & 3);
+  const uint8_t* gA = A + (wave * 32 + srow) * F8_BK + schunk * 16;
+  const uint8_t* gB = B + (wave * 32 + srow) * F8_BK + schunk * 16;
+  auto issue = [&](int stage, int kt) {
+    uint8_t* base = S[stage] + wave * 2048;
+    const uint8_t* a = gA + (long)kt * prm.blkA;
+    const uint8_t* b = gB + (long)kt * prm.blkB;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+      __builtin_amdgcn_global_load_lds((const void*)(a + i * 16 * F8_BK), (lds_void*)(base + i * 1024), 16, 0, 0);
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+      __builtin_amdgcn_global_load_lds((const void*)(b + i * 16 * F8_BK),
+                                       (lds_void*)(base + DM_BM * F8_BK + i * 1024), 16, 0, 0);
+  };
+
+  f32x16 acc[4][2];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
+
+  const int nk = prm.kpad / F8_BK;  // a multiple of 8
+  for (int s = 0; s < F8_STAGES - 1 && s < nk; ++s) issue(s, s);
+
+  const int fr = lane & 31, fh = lane >> 5;
+  const int sw = (fr >> 2) & 3;
+  const int rdA0 = (wm * 128 + fr) * F8_BK + ((2 * fh) ^ sw) * 16;
+  const int rdA1 = (wm * 128 + fr) * F8_BK + ((2 * fh + 1) ^ sw) * 16;
+  const int rdB0 = DM_BM * F8_BK + (wn * 64 + fr) * F8_BK + ((2 * fh) ^ sw) * 16;
+  const int rdB1 = DM_BM * F8_BK + (wn * 64 + fr) * F8_BK + ((2 * fh + 1) ^ sw) * 16;
+  auto readA = [&](int i, const uint8_t* st) { return lds_read32(st, rdA0 + i * 32 * F8_BK, rdA1 + i * 32 * F8_BK); };
+  auto readB = [&](int j, const uint8_t* st) { return lds_read32(st, rdB0 + j * 32 * F8_BK, rdB1 + j * 32 * F8_BK); };
+
+  // as in mod_gemm_fp8_kernel: a[0], a[1], b hold stage kt on entry; a[2],
+  // a[3] are read under the MFMAs of rows 0, 1; after the barrier a[0], a[1]
+  // and bn of stage kt+1 are read under the MFMAs of rows 2, 3
+  i32x8 a[4], b0[2], b1[2];
+  auto stage = [&](int kt, int u, i32x8 (&b)[2], i32x8 (&bn)[2]) {
+    const uint8_t* st = S[u];
+    wait_lgkm0();
+    a[2] = readA(2, st);
+    a[3] = readA(3, st);
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) acc[i][j] = mfma_fp8(a[i], b[j], acc[i][j]);
+    wait_lgkm0();
+    if (kt + 1 < nk) {
+      wait_vmcnt_le(4 * (min(nk - 1, kt + F8_STAGES - 2) - (kt + 1)));
+      __builtin_amdgcn_s_barrier();
+      if (kt + F8_STAGES - 1 < nk) issue((u + F8_STAGES - 1) % F8_STAGES, kt + F8_STAGES - 1);
+    }
+    const uint8_t* sn = S[(u + 1) % F8_STAGES];
+#pragma unroll
+    for (int i = 2; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) acc[i][j] = mfma_fp8(a[i], b[j], acc[i][j]);
+    a[0] = readA(0, sn);
+    a[1] = readA(1, sn);
+#pragma unroll
+    for (int j = 0; j < 2; ++j) bn[j] = readB(j, sn);
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+      __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+      __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);
+    }
+  };
+
+  wait_vmcnt_le(4 * (min(nk, F8_STAGES - 1) - 1));
+  __builtin_amdgcn_s_barrier();
+  a[0] = readA(0, S[0]);
+  a[1] = readA(1, S[0]);
+#pragma unroll
+  for (int j = 0; j < 2; ++j) b0[j] = readB(j, S[0]);
+  for (int kt = 0; kt < nk; kt += 4) {
+    stage(kt, 0, b0, b1);
+    stage(kt + 1, 1, b1, b0);
+    stage(kt + 2, 2, b0, b1);
+    stage(kt + 3, 3, b1, b0);
+  }
+
+  wait_vmcnt_le(0);
+  __syncthreads();
+  w8_store_residues(acc, p, ip, reinterpret_cast<char*>(&S[0][0]),
+                    static_cast<int8_t*>(prm.C) + (long)l * prm.sC + n0 * prm.ldc + m0, prm.ldc);
+}
+
 struct RecParams {
   const void* R;  // residue planes (int16 BF16, int8 FP8)
   long ldr, sR;
@@ -1091,7 +1199,8 @@ static inline long rup(long v, long q) { return (v + q - 1) / q * q; }
 
 enum { WS_SA = 8, WS_SB = 9, WS_SC = 10, WS_SE = 11 };
 
-// BF16 residue GEMM: 0 = 8 waves of 128 x 64, 1 = 4 waves of 128 x 128
+// residue GEMMs (BF16 and FP8): 0 = 8 waves of 128 x 64, 1 = 4 waves of
+// 128 x 128
 static int g_mod_gemm_variant = getenv("XKBLAS_SPLIT_KERNEL") ? atoi(getenv("XKBLAS_SPLIT_KERNEL")) : 0;
 void split_set_kernel(int v) { g_mod_gemm_variant = v; }
 int split_get_kernel() { return g_mod_gemm_variant; }
@@ -1208,7 +1317,9 @@ static void split_gemm_chunk(const Exec& x, int transA, int transB, int m, int n
   g.tiles_m = (int)(mpad / DM_BM);
   g.tiles_n = (int)(npad / DM_BN);
   const long tiles = tri == TRI_FULL ? (long)g.tiles_m * g.tiles_n : (long)g.tiles_m * (g.tiles_m + 1) / 2;
-  if (fp8) hipLaunchKernelGGL(mod_gemm_fp8_kernel, dim3((unsigned)tiles, 1, np), dim3(256), 0, s, g);
+  if (fp8 && g_mod_gemm_variant == 1)
+    hipLaunchKernelGGL(mod_gemm_fp8_kernel, dim3((unsigned)tiles, 1, np), dim3(256), 0, s, g);
+  else if (fp8) hipLaunchKernelGGL(mod_gemm_fp8_w8_kernel, dim3((unsigned)tiles, 1, np), dim3(512), 0, s, g);
   else if (g_mod_gemm_variant == 1)
     hipLaunchKernelGGL(mod_gemm_dma_kernel, dim3((unsigned)tiles, 1, np), dim3(256), 0, s, g);
   else hipLaunchKernelGGL(mod_gemm_w8_kernel, dim3((unsigned)tiles, 1, np), dim3(512), 0, s, g);
