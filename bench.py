@@ -31,8 +31,8 @@ BF16_PEAK = 2500.0
 FP8_PEAK = 5000.0
 # HBM bytes per launch of the BF16 residue GEMM at 16384^3 (13 planes):
 # rocprofv3 FETCH_SIZE x 2 + WRITE_SIZE x 1 (factors from a 1 GiB device copy),
-# profiles/split_dgemm16384_hbm_traffic.md; None until measured for this kernel
-BF16_GEMM_TRAFFIC_16384 = None
+# profiles/split_dgemm16384_hbm_traffic.md
+BF16_GEMM_TRAFFIC_16384 = 167.5e9 + 6.98e9
 
 
 def make_inputs(n, seed):
@@ -194,6 +194,11 @@ def main():
                     help="DPOTRF size for the split-BF16 vs FP64 comparison (0: skip)")
     ap.add_argument("--only-split", type=int, default=0, metavar="MODE",
                     help="profile helper: two split DGEMMs in math mode MODE (2 BF16, 3 FP8), no JSON")
+    ap.add_argument("--only-potrf", type=int, default=-1, metavar="MODE",
+                    help="profile helper: DPOTRF of order --potrf-split-n (block order --potrf-nb) in math mode "
+                         "MODE, twice, no JSON")
+    ap.add_argument("--potrf-nb", type=int, default=2048)
+    ap.add_argument("--only-batched", action="store_true", help="profile helper: the batched leg alone, no JSON")
     ap.add_argument("--copy-probe", action="store_true",
                     help="profile helper: one 1 GiB device-to-device copy (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -212,6 +217,15 @@ def main():
         raise RuntimeError("no GPU visible")
     L.xkblas_init()
     L.xkblas_profile_enable(1)
+    if args.only_potrf >= 0:
+        S, _ = make_spd(xk, args.potrf_split_n, 1024)
+        print(json.dumps(bench_potrf(xk, S, args.potrf_nb, args.only_potrf)[0]), file=sys.stderr)
+        L.xkblas_finalize()
+        return
+    if args.only_batched:
+        print(json.dumps(bench_batched(xk)), file=sys.stderr)
+        L.xkblas_finalize()
+        return
     n = args.n
     print("bench: generating inputs n=%d" % n, file=sys.stderr, flush=True)
     A, B, C = make_inputs(n, 2026)
