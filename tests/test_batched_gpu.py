@@ -137,3 +137,35 @@ def test_batch_edge_sizes(xk, orc):
     L.xkblas_batch_destroy(plan)
     for d in keep:
         d.free()
+
+
+def test_batch_trsm_edge_sizes(xk, orc):
+    """TRSM block boundaries (m = 1, 16, 17, 63, 64, 65, 128, 129, 200) and
+    panel boundaries (n = 1, 64, 65), FP64 and FP32, alpha in {-2, 0, 1}."""
+    L = xk.lib()
+    r = rng(6)
+    probs, keep, checks = [], [], []
+    cases = [(1, 1), (16, 64), (17, 65), (63, 3), (64, 64), (65, 1), (128, 65), (129, 64), (200, 130)]
+    for t, (m, n) in enumerate(cases):
+        for dt, code in ((np.float64, 0), (np.float32, 1)):
+            alpha = (-2.0, 0.0, 1.0)[t % 3]
+            Lm = np.tril(r.uniform(-1, 1, (m, m)))
+            np.fill_diagonal(Lm, 1.0 + np.abs(Lm).sum(axis=1))
+            Lm = np.asfortranarray(Lm.astype(dt))
+            B = np.asfortranarray(r.uniform(-1, 1, (m, n)).astype(dt))
+            dL, dB = xk.DeviceArray(Lm), xk.DeviceArray(B)
+            probs.append(xk.BatchProblem(xk.BATCH_TRSM, code, 111, 111, m, n, m, alpha, 0.0, dL.ptr, m, None, 0,
+                                         dB.ptr, m))
+            keep += [dL, dB]
+            ref = alpha * np.linalg.solve(Lm.astype(np.float64), B.astype(np.float64))
+            checks.append((dB, ref, 1e-13 if dt == np.float64 else 2e-5))
+    arr = (xk.BatchProblem * len(probs))(*probs)
+    plan = L.xkblas_batch_create(ctypes.cast(arr, ctypes.c_void_p), len(arr))
+    assert L.xkblas_batch_run(plan) == 0
+    L.xkblas_sync()
+    for d, ref, tol in checks:
+        got = d.download().astype(np.float64)
+        assert np.abs(got - ref).max() <= tol * max(1.0, np.abs(ref).max())
+    L.xkblas_batch_destroy(plan)
+    for d in keep:
+        d.free()

@@ -13,11 +13,12 @@
 // GEMM tile: 4 waves, 2 x 2 of 32 x 32 quarters as 16 x 16 MFMA blocks
 // (v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32), 16-deep k slices through
 // LDS, edges zero-filled.
-// TRSM panel (Left, Lower, NoTrans, Non-unit; B m x ncols with ncols <= 64):
-// B is scaled by alpha, then for each 64-row block the diagonal block and the
-// panel rows go to LDS, lane c solves column c by forward substitution, the
-// solved rows are written back and the rows below are updated by the GEMM
-// tile routine (C -= L_rb,kb X_kb), all inside the workgroup.
+// TRSM panel (Left, Lower, NoTrans, Non-unit; B m x ncols with ncols <= 64),
+// left-looking over 64-row blocks: one MFMA tile forms alpha B_kb minus the
+// solved blocks' contribution straight into LDS, four 16-row sub-blocks are
+// solved with the solutions in registers (1 / L_ii precomputed), and the
+// block is written back for the next one, all inside the workgroup.
+// Shared memory is 40 KB per workgroup, so four fit on a CU.
 #include "common.h"
 #include "kernels.h"
 #include "runtime.h"
@@ -59,20 +60,39 @@ template <> struct BAcc<float> { typedef bf4 type; };
 
 constexpr int TB_BM = 64, TB_BK = 16, TB_LD = TB_BM + 2;
 
-// C tile (tm, tn) of the 64 x 64 tiling of C <- alpha op(A) op(B) + beta C.
-// All 256 threads participate; smem holds 2 x 2 x 16 x 66 elements of T.
+// Shared memory of one workgroup, in doubles: 40 KB, so that four
+// workgroups fit in the 160 KB of a CU.  Overlaid uses:
+//   GEMM staging  2 buffers x (A, B) x 16 x 66            4224
+//   TRSM          X block 64 x 64 (swizzled) + L strip 64 x 16   5120
+//   item index    one int at the start, between items
+constexpr int BATCH_SMEM = 5120;
+
+// Thread index hidden from loop-invariant code motion: recomputing the
+// swizzled LDS offsets of an item is cheaper than keeping them live (and
+// spilled) across the whole persistent loop.
+__device__ __forceinline__ int item_tid()
+{
+  int t = threadIdx.x;
+  __asm__ volatile("" : "+v"(t));
+  return t;
+}
+
+// Main loop of the 64 x 64 tile (tm, tn) of alpha op(A) op(B): 4 waves as
+// 2 x 2 quarters of 32 x 32, each 2 x 2 MFMA blocks of 16 x 16; 16-deep k
+// slices double-buffered in LDS, the next slice prefetched into registers
+// while the current one is multiplied.  Ends with a barrier after the last
+// LDS read, so the caller may overwrite `smem` at once.
 template <typename T>
-__device__ void gemm_tile64(int ta, int tb, int M, int N, int K, T alpha, const T* __restrict__ A, long lda,
-                            const T* __restrict__ B, long ldb, T beta, T* __restrict__ C, long ldc, int tm, int tn,
-                            T* smem)
+__device__ __forceinline__ void tile64_mainloop(int ta, int tb, int M, int N, int K, const T* __restrict__ A, long lda,
+                                                const T* __restrict__ B, long ldb, int m0, int n0, T* smem,
+                                                typename BAcc<T>::type (&acc)[2][2])
 {
   T(*As)[TB_BK][TB_LD] = reinterpret_cast<T(*)[TB_BK][TB_LD]>(smem);
   T(*Bs)[TB_BK][TB_LD] = reinterpret_cast<T(*)[TB_BK][TB_LD]>(smem + 2 * TB_BK * TB_LD);
   typedef typename BAcc<T>::type acc_t;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tid = item_tid(), lane = tid & 63, wave = tid >> 6;
   const int wm = wave & 1, wn = wave >> 1;
-  const int m0 = tm * TB_BM, n0 = tn * TB_BM;
-  // each thread moves 4 elements of each 64 x 16 slice
+  // each thread moves 4 elements of each 64 x 16 slice of op(A) and op(B)
   T ra[4], rb[4];
   auto load = [&](int k0) {
 #pragma unroll
@@ -106,7 +126,6 @@ __device__ void gemm_tile64(int ta, int tb, int M, int N, int K, T alpha, const 
       Bs[buf][kb][j] = rb[v];
     }
   };
-  acc_t acc[2][2];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -138,14 +157,36 @@ __device__ void gemm_tile64(int ta, int tb, int M, int N, int K, T alpha, const 
       buf ^= 1;
     }
   }
+}
+
+// Tile element of accumulator register r of block (i, j) held by `lane`:
+// row (of op(A) / C) and column (of op(B) / C) inside the 64 x 64 tile.
+template <typename T>
+__device__ __forceinline__ int tile_row(int wave, int lane, int i) { return (wave & 1) * 32 + i * 16 + (lane & 15); }
+template <typename T>
+__device__ __forceinline__ int tile_col(int wave, int lane, int j, int r)
+{
+  return (wave >> 1) * 32 + j * 16 + (sizeof(T) == 8 ? (lane >> 4) + 4 * r : (lane >> 4) * 4 + r);
+}
+
+// C tile (tm, tn) of the 64 x 64 tiling of C <- alpha op(A) op(B) + beta C.
+template <typename T>
+__device__ void gemm_tile64(int ta, int tb, int M, int N, int K, T alpha, const T* __restrict__ A, long lda,
+                            const T* __restrict__ B, long ldb, T beta, T* __restrict__ C, long ldc, int tm, int tn,
+                            T* smem)
+{
+  typename BAcc<T>::type acc[2][2];
+  const int m0 = tm * TB_BM, n0 = tn * TB_BM;
+  tile64_mainloop<T>(ta, tb, M, N, K, A, lda, B, ldb, m0, n0, smem, acc);
+  const int tid = item_tid(), lane = tid & 63, wave = tid >> 6;
 #pragma unroll
   for (int i = 0; i < 2; ++i) {
-    const int gm = m0 + wm * 32 + i * 16 + lr;
+    const int gm = m0 + tile_row<T>(wave, lane, i);
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int gn = n0 + wn * 32 + j * 16 + (sizeof(T) == 8 ? lk + 4 * r : lk * 4 + r);
+        const int gn = n0 + tile_col<T>(wave, lane, j, r);
         if (gm < M && gn < N) {
           T* c = C + gm + (long)gn * ldc;
           T v = alpha * acc[i][j][r];
@@ -155,10 +196,20 @@ __device__ void gemm_tile64(int ta, int tb, int M, int N, int K, T alpha, const 
       }
     }
   }
-  __syncthreads();  // smem reused by the caller's next item
 }
 
-// One 64-column panel of a lower-triangular solve L X = alpha B, in place.
+// One 64-column panel of a lower-triangular solve L X = alpha B, in place,
+// left-looking over 64-row blocks kb:
+//   X_kb <- alpha B_kb - L[kb, 0:kb] X[0:kb]   one MFMA tile, k = 64 kb, into LDS
+//   X_kb <- L_kk^-1 X_kb                       in LDS, 16-row sub-blocks
+//   X_kb -> global                             (read back by later blocks)
+// The X block lives in LDS as Xs[i * 64 + (c ^ i)] (row i, column c): rows
+// and columns are both conflict-free permutations of the banks.  L_kk's lower
+// triangle is read into registers before the block's GEMM and handed to LDS
+// one 16-column strip at a time: Ld[i * 16 + (q ^ (i & 15))] = L(i, 16 s + q),
+// with 1 / L(i, i) in place of the diagonal.  Per strip s, wave 0 (lane =
+// column) solves rows 16 s .. 16 s + 15 with the 16 solutions in registers,
+// then all four waves subtract their contribution from the rows below.
 template <typename T>
 __device__ void trsm_panel(const BatchProblem& p, int panel, T* smem)
 {
@@ -168,70 +219,102 @@ __device__ void trsm_panel(const BatchProblem& p, int panel, T* smem)
   const T* L = reinterpret_cast<const T*>(p.A);
   T* Bp = reinterpret_cast<T*>(p.C) + (long)c0 * p.ldc;
   const long ldl = p.lda, ldb = p.ldc;
-  const int tid = threadIdx.x;
   const T alpha = (T)p.alpha;
-  if (alpha != T(1)) {
-    for (long e = tid; e < (long)m * nc; e += 256) {
-      const int i = (int)(e % m), c = (int)(e / m);
-      Bp[i + (long)c * ldb] *= alpha;
-    }
-    __syncthreads();
-  }
-  constexpr int LD = 65;
-  T* Ls = smem;           // 64 x 65
-  T* Xs = smem + 64 * LD; // 64 x 65 (row i, column c at Xs[i * LD + c])
+  T* Xs = smem;
+  T* Ld = smem + 64 * 64;
   const int nb = (m + 63) / 64;
   for (int kb = 0; kb < nb; ++kb) {
+    const int tid = item_tid(), lane = tid & 63, wave = tid >> 6;
     const int r0 = kb * 64, h = min(64, m - r0);
-    for (int e = tid; e < 64 * 64; e += 256) {
-      const int i = e & 63, j = e >> 6;
-      T l = T(0);
-      if (i < h && j < h) l = (j <= i) ? L[(r0 + i) + (long)(r0 + j) * ldl] : T(0);
-      else if (i == j) l = T(1);
-      Ls[i + j * LD] = l;
-      // X block: row i, column j of the panel
-      Xs[i * LD + j] = (i < h && j < nc) ? Bp[(r0 + i) + (long)j * ldb] : T(0);
+    // L_kk(lane, 4 v + wave), lower triangle; padding rows get a unit diagonal
+    T lreg[16];
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+      const int q = 4 * v + wave;
+      T x = T(0);
+      if (lane < h && q < lane) x = L[(r0 + lane) + (long)(r0 + q) * ldl];
+      else if (q == lane) x = lane < h ? L[(r0 + lane) + (long)(r0 + q) * ldl] : T(1);
+      lreg[v] = x;
     }
-    __syncthreads();
-    if (tid < 64) {
-      const int c = tid;
-      for (int i = 0; i < 64; ++i) {
-        T x = Xs[i * LD + c];
-        for (int j = 0; j < i; ++j) x -= Ls[i + j * LD] * Xs[j * LD + c];
-        Xs[i * LD + c] = x / Ls[i + i * LD];
+    typename BAcc<T>::type acc[2][2];
+    tile64_mainloop<T>(0, 0, h, nc, r0, L + r0, ldl, Bp, ldb, 0, 0, smem, acc);
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      const int row = tile_row<T>(wave, lane, i);
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int col = tile_col<T>(wave, lane, j, r);
+          T v = T(0);
+          if (row < h && col < nc) v = alpha * Bp[(r0 + row) + (long)col * ldb] - acc[i][j][r];
+          Xs[row * 64 + (col ^ row)] = v;
+        }
       }
     }
-    __syncthreads();
-    for (int e = tid; e < 64 * 64; e += 256) {
-      const int i = e & 63, j = e >> 6;
-      if (i < h && j < nc) Bp[(r0 + i) + (long)j * ldb] = Xs[i * LD + j];
-    }
-    __threadfence_block();
-    __syncthreads();
-    // rows below: B[r0+h:, :] -= L[r0+h:, r0:r0+h] X_kb
-    const int rest = m - r0 - h;
-    if (rest > 0) {
-      const int tiles = (rest + 63) / 64;
-      for (int t = 0; t < tiles; ++t)
-        gemm_tile64<T>(0, 0, rest, nc, h, T(-1), L + (r0 + h) + (long)r0 * ldl, ldl, Bp + r0, ldb, T(1),
-                       Bp + r0 + h, ldb, t, 0, smem + 2 * 64 * LD);
-      __threadfence_block();
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      if (16 * s >= h) break;
+      // strip s of L_kk: this thread holds columns q = 4 v + wave, v = 4 s .. 4 s + 3
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int q = 4 * u + wave;  // column inside the strip
+        T x = lreg[4 * s + u];
+        if (16 * s + q == lane) x = T(1) / x;
+        Ld[lane * 16 + (q ^ (lane & 15))] = x;
+      }
+      __syncthreads();
+      if (wave == 0) {
+        const int c = lane;
+        T x[16];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) x[r] = Xs[(16 * s + r) * 64 + (c ^ (16 * s + r))];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const T* lrow = Ld + (16 * s + r) * 16;
+          T v = x[r];
+#pragma unroll
+          for (int q = 0; q < r; ++q) v -= lrow[q ^ r] * x[q];
+          x[r] = v * lrow[0];  // lrow[r ^ r] = 1 / L(r, r)
+        }
+#pragma unroll
+        for (int r = 0; r < 16; ++r) Xs[(16 * s + r) * 64 + (c ^ (16 * s + r))] = x[r];
+      }
+      __syncthreads();
+      if (16 * (s + 1) < h) {
+        const int c = lane;
+        T xq[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) xq[q] = Xs[(16 * s + q) * 64 + (c ^ (16 * s + q))];
+        for (int i = 16 * (s + 1) + wave; i < h; i += 4) {
+          const T* lrow = Ld + i * 16;
+          T v = Xs[i * 64 + (c ^ i)];
+#pragma unroll
+          for (int q = 0; q < 16; ++q) v -= lrow[q ^ (i & 15)] * xq[q];
+          Xs[i * 64 + (c ^ i)] = v;
+        }
+      }
       __syncthreads();
     }
+    // X_kb back to global memory, coalesced along rows
+#pragma unroll 4
+    for (int v = 0; v < 16; ++v) {
+      const int i = lane, c = wave + 4 * v;
+      if (i < h && c < nc) Bp[(r0 + i) + (long)c * ldb] = Xs[i * 64 + (c ^ i)];
+    }
+    __syncthreads();  // X_kb visible to the workgroup's next block; Xs free
   }
 }
 
-__global__ __launch_bounds__(256) void batch_kernel(const BatchProblem* __restrict__ probs,
-                                                    const BatchItem* __restrict__ items, int nitems, int* counter)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void batch_kernel(
+    const BatchProblem* __restrict__ probs, const BatchItem* __restrict__ items, int nitems, int* counter)
 {
-  // union of the TRSM panel buffers (2 x 64 x 65) and one GEMM tile's
-  // staging (4 x 16 x 66), sized for FP64
-  __shared__ double smem_d[2 * 64 * 65 + 4 * TB_BK * TB_LD];
-  __shared__ int next;
+  __shared__ double smem_d[BATCH_SMEM];
+  int* next = reinterpret_cast<int*>(smem_d);
   for (;;) {
-    if (threadIdx.x == 0) next = atomicAdd(counter, 1);
+    if (threadIdx.x == 0) *next = atomicAdd(counter, 1);
     __syncthreads();
-    const int it = next;
+    const int it = __builtin_amdgcn_readfirstlane(*next);
     __syncthreads();
     if (it >= nitems) return;
     const BatchItem w = items[it];
@@ -250,6 +333,7 @@ __global__ __launch_bounds__(256) void batch_kernel(const BatchProblem* __restri
       if (p.dtype == 0) trsm_panel<double>(p, w.tn, smem_d);
       else trsm_panel<float>(p, w.tn, reinterpret_cast<float*>(smem_d));
     }
+    __syncthreads();  // smem free for the next item index
   }
 }
 
@@ -318,8 +402,10 @@ void* xkblas_batch_create(const xkblas_batch_problem_t* probs, int count)
       for (int b = 0; b < np; ++b) {
         Cost c;
         c.w = BatchItem{i, 0, b, 0};
-        // substitution chain (m/64 blocks of 64 dependent rows) + updates
-        c.c = 2.0 * q.m * (double)q.m * 64 * 3.0 + q.m * 4096.0;
+        // left-looking blocks: one tile of k = 64 kb each plus the block's
+        // solve, priced like a tile of k = 128
+        const double nbk = (q.m + 63) / 64;
+        c.c = 2.0 * 64 * 64 * 64 * nbk * (nbk - 1) / 2 + nbk * 2.0 * 64 * 64 * 128;
         work.push_back(c);
       }
       pl->flops += (double)q.m * q.m * q.n;
