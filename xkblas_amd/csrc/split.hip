@@ -1075,7 +1075,7 @@ __global__ __launch_bounds__(512) void mod_gemm_fp8_w8_kernel(ModGemmParams prm)
 struct RecParams {
   const void* R;  // residue planes (int16 BF16, int8 FP8)
   long ldr, sR;
-  int nmod, m, n, tri;
+  int nmod, nplane, m, n, tri;
   unsigned kara;  // bit l: modulus l is a Karatsuba digit triple (3 planes)
   const int* shA;
   const int* shB;
@@ -1117,45 +1117,69 @@ __device__ __forceinline__ double crt_one(const RecParams& rp, double S0, double
   return hi + lo;
 }
 
-// residues of 4 consecutive rows of one plane column
-__device__ __forceinline__ void load4(const int8_t* r, double (&v)[4])
+// residues of 4 consecutive rows of one plane column: int16 planes (BF16
+// products) as one 8-byte word, int8 planes (FP8 products) as one 4-byte word
+template <typename Res> struct ResWord;
+template <> struct ResWord<int16_t> {
+  typedef uint2 type;
+  static constexpr int maxplane = BF16_MAXMOD;
+};
+template <> struct ResWord<int8_t> {
+  typedef unsigned type;
+  static constexpr int maxplane = MAXPLANE;
+};
+
+__device__ __forceinline__ void unpack4(unsigned w, double (&v)[4])
 {
-  const unsigned w = *reinterpret_cast<const unsigned*>(r);
 #pragma unroll
   for (int e = 0; e < 4; ++e) v[e] = (double)(int8_t)(w >> (8 * e));
 }
 
-__device__ __forceinline__ void load4(const int16_t* r, double (&v)[4])
+__device__ __forceinline__ void unpack4(uint2 w, double (&v)[4])
 {
-  const uint2 w = *reinterpret_cast<const uint2*>(r);
   v[0] = (double)(int16_t)(w.x & 0xffffu);
   v[1] = (double)(int16_t)(w.x >> 16);
   v[2] = (double)(int16_t)(w.y & 0xffffu);
   v[3] = (double)(int16_t)(w.y >> 16);
 }
 
+// One thread: 4 consecutive rows of one column of C.  Every plane word of
+// the thread is requested before the first is used (the planes are the
+// kernel's whole input, so this is what keeps enough bytes in flight).
+// Plane order: modulus l owns plane l, except FP8 Karatsuba moduli
+// (l >= FP8_DIRECT), which own planes FP8_DIRECT + 3 (l - FP8_DIRECT) + 0..2.
 template <typename Res>
 __global__ __launch_bounds__(256) void reconstruct_kernel(RecParams rp)
 {
+  typedef typename ResWord<Res>::type Word;
+  constexpr bool F8 = sizeof(Res) == 1;
+  constexpr int NPL = ResWord<Res>::maxplane;
+  constexpr int NMOD = F8 ? MAXMOD : BF16_MAXMOD;
   const int i0 = 4 * (blockIdx.x * blockDim.x + threadIdx.x);
   const int j = blockIdx.y;
   if (i0 >= rp.m || j >= rp.n) return;
   if (rp.tri == TRI_LOWER && i0 + 3 < j) return;
   if (rp.tri == TRI_UPPER && i0 > j) return;
-  double S[4][3] = {};
   const Res* r = static_cast<const Res*>(rp.R) + (long)j * rp.ldr + i0;
-  int q = 0;  // plane of modulus l
-  for (int l = 0; l < rp.nmod; ++l) {
+  // (planes beyond the count in use re-read the last one: no branches, so
+  // the words stay in registers)
+  Word w[NPL];
+#pragma unroll
+  for (int q = 0; q < NPL; ++q) w[q] = *reinterpret_cast<const Word*>(r + (long)min(q, rp.nplane - 1) * rp.sR);
+  double S[4][3] = {};
+#pragma unroll
+  for (int l = 0; l < NMOD; ++l) {
     double v[4];
-    load4(r + (long)q * rp.sR, v);
-    if (!((rp.kara >> l) & 1u)) {
-      q += 1;
+    if (!F8 || l < FP8_DIRECT) {
+      unpack4(w[l], v);
     } else {
       // planes h_a h_b, l_a l_b, s_a s_b (each mod p):
       // r = 256 hh + 16 (ss - hh - ll) + ll, |r| < 2^16 before the reduction
+      const int q = FP8_DIRECT + 3 * (l - FP8_DIRECT);
       double vl[4], vs[4];
-      load4(r + (long)(q + 1) * rp.sR, vl);
-      load4(r + (long)(q + 2) * rp.sR, vs);
+      unpack4(w[q], v);
+      unpack4(w[q + 1], vl);
+      unpack4(w[q + 2], vs);
       const float p = rp.p[l], ip = rp.invp[l];
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
@@ -1163,13 +1187,15 @@ __global__ __launch_bounds__(256) void reconstruct_kernel(RecParams rp)
         const float x = fmaf(256.0f, hh, fmaf(16.0f, ss - hh - ll, ll));
         v[e] = (double)fmaf(-p, rintf(x * ip), x);
       }
-      q += 3;
     }
+    // moduli beyond the count in use are skipped (their W is not set)
+    if (l < rp.nmod) {
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      S[e][0] = fma(v[e], rp.W[l][0], S[e][0]);
-      S[e][1] = fma(v[e], rp.W[l][1], S[e][1]);
-      S[e][2] = fma(v[e], rp.W[l][2], S[e][2]);
+      for (int e = 0; e < 4; ++e) {
+        S[e][0] = fma(v[e], rp.W[l][0], S[e][0]);
+        S[e][1] = fma(v[e], rp.W[l][1], S[e][1]);
+        S[e][2] = fma(v[e], rp.W[l][2], S[e][2]);
+      }
     }
   }
   const int sb = rp.shB[j];
@@ -1328,8 +1354,9 @@ static void split_gemm_chunk(const Exec& x, int transA, int transB, int m, int n
 
   // 5: CRT recombination + alpha/beta in FP64
   R.prof_begin("split_crt", 0.0);
-  RecParams rp;
-  rp.R = RC; rp.ldr = mpad; rp.sR = mpad * npad; rp.nmod = nmod; rp.m = m; rp.n = n; rp.tri = tri;
+  RecParams rp = {};
+  rp.R = RC; rp.ldr = mpad; rp.sR = mpad * npad; rp.nmod = nmod; rp.nplane = np; rp.m = m; rp.n = n;
+  rp.tri = tri;
   rp.kara = M.kara;
   rp.shA = shA; rp.shB = same ? shA : shB; rp.C = C; rp.ldc = ldc; rp.alpha = alpha; rp.beta = beta;
   for (int l = 0; l < nmod; ++l) {
