@@ -89,6 +89,12 @@ int  xkblas_split_side_bits(int k, int nsplit);  /* split-BF16 bits per side t *
  * k and requested count (0 = automatic): returns the bits per side t, stores
  * the number of CRT moduli and of residue GEMMs ("planes"); -1 for another mode */
 int  xkblas_split_info(int mode, int k, int nsplit, int* nmod, int* nplane);
+/* DPOTRF/SPOTRF trailing updates: 1 = paired (default; -1 selects it too):
+ * every other step applies two panels as one rank-2NB update of the trailing
+ * triangle, half at once and half at the next step, while the next block
+ * column is still updated at once for the lookahead; 0 = one rank-NB update
+ * per panel.  Environment XKBLAS_POTRF_PAIR sets the initial value. */
+void xkblas_set_potrf_pairing(int mode);
 
 /* device memory helpers for batched operands */
 void* xkblas_device_malloc(size_t bytes);

@@ -167,6 +167,41 @@ def test_dpotrf_vs_oracle(xk, orc, uplo, n_nb):
     assert np.array_equal(A[other > 0], A0[other > 0])
 
 
+@pytest.mark.parametrize("uplo", [121, 122])
+@pytest.mark.parametrize("pairing", [0, 1])
+@pytest.mark.parametrize("n_nb", [(1000, 128), (1100, 256), (640, 128), (513, 64)])
+def test_dpotrf_paired_updates(xk, orc, uplo, pairing, n_nb):
+    """Paired trailing updates (two panels per rank-2NB update) and the
+    per-panel schedule, forced on and off in FP64, against the oracle: even
+    and odd block counts, ragged last blocks, the info of a failing order."""
+    n, nb = n_nb
+    L = xk.lib()
+    r = rng(n + nb)
+    A0 = spd(r, n)
+    Lref = A0.copy(order="F")
+    assert orc.dpotrf_tiled(uplo, Lref, nb) == 0
+    A = A0.copy(order="F")
+    L.xkblas_set_param(nb, 0)
+    L.xkblas_set_potrf_pairing(pairing)
+    try:
+        info = xk.potrf(uplo, A)
+        xk.invalidate()
+        assert info == 0
+        assert orc.potrf_residual(A0, A, uplo) <= 10.0
+        tri = np.tril if uplo == 122 else np.triu
+        np.testing.assert_allclose(tri(A), tri(Lref), rtol=0, atol=1e-12 * np.abs(Lref).max())
+        other = np.triu(np.ones((n, n)), 1) if uplo == 122 else np.tril(np.ones((n, n)), -1)
+        assert np.array_equal(A[other > 0], A0[other > 0])
+        bad = n - nb // 2  # inside the last block: reached through every deferred update
+        A = A0.copy(order="F")
+        A[bad, bad] = -1e6
+        info = xk.potrf(uplo, A)
+        xk.invalidate()
+        assert info == bad + 1
+    finally:
+        L.xkblas_set_potrf_pairing(-1)
+
+
 def test_dpotrf_exact_integer_factor(xk):
     """A = L0 L0^T with L0 integer and a power-of-two diagonal: the factor is exact."""
     r = rng(5)

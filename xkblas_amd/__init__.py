@@ -91,6 +91,7 @@ SIGNATURES = {
     "xkblas_tune_release": (None, []),
     "xkblas_set_gemm_tile": (None, [_i]),
     "xkblas_set_split_kernel": (None, [_i]),
+    "xkblas_set_potrf_pairing": (None, [_i]),
     "xkblas_split_side_bits": (_i, [_i, _i]),
     "xkblas_split_info": (_i, [_i, _i, _i, _p, _p]),
     "xkblas_dgemm_native_": (None, [_p] * 13),

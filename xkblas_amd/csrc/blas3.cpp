@@ -24,6 +24,7 @@
 #include "split.h"
 #include <algorithm>
 #include <climits>
+#include <cstdlib>
 
 namespace xk {
 
@@ -143,6 +144,12 @@ static void potrf_diag(hipStream_t s, int uplo, int n, T* A, long lda, T* Tinv, 
 
 enum { WS_DIAG0 = 0, WS_DIAG1 = 1, WS_PANEL = 2, WS_INV = 3, WS_TRSM = 4, WS_TRSMW = 5 };
 
+// Trailing updates of POTRF: 1 (default) paired, 0 one per panel.
+// XKBLAS_POTRF_PAIR sets the initial value.
+static int g_potrf_pair = getenv("XKBLAS_POTRF_PAIR") ? (atoi(getenv("XKBLAS_POTRF_PAIR")) != 0) : 1;
+static int potrf_pairing() { return g_potrf_pair; }
+void potrf_set_pairing(int v) { g_potrf_pair = v < 0 ? 1 : (v ? 1 : 0); }
+
 template <typename T>
 void dev_potrf(const Exec& x, int uplo, int n, T* A, long lda, int nb, int* info)
 {
@@ -156,9 +163,25 @@ void dev_potrf(const Exec& x, int uplo, int n, T* A, long lda, int nb, int* info
   XK_HIP(hipEventCreateWithFlags(&ev_look, hipEventDisableTiming));
   const bool lookahead = x.s_diag != nullptr && nt > 1;
   hipStream_t sd = lookahead ? x.s_diag : x.s;
+  // trailing updates of two panels at once (inner dimension 2 nb): longer
+  // products, and half the split modes' CRT passes over the trailing triangle
+  const bool agg = potrf_pairing() == 1;
 
   auto blk = [&](int i, int j) { return A + (long)i * nb + (long)j * nb * lda; };
   auto tsz = [&](int i) { return i == nt - 1 ? n - i * nb : nb; };
+  // part 2 of a paired trailing update: the triangle of order n at C, operand U
+  struct Pending {
+    const T* U;
+    int n, kw;
+    T* C;
+  };
+  Pending pend{nullptr, 0, 0, nullptr};
+  // columns of part 1 for a trailing triangle of order r: (1 - 1/sqrt 2) r
+  // (equal flops in both parts), whole blocks, at least one block
+  auto pair_split = [&](int r) {
+    const int w = (int)((0.2929 * r) / nb + 0.5) * nb;
+    return std::min(r, std::max(nb, w));
+  };
 
   // factor block 0
   if (lookahead) {
@@ -175,45 +198,76 @@ void dev_potrf(const Exec& x, int uplo, int n, T* A, long lda, int nb, int* info
     }
     if (k == nt - 1) break;
     const int rest = n - (k + 1) * nb;
+    // Paired updates: an even step applies panel k only to block column k+1
+    // (the lookahead); the odd step after it applies panels k-1 and k (adjacent
+    // columns of A: one operand U of inner dimension 2 nb) to block column
+    // k+1 and, in two parts split by columns, to the trailing triangle: part
+    // 1 (the first ~29 % of its columns, flops balanced with part 2) at once,
+    // part 2 at the next step, so that every diagonal factorization on the
+    // side stream has trailing work beside it on the compute stream.
+    const bool pair = agg && (k & 1);
+    const Pending prev = pend;
+    pend.n = 0;
+    const int k1 = tsz(k + 1);
+    const int r2 = rest - k1;
+    const int c0 = pair ? k - 1 : k;
+    const int kw = (k + 1 - c0) * nb;
     T* W = (T*)scratch(WS_PANEL, sizeof(T) * (size_t)rest * kk);
     if (uplo == Lower) {
       T* P = blk(k + 1, k);  // rest x kk panel
       copy2d<T>(x.s, rest, kk, P, lda, W, rest);
       dev_gemm<T>(x, NoTrans, Trans, rest, kk, kk, T(1), W, rest, ik, nb, T(0), P, lda);
+      const T* U = A + (long)(k + 1) * nb + (long)c0 * nb * lda;  // rest x kw
       // lookahead: update the next diagonal block and factor it on the side
-      const int k1 = tsz(k + 1);
-      dev_syrk<T>(x, Lower, NoTrans, k1, kk, T(-1), P, lda, T(1), blk(k + 1, k + 1), lda);
-      if (lookahead) {
-        XK_HIP(hipEventRecord(ev_look, x.s));
-        XK_HIP(hipStreamWaitEvent(sd, ev_look, 0));
-        potrf_diag<T>(sd, uplo, k1, blk(k + 1, k + 1), lda, inv + (size_t)((k + 1) & 1) * nb * nb, nb, info,
-                      (k + 1) * nb, ((k + 1) & 1) ? WS_DIAG1 : WS_DIAG0);
-      }
-      // bulk: rows/cols beyond block k+1.  Columns of block k+1 below its
-      // diagonal block, then the trailing triangle.
-      const int r2 = rest - k1;
+      dev_syrk<T>(x, Lower, NoTrans, k1, kw, T(-1), U, lda, T(1), blk(k + 1, k + 1), lda);
+      if (lookahead) XK_HIP(hipEventRecord(ev_look, x.s));
+      // the rest is enqueued before the diagonal factorization's many small
+      // launches, so the compute stream has work while the host issues them:
+      // block column k+1 below its diagonal block, then the trailing triangle
       if (r2 > 0) {
-        dev_gemm<T>(x, NoTrans, Trans, r2, k1, kk, T(-1), P + k1, lda, P, lda, T(1), blk(k + 2, k + 1), lda);
-        dev_syrk<T>(x, Lower, NoTrans, r2, kk, T(-1), P + k1, lda, T(1), blk(k + 2, k + 2), lda);
+        dev_gemm<T>(x, NoTrans, Trans, r2, k1, kw, T(-1), U + k1, lda, U, lda, T(1), blk(k + 2, k + 1), lda);
+        T* C1 = blk(k + 2, k + 2);
+        if (!agg) {
+          dev_syrk<T>(x, Lower, NoTrans, r2, kw, T(-1), U + k1, lda, T(1), C1, lda);
+        } else if (pair) {
+          const int w1 = pair_split(r2);
+          dev_syrk<T>(x, Lower, NoTrans, w1, kw, T(-1), U + k1, lda, T(1), C1, lda);
+          if (w1 < r2) {
+            dev_gemm<T>(x, NoTrans, Trans, r2 - w1, w1, kw, T(-1), U + k1 + w1, lda, U + k1, lda, T(1), C1 + w1, lda);
+            pend = Pending{U + k1 + w1, r2 - w1, kw, C1 + w1 + (long)w1 * lda};
+          }
+        }
       }
+      if (prev.n > 0) dev_syrk<T>(x, Lower, NoTrans, prev.n, prev.kw, T(-1), prev.U, lda, T(1), prev.C, lda);
     } else {
       T* P = blk(k, k + 1);  // kk x rest panel
       copy2d<T>(x.s, kk, rest, P, lda, W, kk);
       dev_gemm<T>(x, Trans, NoTrans, kk, rest, kk, T(1), ik, nb, W, kk, T(0), P, lda);
-      const int k1 = tsz(k + 1);
-      dev_syrk<T>(x, Upper, Trans, k1, kk, T(-1), P, lda, T(1), blk(k + 1, k + 1), lda);
-      if (lookahead) {
-        XK_HIP(hipEventRecord(ev_look, x.s));
-        XK_HIP(hipStreamWaitEvent(sd, ev_look, 0));
-        potrf_diag<T>(sd, uplo, k1, blk(k + 1, k + 1), lda, inv + (size_t)((k + 1) & 1) * nb * nb, nb, info,
-                      (k + 1) * nb, ((k + 1) & 1) ? WS_DIAG1 : WS_DIAG0);
-      }
-      const int r2 = rest - k1;
+      const T* U = A + (long)c0 * nb + (long)(k + 1) * nb * lda;  // kw x rest
+      dev_syrk<T>(x, Upper, Trans, k1, kw, T(-1), U, lda, T(1), blk(k + 1, k + 1), lda);
+      if (lookahead) XK_HIP(hipEventRecord(ev_look, x.s));
       if (r2 > 0) {
-        dev_gemm<T>(x, Trans, NoTrans, k1, r2, kk, T(-1), P, lda, P + (long)k1 * lda, lda, T(1), blk(k + 1, k + 2),
+        dev_gemm<T>(x, Trans, NoTrans, k1, r2, kw, T(-1), U, lda, U + (long)k1 * lda, lda, T(1), blk(k + 1, k + 2),
                     lda);
-        dev_syrk<T>(x, Upper, Trans, r2, kk, T(-1), P + (long)k1 * lda, lda, T(1), blk(k + 2, k + 2), lda);
+        T* C1 = blk(k + 2, k + 2);
+        if (!agg) {
+          dev_syrk<T>(x, Upper, Trans, r2, kw, T(-1), U + (long)k1 * lda, lda, T(1), C1, lda);
+        } else if (pair) {
+          const int w1 = pair_split(r2);
+          dev_syrk<T>(x, Upper, Trans, w1, kw, T(-1), U + (long)k1 * lda, lda, T(1), C1, lda);
+          if (w1 < r2) {
+            dev_gemm<T>(x, Trans, NoTrans, w1, r2 - w1, kw, T(-1), U + (long)k1 * lda, lda,
+                        U + (long)(k1 + w1) * lda, lda, T(1), C1 + (long)w1 * lda, lda);
+            pend = Pending{U + (long)(k1 + w1) * lda, r2 - w1, kw, C1 + w1 + (long)w1 * lda};
+          }
+        }
       }
+      if (prev.n > 0) dev_syrk<T>(x, Upper, Trans, prev.n, prev.kw, T(-1), prev.U, lda, T(1), prev.C, lda);
+    }
+    if (lookahead) {
+      XK_HIP(hipStreamWaitEvent(sd, ev_look, 0));
+      potrf_diag<T>(sd, uplo, k1, blk(k + 1, k + 1), lda, inv + (size_t)((k + 1) & 1) * nb * nb, nb, info,
+                    (k + 1) * nb, ((k + 1) & 1) ? WS_DIAG1 : WS_DIAG0);
     }
     if (!lookahead)
       potrf_diag<T>(x.s, uplo, tsz(k + 1), blk(k + 1, k + 1), lda, inv + (size_t)((k + 1) & 1) * nb * nb, nb,

@@ -36,6 +36,8 @@ void dev_trsm(const Exec& x, int side, int uplo, int transA, int diag, int m, in
 // the caller initialises it.
 template <typename T>
 void dev_potrf(const Exec& x, int uplo, int n, T* A, long lda, int nb, int* info);
+// trailing updates of dev_potrf: 1 paired (default, also for -1), 0 one per panel
+void potrf_set_pairing(int v);
 
 // Inverse of the triangular n x n block A into Tinv (other triangle zeroed).
 template <typename T>

@@ -239,13 +239,14 @@ __device__ __forceinline__ unsigned pack_fp8x4(float a, float b, float c, float 
   return (unsigned)w;
 }
 
-// stores 8 k-consecutive plane values of one row
+// stores 8 k-consecutive plane values of one row (non-temporal: the planes
+// are streamed out once and read back by the residue GEMM from HBM)
 __device__ __forceinline__ void store8(__bf16* out, const float* v)
 {
   bf16x8 o;
 #pragma unroll
   for (int e = 0; e < 8; ++e) o[e] = (__bf16)v[e];
-  *reinterpret_cast<bf16x8*>(out) = o;
+  __builtin_nontemporal_store(o, reinterpret_cast<bf16x8*>(out));
 }
 
 __device__ __forceinline__ void store8(uint8_t* out, const float* v)
@@ -253,7 +254,8 @@ __device__ __forceinline__ void store8(uint8_t* out, const float* v)
   uint2 o;
   o.x = pack_fp8x4(v[0], v[1], v[2], v[3]);
   o.y = pack_fp8x4(v[4], v[5], v[6], v[7]);
-  *reinterpret_cast<uint2*>(out) = o;
+  __builtin_nontemporal_store(o.x, reinterpret_cast<unsigned*>(out));
+  __builtin_nontemporal_store(o.y, reinterpret_cast<unsigned*>(out) + 1);
 }
 
 // Residue planes of the scaled, rounded rows (Elem = __bf16 or E4M3 bytes)

@@ -167,6 +167,12 @@ void xkblas_set_split_kernel(int variant)
   split_set_kernel(variant == 1 ? 1 : 0);
 }
 
+void xkblas_set_potrf_pairing(int mode)
+{
+  std::lock_guard<std::mutex> lock(api_mutex());
+  potrf_set_pairing(mode);
+}
+
 int xkblas_split_side_bits(int k, int nsplit) { return split_side_bits(k, nsplit, false); }
 
 int xkblas_split_info(int mode, int k, int nsplit, int* nmod, int* nplane)
