@@ -67,63 +67,155 @@ __device__ __forceinline__ void lower_inverse_to(const T* Ls, int lane, bool uni
   }
 }
 
-// Cholesky factor and its inverse for one block of order <= 64, in
-// registers: lane i holds row i of the lower-triangular view (x) and later
-// column i of inv(L) (y); the values of other rows come from cross-lane
-// reads (v_readlane), so there is no LDS traffic and no barrier.  Step j
-// takes the pivot from lane j, scales column j and applies the rank-1 update
-// x[k] -= L[i][j] L[k][j] for k > j (entries above the diagonal become
-// garbage and are cleared before they are read).  The inverse is a forward
-// substitution, column `lane` per lane.
+// Cholesky factor and its inverse for one block of order <= 64, blocked by
+// 16 and spread over four waves.  The block and its inverse live in LDS
+// (column-major, stride LLD).  For each 16-column block jb:
+//   wave 0 factors the 16 x 16 diagonal block in registers (lane i holds row
+//   i, cross-lane reads for the broadcasts: at most 15 per step) and forms
+//   its inverse (lane c, column c), which is also the diagonal block of
+//   inv(L);
+//   all threads solve the panel below, L21 = A21 inv(L11)^T, and update the
+//   trailing triangle, A22 -= L21 L21^T.
+// The off-diagonal blocks of inv(L) follow block row by block row:
+//   inv(L)[ib][jb] = -inv(L_ib,ib) sum_{t=jb}^{ib-1} L[ib][t] inv(L)[t][jb].
+// Rows/columns beyond n are the identity.  info receives the first order
+// whose pivot is not positive.
+constexpr int LB = 16;  // inner block of the leaf
+
 template <typename T>
-__global__ __launch_bounds__(64) void potrf_leaf_kernel(int uplo, int n, T* A, long lda, long sA, T* Tinv,
-                                                        long ldinv, long sInv, int* info, int base)
+__global__ __launch_bounds__(256) void potrf_leaf_kernel(int uplo, int n, T* A, long lda, long sA, T* Tinv,
+                                                         long ldinv, long sInv, int* info, int base)
 {
-  const int lane = threadIdx.x;
+  __shared__ T Ls[LEAF * LLD];  // the factor
+  __shared__ T Is[LEAF * LLD];  // its inverse
+  __shared__ T Ts[3 * LB * LB]; // block-row partial products of the inverse
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   A += (long)blockIdx.x * sA;
   Tinv += (long)blockIdx.x * sInv;
-  T x[LEAF];
-#pragma unroll
-  for (int j = 0; j < LEAF; ++j) x[j] = lower_view(A, lda, uplo, n, lane, j);
+  for (int e = tid; e < LEAF * LEAF; e += 256) {
+    const int i = e & 63, k = e >> 6;
+    Ls[i + k * LLD] = lower_view(A, lda, uplo, n, i, k);
+    Is[i + k * LLD] = T(0);
+  }
+  __syncthreads();
 
-  bool failed = false;
+  bool failed = false;  // (wave 0)
+  for (int jb = 0; jb < LEAF / LB; ++jb) {
+    const int c0 = jb * LB;
+    if (wave == 0) {
+      const int i = lane & (LB - 1);
+      T d[LB];
 #pragma unroll
-  for (int j = 0; j < LEAF; ++j) {
-    const T d = bcast(x[j], j);
-    if (!(d > T(0)) && !failed && j < n) {
-      failed = true;
-      if (lane == 0) atomicMin(info, base + j + 1);
+      for (int k = 0; k < LB; ++k) d[k] = Ls[(c0 + i) + (c0 + k) * LLD];
+#pragma unroll
+      for (int j = 0; j < LB; ++j) {
+        const T p = bcast(d[j], j);
+        if (!(p > T(0)) && !failed && c0 + j < n) {
+          failed = true;
+          if (lane == 0) atomicMin(info, base + c0 + j + 1);
+        }
+        const T sq = sqrt(p);
+        d[j] = i > j ? d[j] / sq : (i == j ? sq : T(0));
+#pragma unroll
+        for (int k = j + 1; k < LB; ++k) d[k] = fma(-d[j], bcast(d[j], k), d[k]);
+      }
+      T y[LB];  // column i of the block's inverse
+#pragma unroll
+      for (int r = 0; r < LB; ++r) {
+        T acc = (r == i) ? T(1) : T(0);
+#pragma unroll
+        for (int k = 0; k < r; ++k) acc = fma(-bcast(d[k], r), y[k], acc);
+        y[r] = r < i ? T(0) : acc / bcast(d[r], r);
+      }
+      if (lane < LB) {
+#pragma unroll
+        for (int k = 0; k < LB; ++k) Ls[(c0 + lane) + (c0 + k) * LLD] = k <= lane ? d[k] : T(0);
+#pragma unroll
+        for (int r = 0; r < LB; ++r) Is[(c0 + r) + (c0 + lane) * LLD] = y[r];
+      }
     }
-    const T s = sqrt(d);
-    x[j] = lane > j ? x[j] / s : (lane == j ? s : T(0));
+    __syncthreads();
+    const int m = LEAF - c0 - LB;  // rows below the diagonal block
+    if (m > 0) {
+      // panel: L[c0+LB+r][c0+c] = sum_{t <= c} A[c0+LB+r][c0+t] inv(L11)[c][t]
+      T pv[3];
 #pragma unroll
-    for (int k = j + 1; k < LEAF; ++k) x[k] = fma(-x[j], bcast(x[j], k), x[k]);
+      for (int u = 0; u < 3; ++u) {
+        const int e = tid + 256 * u;
+        pv[u] = T(0);
+        if (e < m * LB) {
+          const int r = e % m, c = e / m;
+          T acc = T(0);
+          for (int t = 0; t <= c; ++t)
+            acc = fma(Ls[(c0 + LB + r) + (c0 + t) * LLD], Is[(c0 + c) + (c0 + t) * LLD], acc);
+          pv[u] = acc;
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int u = 0; u < 3; ++u) {
+        const int e = tid + 256 * u;
+        if (e < m * LB) Ls[(c0 + LB + e % m) + (c0 + e / m) * LLD] = pv[u];
+      }
+      __syncthreads();
+      // trailing triangle: L[c0+LB+r][c0+LB+q] -= sum_t L21[r][t] L21[q][t], r >= q
+      for (int e = tid; e < m * m; e += 256) {
+        const int r = e % m, q = e / m;
+        if (r >= q) {
+          T acc = Ls[(c0 + LB + r) + (c0 + LB + q) * LLD];
+#pragma unroll
+          for (int t = 0; t < LB; ++t)
+            acc = fma(-Ls[(c0 + LB + r) + (c0 + t) * LLD], Ls[(c0 + LB + q) + (c0 + t) * LLD], acc);
+          Ls[(c0 + LB + r) + (c0 + LB + q) * LLD] = acc;
+        }
+      }
+      __syncthreads();
+    }
   }
   // factor back to the stored triangle
-#pragma unroll
-  for (int j = 0; j < LEAF; ++j) {
-    if (lane < n && j < n && j <= lane) {
-      if (uplo == Lower) A[lane + (long)j * lda] = x[j];
-      else A[j + (long)lane * lda] = x[j];
+  for (int e = tid; e < LEAF * LEAF; e += 256) {
+    const int i = e & 63, k = e >> 6;
+    if (i < n && k < n && k <= i) {
+      if (uplo == Lower) A[i + (long)k * lda] = Ls[i + k * LLD];
+      else A[k + (long)i * lda] = Ls[i + k * LLD];
     }
   }
   if (!Tinv) return;
-  // inv(L): lane j computes column j, y[i] = inv(L)[i][j]
-  T y[LEAF];
+  for (int ib = 1; ib < LEAF / LB; ++ib) {
+    // Ts[jb] = sum_{t=jb}^{ib-1} L[ib][t] inv(L)[t][jb], jb < ib
 #pragma unroll
-  for (int i = 0; i < LEAF; ++i) {
-    T acc = (i == lane) ? T(1) : T(0);
-#pragma unroll
-    for (int k = 0; k < i; ++k) acc = fma(-bcast(x[k], i), y[k], acc);
-    y[i] = i < lane ? T(0) : acc / bcast(x[i], i);
-  }
-  if (lane < n) {
-#pragma unroll
-    for (int i = 0; i < LEAF; ++i) {
-      if (i < n) {
-        if (uplo == Lower) Tinv[i + (long)lane * ldinv] = y[i];
-        else Tinv[lane + (long)i * ldinv] = y[i];
+    for (int u = 0; u < 3; ++u) {
+      const int e = tid + 256 * u;
+      if (e < ib * LB * LB) {
+        const int jb = e >> 8, r = e & (LB - 1), c = (e >> 4) & (LB - 1);
+        T acc = T(0);
+        for (int s = jb * LB; s < ib * LB; ++s)
+          acc = fma(Ls[(ib * LB + r) + s * LLD], Is[s + (jb * LB + c) * LLD], acc);
+        Ts[e] = acc;
       }
+    }
+    __syncthreads();
+    // inv(L)[ib][jb] = -inv(L_ib,ib) Ts[jb]
+#pragma unroll
+    for (int u = 0; u < 3; ++u) {
+      const int e = tid + 256 * u;
+      if (e < ib * LB * LB) {
+        const int jb = e >> 8, r = e & (LB - 1), c = (e >> 4) & (LB - 1);
+        T acc = T(0);
+        for (int s = 0; s <= r; ++s)
+          acc = fma(Is[(ib * LB + r) + (ib * LB + s) * LLD], Ts[(jb << 8) + s + (c << 4)], acc);
+        Is[(ib * LB + r) + (jb * LB + c) * LLD] = -acc;
+      }
+    }
+    __syncthreads();
+  }
+  // column j of inv(L) (zeros above the diagonal).  Lower: Tinv[i + j*ld];
+  // Upper (inv(U) = inv(L)^T): Tinv[j + i*ld].
+  for (int e = tid; e < LEAF * LEAF; e += 256) {
+    const int i = e & 63, j = e >> 6;
+    if (i < n && j < n) {
+      if (uplo == Lower) Tinv[i + (long)j * ldinv] = Is[i + j * LLD];
+      else Tinv[j + (long)i * ldinv] = Is[i + j * LLD];
     }
   }
 }
@@ -148,7 +240,7 @@ void potrf_leaf(hipStream_t s, int uplo, int n, T* A, long lda, long sA, T* Tinv
 {
   XK_CHECK(n >= 1 && n <= LEAF, "potrf leaf order must be in [1,64]");
   if (batch <= 0) return;
-  hipLaunchKernelGGL((potrf_leaf_kernel<T>), dim3(batch), dim3(64), 0, s, uplo, n, A, lda, sA, Tinv, ldinv, sInv,
+  hipLaunchKernelGGL((potrf_leaf_kernel<T>), dim3(batch), dim3(256), 0, s, uplo, n, A, lda, sA, Tinv, ldinv, sInv,
                      info, base);
   XK_HIP(hipGetLastError());
 }
