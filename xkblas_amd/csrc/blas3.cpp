@@ -106,7 +106,7 @@ void dev_trtri(const Exec& x, hipStream_t s, int uplo, int diag, int n, const T*
 // for Lower, inv(U) for Upper).
 template <typename T>
 static void potrf_diag(hipStream_t s, int uplo, int n, T* A, long lda, T* Tinv, long ldinv, int* info, int base,
-                       int wslot)
+                       int wslot, bool zeroed = false)
 {
   if (n <= 64) {
     potrf_leaf<T>(s, uplo, n, A, lda, 0, Tinv, ldinv, 0, 1, info, base);
@@ -117,7 +117,7 @@ static void potrf_diag(hipStream_t s, int uplo, int n, T* A, long lda, T* Tinv, 
   T* A22 = A + n1 + (long)n1 * lda;
   T* I11 = Tinv;
   T* I22 = Tinv + n1 + (long)n1 * ldinv;
-  potrf_diag<T>(s, uplo, n1, A11, lda, I11, ldinv, info, base, wslot);
+  potrf_diag<T>(s, uplo, n1, A11, lda, I11, ldinv, info, base, wslot, zeroed);
   T* W = (T*)scratch(wslot, sizeof(T) * (size_t)n1 * n2);
   if (uplo == Lower) {
     T* A21 = A + n1;
@@ -125,20 +125,20 @@ static void potrf_diag(hipStream_t s, int uplo, int n, T* A, long lda, T* Tinv, 
     copy2d<T>(s, n2, n1, A21, lda, W, n2);
     gemm<T>(s, NoTrans, Trans, n2, n1, n1, T(1), W, n2, I11, ldinv, T(0), A21, lda);   // L21 = A21 inv(L11)^T
     gemm<T>(s, NoTrans, Trans, n2, n2, n1, T(-1), A21, lda, A21, lda, T(1), A22, lda, TRI_LOWER);
-    potrf_diag<T>(s, uplo, n2, A22, lda, I22, ldinv, info, base + n1, wslot);
+    potrf_diag<T>(s, uplo, n2, A22, lda, I22, ldinv, info, base + n1, wslot, zeroed);
     gemm<T>(s, NoTrans, NoTrans, n2, n1, n1, T(1), A21, lda, I11, ldinv, T(0), W, n2);  // L21 inv(L11)
     gemm<T>(s, NoTrans, NoTrans, n2, n1, n2, T(-1), I22, ldinv, W, n2, T(0), I21, ldinv);
-    fill_zero<T>(s, n1, n2, Tinv + (long)n1 * ldinv, ldinv);
+    if (!zeroed) fill_zero<T>(s, n1, n2, Tinv + (long)n1 * ldinv, ldinv);
   } else {
     T* A12 = A + (long)n1 * lda;
     T* I12 = Tinv + (long)n1 * ldinv;
     copy2d<T>(s, n1, n2, A12, lda, W, n1);
     gemm<T>(s, Trans, NoTrans, n1, n2, n1, T(1), I11, ldinv, W, n1, T(0), A12, lda);     // U12 = inv(U11)^T A12
     gemm<T>(s, Trans, NoTrans, n2, n2, n1, T(-1), A12, lda, A12, lda, T(1), A22, lda, TRI_UPPER);
-    potrf_diag<T>(s, uplo, n2, A22, lda, I22, ldinv, info, base + n1, wslot);
+    potrf_diag<T>(s, uplo, n2, A22, lda, I22, ldinv, info, base + n1, wslot, zeroed);
     gemm<T>(s, NoTrans, NoTrans, n1, n2, n2, T(1), A12, lda, I22, ldinv, T(0), W, n1);   // U12 inv(U22)
     gemm<T>(s, NoTrans, NoTrans, n1, n2, n1, T(-1), I11, ldinv, W, n1, T(0), I12, ldinv);
-    fill_zero<T>(s, n2, n1, Tinv + n1, ldinv);
+    if (!zeroed) fill_zero<T>(s, n2, n1, Tinv + n1, ldinv);
   }
 }
 
@@ -188,7 +188,10 @@ void dev_potrf(const Exec& x, int uplo, int n, T* A, long lda, int nb, int* info
     XK_HIP(hipEventRecord(ev_look, x.s));
     XK_HIP(hipStreamWaitEvent(sd, ev_look, 0));
   }
-  potrf_diag<T>(sd, uplo, tsz(0), blk(0, 0), lda, inv, nb, info, 0, WS_DIAG0);
+  // the inverses' blocks above the diagonal are never written with nonzeros:
+  // one fill here replaces a fill per recursion node
+  fill_zero<T>(sd, nb, 2 * nb, inv, nb);
+  potrf_diag<T>(sd, uplo, tsz(0), blk(0, 0), lda, inv, nb, info, 0, WS_DIAG0, true);
   for (int k = 0; k < nt; ++k) {
     const int kk = tsz(k);
     T* ik = inv + (size_t)(k & 1) * nb * nb;
@@ -267,11 +270,11 @@ void dev_potrf(const Exec& x, int uplo, int n, T* A, long lda, int nb, int* info
     if (lookahead) {
       XK_HIP(hipStreamWaitEvent(sd, ev_look, 0));
       potrf_diag<T>(sd, uplo, k1, blk(k + 1, k + 1), lda, inv + (size_t)((k + 1) & 1) * nb * nb, nb, info,
-                    (k + 1) * nb, ((k + 1) & 1) ? WS_DIAG1 : WS_DIAG0);
+                    (k + 1) * nb, ((k + 1) & 1) ? WS_DIAG1 : WS_DIAG0, true);
     }
     if (!lookahead)
       potrf_diag<T>(x.s, uplo, tsz(k + 1), blk(k + 1, k + 1), lda, inv + (size_t)((k + 1) & 1) * nb * nb, nb,
-                    info, (k + 1) * nb, WS_DIAG0);
+                    info, (k + 1) * nb, WS_DIAG0, true);
   }
   XK_HIP(hipEventDestroy(ev_diag));
   XK_HIP(hipEventDestroy(ev_look));

@@ -145,8 +145,10 @@ __global__ __launch_bounds__(256) void potrf_leaf_kernel(int uplo, int n, T* A, 
         pv[u] = T(0);
         if (e < m * LB) {
           const int r = e % m, c = e / m;
+          // all 16 terms: inv(L11)[c][t] is 0 for t > c
           T acc = T(0);
-          for (int t = 0; t <= c; ++t)
+#pragma unroll
+          for (int t = 0; t < LB; ++t)
             acc = fma(Ls[(c0 + LB + r) + (c0 + t) * LLD], Is[(c0 + c) + (c0 + t) * LLD], acc);
           pv[u] = acc;
         }
@@ -189,8 +191,13 @@ __global__ __launch_bounds__(256) void potrf_leaf_kernel(int uplo, int n, T* A, 
       if (e < ib * LB * LB) {
         const int jb = e >> 8, r = e & (LB - 1), c = (e >> 4) & (LB - 1);
         T acc = T(0);
-        for (int s = jb * LB; s < ib * LB; ++s)
-          acc = fma(Ls[(ib * LB + r) + s * LLD], Is[s + (jb * LB + c) * LLD], acc);
+        for (int tb = jb; tb < ib; ++tb) {
+#pragma unroll
+          for (int t = 0; t < LB; ++t) {
+            const int s = tb * LB + t;
+            acc = fma(Ls[(ib * LB + r) + s * LLD], Is[s + (jb * LB + c) * LLD], acc);
+          }
+        }
         Ts[e] = acc;
       }
     }
@@ -201,8 +208,9 @@ __global__ __launch_bounds__(256) void potrf_leaf_kernel(int uplo, int n, T* A, 
       const int e = tid + 256 * u;
       if (e < ib * LB * LB) {
         const int jb = e >> 8, r = e & (LB - 1), c = (e >> 4) & (LB - 1);
-        T acc = T(0);
-        for (int s = 0; s <= r; ++s)
+        T acc = T(0);  // all 16 terms: the diagonal block's inverse is 0 above its diagonal
+#pragma unroll
+        for (int s = 0; s < LB; ++s)
           acc = fma(Is[(ib * LB + r) + (ib * LB + s) * LLD], Ts[(jb << 8) + s + (c << 4)], acc);
         Is[(ib * LB + r) + (jb * LB + c) * LLD] = -acc;
       }
