@@ -199,6 +199,8 @@ def main():
                          "MODE, twice, no JSON")
     ap.add_argument("--potrf-nb", type=int, default=2048)
     ap.add_argument("--only-batched", action="store_true", help="profile helper: the batched leg alone, no JSON")
+    ap.add_argument("--batched-parts", action="store_true",
+                    help="with --only-batched: also time the GEMM problems alone and the TRSM problems alone")
     ap.add_argument("--copy-probe", action="store_true",
                     help="profile helper: one 1 GiB device-to-device copy (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -223,7 +225,7 @@ def main():
         L.xkblas_finalize()
         return
     if args.only_batched:
-        print(json.dumps(bench_batched(xk)), file=sys.stderr)
+        print(json.dumps(bench_batched(xk, parts=args.batched_parts)), file=sys.stderr)
         L.xkblas_finalize()
         return
     n = args.n
@@ -359,7 +361,7 @@ def main():
     print(json.dumps(out))
 
 
-def bench_batched(xk, seed=2027, reps=5):
+def bench_batched(xk, seed=2027, reps=5, parts=False):
     """Batched uneven work: 4096 GEMMs (m, n, k log-uniform in [16, 512]) and
     1024 lower-triangular TRSMs (n log-uniform in [32, 1024], 64 right-hand
     sides, diagonally dominant L), all FP64 or all FP32, in one persistent
@@ -435,6 +437,26 @@ def bench_batched(xk, seed=2027, reps=5):
             ms.append(xk.profile("batch")[1])
         res = np.empty(int(offs[-1]), dtype=dt)
         assert L.xkblas_memcpy_dtoh(res.ctypes.data_as(p), pool, res.nbytes) == 0
+        part_ms = {}
+        if parts:
+            # the GEMM problems alone and the TRSM problems alone, same launch
+            for part, lo, hi in (("gemm_only", 0, len(g)), ("trsm_only", len(g), len(arr))):
+                sub = (xk.BatchProblem * (hi - lo))(*arr[lo:hi])
+                sp = L.xkblas_batch_create(ctypes.cast(sub, p), hi - lo)
+                assert sp and L.xkblas_batch_use_graph(sp, 1) == 0
+                L.xkblas_batch_run(sp)
+                L.xkblas_sync()
+                t = []
+                for _ in range(reps):
+                    upload()
+                    L.xkblas_sync()
+                    L.xkblas_profile_reset()
+                    L.xkblas_batch_run(sp)
+                    L.xkblas_sync()
+                    t.append(xk.profile("batch")[1])
+                part_ms[part + "_ms"] = round(min(t), 3)
+                L.xkblas_batch_use_graph(sp, 0)
+                L.xkblas_batch_destroy(sp)
         err = 0.0
         for o, m, n, ref in checks:
             got = res[o:o + m * n].reshape(n, m).T.astype(np.float64)
@@ -458,7 +480,7 @@ def bench_batched(xk, seed=2027, reps=5):
         out[name] = {"ms": round(best, 3), "gflops": round(flops / best / 1e6, 1), "problems": len(arr),
                      "one_launch_per_problem_ms": round(t_each * 1e3, 3),
                      "speedup_vs_one_launch_per_problem": round(t_each * 1e3 / best, 2),
-                     "max_rel_err_sampled": err}
+                     "max_rel_err_sampled": err, **part_ms}
     return out
 
 
