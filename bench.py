@@ -446,15 +446,15 @@ def bench_batched(xk, seed=2027, reps=5, parts=False):
                 assert sp and L.xkblas_batch_use_graph(sp, 1) == 0
                 L.xkblas_batch_run(sp)
                 L.xkblas_sync()
-                t = []
+                part_t = []
                 for _ in range(reps):
                     upload()
                     L.xkblas_sync()
                     L.xkblas_profile_reset()
                     L.xkblas_batch_run(sp)
                     L.xkblas_sync()
-                    t.append(xk.profile("batch")[1])
-                part_ms[part + "_ms"] = round(min(t), 3)
+                    part_t.append(xk.profile("batch")[1])
+                part_ms[part + "_ms"] = round(min(part_t), 3)
                 L.xkblas_batch_use_graph(sp, 0)
                 L.xkblas_batch_destroy(sp)
         err = 0.0
