@@ -1168,6 +1168,14 @@ __global__ __launch_bounds__(256) void reconstruct_kernel(RecParams rp)
   Word w[NPL];
 #pragma unroll
   for (int q = 0; q < NPL; ++q) w[q] = *reinterpret_cast<const Word*>(r + (long)min(q, rp.nplane - 1) * rp.sR);
+  // the old C values (beta != 0) are requested with the planes as well
+  double* c = rp.C + (long)j * rp.ldc;
+  double cold[4] = {0.0, 0.0, 0.0, 0.0};
+  if (rp.beta != 0.0) {
+#pragma unroll
+    for (int e = 0; e < 4; ++e)
+      if (i0 + e < rp.m) cold[e] = c[i0 + e];
+  }
   double S[4][3] = {};
 #pragma unroll
   for (int l = 0; l < NMOD; ++l) {
@@ -1201,7 +1209,6 @@ __global__ __launch_bounds__(256) void reconstruct_kernel(RecParams rp)
     }
   }
   const int sb = rp.shB[j];
-  double* c = rp.C + (long)j * rp.ldc;
 #pragma unroll
   for (int e = 0; e < 4; ++e) {
     const int i = i0 + e;
@@ -1211,7 +1218,7 @@ __global__ __launch_bounds__(256) void reconstruct_kernel(RecParams rp)
     const double x = crt_one(rp, S[e][0], S[e][1], S[e][2]);
     const double res = ldexp(x, -(rp.shA[i] + sb));
     double out = rp.alpha * res;
-    if (rp.beta != 0.0) out = fma(rp.beta, c[i], out);
+    if (rp.beta != 0.0) out = fma(rp.beta, cold[e], out);
     c[i] = out;
   }
 }
