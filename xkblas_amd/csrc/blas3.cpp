@@ -166,6 +166,10 @@ void dev_potrf(const Exec& x, int uplo, int n, T* A, long lda, int nb, int* info
   // trailing updates of two panels at once (inner dimension 2 nb): longer
   // products, and half the split modes' CRT passes over the trailing triangle
   const bool agg = potrf_pairing() == 1;
+  // split modes: the residue planes of each step's panel operand are computed
+  // once for all the products that read it (its rows never change again)
+  const bool split = split_enabled<T>(x);
+  if (split) split_cache_clear();
 
   auto blk = [&](int i, int j) { return A + (long)i * nb + (long)j * nb * lda; };
   auto tsz = [&](int i) { return i == nt - 1 ? n - i * nb : nb; };
@@ -221,6 +225,7 @@ void dev_potrf(const Exec& x, int uplo, int n, T* A, long lda, int nb, int* info
       copy2d<T>(x.s, rest, kk, P, lda, W, rest);
       dev_gemm<T>(x, NoTrans, Trans, rest, kk, kk, T(1), W, rest, ik, nb, T(0), P, lda);
       const T* U = A + (long)(k + 1) * nb + (long)c0 * nb * lda;  // rest x kw
+      if (split) split_cache_register((const double*)U, lda, rest, kw, 1);
       // lookahead: update the next diagonal block and factor it on the side
       dev_syrk<T>(x, Lower, NoTrans, k1, kw, T(-1), U, lda, T(1), blk(k + 1, k + 1), lda);
       if (lookahead) XK_HIP(hipEventRecord(ev_look, x.s));
@@ -247,6 +252,7 @@ void dev_potrf(const Exec& x, int uplo, int n, T* A, long lda, int nb, int* info
       copy2d<T>(x.s, kk, rest, P, lda, W, kk);
       dev_gemm<T>(x, Trans, NoTrans, kk, rest, kk, T(1), ik, nb, W, kk, T(0), P, lda);
       const T* U = A + (long)c0 * nb + (long)(k + 1) * nb * lda;  // kw x rest
+      if (split) split_cache_register((const double*)U, lda, rest, kw, 0);
       dev_syrk<T>(x, Upper, Trans, k1, kw, T(-1), U, lda, T(1), blk(k + 1, k + 1), lda);
       if (lookahead) XK_HIP(hipEventRecord(ev_look, x.s));
       if (r2 > 0) {
@@ -276,6 +282,7 @@ void dev_potrf(const Exec& x, int uplo, int n, T* A, long lda, int nb, int* info
       potrf_diag<T>(x.s, uplo, tsz(k + 1), blk(k + 1, k + 1), lda, inv + (size_t)((k + 1) & 1) * nb * nb, nb,
                     info, (k + 1) * nb, WS_DIAG0, true);
   }
+  if (split) split_cache_clear();
   XK_HIP(hipEventDestroy(ev_diag));
   XK_HIP(hipEventDestroy(ev_look));
 }

@@ -22,6 +22,15 @@ int split_choose_count(int k, int requested, bool fp8);
 int split_side_bits(int k, int nsplit, bool fp8);
 int split_plane_count(int k, int requested, bool fp8);
 
+// Residue planes reused across split products: registers op(X) (rows x k;
+// row_contig: op(X)(i, kk) = X[i + kk ld], else X[kk + i ld]) as unchanging
+// until the next registration or clear; products whose op(A) rows / op(B)
+// columns are runs of its rows (first row a multiple of 256) take their
+// exponents and residue planes from a cache.  The two latest registrations
+// stay valid.
+void split_cache_register(const double* X, long ld, int rows, int k, int row_contig);
+void split_cache_clear();
+
 // BF16 residue GEMM kernel (256 x 256 tiles, direct-to-LDS loads): 0 = 8
 // waves of 128 x 64 (two per SIMD), 1 = 4 waves of 128 x 128.
 void split_set_kernel(int v);

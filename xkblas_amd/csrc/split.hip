@@ -1247,6 +1247,86 @@ int split_side_bits(int k, int nsplit, bool fp8)
   return side_bits(make_moduli(n, fp8), k);
 }
 
+// ---- residue planes reused across split products (opt-in by region)
+// A caller that runs several split products whose op(A) rows / op(B)
+// columns are runs of the rows of one unchanging operand over the same k
+// registers that operand; the exponents and residue planes of each run it
+// needs are then computed once and kept (LRU, one scratch slot per entry).
+// Only runs inside a registered region are cached, so reused buffers such as
+// a workspace copy are never served stale.
+struct OpPlanes {
+  const void* R;        // planes, first row of the run
+  long rpad;            // row pitch of the planes (padded rows of the buffer)
+  const int* shift;     // per-row exponent shifts, first row of the run
+};
+struct PrepRegion {
+  const double* X = nullptr;  // row 0 of op(X)
+  long ld = 0;
+  int rows = 0, k = 0, rc = 0;
+};
+struct PrepEntry {
+  const double* X = nullptr;  // first row of the run
+  long ld = 0;
+  int rows = 0, k = 0, rc = 0, nmod = 0;
+  bool fp8 = false;
+  long rpad = 0;
+  void* planes = nullptr;
+  int* shift = nullptr;
+  unsigned long long stamp = 0;
+};
+constexpr int PREP_ENTRIES = 4, PREP_REGIONS = 2, WS_PREP0 = 12;
+static PrepRegion g_region[PREP_REGIONS];
+static PrepEntry g_prep[PREP_ENTRIES];
+static unsigned long long g_prep_clock = 0;
+
+// row r of op(X) starts at X + r (rc: op(X)(i, kk) = X[i + kk ld]) or X + r ld
+static bool run_offset(const double* base, long ld, int rc, const double* X, long& r)
+{
+  const long d = X - base;
+  if (d < 0) return false;
+  if (rc) {
+    if (d >= ld) return false;
+    r = d;
+  } else {
+    if (d % ld) return false;
+    r = d / ld;
+  }
+  return true;
+}
+
+void split_cache_register(const double* X, long ld, int rows, int k, int row_contig)
+{
+  for (int i = PREP_REGIONS - 1; i > 0; --i) g_region[i] = g_region[i - 1];
+  g_region[0] = PrepRegion{X, ld, rows, k, row_contig ? 1 : 0};
+  // entries outside every registered region are dropped
+  for (auto& e : g_prep) {
+    bool keep = false;
+    for (const auto& g : g_region) {
+      long r;
+      if (e.X && g.X && g.ld == e.ld && g.k == e.k && g.rc == e.rc && run_offset(g.X, g.ld, g.rc, e.X, r) &&
+          r + e.rows <= g.rows)
+        keep = true;
+    }
+    if (!keep) e = PrepEntry();
+  }
+}
+
+void split_cache_clear()
+{
+  for (auto& g : g_region) g = PrepRegion();
+  for (auto& e : g_prep) e = PrepEntry();
+}
+
+static bool in_region(const double* X, long ld, int rows, int k, int rc)
+{
+  for (const auto& g : g_region) {
+    long r;
+    if (g.X && g.ld == ld && g.k == k && g.rc == rc && run_offset(g.X, ld, rc, X, r) && r + rows <= g.rows)
+      return true;
+  }
+  return false;
+}
+
 static void split_gemm_chunk(const Exec& x, int transA, int transB, int m, int n, int k, double alpha,
                              const double* A, long lda, const double* B, long ldb, double beta, double* C, long ldc,
                              int tri);
@@ -1285,65 +1365,96 @@ static void split_gemm_chunk(const Exec& x, int transA, int transB, int m, int n
   const bool same = (A == B) && (lda == ldb) && (transA != transB) && m == n;  // SYRK: op(B) = op(A)^T
   const size_t esz = fp8 ? 1 : sizeof(__bf16);
 
-  char* ws_e = (char*)scratch(WS_SE, sizeof(unsigned long long) * (mpad + npad) + sizeof(int) * (mpad + npad));
-  unsigned long long* bitsA = (unsigned long long*)ws_e;
-  unsigned long long* bitsB = bitsA + mpad;
-  int* shA = (int*)(bitsB + npad);
-  int* shB = shA + mpad;
-  void* RA = scratch(WS_SA, esz * (size_t)np * mpad * kpad);
-  void* RB = same ? RA : scratch(WS_SB, esz * (size_t)np * npad * kpad);
   const size_t rsz = fp8 ? sizeof(int8_t) : sizeof(int16_t);  // residue element
   void* RC = scratch(WS_SC, rsz * (size_t)np * mpad * npad);
   hipStream_t s = x.s;
+  const int kb = fp8 ? F8_BK : DM_BK;  // k-block width: 64 bytes per row
 
-  // 1-2: exponents of the rows of op(A) and columns of op(B)
+  // 1-3: exponents of the rows of op(A) / columns of op(B), then residue planes
   const int rcA = transA == NoTrans;  // op(A)(i,k) = A[i + k lda]
   const int rcB = transB != NoTrans;  // op(B)(k,j) = B[j + k ldb]: row j contiguous
-  Runtime& R = rt();
-  R.prof_begin("split_prep", 0.0);
-  hipLaunchKernelGGL(fill_u64_kernel, dim3((int)((mpad + npad + 255) / 256)), dim3(256), 0, s, bitsA,
-                     (int)(mpad + npad));
+  SplitParams sp;
+  sp.nmod = nmod;
+  sp.kara = M.kara;
+  for (int l = 0; l < nmod; ++l) { sp.pd[l] = M.pd[l]; sp.invpd[l] = M.invpd[l]; }
+  sp.kdim = k;
+  sp.kb = kb;
   auto amax = [&](const double* X, long ld, int rows, int rc, unsigned long long* out) {
     XK_CHECK((rows + 3) / 4 <= 65535, "row-max grid");
     const dim3 grid((unsigned)((k + AM_K - 1) / AM_K), (unsigned)(rc ? (rows + 255) / 256 : (rows + 3) / 4));
     hipLaunchKernelGGL(amax_kernel, grid, dim3(256), 0, s, X, ld, rows, k, rc, out);
   };
-  amax(A, lda, m, rcA, bitsA);
-  hipLaunchKernelGGL(shift_kernel, dim3((m + 255) / 256), dim3(256), 0, s, bitsA, m, tb, shA);
-  if (!same) {
-    amax(B, ldb, n, rcB, bitsB);
-    hipLaunchKernelGGL(shift_kernel, dim3((n + 255) / 256), dim3(256), 0, s, bitsB, n, tb, shB);
-  }
-  XK_HIP(hipGetLastError());
-
-  // 3: residue planes
-  SplitParams sp;
-  sp.nmod = nmod;
-  sp.kara = M.kara;
-  for (int l = 0; l < nmod; ++l) { sp.pd[l] = M.pd[l]; sp.invpd[l] = M.invpd[l]; }
-  sp.X = A; sp.ld = lda; sp.rows = m; sp.kdim = k; sp.row_contig = rcA; sp.shift = shA;
-  sp.kb = fp8 ? F8_BK : DM_BK;  // 64 bytes per row and k-block
-  sp.R = RA; sp.rpad = mpad; sp.rstride = mpad * kpad;
-  auto split = [&](long rows_pad) {
-    const dim3 grid((unsigned)(kpad / 64), (unsigned)(rows_pad / 64));
+  // bits[rpad] are the row maxima, zeroed first; writes planes[rpad] and shift[rows]
+  auto prep = [&](const double* X, long ld, int rows, int rc, long rpad, unsigned long long* bits, int* shift,
+                  void* planes) {
+    hipLaunchKernelGGL(fill_u64_kernel, dim3((int)((rpad + 255) / 256)), dim3(256), 0, s, bits, (int)rpad);
+    amax(X, ld, rows, rc, bits);
+    hipLaunchKernelGGL(shift_kernel, dim3((rows + 255) / 256), dim3(256), 0, s, bits, rows, tb, shift);
+    sp.X = X; sp.ld = ld; sp.rows = rows; sp.row_contig = rc; sp.shift = shift;
+    sp.R = planes; sp.rpad = rpad; sp.rstride = rpad * kpad;
+    const dim3 grid((unsigned)(kpad / 64), (unsigned)(rpad / 64));
     if (fp8) hipLaunchKernelGGL(split_kernel<uint8_t>, grid, dim3(256), 0, s, sp);
     else hipLaunchKernelGGL(split_kernel<__bf16>, grid, dim3(256), 0, s, sp);
+    XK_HIP(hipGetLastError());
   };
-  split(mpad);
-  if (!same) {
-    sp.X = B; sp.ld = ldb; sp.rows = n; sp.row_contig = rcB; sp.shift = shB;
-    sp.R = RB; sp.rpad = npad; sp.rstride = npad * kpad;
-    split(npad);
-  }
-  XK_HIP(hipGetLastError());
+  // a run of a registered operand: from a cache entry, computed into one if absent
+  auto cached = [&](const double* X, long ld, int rows, int rc, OpPlanes& out) {
+    if (!in_region(X, ld, rows, k, rc)) return false;
+    for (auto& e : g_prep) {
+      long r;
+      if (!e.X || e.ld != ld || e.k != k || e.rc != rc || e.nmod != nmod || e.fp8 != fp8) continue;
+      // the tiles read MG_PAD rows from the run's first row: the offset must
+      // keep them inside the entry's padded rows
+      if (!run_offset(e.X, ld, rc, X, r) || r % MG_PAD || r + rows > e.rows) continue;
+      e.stamp = ++g_prep_clock;
+      out = OpPlanes{static_cast<const char*>(e.planes) + esz * (size_t)r * kb, e.rpad, e.shift + r};
+      return true;
+    }
+    int v = 0;
+    for (int i = 1; i < PREP_ENTRIES; ++i)
+      if (g_prep[i].stamp < g_prep[v].stamp) v = i;
+    PrepEntry& e = g_prep[v];
+    const long rpad = rup(rows, MG_PAD);
+    const size_t pb = esz * (size_t)np * rpad * kpad;
+    char* buf = (char*)scratch(WS_PREP0 + v, pb + (sizeof(unsigned long long) + sizeof(int)) * rpad);
+    e = PrepEntry{X, ld, rows, k, rc, nmod, fp8, rpad, buf, (int*)(buf + pb + sizeof(unsigned long long) * rpad),
+                  ++g_prep_clock};
+    prep(X, ld, rows, rc, rpad, (unsigned long long*)(buf + pb), e.shift, e.planes);
+    out = OpPlanes{e.planes, rpad, e.shift};
+    return true;
+  };
 
+  Runtime& R = rt();
+  R.prof_begin("split_prep", 0.0);
+  OpPlanes PA, PB;
+  const bool hitA = cached(A, lda, m, rcA, PA);
+  const bool hitB = same ? hitA : cached(B, ldb, n, rcB, PB);
+  if (same && hitA) PB = PA;
+  if (!hitA || !hitB) {
+    char* ws_e = (char*)scratch(WS_SE, sizeof(unsigned long long) * (mpad + npad) + sizeof(int) * (mpad + npad));
+    unsigned long long* bitsA = (unsigned long long*)ws_e;
+    unsigned long long* bitsB = bitsA + mpad;
+    int* shA = (int*)(bitsB + npad);
+    int* shB = shA + mpad;
+    if (!hitA) {
+      void* RA = scratch(WS_SA, esz * (size_t)np * mpad * kpad);
+      prep(A, lda, m, rcA, mpad, bitsA, shA, RA);
+      PA = OpPlanes{RA, mpad, shA};
+      if (same) PB = PA;
+    }
+    if (!hitB && !same) {
+      void* RB = scratch(WS_SB, esz * (size_t)np * npad * kpad);
+      prep(B, ldb, n, rcB, npad, bitsB, shB, RB);
+      PB = OpPlanes{RB, npad, shB};
+    }
+  }
   R.prof_end();
 
   // 4: one MFMA GEMM per plane
   ModGemmParams g;
-  g.A = RA; g.B = RB; g.C = RC;
-  g.blkA = mpad * sp.kb; g.blkB = (same ? mpad : npad) * sp.kb;
-  g.sA = mpad * kpad; g.sB = same ? mpad * kpad : npad * kpad; g.sC = mpad * npad;
+  g.A = PA.R; g.B = PB.R; g.C = RC;
+  g.blkA = PA.rpad * kb; g.blkB = PB.rpad * kb;
+  g.sA = PA.rpad * kpad; g.sB = PB.rpad * kpad; g.sC = mpad * npad;
   g.ldc = mpad; g.kpad = (int)kpad;
   for (int q = 0; q < np; ++q) { g.p[q] = M.plane_p[q]; g.invp[q] = M.plane_ip[q]; }
   g.tri = tri;
@@ -1367,7 +1478,7 @@ static void split_gemm_chunk(const Exec& x, int transA, int transB, int m, int n
   rp.R = RC; rp.ldr = mpad; rp.sR = mpad * npad; rp.nmod = nmod; rp.nplane = np; rp.m = m; rp.n = n;
   rp.tri = tri;
   rp.kara = M.kara;
-  rp.shA = shA; rp.shB = same ? shA : shB; rp.C = C; rp.ldc = ldc; rp.alpha = alpha; rp.beta = beta;
+  rp.shA = PA.shift; rp.shB = PB.shift; rp.C = C; rp.ldc = ldc; rp.alpha = alpha; rp.beta = beta;
   for (int l = 0; l < nmod; ++l) {
     rp.p[l] = (float)M.pd[l];
     rp.invp[l] = 1.0f / (float)M.pd[l];
