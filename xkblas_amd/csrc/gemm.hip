@@ -22,6 +22,7 @@
 // rastered in groups of 8 tile rows so that concurrently running tiles of one
 // XCD share op(A) panels in its L2.
 #include "common.h"
+#include "tiles.h"
 #include "kernels.h"
 
 namespace xk {
@@ -110,11 +111,8 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmParams<T> p)
       tm = first + r % gm;
       tn = r / gm;
     } else {
-      // triangle of tiles (i >= j) enumerated row by row
-      int i = (int)((sqrtf(8.0f * (float)pid + 1.0f) - 1.0f) * 0.5f);
-      while ((i + 1) * (i + 2) / 2 <= pid) ++i;
-      while (i * (i + 1) / 2 > pid) --i;
-      const int j = pid - i * (i + 1) / 2;
+      int i, j;
+      tri_tile<8>(pid, total, p.tiles_m, i, j);
       if (p.tri == TRI_LOWER) { tm = i; tn = j; }
       else { tm = j; tn = i; }
     }

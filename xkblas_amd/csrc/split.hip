@@ -32,6 +32,7 @@
 // bits and the final rounding to FP64.  N (the split count) is the only
 // accuracy knob: more moduli, larger t.
 #include "split.h"
+#include "tiles.h"
 #include "kernels.h"
 #include "runtime.h"
 #include <algorithm>
@@ -381,7 +382,7 @@ __device__ __forceinline__ void wait_lgkm0()
 // Output tile of this workgroup: consecutive workgroup ids go round-robin
 // over the 8 XCDs, so the remap gives every XCD a contiguous range of ids;
 // full products walk groups of 4 tile rows (their A panels share the XCD's
-// L2), triangles enumerate the lower (or upper) tiles.
+// L2); triangles take the same grouping from tri_tile.
 __device__ __forceinline__ void dm_tile(const ModGemmParams& prm, int& tm, int& tn)
 {
   int pid = blockIdx.x;
@@ -397,10 +398,8 @@ __device__ __forceinline__ void dm_tile(const ModGemmParams& prm, int& tm, int& 
     tm = first + r % gm;
     tn = r / gm;
   } else {
-    int i = (int)((sqrtf(8.0f * (float)pid + 1.0f) - 1.0f) * 0.5f);
-    while ((i + 1) * (i + 2) / 2 <= pid) ++i;
-    while (i * (i + 1) / 2 > pid) --i;
-    const int j = pid - i * (i + 1) / 2;
+    int i, j;
+    tri_tile<4>(blockIdx.x, total, prm.tiles_m, i, j);
     if (prm.tri == TRI_LOWER) { tm = i; tn = j; }
     else { tm = j; tn = i; }
   }
