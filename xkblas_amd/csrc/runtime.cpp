@@ -26,8 +26,11 @@
 #include <algorithm>
 #include <climits>
 #include <cmath>
+#include <condition_variable>
 #include <cstring>
 #include <ctime>
+#include <functional>
+#include <thread>
 #include <mutex>
 #include <string>
 #include <unordered_map>
@@ -99,6 +102,7 @@ void Runtime::evict_for(size_t need, const std::vector<Replica*>& keep)
     }
     writeback(victim, 0, 0, victim->M, victim->N, 0);
     XK_HIP(hipStreamSynchronize(s_d2h));
+    drain_stages();
     drop(victim);
     stats.evictions++;
   }
@@ -129,6 +133,7 @@ Replica* Runtime::replica(const void* host, int M, int N, long ld, int elt, int 
     // a pointer reused with new dimensions gets a fresh mirror.
     writeback(r, 0, 0, r->M, r->N, 0);
     XK_HIP(hipStreamSynchronize(s_d2h));
+    drain_stages();
     drop(r);
   }
   Replica* r = new Replica();
@@ -167,8 +172,7 @@ void Runtime::fetch(Replica* r, int r0, int c0, int m, int n)
       const int col0 = tj * r->nb, col1 = std::min(r->N, (tj + 1) * r->nb);
       const char* src = (const char*)r->host + ((size_t)row0 + (size_t)col0 * r->ld) * r->elt;
       char* dst = r->dev + ((size_t)row0 + (size_t)col0 * r->ldd) * r->elt;
-      XK_HIP(hipMemcpy2DAsync(dst, r->ldd * r->elt, src, r->ld * r->elt, (size_t)(row1 - row0) * r->elt,
-                              (size_t)(col1 - col0), hipMemcpyHostToDevice, s_h2d));
+      copy_h2d(dst, r->ldd * r->elt, src, r->ld * r->elt, (size_t)(row1 - row0) * r->elt, (size_t)(col1 - col0));
       stats.h2d_bytes += (double)(row1 - row0) * (col1 - col0) * r->elt;
       for (int t = ti; t <= te; ++t) r->state[tj * r->mt + t] |= DEV_VALID;
       pending_h2d = true;
@@ -216,8 +220,7 @@ void Runtime::writeback(Replica* r, int r0, int c0, int m, int n, int uplo)
       const int col0 = tj * r->nb, col1 = std::min(r->N, (tj + 1) * r->nb);
       char* dst = (char*)r->host + ((size_t)row0 + (size_t)col0 * r->ld) * r->elt;
       const char* src = r->dev + ((size_t)row0 + (size_t)col0 * r->ldd) * r->elt;
-      XK_HIP(hipMemcpy2DAsync(dst, r->ld * r->elt, src, r->ldd * r->elt, (size_t)(row1 - row0) * r->elt,
-                              (size_t)(col1 - col0), hipMemcpyDeviceToHost, s_d2h));
+      copy_d2h(dst, r->ld * r->elt, src, r->ldd * r->elt, (size_t)(row1 - row0) * r->elt, (size_t)(col1 - col0));
       stats.d2h_bytes += (double)(row1 - row0) * (col1 - col0) * r->elt;
       for (int t = ti; t <= te; ++t) r->state[tj * r->mt + t] |= HOST_VALID;
       any = true;
@@ -227,6 +230,161 @@ void Runtime::writeback(Replica* r, int r0, int c0, int m, int n, int uplo)
   if (any) {
     XK_HIP(hipEventRecord(ev_d2h, s_d2h));
     pending_d2h = true;
+  }
+}
+
+// ---------------------------------------------------------------- staging
+// Copy threads for packing/unpacking staging buffers (the calling thread
+// takes a share too).  Jobs are ranges of columns.
+namespace {
+class CopyPool {
+ public:
+  explicit CopyPool(int n) {
+    for (int i = 0; i < n; ++i) threads_.emplace_back([this, i] { loop(i + 1); });
+  }
+  ~CopyPool() {
+    {
+      std::lock_guard<std::mutex> g(m_);
+      stop_ = true;
+    }
+    cv_.notify_all();
+    for (auto& t : threads_) t.join();
+  }
+  // fn(lo, hi) over [0, count), split into (threads + 1) parts; returns when all are done
+  void run(size_t count, const std::function<void(size_t, size_t)>& fn) {
+    const size_t parts = threads_.size() + 1;
+    {
+      std::lock_guard<std::mutex> g(m_);
+      fn_ = &fn;
+      count_ = count;
+      parts_ = parts;
+      left_ = threads_.size();
+      ++gen_;
+    }
+    cv_.notify_all();
+    slice(0, fn);
+    std::unique_lock<std::mutex> g(m_);
+    done_.wait(g, [this] { return left_ == 0; });
+    fn_ = nullptr;
+  }
+
+ private:
+  void slice(size_t part, const std::function<void(size_t, size_t)>& fn) const {
+    const size_t lo = count_ * part / parts_, hi = count_ * (part + 1) / parts_;
+    if (lo < hi) fn(lo, hi);
+  }
+  void loop(size_t part) {
+    unsigned long long seen = 0;
+    for (;;) {
+      const std::function<void(size_t, size_t)>* fn;
+      {
+        std::unique_lock<std::mutex> g(m_);
+        cv_.wait(g, [&] { return stop_ || gen_ != seen; });
+        if (stop_) return;
+        seen = gen_;
+        fn = fn_;
+      }
+      slice(part, *fn);
+      {
+        std::lock_guard<std::mutex> g(m_);
+        if (--left_ == 0) done_.notify_one();
+      }
+    }
+  }
+  std::vector<std::thread> threads_;
+  std::mutex m_;
+  std::condition_variable cv_, done_;
+  const std::function<void(size_t, size_t)>* fn_ = nullptr;
+  size_t count_ = 0, parts_ = 1, left_ = 0;
+  unsigned long long gen_ = 0;
+  bool stop_ = false;
+};
+
+CopyPool& copy_pool()
+{
+  static CopyPool pool([] {
+    const char* e = getenv("XKBLAS_COPY_THREADS");
+    const int hw = (int)std::thread::hardware_concurrency();
+    const int n = e && *e ? atoi(e) : std::min(8, std::max(1, hw));
+    return std::max(0, n - 1);
+  }());
+  return pool;
+}
+
+// columns j in [0, height): dst + j dpitch <- src + j spitch, width bytes each
+void copy_columns(char* dst, size_t dpitch, const char* src, size_t spitch, size_t width, size_t height)
+{
+  copy_pool().run(height, [&](size_t lo, size_t hi) {
+    for (size_t j = lo; j < hi; ++j) memcpy(dst + j * dpitch, src + j * spitch, width);
+  });
+}
+}  // namespace
+
+bool Runtime::host_pinned(const void* p, size_t bytes) const
+{
+  const uintptr_t a = (uintptr_t)p;
+  for (const auto& r : pinned)
+    if (a >= r.first && a + bytes <= r.first + r.second) return true;
+  return false;
+}
+
+// The next staging buffer, with its previous copy finished (and unpacked).
+int Runtime::take_stage()
+{
+  const int b = stage_next;
+  stage_next = (b + 1) % NSTAGE;
+  if (!stage[b]) XK_HIP(hipHostMalloc((void**)&stage[b], STAGE_BYTES, hipHostMallocDefault));
+  finish_stage(b);
+  return b;
+}
+
+void Runtime::finish_stage(int b)
+{
+  XK_HIP(hipEventSynchronize(stage_ev[b]));
+  Unpack& u = unpack[b];
+  if (u.live) {
+    copy_columns(u.dst, u.dpitch, stage[b], u.width, u.width, u.height);
+    u.live = false;
+  }
+}
+
+void Runtime::drain_stages()
+{
+  for (int i = 0; i < NSTAGE; ++i) finish_stage((stage_next + i) % NSTAGE);
+}
+
+void Runtime::copy_h2d(char* dst, size_t dpitch, const char* src, size_t spitch, size_t width, size_t height)
+{
+  const size_t span = (height - 1) * spitch + width;
+  if (!staging || width > STAGE_BYTES || host_pinned(src, span)) {
+    XK_HIP(hipMemcpy2DAsync(dst, dpitch, src, spitch, width, height, hipMemcpyHostToDevice, s_h2d));
+    return;
+  }
+  drain_stages();  // host data written back through the ring is in place first
+  const size_t per = STAGE_BYTES / width;
+  for (size_t c = 0; c < height; c += per) {
+    const size_t cc = std::min(per, height - c);
+    const int b = take_stage();
+    copy_columns(stage[b], width, src + c * spitch, spitch, width, cc);
+    XK_HIP(hipMemcpy2DAsync(dst + c * dpitch, dpitch, stage[b], width, width, cc, hipMemcpyHostToDevice, s_h2d));
+    XK_HIP(hipEventRecord(stage_ev[b], s_h2d));
+  }
+}
+
+void Runtime::copy_d2h(char* dst, size_t dpitch, const char* src, size_t spitch, size_t width, size_t height)
+{
+  const size_t span = (height - 1) * dpitch + width;
+  if (!staging || width > STAGE_BYTES || host_pinned(dst, span)) {
+    XK_HIP(hipMemcpy2DAsync(dst, dpitch, src, spitch, width, height, hipMemcpyDeviceToHost, s_d2h));
+    return;
+  }
+  const size_t per = STAGE_BYTES / width;
+  for (size_t c = 0; c < height; c += per) {
+    const size_t cc = std::min(per, height - c);
+    const int b = take_stage();
+    XK_HIP(hipMemcpy2DAsync(stage[b], width, src + c * spitch, spitch, width, cc, hipMemcpyDeviceToHost, s_d2h));
+    XK_HIP(hipEventRecord(stage_ev[b], s_d2h));
+    unpack[b] = Unpack{dst + c * dpitch, dpitch, width, cc, true};
   }
 }
 
@@ -268,6 +426,8 @@ void Runtime::init_device()
   XK_HIP(hipEventCreateWithFlags(&ev_comp, hipEventDisableTiming));
   XK_HIP(hipEventCreateWithFlags(&ev_h2d, hipEventDisableTiming));
   XK_HIP(hipEventCreateWithFlags(&ev_d2h, hipEventDisableTiming));
+  for (int b = 0; b < NSTAGE; ++b) XK_HIP(hipEventCreateWithFlags(&stage_ev[b], hipEventDisableTiming));
+  staging = !(getenv("XKBLAS_STAGING") && atoi(getenv("XKBLAS_STAGING")) == 0);
   XK_HIP(hipMalloc(&d_info, sizeof(int) * INFO_SLOTS));
   XK_HIP(hipHostMalloc(&h_info, sizeof(int) * INFO_SLOTS, hipHostMallocDefault));
   for (int i = 0; i < INFO_SLOTS; ++i) h_info[i] = INT_MAX;
@@ -294,6 +454,7 @@ void Runtime::sync_all()
   XK_HIP(hipStreamSynchronize(s_comp));
   XK_HIP(hipStreamSynchronize(s_diag));
   XK_HIP(hipStreamSynchronize(s_d2h));
+  drain_stages();
   prof_collect();
 }
 
@@ -512,13 +673,24 @@ void* xkblas_malloc(size_t sz)
 {
   void* p = nullptr;
   if (hipHostMalloc(&p, sz, hipHostMallocDefault) != hipSuccess) return nullptr;
+  Lock l(g_mu);
+  rt().pinned.emplace_back((uintptr_t)p, sz);
   return p;
 }
 
 void xkblas_free(void* ptr, size_t sz)
 {
   (void)sz;
-  if (ptr) (void)hipHostFree(ptr);
+  if (!ptr) return;
+  {
+    Lock l(g_mu);
+    auto& v = rt().pinned;
+    v.erase(std::remove_if(v.begin(), v.end(), [&](const std::pair<uintptr_t, size_t>& r) {
+              return r.first == (uintptr_t)ptr;
+            }),
+            v.end());
+  }
+  (void)hipHostFree(ptr);
 }
 
 int xkblas_memory_coherent_async(int uplo, int memflag, size_t M, size_t N, void* A, size_t ld, size_t elsize)

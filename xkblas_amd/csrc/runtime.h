@@ -1,6 +1,7 @@
 // runtime.h — state of the single-GPU runtime behind xkblas.h.
 #pragma once
 #include "blas3.h"
+#include <cstdint>
 #include <map>
 #include <mutex>
 #include <string>
@@ -65,6 +66,23 @@ struct Runtime {
   int* h_info = nullptr;
   int next_info = 0;
 
+  // Pageable host memory goes through a ring of pinned staging buffers: a
+  // host-to-device chunk is packed by the copy threads, then sent; a
+  // device-to-host chunk is unpacked when its buffer is next taken or at
+  // sync.  Memory from xkblas_malloc (pinned) is copied directly.
+  static constexpr int NSTAGE = 4;
+  static constexpr size_t STAGE_BYTES = (size_t)64 << 20;
+  bool staging = true;
+  char* stage[NSTAGE] = {};
+  hipEvent_t stage_ev[NSTAGE] = {};
+  struct Unpack {
+    char* dst = nullptr;
+    size_t dpitch = 0, width = 0, height = 0;
+    bool live = false;
+  } unpack[NSTAGE];
+  int stage_next = 0;
+  std::vector<std::pair<uintptr_t, size_t>> pinned;  // xkblas_malloc ranges
+
   struct {
     double h2d_bytes = 0, d2h_bytes = 0;
     long evictions = 0;
@@ -82,6 +100,12 @@ struct Runtime {
   void evict_for(size_t need, const std::vector<Replica*>& keep);
   void drop(Replica* r);
   void before_compute();
+  bool host_pinned(const void* p, size_t bytes) const;
+  void copy_h2d(char* dst, size_t dpitch, const char* src, size_t spitch, size_t width, size_t height);
+  void copy_d2h(char* dst, size_t dpitch, const char* src, size_t spitch, size_t width, size_t height);
+  int take_stage();
+  void finish_stage(int b);
+  void drain_stages();
   Exec exec() const;
   void prof_begin(const char* cls, double flops);
   void prof_end();
