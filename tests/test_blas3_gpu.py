@@ -279,3 +279,55 @@ def test_tile_cache_eviction(xk):
     assert L.xkblas_cache_evictions() > ev0
     L.xkblas_set_cache_limit(1 << 40)
     xk.invalidate()
+
+
+def _pinned(L, shape):
+    """A Fortran-ordered float64 array in xkblas_malloc (pinned) memory."""
+    import ctypes
+    nbytes = int(np.prod(shape)) * 8
+    p = L.xkblas_malloc(nbytes)
+    assert p, "xkblas_malloc failed"
+    flat = np.ctypeslib.as_array(ctypes.cast(p, ctypes.POINTER(ctypes.c_double)), shape=(int(np.prod(shape)),))
+    return p, nbytes, flat.reshape(shape, order="F")
+
+
+def test_tile_cache_staging_chunks_and_pinned(xk):
+    """Tiles larger than one 64 MiB staging buffer (4096 x 4096 FP64 = 2 chunks
+    each, wrapping the 4-buffer ring) round-trip through pageable host memory;
+    the same product on xkblas_malloc memory (direct pinned copies) is bit
+    identical."""
+    L = xk.lib()
+    r = rng(47)
+    n = 4100
+    A, B, C0 = uniform(r, n, n), uniform(r, n, n), uniform(r, n, n)
+    nb0 = L.xkblas_get_param()
+    L.xkblas_set_param(4096, 0)
+    try:
+        C = C0.copy(order="F")
+        xk.gemm(111, 111, 1.0, A, B, 0.5, C)
+        xk.invalidate()
+        ref = A @ B + 0.5 * C0
+        eps = np.finfo(np.float64).eps
+        err = np.abs(C - ref).max() / ((np.abs(A).max() * np.abs(B).max() * n + np.abs(C0).max()) * eps)
+        assert err < 3.0, err
+        bufs = []
+        try:
+            for X in (A, B, C0):
+                p, nb, Y = _pinned(L, X.shape)
+                Y[...] = X
+                bufs.append((p, nb, Y))
+            Ap, Bp, Cp = (b[2] for b in bufs)
+            xk.gemm(111, 111, 1.0, Ap, Bp, 0.5, Cp)
+            xk.invalidate()
+            assert np.array_equal(Cp, C)
+            # pinned operands, pageable output: the write-back goes through the ring
+            C2 = C0.copy(order="F")
+            xk.gemm(111, 111, 1.0, Ap, Bp, 0.5, C2)
+            xk.invalidate()
+            assert np.array_equal(C2, C)
+        finally:
+            for p, nb, _ in bufs:
+                L.xkblas_free(p, nb)
+    finally:
+        L.xkblas_set_param(nb0, 0)
+        xk.invalidate()
