@@ -342,7 +342,10 @@ def main():
             potrf["large_fp64_mfma"] = p64
             potrf["large_split_bf16"] = psp
             potrf["large_split_speedup"] = round(p64["kernel_ms"] / psp["kernel_ms"], 3)
-            del S, F64, Fsp
+            del Fsp
+            print("bench: dposv_ir n=%d nrhs 64" % nbig, file=sys.stderr, flush=True)
+            potrf["posv_ir"] = bench_posv_ir(xk, S, F64, nbsp, p64["kernel_ms"], psp["kernel_ms"])
+            del S, F64
 
     xk.invalidate()
     print("bench: dgemm 2048 tile 512 (the CPU-runnable configuration)", file=sys.stderr, flush=True)
@@ -583,6 +586,64 @@ def bench_potrf(xk, S, nb, mode, dsyrk_tflops=None, samples=96):
     if dsyrk_tflops is not None:
         out["dsyrk_tflops"] = round(dsyrk_tflops, 3)
     return out, F
+
+
+def sampled_backward_error(S, X, B, samples=96, seed=11):
+    """Componentwise (Oettli-Prager) backward error of X for A X = B over
+    sampled rows: max |B - A X|_ij / (|A| |X|)_ij, A symmetric with its lower
+    triangle in S."""
+    n = S.shape[0]
+    rows = np.random.default_rng(seed).integers(0, n, samples)
+    worst = 0.0
+    for i in rows:
+        a = np.concatenate([S[i, :i + 1], S[i + 1:, i]])
+        r = np.abs(B[i] - a @ X)
+        d = np.abs(a) @ np.abs(X)
+        worst = max(worst, float((r / d).max()))
+    return worst
+
+
+def bench_posv_ir(xk, S, F64, nb, p64_ms, psp_ms, nrhs=64, moduli=7):
+    """xkblas_dposv_ir (split-BF16 factor with `moduli` CRT moduli, FP64
+    residuals and corrections) on A = S (lower), B ~ U[-0.5,0.5] n x nrhs,
+    against the FP64 route (DPOTRF kernel time p64_ms + two DTRSM with the FP64
+    factor F64) and the accurate split route (psp_ms + the same DTRSMs).  The
+    call is synchronous; its time is the HIP-event window around it with A
+    already resident in HBM (second call)."""
+    L = xk.lib()
+    n = S.shape[0]
+    B = np.asfortranarray(np.random.default_rng(3).uniform(-0.5, 0.5, (n, nrhs)))
+    L.xkblas_set_modemath(0)
+    L.xkblas_set_param(nb, 0)
+    L.xkblas_set_posv_ir(moduli, 30)
+    best = None
+    for _ in range(2):
+        L.xkblas_profile_reset()
+        X, info, it = xk.posv_ir(122, S, B)
+        ms = xk.profile("posv_ir_f64")[1]
+        best = ms if best is None else min(best, ms)
+    L.xkblas_set_posv_ir(0, -1)
+    xk.invalidate()
+    Y = B.copy(order="F")
+    one = np.array([1.0])
+    vp = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    L.xkblas_set_param(1024, 0)
+    L.xkblas_profile_reset()
+    for ta in (111, 112):
+        L.xkblas_dtrsm_async(141, 122, ta, 131, n, nrhs, vp(one), vp(F64), n, vp(Y), n)
+    L.xkblas_sync()
+    trsm_ms = xk.profile("trsm_f64")[1]
+    xk.coherent(Y)
+    xk.invalidate()
+    fl = n**3 / 3.0 + 2.0 * n * n * nrhs
+    return {"n": n, "nrhs": nrhs, "tile": nb, "factor": "split_bf16, %d moduli" % moduli, "info": info,
+            "iterations": it, "ms": round(best, 3), "tflops_equivalent": round(fl / best / 1e9, 3),
+            "fp64_route_ms": round(p64_ms + trsm_ms, 3), "split_accurate_route_ms": round(psp_ms + trsm_ms, 3),
+            "two_dtrsm_ms": round(trsm_ms, 3), "speedup_vs_fp64_route": round((p64_ms + trsm_ms) / best, 3),
+            "speedup_vs_split_accurate_route": round((psp_ms + trsm_ms) / best, 3),
+            "sampled_backward_error": sampled_backward_error(S, X, B),
+            "fp64_route_sampled_backward_error": sampled_backward_error(S, Y, B),
+            "max_rel_diff_vs_fp64_route": float(np.abs(X - Y).max() / np.abs(Y).max())}
 
 
 if __name__ == "__main__":

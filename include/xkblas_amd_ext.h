@@ -96,6 +96,25 @@ int  xkblas_split_info(int mode, int k, int nsplit, int* nmod, int* nplane);
  * per panel.  Environment XKBLAS_POTRF_PAIR sets the initial value. */
 void xkblas_set_potrf_pairing(int mode);
 
+/* Mixed-precision solve of A X = B, A symmetric positive definite (the uplo
+ * triangle of A is read), B and X N x NRHS (X must not be B).  No counterpart
+ * in the reference; argument order, codes and *iter follow LAPACK dsposv:
+ * the Cholesky factor is computed in split-BF16 mode with a reduced number of
+ * CRT moduli (diagonal blocks in FP64), the residual B - A X and the updates
+ * are FP64, and refinement stops when every column has
+ * ||r||_inf < sqrt(N) ||x||_inf ||A||_inf eps.  *iter = corrections applied;
+ * -3 if the reduced factor failed, -(itermax + 1) after itermax corrections:
+ * then X comes from the factor of the current mode (as dsposv falls back to
+ * dpotrf).  Returns 0, > 0 (LAPACK info: A not positive definite) or < 0
+ * (argument).  Synchronous; X is then valid on the device and reaches host
+ * memory by xkblas_memory_coherent_async + xkblas_sync, like every result. */
+int  xkblas_dposv_ir(int uplo, int N, int NRHS, const double* A, int LDA, const double* B, int LDB, double* X,
+                     int LDX, int* iter);
+/* CRT moduli of the reduced factor (default 7: about 24 bits per side at
+ * inner dimension 4096; clamped to 4..13) and the correction limit (default
+ * 30, as dsposv); values <= 0 (< 0 for itermax) restore the defaults. */
+void xkblas_set_posv_ir(int moduli, int itermax);
+
 /* device memory helpers for batched operands */
 void* xkblas_device_malloc(size_t bytes);
 void  xkblas_device_free(void* ptr);

@@ -134,6 +134,31 @@ print(L.xkblas_dgemm_async(999, 111, 2, 2, 2, one, buf, 2, buf, 2, one, buf, 2),
     assert out.stdout.split() == ["-1", "-8", "-13", "-2", "-3", "-8"]
 
 
+def test_posv_ir_illegal_arguments_return_codes():
+    """xkblas_dposv_ir checks its arguments in LAPACK dsposv's order before
+    any device use."""
+    code = r'''
+import ctypes, sys
+sys.path.insert(0, %r)
+import xkblas_amd as x
+L = x.lib()
+a, b, xx = (ctypes.c_double * 16)(), (ctypes.c_double * 16)(), (ctypes.c_double * 16)()
+it = ctypes.c_int(7)
+print(L.xkblas_dposv_ir(999, 4, 1, a, 4, b, 4, xx, 4, ctypes.byref(it)),
+      L.xkblas_dposv_ir(122, -1, 1, a, 4, b, 4, xx, 4, ctypes.byref(it)),
+      L.xkblas_dposv_ir(122, 4, -1, a, 4, b, 4, xx, 4, ctypes.byref(it)),
+      L.xkblas_dposv_ir(122, 4, 1, a, 2, b, 4, xx, 4, ctypes.byref(it)),
+      L.xkblas_dposv_ir(122, 4, 1, a, 4, b, 2, xx, 4, ctypes.byref(it)),
+      L.xkblas_dposv_ir(122, 4, 1, a, 4, b, 4, xx, 2, ctypes.byref(it)),
+      L.xkblas_dposv_ir(122, 4, 1, a, 4, b, 4, b, 4, ctypes.byref(it)),
+      L.xkblas_dposv_ir(122, 0, 1, a, 1, b, 1, xx, 1, ctypes.byref(it)), it.value)
+''' % ROOT
+    env = dict(os.environ, XKBLAS_ERROR_RETURN="1")
+    out = subprocess.run(["python", "-c", code], capture_output=True, text=True, env=env, timeout=120)
+    assert out.returncode == 0, out.stderr
+    assert out.stdout.split() == ["-1", "-2", "-3", "-5", "-7", "-9", "-8", "0", "0"]
+
+
 def test_illegal_argument_aborts_by_default():
     """kaapi_error semantics: print and abort (blas/common.gen:312-316)."""
     code = r'''

@@ -127,6 +127,23 @@ def test_dtrsm_vs_oracle(xk, orc, side, uplo, ta, diag):
     assert orc.trsm_residual(A, uplo, diag, Xref, X) <= 10.0
 
 
+@pytest.mark.parametrize("uplo", [121, 122])
+@pytest.mark.parametrize("ta", TR)
+def test_dtrsm_batched_block_inverses(xk, orc, uplo, ta):
+    """Order 1500 with 512-blocks: two full diagonal blocks inverted in one
+    batched recursion (three levels) plus a ragged 476 block."""
+    M, N = 1500, 64
+    r = rng(uplo * 3 + ta)
+    A = tri_matrix(r, M, uplo == 122, False)
+    B0 = uniform(r, M, N, 0.0, 1.0)
+    Xref = orc.dtrsm_tiled(141, uplo, ta, 131, 1.0, A, B0.copy(order="F"), 512)
+    X = B0.copy(order="F")
+    xk.lib().xkblas_set_param(512, 0)
+    xk.trsm(141, uplo, ta, 131, 1.0, A, X)
+    xk.invalidate()
+    assert orc.trsm_residual(A, uplo, 131, Xref, X) <= 10.0
+
+
 def test_dtrsm_exact_unit_lower(xk):
     """L unit lower with small integers, X integers: B = L X exactly, solve returns X."""
     r = rng(21)

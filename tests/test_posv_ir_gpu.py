@@ -1,0 +1,133 @@
+"""Mixed-precision SPD solve with FP64 iterative refinement (xkblas_dposv_ir).
+
+The stopping rule is LAPACK dsposv's: every column of r = B - A X satisfies
+||r||_inf < sqrt(n) ||x||_inf ||A||_inf eps, eps = 2^-53.  The tests check it
+on the host from the returned X (FP64 NumPy residual, a factor 2 of slack for
+the different summation order), compare X with LAPACK's FP64 solve
+(numpy.linalg.solve), check that only the uplo triangle of A is read (the
+other one holds NaN), and force both fallbacks of dsposv: the correction
+limit (itermax 0) and a matrix that is not positive definite.
+"""
+import numpy as np
+import pytest
+
+from helpers import rng, spd, uniform
+
+pytestmark = pytest.mark.gpu
+
+LOWER, UPPER = 122, 121
+EPS = 2.0 ** -53
+
+
+def _stored(A, uplo):
+    """A with the triangle that is not read set to NaN."""
+    S = A.copy(order="F")
+    iu = np.triu_indices(A.shape[0], 1) if uplo == LOWER else np.tril_indices(A.shape[0], -1)
+    S[iu] = np.nan
+    return S
+
+
+def _criterion(A, B, X):
+    n = A.shape[0]
+    R = B - A @ X
+    rn = np.abs(R).max(axis=0)
+    xn = np.abs(X).max(axis=0)
+    anrm = np.abs(A).sum(axis=1).max()
+    return rn / (xn * anrm * EPS * np.sqrt(n))
+
+
+@pytest.mark.parametrize("uplo", [LOWER, UPPER])
+@pytest.mark.parametrize("n,nrhs,nb", [(300, 1, 128), (1000, 37, 256), (2048, 64, 512), (1100, 5, 1100)])
+def test_dposv_ir_converges(xk, uplo, n, nrhs, nb):
+    L = xk.lib()
+    r = rng(n * 7 + nrhs)
+    A = spd(r, n)
+    B = uniform(r, n, nrhs)
+    nb0 = L.xkblas_get_param()
+    L.xkblas_set_param(nb, 0)
+    try:
+        X, info, it = xk.posv_ir(uplo, _stored(A, uplo), B)
+    finally:
+        L.xkblas_set_param(nb0, 0)
+        xk.invalidate()
+    assert info == 0
+    if nb >= n:
+        # one diagonal block: the whole factor is FP64, nothing to refine
+        assert it == 0
+    else:
+        # split products of the reduced factor are not FP64-accurate
+        assert 1 <= it <= 10, it
+    assert np.all(_criterion(A, B, X) < 2.0)
+    Xr = np.linalg.solve(A, B)
+    assert np.abs(X - Xr).max() / np.abs(Xr).max() < 1e-13
+
+
+@pytest.mark.parametrize("mode", [0, 2])
+def test_dposv_ir_fallback_after_itermax(xk, mode):
+    """itermax 0: no correction allowed, so X comes from the factor of the
+    current mode (FP64 MFMA or 13-moduli split-BF16) and *iter = -1."""
+    L = xk.lib()
+    r = rng(91 + mode)
+    n, nrhs = 700, 9
+    A = spd(r, n)
+    B = uniform(r, n, nrhs)
+    nb0 = L.xkblas_get_param()
+    L.xkblas_set_param(128, 0)
+    L.xkblas_set_posv_ir(7, 0)
+    L.xkblas_set_modemath(mode)
+    try:
+        X, info, it = xk.posv_ir(LOWER, _stored(A, LOWER), B)
+    finally:
+        L.xkblas_set_posv_ir(0, -1)
+        L.xkblas_set_modemath(0)
+        L.xkblas_set_param(nb0, 0)
+        xk.invalidate()
+    assert (info, it) == (0, -1)
+    assert np.all(_criterion(A, B, X) < 2.0)
+    Xr = np.linalg.solve(A, B)
+    assert np.abs(X - Xr).max() / np.abs(Xr).max() < 1e-13
+
+
+def test_dposv_ir_more_moduli_fewer_corrections(xk):
+    """More CRT moduli make the reduced factor more accurate: never more
+    corrections."""
+    L = xk.lib()
+    r = rng(93)
+    n = 1024
+    A = spd(r, n)
+    B = uniform(r, n, 3)
+    its = []
+    nb0 = L.xkblas_get_param()
+    L.xkblas_set_param(256, 0)
+    try:
+        for m in (6, 7, 9):
+            L.xkblas_set_posv_ir(m, 30)
+            X, info, it = xk.posv_ir(UPPER, _stored(A, UPPER), B)
+            xk.invalidate()
+            assert info == 0 and it >= 0
+            assert np.all(_criterion(A, B, X) < 2.0)
+            its.append(it)
+    finally:
+        L.xkblas_set_posv_ir(0, -1)
+        L.xkblas_set_param(nb0, 0)
+    assert its[0] >= its[1] >= its[2] and its[0] >= 1, its
+
+
+@pytest.mark.parametrize("bad", [0, 333, 999])
+def test_dposv_ir_not_positive_definite(xk, bad):
+    """A negative pivot at order bad+1: the reduced factor fails (*iter = -3)
+    and so does the accurate one, whose LAPACK info is returned."""
+    L = xk.lib()
+    r = rng(97)
+    n = 1000
+    A = np.asfortranarray(np.diag(r.uniform(1.0, 2.0, n)))
+    A[bad, bad] = -1.0
+    B = uniform(r, n, 2)
+    nb0 = L.xkblas_get_param()
+    L.xkblas_set_param(256, 0)
+    try:
+        X, info, it = xk.posv_ir(LOWER, A, B)
+    finally:
+        L.xkblas_set_param(nb0, 0)
+        xk.invalidate()
+    assert (info, it) == (bad + 1, -3)

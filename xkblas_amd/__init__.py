@@ -92,6 +92,8 @@ SIGNATURES = {
     "xkblas_set_gemm_tile": (None, [_i]),
     "xkblas_set_split_kernel": (None, [_i]),
     "xkblas_set_potrf_pairing": (None, [_i]),
+    "xkblas_dposv_ir": (_i, [_i, _i, _i, _p, _i, _p, _i, _p, _i, _p]),
+    "xkblas_set_posv_ir": (None, [_i, _i]),
     "xkblas_split_side_bits": (_i, [_i, _i]),
     "xkblas_split_info": (_i, [_i, _i, _i, _p, _p]),
     "xkblas_dgemm_native_": (None, [_p] * 13),
@@ -188,6 +190,24 @@ def potrf(uplo, A, sync=True):
         return rc
     coherent(A, uplo)
     return lib().xkblas_potrf_info(_ptr(A))
+
+
+def posv_ir(uplo, A, B):
+    """X solving A X = B (A SPD, uplo triangle) by xkblas_dposv_ir; returns
+    (X, info, iter) with X coherent in host memory."""
+    _check(A, np.float64)
+    _check(B, np.float64)
+    N, NRHS = B.shape
+    X = np.zeros((N, NRHS), order="F")
+    it = ctypes.c_int(0)
+    info = lib().xkblas_dposv_ir(uplo, N, NRHS, _ptr(A), max(1, A.shape[0]), _ptr(B), max(1, N), _ptr(X), max(1, N),
+                                 ctypes.byref(it))
+    if info >= 0 and N and NRHS:
+        coherent(X)
+    # X is this call's own array: drop its device mirror, so that a later
+    # array at the same address is not taken for it
+    lib().xkblas_memory_invalidate(_ptr(X))
+    return X, info, it.value
 
 
 def coherent(X, uplo=0):

@@ -24,7 +24,9 @@
 #include "split.h"
 #include <algorithm>
 #include <climits>
+#include <cmath>
 #include <cstdlib>
+#include <vector>
 
 namespace xk {
 
@@ -287,26 +289,70 @@ void dev_potrf(const Exec& x, int uplo, int n, T* A, long lda, int nb, int* info
   XK_HIP(hipEventDestroy(ev_look));
 }
 
+// Diagonal block order of the TRSM sweep for a triangle of order na.
+static inline int trsm_block(int nb, int na) { return std::max(64, std::min(nb, na)); }
+
+// dev_trtri over `cnt` triangular blocks of order n at once: block b is
+// A + b sA (leading dimension lda), its inverse Tinv + b sInv (ldinv).  Every
+// level of the recursion is one batched launch per step for all blocks.
 template <typename T>
-void dev_trsm(const Exec& x, int side, int uplo, int transA, int diag, int m, int n, T alpha, const T* A, long lda,
-              T* B, long ldb, int nb)
+static void trtri_batched(hipStream_t s, int uplo, int diag, int n, const T* A, long lda, long sA, T* Tinv,
+                          long ldinv, long sInv, int cnt, int wslot)
 {
-  if (m <= 0 || n <= 0) return;
-  if (alpha == T(0)) {
-    fill_zero<T>(x.s, m, n, B, ldb);
+  if (n <= 64) {
+    trtri_leaf<T>(s, uplo, diag, n, A, lda, sA, Tinv, ldinv, sInv, cnt);
     return;
   }
+  const int n1 = half64(n), n2 = n - n1;
+  trtri_batched<T>(s, uplo, diag, n1, A, lda, sA, Tinv, ldinv, sInv, cnt, wslot);
+  trtri_batched<T>(s, uplo, diag, n2, A + n1 + (long)n1 * lda, lda, sA, Tinv + n1 + (long)n1 * ldinv, ldinv, sInv,
+                   cnt, wslot);
+  const long sW = (long)n1 * n2;
+  T* W = (T*)scratch(wslot, sizeof(T) * (size_t)sW * cnt);
+  const T* I11 = Tinv;
+  const T* I22 = Tinv + n1 + (long)n1 * ldinv;
+  if (uplo == Lower) {
+    // I21 = -I22 A21 I11, strictly upper block zeroed
+    gemm<T>(s, NoTrans, NoTrans, n2, n1, n1, T(1), A + n1, lda, I11, ldinv, T(0), W, n2, TRI_FULL, cnt, sA, sInv, sW);
+    gemm<T>(s, NoTrans, NoTrans, n2, n1, n2, T(-1), I22, ldinv, W, n2, T(0), Tinv + n1, ldinv, TRI_FULL, cnt, sInv,
+            sW, sInv);
+    T* Z = Tinv + (long)n1 * ldinv;
+    gemm<T>(s, NoTrans, NoTrans, n1, n2, 0, T(0), Z, ldinv, Z, ldinv, T(0), Z, ldinv, TRI_FULL, cnt, 0, 0, sInv);
+  } else {
+    // I12 = -I11 A12 I22, strictly lower block zeroed
+    gemm<T>(s, NoTrans, NoTrans, n1, n2, n2, T(1), A + (long)n1 * lda, lda, I22, ldinv, T(0), W, n1, TRI_FULL, cnt,
+            sA, sInv, sW);
+    gemm<T>(s, NoTrans, NoTrans, n1, n2, n1, T(-1), I11, ldinv, W, n1, T(0), Tinv + (long)n1 * ldinv, ldinv,
+            TRI_FULL, cnt, sInv, sW, sInv);
+    T* Z = Tinv + n1;
+    gemm<T>(s, NoTrans, NoTrans, n2, n1, 0, T(0), Z, ldinv, Z, ldinv, T(0), Z, ldinv, TRI_FULL, cnt, 0, 0, sInv);
+  }
+}
+
+// Inverses of all nb x nb diagonal blocks of the triangular na x na matrix A,
+// stored one after another (nb x nb each) in inv: the full-order blocks in
+// one batched recursion, a shorter last block on its own.
+template <typename T>
+static void trsm_inverses(const Exec& x, int uplo, int diag, int na, const T* A, long lda, int nb, T* inv)
+{
+  const int full = na / nb, last = na - full * nb;
+  if (full > 0)
+    trtri_batched<T>(x.s, uplo, diag, nb, A, lda, (long)nb * (1 + lda), inv, nb, (long)nb * nb, full, WS_DIAG0);
+  if (last > 0)
+    dev_trtri<T>(x, x.s, uplo, diag, last, A + (long)full * nb * (1 + lda), lda, inv + (size_t)full * nb * nb, nb,
+                 WS_DIAG0);
+}
+
+// The block sweep of dev_trsm with the diagonal-block inverses given.
+template <typename T>
+static void trsm_sweep(const Exec& x, int side, int uplo, int transA, int m, int n, T alpha, const T* A, long lda,
+                       T* B, long ldb, int nb, const T* inv)
+{
   const bool tr = transA != NoTrans;
   const bool lower_eff = (uplo == Lower) != tr;
   const int na = side == Left ? m : n;
-  nb = std::max(64, std::min(nb, na));
   const int nblk = (na + nb - 1) / nb;
   auto bsz = [&](int i) { return i == nblk - 1 ? na - i * nb : nb; };
-  // inverses of all diagonal blocks of A, each nb x nb
-  T* inv = (T*)scratch(WS_TRSM, sizeof(T) * (size_t)nblk * nb * nb);
-  for (int d = 0; d < nblk; ++d)
-    dev_trtri<T>(x, x.s, uplo, diag, bsz(d), A + (long)d * nb * (1 + lda), lda, inv + (size_t)d * nb * nb, nb,
-                 WS_DIAG0);
   // op(A) block (i,j) pointer and the transpose flag to use with it
   auto opblk = [&](int i, int j) -> const T* {
     return tr ? A + (long)j * nb + (long)i * nb * lda : A + (long)i * nb + (long)j * nb * lda;
@@ -350,6 +396,111 @@ void dev_trsm(const Exec& x, int side, int uplo, int transA, int diag, int m, in
       }
     }
   }
+}
+
+template <typename T>
+void dev_trsm(const Exec& x, int side, int uplo, int transA, int diag, int m, int n, T alpha, const T* A, long lda,
+              T* B, long ldb, int nb)
+{
+  if (m <= 0 || n <= 0) return;
+  if (alpha == T(0)) {
+    fill_zero<T>(x.s, m, n, B, ldb);
+    return;
+  }
+  const int na = side == Left ? m : n;
+  nb = trsm_block(nb, na);
+  T* inv = (T*)scratch(WS_TRSM, sizeof(T) * (size_t)((na + nb - 1) / nb) * nb * nb);
+  trsm_inverses<T>(x, uplo, diag, na, A, lda, nb, inv);
+  trsm_sweep<T>(x, side, uplo, transA, m, n, alpha, A, lda, B, ldb, nb, inv);
+}
+
+// ---------------------------------------------------------------- SPD solve
+// LAPACK dsposv's mixed-precision scheme (factor in low precision, residuals
+// and updates in FP64, stop when every column satisfies
+// ||r||_inf < sqrt(n) ||x||_inf ||A||_inf eps, fall back to the accurate
+// factor after itermax corrections or a failed low-precision factor), with
+// the low-precision factor a reduced-moduli split-BF16 Cholesky whose
+// diagonal blocks stay FP64.  A's full symmetric image is kept for the
+// residual GEMM (B - A X); the solves reuse one set of diagonal-block
+// inverses of the factor.
+enum { WS_IR_A = 16, WS_IR_F = 17, WS_IR_R = 18, WS_IR_V = 19 };
+
+int dev_posv_ir(const Exec& x, const Exec& xl, const Exec& xf, int uplo, int n, int nrhs, const double* A, long lda,
+                const double* B, long ldb, double* X, long ldx, int nb, int itermax, int* iter)
+{
+  *iter = 0;
+  if (n <= 0 || nrhs <= 0) return 0;
+  const long ldf = (n + 31) / 32 * 32;
+  const int tnb = trsm_block(512, n);
+  const int nblk = (n + tnb - 1) / tnb;
+  double* Af = (double*)scratch(WS_IR_A, sizeof(double) * (size_t)ldf * n);
+  double* F = (double*)scratch(WS_IR_F, sizeof(double) * (size_t)ldf * n);
+  // R (n x nrhs) then the diagonal-block inverses of the factor
+  double* R = (double*)scratch(WS_IR_R, sizeof(double) * ((size_t)ldf * nrhs + (size_t)nblk * tnb * tnb));
+  double* inv = R + (size_t)ldf * nrhs;
+  // n column sums of |A|, nrhs residual norms, nrhs solution norms, info
+  double* V = (double*)scratch(WS_IR_V, sizeof(double) * ((size_t)n + 2 * nrhs + 1));
+  int* dinfo = (int*)(V + n + 2 * nrhs);
+  std::vector<double> hv((size_t)n + 2 * nrhs);
+  int hinfo = 0;
+
+  const int tA = uplo == Lower ? NoTrans : Trans;  // op of the first solve's factor
+  auto solve = [&](double* Y, long ldy) {          // Y <- F^-1 Y (F = L L^T or U^T U)
+    trsm_sweep<double>(x, Left, uplo, tA, n, nrhs, 1.0, F, ldf, Y, ldy, tnb, inv);
+    trsm_sweep<double>(x, Left, uplo, tA == NoTrans ? Trans : NoTrans, n, nrhs, 1.0, F, ldf, Y, ldy, tnb, inv);
+  };
+  auto factor = [&](const Exec& e) {  // F <- Cholesky factor of A's image under e; returns info
+    fill_int(x.s, dinfo, 1, INT_MAX);
+    dev_potrf<double>(e, uplo, n, F, ldf, nb, dinfo);
+    XK_HIP(hipMemcpyAsync(&hinfo, dinfo, sizeof(int), hipMemcpyDeviceToHost, x.s));
+    XK_HIP(hipStreamSynchronize(x.s));
+    return hinfo == INT_MAX ? 0 : hinfo;
+  };
+
+  symm_expand(x.s, uplo, n, A, lda, Af, ldf, F, ldf);
+  col_reduce(x.s, 1, n, n, Af, ldf, V);  // row sums of |A| = column sums (symmetric)
+  XK_HIP(hipMemcpyAsync(hv.data(), V, sizeof(double) * n, hipMemcpyDeviceToHost, x.s));
+  XK_HIP(hipStreamSynchronize(x.s));
+  double anrm = 0.0;
+  for (int j = 0; j < n; ++j) anrm = (hv[j] != hv[j] || hv[j] > anrm) ? hv[j] : anrm;
+  const double cte = anrm * 1.1102230246251565e-16 * std::sqrt((double)n);  // eps = 2^-53
+
+  int fallback = 0;
+  if (factor(xl) != 0) {
+    fallback = -3;
+  } else {
+    trsm_inverses<double>(x, uplo, NonUnit, n, F, ldf, tnb, inv);
+    copy2d<double>(x.s, n, nrhs, B, ldb, X, ldx);
+    solve(X, ldx);
+    for (int it = 0;; ++it) {
+      copy2d<double>(x.s, n, nrhs, B, ldb, R, ldf);
+      dev_gemm<double>(x, NoTrans, NoTrans, n, nrhs, n, -1.0, Af, ldf, X, ldx, 1.0, R, ldf);
+      col_reduce(x.s, 0, n, nrhs, R, ldf, V + n);
+      col_reduce(x.s, 0, n, nrhs, X, ldx, V + n + nrhs);
+      XK_HIP(hipMemcpyAsync(hv.data() + n, V + n, sizeof(double) * 2 * nrhs, hipMemcpyDeviceToHost, x.s));
+      XK_HIP(hipStreamSynchronize(x.s));
+      bool done = true;
+      for (int j = 0; j < nrhs && done; ++j) done = hv[n + j] < hv[n + nrhs + j] * cte;
+      if (done) {
+        *iter = it;
+        return 0;
+      }
+      if (it == itermax) break;
+      solve(R, ldf);
+      add2d(x.s, n, nrhs, R, ldf, X, ldx);
+    }
+    fallback = -(itermax + 1);
+  }
+  // the accurate factor, as LAPACK dsposv falls back to dpotrf
+  *iter = fallback;
+  symm_expand(x.s, uplo, n, A, lda, F, ldf, nullptr, 0);
+  const int info = factor(xf);
+  if (info != 0) return info;
+  trsm_inverses<double>(x, uplo, NonUnit, n, F, ldf, tnb, inv);
+  copy2d<double>(x.s, n, nrhs, B, ldb, X, ldx);
+  solve(X, ldx);
+  XK_HIP(hipStreamSynchronize(x.s));
+  return 0;
 }
 
 #define XK_INST(T)                                                                                             \

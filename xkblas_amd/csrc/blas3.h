@@ -39,6 +39,15 @@ void dev_potrf(const Exec& x, int uplo, int n, T* A, long lda, int nb, int* info
 // trailing updates of dev_potrf: 1 paired (default, also for -1), 0 one per panel
 void potrf_set_pairing(int v);
 
+// Solution of A X = B (A SPD, uplo triangle stored; B, X n x nrhs) from a
+// factor computed under `xl` (a reduced-precision split mode) and FP64
+// iterative refinement under `x`: LAPACK dsposv's scheme.  Returns the
+// LAPACK info of the factor finally used; *iter = corrections applied, or
+// -3 (reduced factor failed) / -(itermax + 1) (no convergence) when the
+// factor of mode `xf` was used instead.  Synchronises the stream.
+int dev_posv_ir(const Exec& x, const Exec& xl, const Exec& xf, int uplo, int n, int nrhs, const double* A, long lda,
+                const double* B, long ldb, double* X, long ldx, int nb, int itermax, int* iter);
+
 // Inverse of the triangular n x n block A into Tinv (other triangle zeroed).
 template <typename T>
 void dev_trtri(const Exec& x, hipStream_t s, int uplo, int diag, int n, const T* A, long lda, T* Tinv, long ldinv,

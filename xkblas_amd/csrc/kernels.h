@@ -47,4 +47,15 @@ void zero_triangle(hipStream_t s, int uplo, int n, T* A, long lda);
 // Fill an int array with a value (stream-ordered, graph-capturable).
 void fill_int(hipStream_t s, int* p, int n, int v);
 
+// Y (and Z when non-null) <- the full symmetric n x n matrix whose uplo
+// triangle is stored in A.
+void symm_expand(hipStream_t s, int uplo, int n, const double* A, long lda, double* Y, long ldy, double* Z, long ldz);
+
+// out[j] <- max_i |X(i, j)| (op 0, NaN-propagating) or sum_i |X(i, j)| (op 1)
+// for the n columns of the m x n block X.
+void col_reduce(hipStream_t s, int op, int m, int n, const double* X, long ldx, double* out);
+
+// X <- X + D (m x n blocks).
+void add2d(hipStream_t s, int m, int n, const double* D, long ldd, double* X, long ldx);
+
 }  // namespace xk

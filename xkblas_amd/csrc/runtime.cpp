@@ -975,6 +975,56 @@ int xkblas_spotrf_async(int uplo, int N, float* A, int LDA)
   return potrf_async<float>("xkblas_spotrf_async", uplo, N, A, LDA);
 }
 
+// ---- mixed-precision SPD solve with FP64 iterative refinement (no
+// counterpart in the reference; LAPACK dsposv's argument order and codes)
+static int g_ir_moduli = 7, g_ir_itermax = 30;
+
+void xkblas_set_posv_ir(int moduli, int itermax)
+{
+  Lock l(g_mu);
+  g_ir_moduli = moduli > 0 ? std::min(std::max(moduli, 4), 13) : 7;
+  g_ir_itermax = itermax >= 0 ? itermax : 30;
+}
+
+int xkblas_dposv_ir(int uplo, int N, int NRHS, const double* A, int LDA, const double* B, int LDB, double* X,
+                    int LDX, int* iter)
+{
+  const char* fn = "xkblas_dposv_ir";
+  if (uplo != Upper && uplo != Lower) return arg_error(fn, "illegal value of uplo", -1);
+  if (N < 0) return arg_error(fn, "illegal value of N", -2);
+  if (NRHS < 0) return arg_error(fn, "illegal value of NRHS", -3);
+  if (LDA < std::max(1, N)) return arg_error(fn, "illegal value of LDA", -5);
+  if (LDB < std::max(1, N)) return arg_error(fn, "illegal value of LDB", -7);
+  if (LDX < std::max(1, N)) return arg_error(fn, "illegal value of LDX", -9);
+  if (!iter) return arg_error(fn, "iter is NULL", -10);
+  if ((const void*)X == (const void*)B) return arg_error(fn, "X must not be B", -8);
+  *iter = 0;
+  if (N == 0 || NRHS == 0) return 0;
+  Lock l(g_mu);
+  ensure_init();
+  Runtime& r = rt();
+  const int nb = (int)xkblas_auto_tilesize(nullptr, KERN_POTRF, N, N, N);
+  std::vector<Replica*> keep;
+  Replica* rA = r.replica(A, N, N, LDA, sizeof(double), nb, keep);
+  Replica* rB = r.replica(B, N, NRHS, LDB, sizeof(double), nb, keep);
+  Replica* rX = r.replica(X, N, NRHS, LDX, sizeof(double), nb, keep);
+  r.fetch(rA, 0, 0, N, N);
+  r.fetch(rB, 0, 0, N, NRHS);
+  r.before_compute();
+  Exec xf = r.exec();              // the accurate factor: the current mode
+  Exec x = xf;                     // residuals, updates and solves: FP64
+  x.mode = XKBLAS_DEFAULT_MATH;
+  Exec xl = xf;                    // the reduced factor: split-BF16, fewer moduli
+  xl.mode = XKBLAS_SPLIT_BF16_MATH;
+  xl.nsplit = g_ir_moduli;
+  r.prof_begin("posv_ir_f64", (double)N * N * N / 3.0 + 2.0 * N * N * (double)NRHS);
+  const int info = dev_posv_ir(x, xl, xf, uplo, N, NRHS, (const double*)rA->dev, rA->ldd, (const double*)rB->dev,
+                               rB->ldd, (double*)rX->dev, rX->ldd, nb, g_ir_itermax, iter);
+  r.prof_end();
+  r.mark_written(rX, 0, 0, N, NRHS, 0);
+  return info;
+}
+
 // ---- profiling extension
 void xkblas_profile_enable(int on)
 {

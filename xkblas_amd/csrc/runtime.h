@@ -40,7 +40,7 @@ struct ProfStat {
 };
 
 struct Runtime {
-  static constexpr int NSCRATCH = 16;
+  static constexpr int NSCRATCH = 20;
   static constexpr int INFO_SLOTS = 1024;
 
   bool device_ready = false;
