@@ -225,6 +225,9 @@ def main():
                     help="profile helper: DPOTRF of order --potrf-split-n (block order --potrf-nb) in math mode "
                          "MODE, twice, no JSON")
     ap.add_argument("--potrf-nb", type=int, default=2048)
+    ap.add_argument("--only-posv-ir", action="store_true",
+                    help="profile helper: xkblas_dposv_ir of order --potrf-split-n, 64 right-hand sides, block "
+                         "order --potrf-nb, twice, no JSON")
     ap.add_argument("--only-batched", action="store_true", help="profile helper: the batched leg alone, no JSON")
     ap.add_argument("--batched-parts", action="store_true",
                     help="with --only-batched: also time the GEMM problems alone and the TRSM problems alone")
@@ -249,6 +252,17 @@ def main():
     if args.only_potrf >= 0:
         S, _ = make_spd(xk, args.potrf_split_n, 1024)
         print(json.dumps(bench_potrf(xk, S, args.potrf_nb, args.only_potrf)[0]), file=sys.stderr)
+        L.xkblas_finalize()
+        return
+    if args.only_posv_ir:
+        S, _ = make_spd(xk, args.potrf_split_n, 1024)
+        B = np.asfortranarray(np.random.default_rng(3).uniform(-0.5, 0.5, (S.shape[0], 64)))
+        L.xkblas_set_param(args.potrf_nb, 0)
+        for _ in range(2):
+            L.xkblas_profile_reset()
+            X, info, it = xk.posv_ir(122, S, B)
+            print(json.dumps({"info": info, "iterations": it, "ms": xk.profile("posv_ir_f64")[1]}), file=sys.stderr,
+                  flush=True)
         L.xkblas_finalize()
         return
     if args.only_batched:

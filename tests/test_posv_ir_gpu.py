@@ -4,7 +4,8 @@ The stopping rule is LAPACK dsposv's: every column of r = B - A X satisfies
 ||r||_inf < sqrt(n) ||x||_inf ||A||_inf eps, eps = 2^-53.  The tests check it
 on the host from the returned X (FP64 NumPy residual, a factor 2 of slack for
 the different summation order), compare X with LAPACK's FP64 solve
-(numpy.linalg.solve), check that only the uplo triangle of A is read (the
+(numpy.linalg.solve) and with the oracle's tiled Cholesky + two
+tiled triangular solves, check that only the uplo triangle of A is read (the
 other one holds NaN), and force both fallbacks of dsposv: the correction
 limit (itermax 0) and a matrix that is not positive definite.
 """
@@ -36,9 +37,20 @@ def _criterion(A, B, X):
     return rn / (xn * anrm * EPS * np.sqrt(n))
 
 
+def _oracle_solve(orc, uplo, A, B, nb):
+    """X from the oracle's tiled Cholesky and two tiled triangular solves."""
+    F = A.copy(order="F")
+    assert orc.dpotrf_tiled(uplo, F, nb) == 0
+    X = B.copy(order="F")
+    first, second = (111, 112) if uplo == LOWER else (112, 111)
+    orc.dtrsm_tiled(141, uplo, first, 131, 1.0, F, X, nb)
+    orc.dtrsm_tiled(141, uplo, second, 131, 1.0, F, X, nb)
+    return X
+
+
 @pytest.mark.parametrize("uplo", [LOWER, UPPER])
 @pytest.mark.parametrize("n,nrhs,nb", [(300, 1, 128), (1000, 37, 256), (2048, 64, 512), (1100, 5, 1100)])
-def test_dposv_ir_converges(xk, uplo, n, nrhs, nb):
+def test_dposv_ir_converges(xk, orc, uplo, n, nrhs, nb):
     L = xk.lib()
     r = rng(n * 7 + nrhs)
     A = spd(r, n)
@@ -58,7 +70,7 @@ def test_dposv_ir_converges(xk, uplo, n, nrhs, nb):
         # split products of the reduced factor are not FP64-accurate
         assert 1 <= it <= 10, it
     assert np.all(_criterion(A, B, X) < 2.0)
-    Xr = np.linalg.solve(A, B)
+    Xr = _oracle_solve(orc, uplo, A, B, nb)
     assert np.abs(X - Xr).max() / np.abs(Xr).max() < 1e-13
 
 

@@ -431,7 +431,10 @@ int dev_posv_ir(const Exec& x, const Exec& xl, const Exec& xf, int uplo, int n, 
   *iter = 0;
   if (n <= 0 || nrhs <= 0) return 0;
   const long ldf = (n + 31) / 32 * 32;
-  const int tnb = trsm_block(512, n);
+  // diagonal block order of the solves: few right-hand sides make each sweep
+  // step latency-bound, so fewer, larger steps pay (XKBLAS_IR_TRSM_NB)
+  static const int ir_nb = getenv("XKBLAS_IR_TRSM_NB") ? atoi(getenv("XKBLAS_IR_TRSM_NB")) : 1024;
+  const int tnb = trsm_block(ir_nb > 0 ? ir_nb / 64 * 64 : 1024, n);
   const int nblk = (n + tnb - 1) / tnb;
   double* Af = (double*)scratch(WS_IR_A, sizeof(double) * (size_t)ldf * n);
   double* F = (double*)scratch(WS_IR_F, sizeof(double) * (size_t)ldf * n);

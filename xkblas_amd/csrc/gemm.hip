@@ -293,9 +293,19 @@ void gemm(hipStream_t s, int transA, int transB, int m, int n, int k, T alpha, c
     // there are enough tiles to fill the chip
     long t128 = (long)((m + 127) / 128) * ((n + 127) / 128) * batch;
     bm = (sizeof(T) == 8 || m <= 64 || n <= 64 || t128 < 160) ? 64 : 128;
+    // FP64 products too small to give every CU two 64 x 64 tiles (the
+    // latency-bound steps of TRSM sweeps and diagonal-block chains): 32 x 32
+    // tiles, four times the workgroups; the k order per element, hence the
+    // result, is the same
+    long t64 = (long)((m + 63) / 64) * ((n + 63) / 64);
+    if (p.tri != TRI_FULL) t64 = t64 / 2 + (m + 63) / 64;
+    if (sizeof(T) == 8 && t64 * batch < 512) bm = 32;
   }
   if (bm == 128) launch_shape<T, 128, 16>(s, ta, tb, p, batch);
-  else launch_shape<T, 64, 16>(s, ta, tb, p, batch);
+  else if (bm == 32) {
+    if constexpr (sizeof(T) == 8) launch_shape<T, 32, 16>(s, ta, tb, p, batch);
+    else launch_shape<T, 64, 16>(s, ta, tb, p, batch);
+  } else launch_shape<T, 64, 16>(s, ta, tb, p, batch);
 }
 
 template void gemm<double>(hipStream_t, int, int, int, int, int, double, const double*, long, const double*, long,
