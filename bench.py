@@ -364,6 +364,8 @@ def main():
     xk.invalidate()
     print("bench: dgemm 2048 tile 512 (the CPU-runnable configuration)", file=sys.stderr, flush=True)
     c1 = bench_c1(xk)
+    print("bench: dgemmt/dsyr2k/dsymm/dtrmm 8192", file=sys.stderr, flush=True)
+    l3_more = bench_l3_more(xk)
     cpu = None if args.no_cpu else cpu_baseline()
 
     value = split["tflops"]
@@ -390,6 +392,7 @@ def main():
         "split_bf16": split,
         "split_fp8": split_fp8,
         "dgemm_2048_tile512": c1,
+        "l3_more_fp64": l3_more,
         "dpotrf": potrf,
         "batched": batched,
         "roofline": {"bound": "mfma", "achieved": round(bf16_achieved, 1), "peak": BF16_PEAK, "unit": "TFLOP/s",
@@ -600,6 +603,37 @@ def bench_potrf(xk, S, nb, mode, dsyrk_tflops=None, samples=96):
     if dsyrk_tflops is not None:
         out["dsyrk_tflops"] = round(dsyrk_tflops, 3)
     return out, F
+
+
+def bench_l3_more(xk, n=8192):
+    """DGEMMT (lower, NT), DSYR2K (lower, N), DSYMM (left, lower), DTRMM
+    (left, lower, N, non-unit) of order n in FP64 MFMA, operands resident in
+    HBM: the second of two calls, HIP-event kernel time.  TFLOP/s from the
+    textbook flop counts (triangle-only for GEMMT/SYR2K, m^2 n for TRMM)."""
+    L = xk.lib()
+    L.xkblas_set_modemath(0)
+    L.xkblas_set_param(0, 0)
+    r = np.random.default_rng(17)
+    A, B, C = (np.asfortranarray(r.uniform(-0.5, 0.5, (n, n))) for _ in range(3))
+    runs = {
+        "dgemmt": ("gemmt_f64", float(n) * n * (n + 1), lambda: xk.gemmt(122, 111, 112, 1.0, A, B, 0.5, C, sync=False)),
+        "dsyr2k": ("syr2k_f64", 2.0 * n * n * (n + 1), lambda: xk.syr2k(122, 111, 1.0, A, B, 0.5, C, sync=False)),
+        "dsymm": ("symm_f64", 2.0 * n ** 3, lambda: xk.symm(141, 122, 1.0, A, B, 0.5, C, sync=False)),
+        "dtrmm": ("trmm_f64", float(n) ** 3, lambda: xk.trmm(141, 122, 111, 131, 1.0, A, C, sync=False)),
+    }
+    out = {"n": n}
+    for name, (cls, fl, call) in runs.items():
+        best = None
+        for _ in range(2):
+            L.xkblas_profile_reset()
+            call()
+            L.xkblas_sync()
+            ms = xk.profile(cls)[1]
+            best = ms if best is None else min(best, ms)
+        out[name] = {"ms": round(best, 3), "tflops": round(fl / best / 1e9, 3),
+                     "frac_of_fp64_peak": round(fl / best / 1e9 / FP64_PEAK, 4)}
+    xk.invalidate()
+    return out
 
 
 def sampled_backward_error(S, X, B, samples=96, seed=11):

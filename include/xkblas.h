@@ -101,6 +101,25 @@ int xkblas_dtrsm_async(int side, int uplo, int transA, int diag, int N, int NRHS
  * with xkblas_potrf_info(A). */
 int xkblas_dpotrf_async(int uplo, int N, double* A, int LDA);
 
+/* The other L3 tile algorithms (reference blas/zgemmt.c, zsyr2k.c, zsymm.c,
+ * ztrmm.c), same argument order and codes.  GEMMT writes only the uplo
+ * triangle of C (all of C for uplo 0), as the host BLAS does. */
+int xkblas_dgemmt_async(int uplo, int transA, int transB, int N, int K,
+                        const double* alpha, const double* A, int LDA,
+                        const double* B, int LDB,
+                        const double* beta, double* C, int LDC);      /* ref blas/zxkblas.h:57, zgemmt.c:114 */
+int xkblas_dsyr2k_async(int uplo, int trans, int N, int K,
+                        const double* alpha, const double* A, int LDA,
+                        const double* B, int LDB,
+                        const double* beta, double* C, int LDC);      /* ref blas/zxkblas.h:107, zsyr2k.c:107 */
+int xkblas_dsymm_async(int side, int uplo, int M, int N,
+                       const double* alpha, const double* A, int LDA,
+                       const double* B, int LDB,
+                       const double* beta, double* C, int LDC);       /* ref blas/zxkblas.h:85, zsymm.c:107 */
+int xkblas_dtrmm_async(int side, int uplo, int transA, int diag, int N, int NRHS,
+                       const double* alpha, const double* A, int LDA,
+                       double* B, int LDB);                           /* ref blas/zxkblas.h:77, ztrmm.c:101 */
+
 /* Host-BLAS forwarders (reference blas/zgemm.c:373-414, zsyrk.c:322-357,
  * ztrsm.c:502-538): call the vendor BLAS named by XKBLAS_BLASLIB. */
 int xkblas_dgemm_native(int transA, int transB, int M, int N, int K,
@@ -127,6 +146,21 @@ int xkblas_strsm_async(int side, int uplo, int transA, int diag, int N, int NRHS
                        const float* alpha, const float* A, int LDA,
                        float* B, int LDB);
 int xkblas_spotrf_async(int uplo, int N, float* A, int LDA);
+int xkblas_sgemmt_async(int uplo, int transA, int transB, int N, int K,
+                        const float* alpha, const float* A, int LDA,
+                        const float* B, int LDB,
+                        const float* beta, float* C, int LDC);
+int xkblas_ssyr2k_async(int uplo, int trans, int N, int K,
+                        const float* alpha, const float* A, int LDA,
+                        const float* B, int LDB,
+                        const float* beta, float* C, int LDC);
+int xkblas_ssymm_async(int side, int uplo, int M, int N,
+                       const float* alpha, const float* A, int LDA,
+                       const float* B, int LDB,
+                       const float* beta, float* C, int LDC);
+int xkblas_strmm_async(int side, int uplo, int transA, int diag, int N, int NRHS,
+                       const float* alpha, const float* A, int LDA,
+                       float* B, int LDB);
 
 /* ---- extensions (no reference counterpart) -------------------------------- */
 /* LAPACK-style info of the last ?potrf_async on A (0 = success, k>0 = the

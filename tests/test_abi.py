@@ -159,6 +159,32 @@ print(L.xkblas_dposv_ir(999, 4, 1, a, 4, b, 4, xx, 4, ctypes.byref(it)),
     assert out.stdout.split() == ["-1", "-2", "-3", "-5", "-7", "-9", "-8", "0", "0"]
 
 
+def test_more_l3_illegal_arguments_return_codes():
+    """GEMMT/SYR2K/SYMM/TRMM argument codes of blas/zgemmt.c:123-159,
+    zsyr2k.c:119-150, zsymm.c:115-142, ztrmm.c:116-149."""
+    code = r'''
+import ctypes, sys
+sys.path.insert(0, %r)
+import xkblas_amd as x
+L = x.lib()
+one = (ctypes.c_double * 1)(1.0)
+buf = (ctypes.c_double * 64)()
+print(L.xkblas_dgemmt_async(122, 999, 111, 2, 2, one, buf, 2, buf, 2, one, buf, 2),
+      L.xkblas_dgemmt_async(122, 111, 111, 4, 2, one, buf, 2, buf, 2, one, buf, 4),
+      L.xkblas_dgemmt_async(0, 111, 111, 2, 2, one, buf, 2, buf, 2, one, buf, 1),
+      L.xkblas_dsyr2k_async(122, 113, 2, 2, one, buf, 2, buf, 2, one, buf, 2),
+      L.xkblas_dsyr2k_async(122, 111, 4, 2, one, buf, 4, buf, 2, one, buf, 4),
+      L.xkblas_dsymm_async(141, 122, 4, 2, one, buf, 2, buf, 4, one, buf, 4),
+      L.xkblas_dsymm_async(141, 122, 2, 2, one, buf, 2, buf, 2, one, buf, 1),
+      L.xkblas_dtrmm_async(141, 122, 111, 999, 2, 2, one, buf, 2, buf, 2),
+      L.xkblas_dtrmm_async(142, 122, 111, 131, 2, 4, one, buf, 2, buf, 2))
+''' % ROOT
+    env = dict(os.environ, XKBLAS_ERROR_RETURN="1")
+    out = subprocess.run(["python", "-c", code], capture_output=True, text=True, env=env, timeout=120)
+    assert out.returncode == 0, out.stderr
+    assert out.stdout.split() == ["-1", "-8", "-13", "-2", "-9", "-7", "-12", "-4", "-8"]
+
+
 def test_illegal_argument_aborts_by_default():
     """kaapi_error semantics: print and abort (blas/common.gen:312-316)."""
     code = r'''

@@ -92,6 +92,14 @@ SIGNATURES = {
     "xkblas_set_gemm_tile": (None, [_i]),
     "xkblas_set_split_kernel": (None, [_i]),
     "xkblas_set_potrf_pairing": (None, [_i]),
+    "xkblas_dgemmt_async": (_i, [_i, _i, _i, _i, _i, _p, _p, _i, _p, _i, _p, _p, _i]),
+    "xkblas_sgemmt_async": (_i, [_i, _i, _i, _i, _i, _p, _p, _i, _p, _i, _p, _p, _i]),
+    "xkblas_dsyr2k_async": (_i, [_i, _i, _i, _i, _p, _p, _i, _p, _i, _p, _p, _i]),
+    "xkblas_ssyr2k_async": (_i, [_i, _i, _i, _i, _p, _p, _i, _p, _i, _p, _p, _i]),
+    "xkblas_dsymm_async": (_i, [_i, _i, _i, _i, _p, _p, _i, _p, _i, _p, _p, _i]),
+    "xkblas_ssymm_async": (_i, [_i, _i, _i, _i, _p, _p, _i, _p, _i, _p, _p, _i]),
+    "xkblas_dtrmm_async": (_i, [_i, _i, _i, _i, _i, _i, _p, _p, _i, _p, _i]),
+    "xkblas_strmm_async": (_i, [_i, _i, _i, _i, _i, _i, _p, _p, _i, _p, _i]),
     "xkblas_dposv_ir": (_i, [_i, _i, _i, _p, _i, _p, _i, _p, _i, _p]),
     "xkblas_set_posv_ir": (None, [_i, _i]),
     "xkblas_split_side_bits": (_i, [_i, _i]),
@@ -190,6 +198,63 @@ def potrf(uplo, A, sync=True):
         return rc
     coherent(A, uplo)
     return lib().xkblas_potrf_info(_ptr(A))
+
+
+def gemmt(uplo, ta, tb, alpha, A, B, beta, C, sync=True):
+    """C <- alpha op(A) op(B) + beta C on the uplo triangle (0: all of C)."""
+    dt = C.dtype
+    for X in (A, B, C):
+        _check(X, dt)
+    N = C.shape[0]
+    K = A.shape[1] if ta == NoTrans else A.shape[0]
+    fn = lib().xkblas_dgemmt_async if dt == np.float64 else lib().xkblas_sgemmt_async
+    al, be = _scalar(alpha, dt), _scalar(beta, dt)
+    rc = fn(uplo, ta, tb, N, K, _ptr(al), _ptr(A), max(1, A.shape[0]), _ptr(B), max(1, B.shape[0]), _ptr(be), _ptr(C),
+            max(1, N))
+    if sync:
+        coherent(C)
+    return rc
+
+
+def syr2k(uplo, trans, alpha, A, B, beta, C, sync=True):
+    dt = C.dtype
+    for X in (A, B, C):
+        _check(X, dt)
+    N = C.shape[0]
+    K = A.shape[1] if trans == NoTrans else A.shape[0]
+    fn = lib().xkblas_dsyr2k_async if dt == np.float64 else lib().xkblas_ssyr2k_async
+    al, be = _scalar(alpha, dt), _scalar(beta, dt)
+    rc = fn(uplo, trans, N, K, _ptr(al), _ptr(A), max(1, A.shape[0]), _ptr(B), max(1, B.shape[0]), _ptr(be), _ptr(C),
+            max(1, N))
+    if sync:
+        coherent(C, uplo)
+    return rc
+
+
+def symm(side, uplo, alpha, A, B, beta, C, sync=True):
+    dt = C.dtype
+    for X in (A, B, C):
+        _check(X, dt)
+    M, N = C.shape
+    fn = lib().xkblas_dsymm_async if dt == np.float64 else lib().xkblas_ssymm_async
+    al, be = _scalar(alpha, dt), _scalar(beta, dt)
+    rc = fn(side, uplo, M, N, _ptr(al), _ptr(A), max(1, A.shape[0]), _ptr(B), max(1, M), _ptr(be), _ptr(C), max(1, M))
+    if sync:
+        coherent(C)
+    return rc
+
+
+def trmm(side, uplo, ta, diag, alpha, A, B, sync=True):
+    dt = B.dtype
+    for X in (A, B):
+        _check(X, dt)
+    M, N = B.shape
+    fn = lib().xkblas_dtrmm_async if dt == np.float64 else lib().xkblas_strmm_async
+    al = _scalar(alpha, dt)
+    rc = fn(side, uplo, ta, diag, M, N, _ptr(al), _ptr(A), max(1, A.shape[0]), _ptr(B), max(1, M))
+    if sync:
+        coherent(B)
+    return rc
 
 
 def posv_ir(uplo, A, B):

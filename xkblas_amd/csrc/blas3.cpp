@@ -414,6 +414,92 @@ void dev_trsm(const Exec& x, int side, int uplo, int transA, int diag, int m, in
   trsm_sweep<T>(x, side, uplo, transA, m, n, alpha, A, lda, B, ldb, nb, inv);
 }
 
+// ------------------------------------------------ GEMMT, SYR2K, SYMM, TRMM
+// The reference's tile algorithms for these (blas/zgemmt.c:253-380,
+// zsyr2k.c:235-325, zsymm.c:228-394, ztrmm.c:222-460) are GEMM tile loops
+// over the output; here each is one or two launches of the same kernels.
+enum { WS_X0 = 6, WS_X1 = 7 };
+
+template <typename T>
+void dev_gemmt(const Exec& x, int uplo, int transA, int transB, int n, int k, T alpha, const T* A, long lda,
+               const T* B, long ldb, T beta, T* C, long ldc)
+{
+  if (n <= 0) return;
+  if (alpha == T(0)) k = 0;  // A and B are not read
+  if (uplo != Lower && uplo != Upper) {
+    dev_gemm<T>(x, transA, transB, n, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+    return;
+  }
+  const int tri = uplo == Lower ? TRI_LOWER : TRI_UPPER;
+  if (split_enabled<T>(x) && k > 0 && alpha != T(0)) {
+    split_gemm<T>(x, transA, transB, n, n, k, alpha, A, lda, B, ldb, beta, C, ldc, tri);
+    return;
+  }
+  gemm<T>(x.s, transA, transB, n, n, k, alpha, A, lda, B, ldb, beta, C, ldc, tri);
+}
+
+template <typename T>
+void dev_syr2k(const Exec& x, int uplo, int trans, int n, int k, T alpha, const T* A, long lda, const T* B, long ldb,
+               T beta, T* C, long ldc)
+{
+  // C <- alpha (op(A) op(B)^T + op(B) op(A)^T) + beta C on the uplo triangle
+  const int ta = trans == NoTrans ? NoTrans : Trans;
+  const int tb = trans == NoTrans ? Trans : NoTrans;
+  dev_gemmt<T>(x, uplo, ta, tb, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+  if (k > 0 && alpha != T(0)) dev_gemmt<T>(x, uplo, ta, tb, n, k, alpha, B, ldb, A, lda, T(1), C, ldc);
+}
+
+template <typename T>
+void dev_symm(const Exec& x, int side, int uplo, int m, int n, T alpha, const T* A, long lda, const T* B, long ldb,
+              T beta, T* C, long ldc)
+{
+  if (m <= 0 || n <= 0) return;
+  const int ka = side == Left ? m : n;
+  if (alpha == T(0)) {
+    dev_gemm<T>(x, NoTrans, NoTrans, m, n, 0, T(0), B, ldb, B, ldb, beta, C, ldc);
+    return;
+  }
+  // the full symmetric operand, then one GEMM
+  const long ldf = (ka + 31) / 32 * 32;
+  T* F = (T*)scratch(WS_X0, sizeof(T) * (size_t)ldf * ka);
+  symm_expand<T>(x.s, uplo, ka, A, lda, F, ldf, nullptr, 0);
+  if (side == Left) dev_gemm<T>(x, NoTrans, NoTrans, m, n, m, alpha, F, ldf, B, ldb, beta, C, ldc);
+  else dev_gemm<T>(x, NoTrans, NoTrans, m, n, n, alpha, B, ldb, F, ldf, beta, C, ldc);
+}
+
+template <typename T>
+void dev_trmm(const Exec& x, int side, int uplo, int transA, int diag, int m, int n, T alpha, const T* A, long lda,
+              T* B, long ldb)
+{
+  // B <- alpha op(A) B (Left) or alpha B op(A) (Right), A triangular
+  if (m <= 0 || n <= 0) return;
+  if (alpha == T(0)) {
+    fill_zero<T>(x.s, m, n, B, ldb);
+    return;
+  }
+  const int ka = side == Left ? m : n;
+  const long ldf = (ka + 31) / 32 * 32, ldw = (m + 31) / 32 * 32;
+  T* F = (T*)scratch(WS_X0, sizeof(T) * (size_t)ldf * ka);
+  T* W = (T*)scratch(WS_X1, sizeof(T) * (size_t)ldw * n);
+  tri_expand<T>(x.s, uplo, diag == Unit, ka, A, lda, F, ldf);
+  copy2d<T>(x.s, m, n, B, ldb, W, ldw);
+  const int ta = transA == NoTrans ? NoTrans : Trans;
+  // op(A) is lower when (uplo == Lower) == (no transpose)
+  const bool op_lower = (uplo == Lower) == (ta == NoTrans);
+  if (split_enabled<T>(x)) {
+    if (side == Left) dev_gemm<T>(x, ta, NoTrans, m, n, m, alpha, F, ldf, W, ldw, T(0), B, ldb);
+    else dev_gemm<T>(x, NoTrans, ta, m, n, n, alpha, W, ldw, F, ldf, T(0), B, ldb);
+    return;
+  }
+  // only the inner range where the triangle is not zero
+  if (side == Left)
+    gemm<T>(x.s, ta, NoTrans, m, n, m, alpha, F, ldf, W, ldw, T(0), B, ldb, TRI_FULL, 1, 0, 0, 0,
+            op_lower ? KRANGE_A_LOWER : KRANGE_A_UPPER);
+  else
+    gemm<T>(x.s, NoTrans, ta, m, n, n, alpha, W, ldw, F, ldf, T(0), B, ldb, TRI_FULL, 1, 0, 0, 0,
+            op_lower ? KRANGE_B_LOWER : KRANGE_B_UPPER);
+}
+
 // ---------------------------------------------------------------- SPD solve
 // LAPACK dsposv's mixed-precision scheme (factor in low precision, residuals
 // and updates in FP64, stop when every column satisfies
@@ -496,7 +582,7 @@ int dev_posv_ir(const Exec& x, const Exec& xl, const Exec& xf, int uplo, int n, 
   }
   // the accurate factor, as LAPACK dsposv falls back to dpotrf
   *iter = fallback;
-  symm_expand(x.s, uplo, n, A, lda, F, ldf, nullptr, 0);
+  symm_expand<double>(x.s, uplo, n, A, lda, F, ldf, nullptr, 0);
   const int info = factor(xf);
   if (info != 0) return info;
   trsm_inverses<double>(x, uplo, NonUnit, n, F, ldf, tnb, inv);
@@ -511,7 +597,12 @@ int dev_posv_ir(const Exec& x, const Exec& xl, const Exec& xf, int uplo, int n, 
   template void dev_syrk<T>(const Exec&, int, int, int, int, T, const T*, long, T, T*, long);                     \
   template void dev_trsm<T>(const Exec&, int, int, int, int, int, int, T, const T*, long, T*, long, int);        \
   template void dev_potrf<T>(const Exec&, int, int, T*, long, int, int*);                                         \
-  template void dev_trtri<T>(const Exec&, hipStream_t, int, int, int, const T*, long, T*, long, int);
+  template void dev_trtri<T>(const Exec&, hipStream_t, int, int, int, const T*, long, T*, long, int);        \
+  template void dev_gemmt<T>(const Exec&, int, int, int, int, int, T, const T*, long, const T*, long, T, T*,     \
+                             long);                                                                             \
+  template void dev_syr2k<T>(const Exec&, int, int, int, int, T, const T*, long, const T*, long, T, T*, long);  \
+  template void dev_symm<T>(const Exec&, int, int, int, int, T, const T*, long, const T*, long, T, T*, long);   \
+  template void dev_trmm<T>(const Exec&, int, int, int, int, int, int, T, const T*, long, T*, long);
 XK_INST(double)
 XK_INST(float)
 #undef XK_INST

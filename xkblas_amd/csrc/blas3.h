@@ -39,6 +39,29 @@ void dev_potrf(const Exec& x, int uplo, int n, T* A, long lda, int nb, int* info
 // trailing updates of dev_potrf: 1 paired (default, also for -1), 0 one per panel
 void potrf_set_pairing(int v);
 
+// C <- alpha op(A) op(B) + beta C on the uplo triangle of the n x n C (uplo
+// neither Lower nor Upper: all of C).
+template <typename T>
+void dev_gemmt(const Exec& x, int uplo, int transA, int transB, int n, int k, T alpha, const T* A, long lda,
+               const T* B, long ldb, T beta, T* C, long ldc);
+
+// C <- alpha (op(A) op(B)^T + op(B) op(A)^T) + beta C on the uplo triangle
+// (op = identity for trans NoTrans: A, B n x k; else transposed: k x n).
+template <typename T>
+void dev_syr2k(const Exec& x, int uplo, int trans, int n, int k, T alpha, const T* A, long lda, const T* B, long ldb,
+               T beta, T* C, long ldc);
+
+// C <- alpha A B + beta C (Left) or alpha B A + beta C (Right), A symmetric
+// with its uplo triangle stored; C, B m x n.
+template <typename T>
+void dev_symm(const Exec& x, int side, int uplo, int m, int n, T alpha, const T* A, long lda, const T* B, long ldb,
+              T beta, T* C, long ldc);
+
+// B <- alpha op(A) B (Left) or alpha B op(A) (Right), A triangular; B m x n.
+template <typename T>
+void dev_trmm(const Exec& x, int side, int uplo, int transA, int diag, int m, int n, T alpha, const T* A, long lda,
+              T* B, long ldb);
+
 // Solution of A X = B (A SPD, uplo triangle stored; B, X n x nrhs) from a
 // factor computed under `xl` (a reduced-precision split mode) and FP64
 // iterative refinement under `x`: LAPACK dsposv's scheme.  Returns the

@@ -16,6 +16,12 @@ enum : int { Left = 141, Right = 142 };
 // LOWER (UPPER) writes only entries with row >= col (row <= col) of C.
 enum : int { TRI_FULL = 0, TRI_LOWER = 1, TRI_UPPER = 2 };
 
+// Inner range of a GEMM tile when op(A) or op(B) is triangular (zeros
+// outside): op(A) lower, rows i < m0 + BM need k < m0 + BM; op(A) upper,
+// k >= m0; op(B) upper, columns j < n0 + BN need k < n0 + BN; op(B) lower,
+// k >= n0.
+enum : int { KRANGE_ALL = 0, KRANGE_A_LOWER = 1, KRANGE_A_UPPER = 2, KRANGE_B_UPPER = 3, KRANGE_B_LOWER = 4 };
+
 [[noreturn]] void fatal(const char* what, const char* file, int line, const char* msg);
 
 }  // namespace xk

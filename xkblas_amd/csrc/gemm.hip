@@ -48,6 +48,7 @@ struct GemmParams {
   int m, n, k;
   int tiles_m, tiles_n;
   int tri;
+  int kmode;       // inner range per tile for a triangular operand (KRANGE_*)
   int vecA, vecB;  // 16-byte vector loads allowed (base and ld aligned)
   T alpha, beta;
   const T* A;
@@ -101,7 +102,11 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmParams<T> p)
     int pid = blockIdx.x;
     const int total = gridDim.x;
     if (p.tri == TRI_FULL) {
-      if ((total & 7) == 0) pid = (pid & 7) * (total >> 3) + (pid >> 3);
+      // a limited inner range makes the work per tile grow along the tile
+      // rows or columns: no XCD blocking then (consecutive ids go to
+      // different XCDs, which then share light and heavy tiles alike), and
+      // the heaviest tiles are dispatched first
+      if (p.kmode == KRANGE_ALL && (total & 7) == 0) pid = (pid & 7) * (total >> 3) + (pid >> 3);
       constexpr int GM = 8;
       const int group = GM * p.tiles_n;
       const int g = pid / group;
@@ -110,6 +115,8 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmParams<T> p)
       const int r = pid - g * group;
       tm = first + r % gm;
       tn = r / gm;
+      if (p.kmode == KRANGE_A_LOWER) tm = p.tiles_m - 1 - tm;
+      else if (p.kmode == KRANGE_B_UPPER) tn = p.tiles_n - 1 - tn;
     } else {
       int i, j;
       tri_tile<8>(pid, total, p.tiles_m, i, j);
@@ -194,14 +201,22 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmParams<T> p)
 #pragma unroll
     for (int j = 0; j < TN; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
 
-  const int nk = (K + BK - 1) / BK;
+  // inner range: all of k, or the part where a triangular op(A) / op(B) of
+  // this tile row / column is not zero (the rest would add exact zeros)
+  int kbeg = 0, kend = K;
+  if (p.kmode == KRANGE_A_LOWER) kend = min(K, m0 + BM);
+  else if (p.kmode == KRANGE_A_UPPER) kbeg = m0;
+  else if (p.kmode == KRANGE_B_UPPER) kend = min(K, n0 + BN);
+  else if (p.kmode == KRANGE_B_LOWER) kbeg = n0;
+  const int kt0 = kbeg / BK;
+  const int nk = (kend + BK - 1) / BK;
   const int lr = lane & 15, lk = lane >> 4;
-  if (nk > 0) {
-    load_slice(0);
+  if (nk > kt0) {
+    load_slice(kt0 * BK);
     store_slice(0);
     __syncthreads();
     int buf = 0;
-    for (int kt = 0; kt < nk; ++kt) {
+    for (int kt = kt0; kt < nk; ++kt) {
       const bool more = kt + 1 < nk;
       if (more) load_slice((kt + 1) * BK);
 #pragma unroll
@@ -270,12 +285,12 @@ static void launch_shape(hipStream_t s, bool ta, bool tb, GemmParams<T>& p, int 
 
 template <typename T>
 void gemm(hipStream_t s, int transA, int transB, int m, int n, int k, T alpha, const T* A, long lda,
-          const T* B, long ldb, T beta, T* C, long ldc, int tri, int batch, long sA, long sB, long sC)
+          const T* B, long ldb, T beta, T* C, long ldc, int tri, int batch, long sA, long sB, long sC, int kmode)
 {
   if (m <= 0 || n <= 0 || batch <= 0) return;
   if (tri != TRI_FULL) XK_CHECK(m == n, "triangular gemm output must be square");
   GemmParams<T> p;
-  p.m = m; p.n = n; p.k = k; p.tri = tri;
+  p.m = m; p.n = n; p.k = k; p.tri = tri; p.kmode = kmode;
   p.alpha = alpha; p.beta = beta;
   p.A = A; p.lda = lda; p.sA = sA;
   p.B = B; p.ldb = ldb; p.sB = sB;
@@ -309,8 +324,8 @@ void gemm(hipStream_t s, int transA, int transB, int m, int n, int k, T alpha, c
 }
 
 template void gemm<double>(hipStream_t, int, int, int, int, int, double, const double*, long, const double*, long,
-                           double, double*, long, int, int, long, long, long);
+                           double, double*, long, int, int, long, long, long, int);
 template void gemm<float>(hipStream_t, int, int, int, int, int, float, const float*, long, const float*, long,
-                          float, float*, long, int, int, long, long, long);
+                          float, float*, long, int, int, long, long, long, int);
 
 }  // namespace xk

@@ -11,10 +11,12 @@ namespace xk {
 // `batch` > 1 runs independent problems at element strides sA, sB, sC.
 // beta == 0 never reads C.  FP64 uses v_mfma_f64_16x16x4f64, FP32 uses
 // v_mfma_f32_16x16x4f32; tiles are staged through LDS, double-buffered.
+// `kmode` (KRANGE_*) limits each tile's inner range to where a triangular
+// op(A) (its rows) or op(B) (its columns) is not zero.
 template <typename T>
 void gemm(hipStream_t s, int transA, int transB, int m, int n, int k, T alpha,
           const T* A, long lda, const T* B, long ldb, T beta, T* C, long ldc,
-          int tri = TRI_FULL, int batch = 1, long sA = 0, long sB = 0, long sC = 0);
+          int tri = TRI_FULL, int batch = 1, long sA = 0, long sB = 0, long sC = 0, int kmode = 0);
 
 // Forces the macro-tile shape of gemm(): 0 = automatic, 64 or 128.
 void gemm_set_tile(int bm);
@@ -49,7 +51,13 @@ void fill_int(hipStream_t s, int* p, int n, int v);
 
 // Y (and Z when non-null) <- the full symmetric n x n matrix whose uplo
 // triangle is stored in A.
-void symm_expand(hipStream_t s, int uplo, int n, const double* A, long lda, double* Y, long ldy, double* Z, long ldz);
+template <typename T>
+void symm_expand(hipStream_t s, int uplo, int n, const T* A, long lda, T* Y, long ldy, T* Z, long ldz);
+
+// Y <- the n x n triangular matrix whose uplo triangle is stored in A (zeros
+// outside it, a unit diagonal when unit != 0).
+template <typename T>
+void tri_expand(hipStream_t s, int uplo, int unit, int n, const T* A, long lda, T* Y, long ldy);
 
 // out[j] <- max_i |X(i, j)| (op 0, NaN-propagating) or sum_i |X(i, j)| (op 1)
 // for the n columns of the m x n block X.

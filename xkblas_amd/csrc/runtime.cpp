@@ -925,9 +925,188 @@ int potrf_async(const char* fn, int uplo, int N, T* A, int LDA)
   return 0;
 }
 
+template <typename T>
+int gemmt_async(const char* fn, int uplo, int transA, int transB, int N, int K, const T* alpha, const T* A, int LDA,
+                const T* B, int LDB, const T* beta, T* C, int LDC)
+{
+  // checks and codes of blas/zgemmt.c:123-165 (uplo 0 = all of C, as there;
+  // any other value than Upper/Lower/0 is refused here)
+  if (uplo != 0 && uplo != Upper && uplo != Lower) return arg_error(fn, "illegal value of uplo", -1);
+  if (transA < NoTrans || transA > ConjTrans) return arg_error(fn, "illegal value of transA", -1);
+  if (transB < NoTrans || transB > ConjTrans) return arg_error(fn, "illegal value of transB", -2);
+  const int Am = transA == NoTrans ? N : K, An = transA == NoTrans ? K : N;
+  const int Bm = transB == NoTrans ? K : N, Bn = transB == NoTrans ? N : K;
+  if (N < 0) return arg_error(fn, "illegal value of N", -4);
+  if (K < 0) return arg_error(fn, "illegal value of K", -5);
+  if (LDA < std::max(1, Am)) return arg_error(fn, "illegal value of LDA", -8);
+  if (LDB < std::max(1, Bm)) return arg_error(fn, "illegal value of LDB", -10);
+  if (LDC < std::max(1, N)) return arg_error(fn, "illegal value of LDC", -13);
+  if (N == 0 || ((*alpha == T(0) || K == 0) && *beta == T(1))) return 0;
+
+  Lock l(g_mu);
+  ensure_init();
+  Runtime& r = rt();
+  const int nb = (int)xkblas_auto_tilesize(nullptr, KERN_GEMMT, N, N, K);
+  std::vector<Replica*> keep;
+  const bool use_ab = K > 0 && *alpha != T(0);
+  Replica* rA = use_ab ? r.replica(A, Am, An, LDA, sizeof(T), nb, keep) : nullptr;
+  Replica* rB = use_ab ? r.replica(B, Bm, Bn, LDB, sizeof(T), nb, keep) : nullptr;
+  Replica* rC = r.replica(C, N, N, LDC, sizeof(T), nb, keep);
+  if (rA) r.fetch(rA, 0, 0, Am, An);
+  if (rB) r.fetch(rB, 0, 0, Bm, Bn);
+  r.fetch(rC, 0, 0, N, N);  // the other triangle stays intact on the device
+  r.before_compute();
+  Exec x = r.exec();
+  const bool tri = uplo == Upper || uplo == Lower;
+  r.prof_begin(sizeof(T) == 8 ? "gemmt_f64" : "gemmt_f32", tri ? (double)K * N * (N + 1.0) : 2.0 * N * N * (double)K);
+  dev_gemmt<T>(x, uplo, transA, transB, N, use_ab ? K : 0, use_ab ? *alpha : T(0),
+               rA ? (const T*)rA->dev : (const T*)rC->dev, rA ? rA->ldd : rC->ldd,
+               rB ? (const T*)rB->dev : (const T*)rC->dev, rB ? rB->ldd : rC->ldd, *beta, (T*)rC->dev, rC->ldd);
+  r.prof_end();
+  r.mark_written(rC, 0, 0, N, N, tri ? uplo : 0);
+  return 0;
+}
+
+template <typename T>
+int syr2k_async(const char* fn, int uplo, int trans, int N, int K, const T* alpha, const T* A, int LDA, const T* B,
+                int LDB, const T* beta, T* C, int LDC)
+{
+  // checks and codes of blas/zsyr2k.c:119-156
+  if (uplo != Upper && uplo != Lower) return arg_error(fn, "illegal value of uplo", -1);
+  if (trans != NoTrans && trans != Trans) return arg_error(fn, "illegal value of trans", -2);
+  const int Am = trans == NoTrans ? N : K, An = trans == NoTrans ? K : N;
+  if (N < 0) return arg_error(fn, "illegal value of N", -3);
+  if (K < 0) return arg_error(fn, "illegal value of K", -4);
+  if (LDA < std::max(1, Am)) return arg_error(fn, "illegal value of LDA", -7);
+  if (LDB < std::max(1, Am)) return arg_error(fn, "illegal value of LDB", -9);
+  if (LDC < std::max(1, N)) return arg_error(fn, "illegal value of LDC", -12);
+  if (N == 0 || ((*alpha == T(0) || K == 0) && *beta == T(1))) return 0;
+
+  Lock l(g_mu);
+  ensure_init();
+  Runtime& r = rt();
+  const int nb = (int)xkblas_auto_tilesize(nullptr, KERN_SYR2K, N, N, K);
+  std::vector<Replica*> keep;
+  const bool use_ab = K > 0 && *alpha != T(0);
+  Replica* rA = use_ab ? r.replica(A, Am, An, LDA, sizeof(T), nb, keep) : nullptr;
+  Replica* rB = use_ab ? r.replica(B, Am, An, LDB, sizeof(T), nb, keep) : nullptr;
+  Replica* rC = r.replica(C, N, N, LDC, sizeof(T), nb, keep);
+  if (rA) r.fetch(rA, 0, 0, Am, An);
+  if (rB) r.fetch(rB, 0, 0, Am, An);
+  r.fetch(rC, 0, 0, N, N);
+  r.before_compute();
+  Exec x = r.exec();
+  r.prof_begin(sizeof(T) == 8 ? "syr2k_f64" : "syr2k_f32", 2.0 * K * N * (N + 1.0));
+  dev_syr2k<T>(x, uplo, trans, N, use_ab ? K : 0, use_ab ? *alpha : T(0), rA ? (const T*)rA->dev : (const T*)rC->dev,
+               rA ? rA->ldd : rC->ldd, rB ? (const T*)rB->dev : (const T*)rC->dev, rB ? rB->ldd : rC->ldd, *beta,
+               (T*)rC->dev, rC->ldd);
+  r.prof_end();
+  r.mark_written(rC, 0, 0, N, N, uplo);
+  return 0;
+}
+
+template <typename T>
+int symm_async(const char* fn, int side, int uplo, int M, int N, const T* alpha, const T* A, int LDA, const T* B,
+               int LDB, const T* beta, T* C, int LDC)
+{
+  // checks and codes of blas/zsymm.c:115-148
+  if (side != Left && side != Right) return arg_error(fn, "illegal value of side", -1);
+  if (uplo != Lower && uplo != Upper) return arg_error(fn, "illegal value of uplo", -2);
+  const int Am = side == Left ? M : N;
+  if (M < 0) return arg_error(fn, "illegal value of M", -3);
+  if (N < 0) return arg_error(fn, "illegal value of N", -4);
+  if (LDA < std::max(1, Am)) return arg_error(fn, "illegal value of LDA", -7);
+  if (LDB < std::max(1, M)) return arg_error(fn, "illegal value of LDB", -9);
+  if (LDC < std::max(1, M)) return arg_error(fn, "illegal value of LDC", -12);
+  if (M == 0 || N == 0 || (*alpha == T(0) && *beta == T(1))) return 0;
+
+  Lock l(g_mu);
+  ensure_init();
+  Runtime& r = rt();
+  const int nb = (int)xkblas_auto_tilesize(nullptr, KERN_SYMM, M, N, Am);
+  std::vector<Replica*> keep;
+  const bool use_ab = *alpha != T(0);
+  Replica* rA = use_ab ? r.replica(A, Am, Am, LDA, sizeof(T), nb, keep) : nullptr;
+  Replica* rB = use_ab ? r.replica(B, M, N, LDB, sizeof(T), nb, keep) : nullptr;
+  Replica* rC = r.replica(C, M, N, LDC, sizeof(T), nb, keep);
+  if (rA) r.fetch(rA, 0, 0, Am, Am);
+  if (rB) r.fetch(rB, 0, 0, M, N);
+  if (*beta != T(0)) r.fetch(rC, 0, 0, M, N);
+  r.before_compute();
+  Exec x = r.exec();
+  r.prof_begin(sizeof(T) == 8 ? "symm_f64" : "symm_f32", 2.0 * M * N * (double)Am);
+  dev_symm<T>(x, side, uplo, M, N, use_ab ? *alpha : T(0), rA ? (const T*)rA->dev : (const T*)rC->dev,
+              rA ? rA->ldd : rC->ldd, rB ? (const T*)rB->dev : (const T*)rC->dev, rB ? rB->ldd : rC->ldd, *beta,
+              (T*)rC->dev, rC->ldd);
+  r.prof_end();
+  r.mark_written(rC, 0, 0, M, N, 0);
+  return 0;
+}
+
+template <typename T>
+int trmm_async(const char* fn, int side, int uplo, int transA, int diag, int N, int NRHS, const T* alpha, const T* A,
+               int LDA, T* B, int LDB)
+{
+  // checks of blas/ztrmm.c:116-153; N = rows of B, NRHS = columns
+  const int NA = side == Left ? N : NRHS;
+  if (side != Left && side != Right) return arg_error(fn, "illegal value of side", -1);
+  if (uplo != Upper && uplo != Lower) return arg_error(fn, "illegal value of uplo", -2);
+  if (transA < NoTrans || transA > ConjTrans) return arg_error(fn, "illegal value of transA", -3);
+  if (diag != Unit && diag != NonUnit) return arg_error(fn, "illegal value of diag", -4);
+  if (N < 0) return arg_error(fn, "illegal value of N", -5);
+  if (NRHS < 0) return arg_error(fn, "illegal value of NRHS", -6);
+  if (LDA < std::max(1, NA)) return arg_error(fn, "illegal value of LDA", -8);
+  if (LDB < std::max(1, N)) return arg_error(fn, "illegal value of LDB", -10);
+  if (std::min(N, NRHS) == 0) return 0;
+
+  Lock l(g_mu);
+  ensure_init();
+  Runtime& r = rt();
+  const int nb = (int)xkblas_auto_tilesize(nullptr, KERN_TRMM, N, NRHS, NA);
+  std::vector<Replica*> keep;
+  Replica* rA = r.replica(A, NA, NA, LDA, sizeof(T), nb, keep);
+  Replica* rB = r.replica(B, N, NRHS, LDB, sizeof(T), nb, keep);
+  r.fetch(rA, 0, 0, NA, NA);
+  r.fetch(rB, 0, 0, N, NRHS);
+  r.before_compute();
+  Exec x = r.exec();
+  const double flops = side == Left ? (double)N * N * NRHS : (double)N * NRHS * NRHS;
+  r.prof_begin(sizeof(T) == 8 ? "trmm_f64" : "trmm_f32", flops);
+  dev_trmm<T>(x, side, uplo, transA, diag, N, NRHS, *alpha, (const T*)rA->dev, rA->ldd, (T*)rB->dev, rB->ldd);
+  r.prof_end();
+  r.mark_written(rB, 0, 0, N, NRHS, 0);
+  return 0;
+}
+
 }  // namespace
 
 extern "C" {
+
+#define XK_NEXT_OPS(P, T)                                                                                         \
+  int xkblas_##P##gemmt_async(int uplo, int transA, int transB, int N, int K, const T* alpha, const T* A, int LDA, \
+                              const T* B, int LDB, const T* beta, T* C, int LDC)                                  \
+  {                                                                                                               \
+    return gemmt_async<T>("xkblas_" #P "gemmt_async", uplo, transA, transB, N, K, alpha, A, LDA, B, LDB, beta, C, \
+                          LDC);                                                                                   \
+  }                                                                                                               \
+  int xkblas_##P##syr2k_async(int uplo, int trans, int N, int K, const T* alpha, const T* A, int LDA, const T* B,  \
+                              int LDB, const T* beta, T* C, int LDC)                                              \
+  {                                                                                                               \
+    return syr2k_async<T>("xkblas_" #P "syr2k_async", uplo, trans, N, K, alpha, A, LDA, B, LDB, beta, C, LDC);    \
+  }                                                                                                               \
+  int xkblas_##P##symm_async(int side, int uplo, int M, int N, const T* alpha, const T* A, int LDA, const T* B,    \
+                             int LDB, const T* beta, T* C, int LDC)                                               \
+  {                                                                                                               \
+    return symm_async<T>("xkblas_" #P "symm_async", side, uplo, M, N, alpha, A, LDA, B, LDB, beta, C, LDC);       \
+  }                                                                                                               \
+  int xkblas_##P##trmm_async(int side, int uplo, int transA, int diag, int N, int NRHS, const T* alpha,           \
+                             const T* A, int LDA, T* B, int LDB)                                                  \
+  {                                                                                                               \
+    return trmm_async<T>("xkblas_" #P "trmm_async", side, uplo, transA, diag, N, NRHS, alpha, A, LDA, B, LDB);    \
+  }
+XK_NEXT_OPS(d, double)
+XK_NEXT_OPS(s, float)
+#undef XK_NEXT_OPS
 
 int xkblas_dgemm_async(int transA, int transB, int M, int N, int K, const double* alpha, const double* A, int LDA,
                        const double* B, int LDB, const double* beta, double* C, int LDC)
