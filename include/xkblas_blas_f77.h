@@ -25,6 +25,16 @@ void dsyrk_(const char* uplo, const char* trans, const int* n, const int* k, con
 void dtrsm_(const char* side, const char* uplo, const char* transa, const char* diag, const int* m, const int* n,
             const double* alpha, const double* A, const int* lda, double* B, const int* ldb);
 void dpotrf_(const char* uplo, const int* n, double* A, const int* lda, int* info);
+/* reference blas/zlibxkblas_wrapper.c:85-300 (gemmt, trmm, symm, syr2k) */
+void dgemmt_(const char* uplo, const char* transa, const char* transb, const int* n, const int* k,
+             const double* alpha, const double* A, const int* lda, const double* B, const int* ldb,
+             const double* beta, double* C, const int* ldc);
+void dsyr2k_(const char* uplo, const char* trans, const int* n, const int* k, const double* alpha, const double* A,
+             const int* lda, const double* B, const int* ldb, const double* beta, double* C, const int* ldc);
+void dsymm_(const char* side, const char* uplo, const int* m, const int* n, const double* alpha, const double* A,
+            const int* lda, const double* B, const int* ldb, const double* beta, double* C, const int* ldc);
+void dtrmm_(const char* side, const char* uplo, const char* transa, const char* diag, const int* m, const int* n,
+            const double* alpha, const double* A, const int* lda, double* B, const int* ldb);
 
 #ifdef __cplusplus
 }

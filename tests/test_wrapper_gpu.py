@@ -35,7 +35,26 @@ F = S.copy(order="F")
 info = ctypes.c_int(-7)
 W.dpotrf_(b"L", i(n), p(F), i(n), ctypes.byref(info))
 res_potrf = oracle.potrf_residual(S, F, 122)
-print(json.dumps({"gemm": res_gemm, "potrf": res_potrf, "info": info.value}))
+# dgemmt_, dsyr2k_, dsymm_, dtrmm_ against their textbook definitions
+n, k = 260, 140
+A, B, C0 = uniform(r, n, k), uniform(r, n, k), uniform(r, n, n)
+nrm = lambda X: float(np.abs(X).max())
+C = C0.copy(order="F")
+W.dgemmt_(b"L", b"N", b"T", i(n), i(k), d(1.5), p(A), i(n), p(B), i(n), d(-0.5), p(C), i(n))
+err = {"gemmt": nrm(np.tril(C - (1.5 * A @ B.T - 0.5 * C0))) / nrm(C0), "gemmt_other": nrm(np.triu(C - C0, 1))}
+C = C0.copy(order="F")
+W.dsyr2k_(b"U", b"N", i(n), i(k), d(0.5), p(A), i(n), p(B), i(n), d(2.0), p(C), i(n))
+err["syr2k"] = nrm(np.triu(C - (0.5 * (A @ B.T + B @ A.T) + 2.0 * C0))) / nrm(C0)
+S2 = np.asfortranarray(C0 + C0.T)
+X = uniform(r, n, 70)
+Y = np.zeros((n, 70), order="F")
+W.dsymm_(b"L", b"L", i(n), i(70), d(1.0), p(np.asfortranarray(np.tril(S2))), i(n), p(X), i(n), d(0.0), p(Y), i(n))
+err["symm"] = nrm(Y - S2 @ X) / nrm(S2 @ X)
+T = np.asfortranarray(np.triu(C0))
+Y = X.copy(order="F")
+W.dtrmm_(b"L", b"U", b"T", b"N", i(n), i(70), d(2.0), p(T), i(n), p(Y), i(n))
+err["trmm"] = nrm(Y - 2.0 * T.T @ X) / nrm(2.0 * T.T @ X)
+print(json.dumps({"gemm": res_gemm, "potrf": res_potrf, "info": info.value, "more": err}))
 '''
 
 
@@ -49,3 +68,5 @@ def test_wrapper_dispatch(threshold):
     assert res["info"] == 0
     assert res["gemm"] <= 3.0
     assert res["potrf"] <= 10.0
+    assert res["more"]["gemmt_other"] == 0.0
+    assert max(v for key, v in res["more"].items()) <= 1e-13, res["more"]

@@ -1,6 +1,7 @@
 // wrapper.cpp — libxkblas_amd_wrapper.so: Fortran-ABI drop-in dgemm_,
-// dsyrk_, dtrsm_, dpotrf_ (and s-variants of the first three) for
-// LD_PRELOAD or link-time substitution of a host BLAS.
+// dsyrk_, dtrsm_, dpotrf_, sgemm_, and dgemmt_, dsyr2k_, dsymm_, dtrmm_
+// (blas/zlibxkblas_wrapper.c:85-300) for LD_PRELOAD or link-time
+// substitution of a host BLAS.
 //
 // Restates the reference's unbuilt wrapper (blas/libxkblas_wrapper.c:88-187,
 // blas/zlibxkblas_wrapper.c:46-76): quick return, then a CPU/GPU choice on
@@ -18,8 +19,10 @@
 
 namespace {
 
-enum Kern { K_DGEMM, K_SGEMM, K_DSYRK, K_SSYRK, K_DTRSM, K_STRSM, K_DPOTRF, K_LAST };
-const char* kKernName[K_LAST] = {"DGEMM", "SGEMM", "DSYRK", "SSYRK", "DTRSM", "STRSM", "DPOTRF"};
+enum Kern { K_DGEMM, K_SGEMM, K_DSYRK, K_SSYRK, K_DTRSM, K_STRSM, K_DPOTRF, K_DGEMMT, K_DSYR2K, K_DSYMM, K_DTRMM,
+            K_LAST };
+const char* kKernName[K_LAST] = {"DGEMM", "SGEMM", "DSYRK", "SSYRK", "DTRSM", "STRSM", "DPOTRF",
+                                 "DGEMMT", "DSYR2K", "DSYMM", "DTRMM"};
 double g_threshold[K_LAST];
 bool g_gpu_ready = false;
 
@@ -74,6 +77,29 @@ double ai_trsm(bool left, double m, double n, double elt)
   const double f = left ? n * m * (m + 1.0) : m * n * (n + 1.0);
   const double d = left ? (0.5 * m * m + 2.0 * m * n) : (0.5 * n * n + 2.0 * n * m);
   return f / (elt * d);
+}
+
+// FLOPS_DGEMMT / DATA_DGEMMT, FLOPS_DSYR2K / DATA_DSYR2K, FLOPS_DSYMM /
+// DATA_DSYMM, FLOPS_DTRMM / DATA_DTRMM (blas/flops.h:55-75,375-417)
+double ai_gemmt(double n, double k, double elt) { return n * n * k / (elt * (0.5 * n * n + 2.0 * n * k)); }
+double ai_syr2k(double n, double k, double elt) { return (2.0 * k * n * n + n) / (elt * (0.5 * n * n + 2.0 * n * k)); }
+double ai_symm(bool left, double m, double n, double elt)
+{
+  return left ? 2.0 * m * m * n / (elt * (m * m + 2.0 * m * n)) : 2.0 * m * n * n / (elt * (n * n + 2.0 * m * n));
+}
+double ai_trmm(bool left, double m, double n, double elt)
+{
+  const double f = left ? n * m * m : m * n * n;
+  const double d = left ? (0.5 * m * m + 2.0 * m * n) : (0.5 * n * n + 2.0 * n * m);
+  return f / (elt * d);
+}
+
+// the host BLAS routine `name` of XKBLAS_BLASLIB, loaded on first use
+template <typename F>
+F host_sym(F& f, const char* name)
+{
+  if (!f) xkblas_load_sym((void**)&f, name);
+  return f;
 }
 
 template <typename T>
@@ -143,6 +169,73 @@ void dtrsm_(const char* side, const char* uplo, const char* transa, const char* 
   }
   gpu_ready();
   xkblas_dtrsm_async(side_of(side), uplo_of(uplo), trans_of(transa), diag_of(diag), *m, *n, alpha, A, *lda, B,
+                     *ldb);
+  coherent_sync<double>(0, *m, *n, B, *ldb);
+}
+
+void dgemmt_(const char* uplo, const char* transa, const char* transb, const int* n, const int* k,
+             const double* alpha, const double* A, const int* lda, const double* B, const int* ldb,
+             const double* beta, double* C, const int* ldc)
+{
+  if (*n == 0 || ((*alpha == 0.0 || *k == 0) && *beta == 1.0)) return;
+  if (ai_gemmt(*n, *k, 8.0) < g_threshold[K_DGEMMT]) {
+    typedef void (*f_t)(const char*, const char*, const char*, const int*, const int*, const double*,
+                        const double*, const int*, const double*, const int*, const double*, double*, const int*);
+    static f_t f = nullptr;
+    host_sym(f, "dgemmt_")(uplo, transa, transb, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+    return;
+  }
+  gpu_ready();
+  xkblas_dgemmt_async(uplo_of(uplo), trans_of(transa), trans_of(transb), *n, *k, alpha, A, *lda, B, *ldb, beta, C,
+                      *ldc);
+  coherent_sync<double>(uplo_of(uplo), *n, *n, C, *ldc);
+}
+
+void dsyr2k_(const char* uplo, const char* trans, const int* n, const int* k, const double* alpha, const double* A,
+             const int* lda, const double* B, const int* ldb, const double* beta, double* C, const int* ldc)
+{
+  if (*n == 0 || ((*alpha == 0.0 || *k == 0) && *beta == 1.0)) return;
+  if (ai_syr2k(*n, *k, 8.0) < g_threshold[K_DSYR2K]) {
+    typedef void (*f_t)(const char*, const char*, const int*, const int*, const double*, const double*, const int*,
+                        const double*, const int*, const double*, double*, const int*);
+    static f_t f = nullptr;
+    host_sym(f, "dsyr2k_")(uplo, trans, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+    return;
+  }
+  gpu_ready();
+  xkblas_dsyr2k_async(uplo_of(uplo), trans_of(trans), *n, *k, alpha, A, *lda, B, *ldb, beta, C, *ldc);
+  coherent_sync<double>(uplo_of(uplo), *n, *n, C, *ldc);
+}
+
+void dsymm_(const char* side, const char* uplo, const int* m, const int* n, const double* alpha, const double* A,
+            const int* lda, const double* B, const int* ldb, const double* beta, double* C, const int* ldc)
+{
+  if (*m == 0 || *n == 0 || (*alpha == 0.0 && *beta == 1.0)) return;
+  if (ai_symm(side_of(side) == CblasLeft, *m, *n, 8.0) < g_threshold[K_DSYMM]) {
+    typedef void (*f_t)(const char*, const char*, const int*, const int*, const double*, const double*, const int*,
+                        const double*, const int*, const double*, double*, const int*);
+    static f_t f = nullptr;
+    host_sym(f, "dsymm_")(side, uplo, m, n, alpha, A, lda, B, ldb, beta, C, ldc);
+    return;
+  }
+  gpu_ready();
+  xkblas_dsymm_async(side_of(side), uplo_of(uplo), *m, *n, alpha, A, *lda, B, *ldb, beta, C, *ldc);
+  coherent_sync<double>(0, *m, *n, C, *ldc);
+}
+
+void dtrmm_(const char* side, const char* uplo, const char* transa, const char* diag, const int* m, const int* n,
+            const double* alpha, const double* A, const int* lda, double* B, const int* ldb)
+{
+  if (*m == 0 || *n == 0) return;
+  if (ai_trmm(side_of(side) == CblasLeft, *m, *n, 8.0) < g_threshold[K_DTRMM]) {
+    typedef void (*f_t)(const char*, const char*, const char*, const char*, const int*, const int*, const double*,
+                        const double*, const int*, double*, const int*);
+    static f_t f = nullptr;
+    host_sym(f, "dtrmm_")(side, uplo, transa, diag, m, n, alpha, A, lda, B, ldb);
+    return;
+  }
+  gpu_ready();
+  xkblas_dtrmm_async(side_of(side), uplo_of(uplo), trans_of(transa), diag_of(diag), *m, *n, alpha, A, *lda, B,
                      *ldb);
   coherent_sync<double>(0, *m, *n, B, *ldb);
 }
