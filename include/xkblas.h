@@ -133,6 +133,21 @@ int xkblas_dtrsm_native(int side, int uplo, int transA, int diag, int N, int NRH
                         const double* alpha, const double* A, int LDA,
                         double* B, int LDB);
 int xkblas_dpotrf_native(int uplo, int N, double* A, int LDA);
+int xkblas_dgemmt_native(int uplo, int transA, int transB, int N, int K,
+                         const double* alpha, const double* A, int LDA,
+                         const double* B, int LDB,
+                         const double* beta, double* C, int LDC);   /* ref blas/zgemmt.c:432 */
+int xkblas_dsyr2k_native(int uplo, int trans, int N, int K,
+                         const double* alpha, const double* A, int LDA,
+                         const double* B, int LDB,
+                         const double* beta, double* C, int LDC);   /* ref blas/zxkblas.h:113 */
+int xkblas_dsymm_native(int side, int uplo, int M, int N,
+                        const double* alpha, const double* A, int LDA,
+                        const double* B, int LDB,
+                        const double* beta, double* C, int LDC);    /* ref blas/zsymm.c:426 */
+int xkblas_dtrmm_native(int side, int uplo, int transA, int diag, int N, int NRHS,
+                        const double* alpha, const double* A, int LDA,
+                        double* B, int LDB);                        /* ref blas/zxkblas.h:81 */
 
 /* ---- FP32 compute (gen s of reference blas/zxkblas.h) --------------------- */
 int xkblas_sgemm_async(int transA, int transB, int M, int N, int K,

@@ -22,6 +22,19 @@ void xkblas_dtrsm_native_(const char* side, const char* uplo, const char* transa
                           const int* n, const double* alpha, const double* A, const int* lda, double* B,
                           const int* ldb);
 void xkblas_dpotrf_native_(const char* uplo, const int* n, double* A, const int* lda, int* info);
+/* reference blas/zgemmt.c:402, zsymm.c:409, ztrmm.c:473, zsyr2k.c:339 */
+void xkblas_dgemmt_native_(const char* uplo, const char* transa, const char* transb, const int* n, const int* k,
+                           const double* alpha, const double* A, const int* lda, const double* B, const int* ldb,
+                           const double* beta, double* C, const int* ldc);
+void xkblas_dsyr2k_native_(const char* uplo, const char* trans, const int* n, const int* k, const double* alpha,
+                           const double* A, const int* lda, const double* B, const int* ldb, const double* beta,
+                           double* C, const int* ldc);
+void xkblas_dsymm_native_(const char* side, const char* uplo, const int* m, const int* n, const double* alpha,
+                          const double* A, const int* lda, const double* B, const int* ldb, const double* beta,
+                          double* C, const int* ldc);
+void xkblas_dtrmm_native_(const char* side, const char* uplo, const char* transa, const char* diag, const int* m,
+                          const int* n, const double* alpha, const double* A, const int* lda, double* B,
+                          const int* ldb);
 
 #ifdef __cplusplus
 }

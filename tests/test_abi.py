@@ -200,3 +200,32 @@ L.xkblas_dgemm_async(999, 111, 2, 2, 2, one, buf, 2, buf, 2, one, buf, 2)
     out = subprocess.run(["python", "-c", code], capture_output=True, text=True, env=env, timeout=120)
     assert out.returncode != 0
     assert "illegal value of transA" in out.stderr
+
+
+def test_native_forwarders_of_gemmt_syr2k_symm_trmm():
+    """xkblas_d{gemmt,syr2k,symm,trmm}_native run the host BLAS (no GPU):
+    checked against their textbook definitions."""
+    import numpy as np
+    import xkblas_amd
+    L = xkblas_amd.lib()
+    r = np.random.default_rng(5)
+    n, k = 90, 40
+    A, B = (np.asfortranarray(r.uniform(-1, 1, (n, k))) for _ in range(2))
+    C0 = np.asfortranarray(r.uniform(-1, 1, (n, n)))
+    p = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    one, half = np.array([1.0]), np.array([0.5])
+    C = C0.copy(order="F")
+    L.xkblas_dgemmt_native(122, 111, 112, n, k, p(one), p(A), n, p(B), n, p(half), p(C), n)
+    assert np.allclose(np.tril(C), np.tril(A @ B.T + 0.5 * C0), atol=1e-13)
+    assert np.array_equal(np.triu(C, 1), np.triu(C0, 1))
+    C = C0.copy(order="F")
+    L.xkblas_dsyr2k_native(121, 111, n, k, p(one), p(A), n, p(B), n, p(half), p(C), n)
+    assert np.allclose(np.triu(C), np.triu(A @ B.T + B @ A.T + 0.5 * C0), atol=1e-13)
+    S = np.asfortranarray(C0 + C0.T)
+    C = np.zeros((n, k), order="F")
+    L.xkblas_dsymm_native(141, 122, n, k, p(one), p(np.asfortranarray(np.tril(S))), n, p(A), n, p(half), p(C), n)
+    assert np.allclose(C, S @ A, atol=1e-13)
+    T = np.asfortranarray(np.triu(C0))
+    X = A.copy(order="F")
+    L.xkblas_dtrmm_native(141, 121, 112, 131, n, k, p(one), p(T), n, p(X), n)
+    assert np.allclose(X, T.T @ A, atol=1e-13)

@@ -56,6 +56,10 @@ SIGNATURES = {
     "xkblas_dsyrk_native": (_i, [_i, _i, _i, _i, _p, _p, _i, _p, _p, _i]),
     "xkblas_dtrsm_native": (_i, [_i, _i, _i, _i, _i, _i, _p, _p, _i, _p, _i]),
     "xkblas_dpotrf_native": (_i, [_i, _i, _p, _i]),
+    "xkblas_dgemmt_native": (_i, [_i, _i, _i, _i, _i, _p, _p, _i, _p, _i, _p, _p, _i]),
+    "xkblas_dsyr2k_native": (_i, [_i, _i, _i, _i, _p, _p, _i, _p, _i, _p, _p, _i]),
+    "xkblas_dsymm_native": (_i, [_i, _i, _i, _i, _p, _p, _i, _p, _i, _p, _p, _i]),
+    "xkblas_dtrmm_native": (_i, [_i, _i, _i, _i, _i, _i, _p, _p, _i, _p, _i]),
     "xkblas_sgemm_async": (_i, [_i, _i, _i, _i, _i, _p, _p, _i, _p, _i, _p, _p, _i]),
     "xkblas_ssyrk_async": (_i, [_i, _i, _i, _i, _p, _p, _i, _p, _p, _i]),
     "xkblas_strsm_async": (_i, [_i, _i, _i, _i, _i, _i, _p, _p, _i, _p, _i]),
@@ -108,6 +112,10 @@ SIGNATURES = {
     "xkblas_dsyrk_native_": (None, [_p] * 10),
     "xkblas_dtrsm_native_": (None, [_p] * 11),
     "xkblas_dpotrf_native_": (None, [_p] * 5),
+    "xkblas_dgemmt_native_": (None, [_p] * 13),
+    "xkblas_dsyr2k_native_": (None, [_p] * 12),
+    "xkblas_dsymm_native_": (None, [_p] * 12),
+    "xkblas_dtrmm_native_": (None, [_p] * 11),
 }
 
 _lib = None

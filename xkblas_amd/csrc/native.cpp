@@ -53,6 +53,19 @@ typedef void (*dtrsm_t)(const char*, const char*, const char*, const char*, cons
                         const double*, const int*, double*, const int*);
 typedef void (*dpotrf_t)(const char*, const int*, double*, const int*, int*);
 
+typedef void (*dgemmt_t)(const char*, const char*, const char*, const int*, const int*, const double*, const double*,
+                         const int*, const double*, const int*, const double*, double*, const int*);
+typedef void (*dsyr2k_t)(const char*, const char*, const int*, const int*, const double*, const double*, const int*,
+                         const double*, const int*, const double*, double*, const int*);
+typedef void (*dsymm_t)(const char*, const char*, const int*, const int*, const double*, const double*, const int*,
+                        const double*, const int*, const double*, double*, const int*);
+typedef void (*dtrmm_t)(const char*, const char*, const char*, const char*, const int*, const int*, const double*,
+                        const double*, const int*, double*, const int*);
+
+dgemmt_t p_dgemmt = nullptr;
+dsyr2k_t p_dsyr2k = nullptr;
+dsymm_t p_dsymm = nullptr;
+dtrmm_t p_dtrmm = nullptr;
 dgemm_t p_dgemm = nullptr;
 dsyrk_t p_dsyrk = nullptr;
 dtrsm_t p_dtrsm = nullptr;
@@ -101,6 +114,72 @@ void xkblas_dpotrf_native_(const char* uplo, const int* n, double* A, const int*
 {
   if (!p_dpotrf) xkblas_load_sym((void**)&p_dpotrf, "dpotrf_");
   p_dpotrf(uplo, n, A, lda, info);
+}
+
+// reference blas/zgemmt.c:402-446, zsyr2k.c:339-369, zsymm.c:409-440, ztrmm.c:473-500
+void xkblas_dgemmt_native_(const char* uplo, const char* transa, const char* transb, const int* n, const int* k,
+                           const double* alpha, const double* A, const int* lda, const double* B, const int* ldb,
+                           const double* beta, double* C, const int* ldc)
+{
+  if (!p_dgemmt) xkblas_load_sym((void**)&p_dgemmt, "dgemmt_");
+  p_dgemmt(uplo, transa, transb, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+
+void xkblas_dsyr2k_native_(const char* uplo, const char* trans, const int* n, const int* k, const double* alpha,
+                           const double* A, const int* lda, const double* B, const int* ldb, const double* beta,
+                           double* C, const int* ldc)
+{
+  if (!p_dsyr2k) xkblas_load_sym((void**)&p_dsyr2k, "dsyr2k_");
+  p_dsyr2k(uplo, trans, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+
+void xkblas_dsymm_native_(const char* side, const char* uplo, const int* m, const int* n, const double* alpha,
+                          const double* A, const int* lda, const double* B, const int* ldb, const double* beta,
+                          double* C, const int* ldc)
+{
+  if (!p_dsymm) xkblas_load_sym((void**)&p_dsymm, "dsymm_");
+  p_dsymm(side, uplo, m, n, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+
+void xkblas_dtrmm_native_(const char* side, const char* uplo, const char* transa, const char* diag, const int* m,
+                          const int* n, const double* alpha, const double* A, const int* lda, double* B,
+                          const int* ldb)
+{
+  if (!p_dtrmm) xkblas_load_sym((void**)&p_dtrmm, "dtrmm_");
+  p_dtrmm(side, uplo, transa, diag, m, n, alpha, A, lda, B, ldb);
+}
+
+int xkblas_dgemmt_native(int uplo, int transA, int transB, int N, int K, const double* alpha, const double* A,
+                         int LDA, const double* B, int LDB, const double* beta, double* C, int LDC)
+{
+  char u = uplo_char(uplo), ta = trans_char(transA), tb = trans_char(transB);
+  xkblas_dgemmt_native_(&u, &ta, &tb, &N, &K, alpha, A, &LDA, B, &LDB, beta, C, &LDC);
+  return 0;
+}
+
+int xkblas_dsyr2k_native(int uplo, int trans, int N, int K, const double* alpha, const double* A, int LDA,
+                         const double* B, int LDB, const double* beta, double* C, int LDC)
+{
+  char u = uplo_char(uplo), t = trans_char(trans);
+  xkblas_dsyr2k_native_(&u, &t, &N, &K, alpha, A, &LDA, B, &LDB, beta, C, &LDC);
+  return 0;
+}
+
+int xkblas_dsymm_native(int side, int uplo, int M, int N, const double* alpha, const double* A, int LDA,
+                        const double* B, int LDB, const double* beta, double* C, int LDC)
+{
+  char s = side == CblasLeft ? 'L' : 'R', u = uplo_char(uplo);
+  xkblas_dsymm_native_(&s, &u, &M, &N, alpha, A, &LDA, B, &LDB, beta, C, &LDC);
+  return 0;
+}
+
+int xkblas_dtrmm_native(int side, int uplo, int transA, int diag, int N, int NRHS, const double* alpha,
+                        const double* A, int LDA, double* B, int LDB)
+{
+  char s = side == CblasLeft ? 'L' : 'R', u = uplo_char(uplo), t = trans_char(transA);
+  char d = diag == CblasUnit ? 'U' : 'N';
+  xkblas_dtrmm_native_(&s, &u, &t, &d, &N, &NRHS, alpha, A, &LDA, B, &LDB);
+  return 0;
 }
 
 int xkblas_dgemm_native(int transA, int transB, int M, int N, int K, const double* alpha, const double* A, int LDA,

@@ -94,14 +94,6 @@ double ai_trmm(bool left, double m, double n, double elt)
   return f / (elt * d);
 }
 
-// the host BLAS routine `name` of XKBLAS_BLASLIB, loaded on first use
-template <typename F>
-F host_sym(F& f, const char* name)
-{
-  if (!f) xkblas_load_sym((void**)&f, name);
-  return f;
-}
-
 template <typename T>
 void coherent_sync(int uplo, int m, int n, T* C, int ldc)
 {
@@ -179,10 +171,7 @@ void dgemmt_(const char* uplo, const char* transa, const char* transb, const int
 {
   if (*n == 0 || ((*alpha == 0.0 || *k == 0) && *beta == 1.0)) return;
   if (ai_gemmt(*n, *k, 8.0) < g_threshold[K_DGEMMT]) {
-    typedef void (*f_t)(const char*, const char*, const char*, const int*, const int*, const double*,
-                        const double*, const int*, const double*, const int*, const double*, double*, const int*);
-    static f_t f = nullptr;
-    host_sym(f, "dgemmt_")(uplo, transa, transb, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+    xkblas_dgemmt_native_(uplo, transa, transb, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
     return;
   }
   gpu_ready();
@@ -196,10 +185,7 @@ void dsyr2k_(const char* uplo, const char* trans, const int* n, const int* k, co
 {
   if (*n == 0 || ((*alpha == 0.0 || *k == 0) && *beta == 1.0)) return;
   if (ai_syr2k(*n, *k, 8.0) < g_threshold[K_DSYR2K]) {
-    typedef void (*f_t)(const char*, const char*, const int*, const int*, const double*, const double*, const int*,
-                        const double*, const int*, const double*, double*, const int*);
-    static f_t f = nullptr;
-    host_sym(f, "dsyr2k_")(uplo, trans, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+    xkblas_dsyr2k_native_(uplo, trans, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
     return;
   }
   gpu_ready();
@@ -212,10 +198,7 @@ void dsymm_(const char* side, const char* uplo, const int* m, const int* n, cons
 {
   if (*m == 0 || *n == 0 || (*alpha == 0.0 && *beta == 1.0)) return;
   if (ai_symm(side_of(side) == CblasLeft, *m, *n, 8.0) < g_threshold[K_DSYMM]) {
-    typedef void (*f_t)(const char*, const char*, const int*, const int*, const double*, const double*, const int*,
-                        const double*, const int*, const double*, double*, const int*);
-    static f_t f = nullptr;
-    host_sym(f, "dsymm_")(side, uplo, m, n, alpha, A, lda, B, ldb, beta, C, ldc);
+    xkblas_dsymm_native_(side, uplo, m, n, alpha, A, lda, B, ldb, beta, C, ldc);
     return;
   }
   gpu_ready();
@@ -228,10 +211,7 @@ void dtrmm_(const char* side, const char* uplo, const char* transa, const char* 
 {
   if (*m == 0 || *n == 0) return;
   if (ai_trmm(side_of(side) == CblasLeft, *m, *n, 8.0) < g_threshold[K_DTRMM]) {
-    typedef void (*f_t)(const char*, const char*, const char*, const char*, const int*, const int*, const double*,
-                        const double*, const int*, double*, const int*);
-    static f_t f = nullptr;
-    host_sym(f, "dtrmm_")(side, uplo, transa, diag, m, n, alpha, A, lda, B, ldb);
+    xkblas_dtrmm_native_(side, uplo, transa, diag, m, n, alpha, A, lda, B, ldb);
     return;
   }
   gpu_ready();
