@@ -20,6 +20,18 @@ void xkblas_profile_enable(int on);
 void xkblas_profile_reset(void);
 int  xkblas_profile_get(const char* cls, long* launches, double* total_ms, double* flops);
 
+/* Trace in the CSV schema of the reference's trace reader (bin/ukilli.cpp:
+ * 848-1113, the files it writes from a KAAPI_RECORD_TRACE event file):
+ * kern.csv and task.csv get one row per call's device work (bracketed by
+ * HIP events on the compute stream; Name = the profiling class), cpy.csv one
+ * row per host<->device copy (Dir 1 = H2D, 2 = D2H, Size in bytes; Stream 2 =
+ * H2D, 3 = D2H), parallel.csv the traced region.  Times are seconds from the
+ * start of the trace; rows are written at each xkblas_sync().  Also started at
+ * initialisation by XKBLAS_TRACE_DIR=<dir>, and stopped by xkblas_finalize.
+ * Returns 0, or -1 if the files cannot be created. */
+int  xkblas_trace_start(const char* dir);
+int  xkblas_trace_stop(void);
+
 /* Factor the next diagonal block of POTRF on a side stream while the
  * trailing update runs (default on). */
 void xkblas_set_lookahead(int on);

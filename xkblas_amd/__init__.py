@@ -69,6 +69,8 @@ SIGNATURES = {
     "xkblas_get_split_count": (_i, []),
     "xkblas_profile_enable": (None, [_i]),
     "xkblas_profile_reset": (None, []),
+    "xkblas_trace_start": (_i, [ctypes.c_char_p]),
+    "xkblas_trace_stop": (_i, []),
     "xkblas_profile_get": (_i, [ctypes.c_char_p, _p, _p, _p]),
     "xkblas_set_lookahead": (None, [_i]),
     "xkblas_cache_bytes": (_l, []),

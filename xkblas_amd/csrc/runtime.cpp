@@ -356,8 +356,11 @@ void Runtime::drain_stages()
 void Runtime::copy_h2d(char* dst, size_t dpitch, const char* src, size_t spitch, size_t width, size_t height)
 {
   const size_t span = (height - 1) * spitch + width;
+  TraceCopy tc{nullptr, nullptr, 0, width * height, 1, 2};
+  if (tracing) trace_copy_begin(s_h2d, tc);
   if (!staging || width > STAGE_BYTES || host_pinned(src, span)) {
     XK_HIP(hipMemcpy2DAsync(dst, dpitch, src, spitch, width, height, hipMemcpyHostToDevice, s_h2d));
+    if (tracing) trace_copy_end(s_h2d, tc);
     return;
   }
   drain_stages();  // host data written back through the ring is in place first
@@ -369,13 +372,17 @@ void Runtime::copy_h2d(char* dst, size_t dpitch, const char* src, size_t spitch,
     XK_HIP(hipMemcpy2DAsync(dst + c * dpitch, dpitch, stage[b], width, width, cc, hipMemcpyHostToDevice, s_h2d));
     XK_HIP(hipEventRecord(stage_ev[b], s_h2d));
   }
+  if (tracing) trace_copy_end(s_h2d, tc);
 }
 
 void Runtime::copy_d2h(char* dst, size_t dpitch, const char* src, size_t spitch, size_t width, size_t height)
 {
   const size_t span = (height - 1) * dpitch + width;
+  TraceCopy tc{nullptr, nullptr, 0, width * height, 2, 3};
+  if (tracing) trace_copy_begin(s_d2h, tc);
   if (!staging || width > STAGE_BYTES || host_pinned(dst, span)) {
     XK_HIP(hipMemcpy2DAsync(dst, dpitch, src, spitch, width, height, hipMemcpyDeviceToHost, s_d2h));
+    if (tracing) trace_copy_end(s_d2h, tc);
     return;
   }
   const size_t per = STAGE_BYTES / width;
@@ -386,6 +393,7 @@ void Runtime::copy_d2h(char* dst, size_t dpitch, const char* src, size_t spitch,
     XK_HIP(hipEventRecord(stage_ev[b], s_d2h));
     unpack[b] = Unpack{dst + c * dpitch, dpitch, width, cc, true};
   }
+  if (tracing) trace_copy_end(s_d2h, tc);
 }
 
 // Order the compute stream after the pending copies.
@@ -445,6 +453,8 @@ void Runtime::init_device()
     cache_limit = dflt;
   }
   device_ready = true;
+  if (const char* td = getenv("XKBLAS_TRACE_DIR"))
+    if (*td && !trace_start(td)) fprintf(stderr, "[xkblas_amd] cannot write the trace into '%s'\n", td);
 }
 
 void Runtime::sync_all()
@@ -459,12 +469,20 @@ void Runtime::sync_all()
 }
 
 // ------------------------------------------------------------- profiling
+static uint64_t now_ns()
+{
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return (uint64_t)ts.tv_sec * 1000000000ull + (uint64_t)ts.tv_nsec;
+}
+
 void Runtime::prof_begin(const char* cls, double flops)
 {
-  if (!profiling) return;
+  if (!profiling && !tracing) return;
   ProfEntry e;
   e.cls = cls;
   e.flops = flops;
+  e.push_ns = now_ns();
   XK_HIP(hipEventCreate(&e.b));
   XK_HIP(hipEventCreate(&e.e));
   XK_HIP(hipEventRecord(e.b, s_comp));
@@ -473,7 +491,7 @@ void Runtime::prof_begin(const char* cls, double flops)
 
 void Runtime::prof_end()
 {
-  if (!profiling || prof_open.empty()) return;
+  if (prof_open.empty()) return;
   ProfEntry e = prof_open.back();
   prof_open.pop_back();
   XK_HIP(hipEventRecord(e.e, s_comp));
@@ -486,14 +504,102 @@ void Runtime::prof_collect()
     float ms = 0.0f;
     XK_HIP(hipEventSynchronize(e.e));
     XK_HIP(hipEventElapsedTime(&ms, e.b, e.e));
-    ProfStat& st = prof[e.cls];
-    st.count += 1;
-    st.ms += ms;
-    st.flops += e.flops;
+    if (profiling) {
+      ProfStat& st = prof[e.cls];
+      st.count += 1;
+      st.ms += ms;
+      st.flops += e.flops;
+    }
+    if (tracing) {
+      // "Resource,Type,Stream,Push,Start,End,Duration,Name,TaskId" and
+      // "Resource,Type,Push,Start,End,Duration,Name,TaskId"
+      const double push = (double)(e.push_ns - trace_t0) * 1e-9, t0 = trace_sec(e.b), t1 = trace_sec(e.e);
+      const long id = ++trace_id;
+      fprintf(f_kern, "1,HIP,1,%.15f,%.15f,%.15f,%.15f,%s,%ld\n", push, t0, t1, t1 - t0, e.cls, id);
+      fprintf(f_task, "1,HIP,%.15f,%.15f,%.15f,%.15f,%s,%ld\n", push, t0, t1, t1 - t0, e.cls, id);
+    }
     XK_HIP(hipEventDestroy(e.b));
     XK_HIP(hipEventDestroy(e.e));
   }
   prof_pending.clear();
+  if (tracing) {
+    for (auto& c : trace_copies) {
+      XK_HIP(hipEventSynchronize(c.e));
+      const double push = (double)(c.push_ns - trace_t0) * 1e-9, t0 = trace_sec(c.b), t1 = trace_sec(c.e);
+      // "Resource,Type,Stream,Push,Start,End,Duration,Size,Dir"
+      fprintf(f_cpy, "1,HIP,%d,%.15f,%.15f,%.15f,%.15f,%zu,%d\n", c.stream, push, t0, t1, t1 - t0, c.bytes, c.dir);
+      XK_HIP(hipEventDestroy(c.b));
+      XK_HIP(hipEventDestroy(c.e));
+    }
+    trace_copies.clear();
+    fflush(f_kern);
+    fflush(f_task);
+    fflush(f_cpy);
+  }
+}
+
+// seconds from the trace's base event to `ev` (both complete)
+double Runtime::trace_sec(hipEvent_t ev)
+{
+  float ms = 0.0f;
+  XK_HIP(hipEventElapsedTime(&ms, trace_base, ev));
+  return (double)ms * 1e-3;
+}
+
+bool Runtime::trace_start(const char* dir)
+{
+  if (tracing) trace_stop();
+  std::string d = dir && *dir ? dir : ".";
+  auto open = [&](const char* name, const char* header) {
+    FILE* f = fopen((d + "/" + name).c_str(), "w");
+    if (f) fprintf(f, "%s\n", header);
+    return f;
+  };
+  f_kern = open("kern.csv", "Resource,Type,Stream,Push,Start,End,Duration,Name,TaskId");
+  f_task = open("task.csv", "Resource,Type,Push,Start,End,Duration,Name,TaskId");
+  f_cpy = open("cpy.csv", "Resource,Type,Stream,Push,Start,End,Duration,Size,Dir");
+  f_par = open("parallel.csv", "Resource,Type,Start,End,Duration,Name");
+  if (!f_kern || !f_task || !f_cpy || !f_par) {
+    for (FILE* f : {f_kern, f_task, f_cpy, f_par})
+      if (f) fclose(f);
+    f_kern = f_task = f_cpy = f_par = nullptr;
+    return false;
+  }
+  sync_all();
+  XK_HIP(hipEventCreate(&trace_base));
+  XK_HIP(hipEventRecord(trace_base, s_comp));
+  XK_HIP(hipEventSynchronize(trace_base));
+  trace_t0 = now_ns();
+  trace_id = 0;
+  tracing = true;
+  return true;
+}
+
+void Runtime::trace_stop()
+{
+  if (!tracing) return;
+  sync_all();
+  const double t1 = (double)(now_ns() - trace_t0) * 1e-9;
+  fprintf(f_par, "0,CPU,%.15f,%.15f,%.15f,xkblas\n", 0.0, t1, t1);
+  for (FILE* f : {f_kern, f_task, f_cpy, f_par}) fclose(f);
+  f_kern = f_task = f_cpy = f_par = nullptr;
+  XK_HIP(hipEventDestroy(trace_base));
+  trace_base = nullptr;
+  tracing = false;
+}
+
+void Runtime::trace_copy_begin(hipStream_t s, TraceCopy& c)
+{
+  c.push_ns = now_ns();
+  XK_HIP(hipEventCreate(&c.b));
+  XK_HIP(hipEventCreate(&c.e));
+  XK_HIP(hipEventRecord(c.b, s));
+}
+
+void Runtime::trace_copy_end(hipStream_t s, TraceCopy& c)
+{
+  XK_HIP(hipEventRecord(c.e, s));
+  trace_copies.push_back(c);
 }
 
 }  // namespace xk
@@ -558,6 +664,7 @@ int xkblas_finalize(void)
   if (r.refcount > 0) r.refcount--;
   if (r.refcount == 0 && r.device_ready) {
     r.sync_all();
+    r.trace_stop();
     if (getenv("XKBLAS_VERBOSE") && atoi(getenv("XKBLAS_VERBOSE")) > 0) {
       fprintf(stderr, "[xkblas_amd] h2d %.3f GB  d2h %.3f GB  evictions %ld  mirrors %zu (%.3f GB)\n",
               r.stats.h2d_bytes / 1e9, r.stats.d2h_bytes / 1e9, r.stats.evictions, r.map.size(),
@@ -1202,6 +1309,21 @@ int xkblas_dposv_ir(int uplo, int N, int NRHS, const double* A, int LDA, const d
   r.prof_end();
   r.mark_written(rX, 0, 0, N, NRHS, 0);
   return info;
+}
+
+// ---- trace in the reference trace reader's CSV schema (bin/ukilli.cpp:848-1113)
+int xkblas_trace_start(const char* dir)
+{
+  Lock l(g_mu);
+  ensure_init();
+  return rt().trace_start(dir) ? 0 : -1;
+}
+
+int xkblas_trace_stop(void)
+{
+  Lock l(g_mu);
+  rt().trace_stop();
+  return 0;
 }
 
 // ---- profiling extension

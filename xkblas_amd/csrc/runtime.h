@@ -33,6 +33,16 @@ struct ProfEntry {
   const char* cls;
   double flops;
   hipEvent_t b, e;
+  uint64_t push_ns;  // host time of the enqueue (trace)
+};
+
+// One host<->device copy seen by the trace: events around it on its stream.
+struct TraceCopy {
+  hipEvent_t b, e;
+  uint64_t push_ns;
+  size_t bytes;
+  int dir;     // 1 H2D, 2 D2H (the reference's trace codes)
+  int stream;  // 1 compute, 2 H2D, 3 D2H
 };
 struct ProfStat {
   long count = 0;
@@ -91,6 +101,16 @@ struct Runtime {
   std::vector<ProfEntry> prof_open, prof_pending;
   std::map<std::string, ProfStat> prof;
 
+  // Trace in the CSV schema of the reference's trace reader (bin/ukilli.cpp):
+  // kern.csv / task.csv (one row per call's device work), cpy.csv (copies),
+  // parallel.csv (the traced region).  Times in seconds from trace start.
+  bool tracing = false;
+  hipEvent_t trace_base = nullptr;
+  uint64_t trace_t0 = 0;
+  long trace_id = 0;
+  FILE *f_kern = nullptr, *f_task = nullptr, *f_cpy = nullptr, *f_par = nullptr;
+  std::vector<TraceCopy> trace_copies;
+
   void init_device();
   void sync_all();
   Replica* replica(const void* host, int M, int N, long ld, int elt, int nb, std::vector<Replica*>& keep);
@@ -110,6 +130,11 @@ struct Runtime {
   void prof_begin(const char* cls, double flops);
   void prof_end();
   void prof_collect();
+  bool trace_start(const char* dir);
+  void trace_stop();
+  double trace_sec(hipEvent_t ev);
+  void trace_copy_begin(hipStream_t s, TraceCopy& c);
+  void trace_copy_end(hipStream_t s, TraceCopy& c);
 };
 
 Runtime& rt();
