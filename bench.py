@@ -607,19 +607,22 @@ def bench_potrf(xk, S, nb, mode, dsyrk_tflops=None, samples=96):
 
 def bench_l3_more(xk, n=8192):
     """DGEMMT (lower, NT), DSYR2K (lower, N), DSYMM (left, lower), DTRMM
-    (left, lower, N, non-unit) of order n in FP64 MFMA, operands resident in
-    HBM: the second of two calls, HIP-event kernel time.  TFLOP/s from the
-    textbook flop counts (triangle-only for GEMMT/SYR2K, m^2 n for TRMM)."""
+    (left, lower, N, non-unit) and ZGEMM (N, C) of order n in FP64 MFMA,
+    operands resident in HBM: the second of two calls, HIP-event kernel time.
+    TFLOP/s from the textbook flop counts (triangle-only for GEMMT/SYR2K, m^2 n
+    for TRMM, 8 n^3 real flop for ZGEMM)."""
     L = xk.lib()
     L.xkblas_set_modemath(0)
     L.xkblas_set_param(0, 0)
     r = np.random.default_rng(17)
     A, B, C = (np.asfortranarray(r.uniform(-0.5, 0.5, (n, n))) for _ in range(3))
+    ZA, ZB, ZC = (np.asfortranarray(X + 0.5j * Y) for X, Y in ((A, B), (B, C), (C, A)))
     runs = {
         "dgemmt": ("gemmt_f64", float(n) * n * (n + 1), lambda: xk.gemmt(122, 111, 112, 1.0, A, B, 0.5, C, sync=False)),
         "dsyr2k": ("syr2k_f64", 2.0 * n * n * (n + 1), lambda: xk.syr2k(122, 111, 1.0, A, B, 0.5, C, sync=False)),
         "dsymm": ("symm_f64", 2.0 * n ** 3, lambda: xk.symm(141, 122, 1.0, A, B, 0.5, C, sync=False)),
         "dtrmm": ("trmm_f64", float(n) ** 3, lambda: xk.trmm(141, 122, 111, 131, 1.0, A, C, sync=False)),
+        "zgemm": ("gemm_c128", 8.0 * n ** 3, lambda: xk.gemm(111, 113, 1.0 - 0.5j, ZA, ZB, 0.5j, ZC, sync=False)),
     }
     out = {"n": n}
     for name, (cls, fl, call) in runs.items():

@@ -38,6 +38,20 @@ typedef enum { CblasLeft = 141, CblasRight = 142 } CBLAS_SIDE;
 typedef CBLAS_ORDER CBLAS_LAYOUT;
 #endif
 
+/* Complex element types (reference blas/xkblas.h:62-64): interleaved
+ * (re, im), layout-compatible between C99 complex and std::complex. */
+#ifdef __cplusplus
+}
+#include <complex>
+typedef std::complex<float> Complex32_t;
+typedef std::complex<double> Complex64_t;
+extern "C" {
+#else
+#include <complex.h>
+typedef float complex Complex32_t;
+typedef double complex Complex64_t;
+#endif
+
 /* Kernel list (reference blas/xkblas.h:124-139). */
 typedef enum {
   KERN_VOID, KERN_GEMM, KERN_GEMMT, KERN_TRMM, KERN_TRSM, KERN_SYMM, KERN_SYRK,
@@ -100,6 +114,18 @@ int xkblas_dtrsm_async(int side, int uplo, int transA, int diag, int N, int NRHS
  * Returns 0 when enqueued; the LAPACK-style info is read after xkblas_sync()
  * with xkblas_potrf_info(A). */
 int xkblas_dpotrf_async(int uplo, int N, double* A, int LDA);
+
+/* Complex GEMM (gen z / c of reference blas/zxkblas.h:45, zgemm.c:118): same
+ * arguments and codes; computed as one real GEMM of order (2M, N, 2K) on the
+ * MFMA kernel (the split modes apply to the FP64 one). */
+int xkblas_zgemm_async(int transA, int transB, int M, int N, int K,
+                       const Complex64_t* alpha, const Complex64_t* A, int LDA,
+                       const Complex64_t* B, int LDB,
+                       const Complex64_t* beta, Complex64_t* C, int LDC);
+int xkblas_cgemm_async(int transA, int transB, int M, int N, int K,
+                       const Complex32_t* alpha, const Complex32_t* A, int LDA,
+                       const Complex32_t* B, int LDB,
+                       const Complex32_t* beta, Complex32_t* C, int LDC);
 
 /* The other L3 tile algorithms (reference blas/zgemmt.c, zsyr2k.c, zsymm.c,
  * ztrmm.c), same argument order and codes.  GEMMT writes only the uplo

@@ -1,0 +1,97 @@
+"""ZGEMM / CGEMM (reference blas/zgemm.c, gen z and c) through the C ABI:
+every op(A) x op(B) combination of N, T, C against the textbook complex
+product in NumPy, with the reference driver's normwise GEMM bar
+(testing/ztesting_gemm.c:288-306, <= 3 in units of eps, complex moduli);
+small Gaussian integers must be bit-exact; beta = 0 never reads C."""
+import numpy as np
+import pytest
+
+from helpers import rng
+
+pytestmark = pytest.mark.gpu
+
+OPS = [111, 112, 113]
+EPS = 2.0 ** -53
+
+
+def _c(r, m, n, dtype=np.complex128):
+    return np.asfortranarray((r.uniform(-0.5, 0.5, (m, n)) + 1j * r.uniform(-0.5, 0.5, (m, n))).astype(dtype))
+
+
+def _op(X, t):
+    return X if t == 111 else (X.T if t == 112 else X.conj().T)
+
+
+def _inf(X):
+    return float(np.abs(X).sum(axis=1).max()) if X.size else 0.0
+
+
+def _bar(alpha, oA, oB, beta, C0, ref, out):
+    K = oA.shape[1]
+    norm = np.sqrt(K + 2.0) * abs(alpha) * _inf(oA) * _inf(oB) + np.sqrt(2.0) * abs(beta) * _inf(C0)
+    return _inf(ref - out) / norm / EPS
+
+
+@pytest.mark.parametrize("ta", OPS)
+@pytest.mark.parametrize("tb", OPS)
+@pytest.mark.parametrize("shape", [(130, 70, 90), (1, 1, 1), (257, 3, 300)])
+def test_zgemm(xk, ta, tb, shape):
+    M, N, K = shape
+    r = rng(M + N + K + ta * 3 + tb)
+    A = _c(r, M, K) if ta == 111 else _c(r, K, M)
+    B = _c(r, K, N) if tb == 111 else _c(r, N, K)
+    C0 = _c(r, M, N)
+    alpha, beta = 1.5 - 0.5j, -0.5 + 0.25j
+    C = C0.copy(order="F")
+    xk.gemm(ta, tb, alpha, A, B, beta, C)
+    xk.invalidate()
+    ref = alpha * (_op(A, ta) @ _op(B, tb)) + beta * C0
+    assert _bar(alpha, _op(A, ta), _op(B, tb), beta, C0, ref, C) <= 3.0
+
+
+def test_zgemm_special_scalars_and_exact(xk):
+    r = rng(61)
+    M, N, K = 96, 80, 120
+    A = np.asfortranarray(r.integers(-4, 5, (M, K)) + 1j * r.integers(-4, 5, (M, K)))
+    B = np.asfortranarray(r.integers(-4, 5, (N, K)) + 1j * r.integers(-4, 5, (N, K)))
+    C = np.full((M, N), np.nan + 1j * np.nan, order="F")
+    xk.gemm(111, 113, 1.0, A, B, 0.0, C)  # beta = 0: C is not read
+    xk.invalidate()
+    assert np.array_equal(C, A @ B.conj().T)
+    B = np.asfortranarray(B.T)
+    C0 = np.asfortranarray(r.integers(-4, 5, (M, N)) + 1j * r.integers(-4, 5, (M, N)))
+    C = C0.copy(order="F")
+    nanA = np.full((M, K), np.nan, dtype=np.complex128, order="F")
+    xk.gemm(111, 111, 0.0, nanA, B, 2.0 - 1.0j, C)  # alpha = 0: A, B are not read
+    xk.invalidate()
+    assert np.array_equal(C, (2.0 - 1.0j) * C0)
+
+
+def test_zgemm_split_mode(xk):
+    L = xk.lib()
+    r = rng(67)
+    M, N, K = 200, 150, 333
+    A, B, C0 = _c(r, M, K), _c(r, N, K), _c(r, M, N)
+    L.xkblas_set_modemath(2)
+    try:
+        C = C0.copy(order="F")
+        xk.gemm(111, 113, 1.0 + 1.0j, A, B, 0.5, C)
+    finally:
+        L.xkblas_set_modemath(0)
+        xk.invalidate()
+    ref = (1.0 + 1.0j) * (A @ B.conj().T) + 0.5 * C0
+    assert _bar(1.0 + 1.0j, A, B.conj().T, 0.5, C0, ref, C) <= 3.0
+
+
+@pytest.mark.parametrize("ta,tb", [(111, 111), (113, 112)])
+def test_cgemm(xk, ta, tb):
+    r = rng(71 + ta + tb)
+    M, N, K = 180, 90, 150
+    A = _c(r, M, K, np.complex64) if ta == 111 else _c(r, K, M, np.complex64)
+    B = _c(r, K, N, np.complex64) if tb == 111 else _c(r, N, K, np.complex64)
+    C0 = _c(r, M, N, np.complex64)
+    C = C0.copy(order="F")
+    xk.gemm(ta, tb, np.complex64(0.5 + 2.0j), A, B, np.complex64(1.0), C)
+    xk.invalidate()
+    ref = (0.5 + 2.0j) * (_op(A.astype(np.complex128), ta) @ _op(B.astype(np.complex128), tb)) + C0
+    assert np.abs(C - ref).max() / np.abs(ref).max() < 2e-6

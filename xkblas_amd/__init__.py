@@ -98,6 +98,8 @@ SIGNATURES = {
     "xkblas_set_gemm_tile": (None, [_i]),
     "xkblas_set_split_kernel": (None, [_i]),
     "xkblas_set_potrf_pairing": (None, [_i]),
+    "xkblas_zgemm_async": (_i, [_i, _i, _i, _i, _i, _p, _p, _i, _p, _i, _p, _p, _i]),
+    "xkblas_cgemm_async": (_i, [_i, _i, _i, _i, _i, _p, _p, _i, _p, _i, _p, _p, _i]),
     "xkblas_dgemmt_async": (_i, [_i, _i, _i, _i, _i, _p, _p, _i, _p, _i, _p, _p, _i]),
     "xkblas_sgemmt_async": (_i, [_i, _i, _i, _i, _i, _p, _p, _i, _p, _i, _p, _p, _i]),
     "xkblas_dsyr2k_async": (_i, [_i, _i, _i, _i, _p, _p, _i, _p, _i, _p, _p, _i]),
@@ -161,7 +163,8 @@ def gemm(ta, tb, alpha, A, B, beta, C, sync=True):
         _check(X, dt)
     M, N = C.shape
     K = A.shape[1] if ta == NoTrans else A.shape[0]
-    fn = lib().xkblas_dgemm_async if dt == np.float64 else lib().xkblas_sgemm_async
+    fn = {np.dtype(np.float64): lib().xkblas_dgemm_async, np.dtype(np.float32): lib().xkblas_sgemm_async,
+          np.dtype(np.complex128): lib().xkblas_zgemm_async, np.dtype(np.complex64): lib().xkblas_cgemm_async}[dt]
     al, be = _scalar(alpha, dt), _scalar(beta, dt)
     rc = fn(ta, tb, M, N, K, _ptr(al), _ptr(A), max(1, A.shape[0]), _ptr(B), max(1, B.shape[0]), _ptr(be), _ptr(C),
             max(1, M))

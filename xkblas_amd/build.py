@@ -21,7 +21,7 @@ FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-u
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          "-I" + CSRC, "-I" + os.path.join(HERE, "..", "include")]
 
-SOURCES = ["gemm.hip", "factor.hip", "split.hip", "batched.hip", "mlp.hip", "blas3.cpp", "runtime.cpp", "native.cpp", "tune.hip", "refine.hip"]
+SOURCES = ["gemm.hip", "factor.hip", "split.hip", "batched.hip", "mlp.hip", "blas3.cpp", "runtime.cpp", "native.cpp", "tune.hip", "refine.hip", "complex.hip"]
 
 
 def _compile(src, force):

@@ -500,6 +500,42 @@ void dev_trmm(const Exec& x, int side, int uplo, int transA, int diag, int m, in
             op_lower ? KRANGE_B_LOWER : KRANGE_B_UPPER);
 }
 
+// ---------------------------------------------------------------- complex
+// [Pr; Pi] = [Xr -Xi; Xi Xr] [Yr; Yi] for X = op(A), Y = op(B): one real
+// GEMM (2m x n x 2k); for a (conjugate) transposed operand the transpose of
+// its block matrix is built and read transposed.
+enum { WS_ZA = 20, WS_ZB = 21, WS_ZP = 22 };
+
+template <typename T>
+void dev_cgemm(const Exec& x, int transA, int transB, int m, int n, int k, const T* alpha, const T* A, long lda,
+               const T* B, long ldb, const T* beta, T* C, long ldc)
+{
+  if (m <= 0 || n <= 0) return;
+  if (k == 0 || (alpha[0] == T(0) && alpha[1] == T(0))) {
+    cplx_combine<T>(x.s, m, n, nullptr, 0, alpha, beta, C, ldc);
+    return;
+  }
+  auto rup = [](long v) { return (v + 31) / 32 * 32; };
+  // op(A): NoTrans m x k -> [Ar -Ai; Ai Ar]; Trans (k x m stored) -> [Ar Ai;
+  // -Ai Ar] read transposed; ConjTrans -> [Ar -Ai; Ai Ar] read transposed
+  const bool ta = transA != NoTrans;
+  const int ar = ta ? k : m, ac = ta ? m : k;
+  const long ldya = rup(2L * ar);
+  T* YA = (T*)scratch(WS_ZA, sizeof(T) * (size_t)ldya * 2 * ac);
+  cplx_pack_a<T>(x.s, transA == Trans ? -1 : 1, ar, ac, A, lda, YA, ldya);
+  // op(B): NoTrans -> [Br; Bi]; Trans (n x k stored) -> [Br, Bi] read
+  // transposed; ConjTrans -> [Br, -Bi] read transposed
+  const bool tb = transB != NoTrans;
+  const int br = tb ? n : k, bc = tb ? k : n;
+  const long ldyb = rup(tb ? (long)br : 2L * br);
+  T* YB = (T*)scratch(WS_ZB, sizeof(T) * (size_t)ldyb * (tb ? 2 * bc : bc));
+  cplx_pack_b<T>(x.s, tb ? 0 : 1, transB == ConjTrans ? -1 : 1, br, bc, B, ldb, YB, ldyb);
+  const long ldp = rup(2L * m);
+  T* P = (T*)scratch(WS_ZP, sizeof(T) * (size_t)ldp * n);
+  dev_gemm<T>(x, ta ? Trans : NoTrans, tb ? Trans : NoTrans, 2 * m, n, 2 * k, T(1), YA, ldya, YB, ldyb, T(0), P, ldp);
+  cplx_combine<T>(x.s, m, n, P, ldp, alpha, beta, C, ldc);
+}
+
 // ---------------------------------------------------------------- SPD solve
 // LAPACK dsposv's mixed-precision scheme (factor in low precision, residuals
 // and updates in FP64, stop when every column satisfies
@@ -602,7 +638,9 @@ int dev_posv_ir(const Exec& x, const Exec& xl, const Exec& xf, int uplo, int n, 
                              long);                                                                             \
   template void dev_syr2k<T>(const Exec&, int, int, int, int, T, const T*, long, const T*, long, T, T*, long);  \
   template void dev_symm<T>(const Exec&, int, int, int, int, T, const T*, long, const T*, long, T, T*, long);   \
-  template void dev_trmm<T>(const Exec&, int, int, int, int, int, int, T, const T*, long, T*, long);
+  template void dev_trmm<T>(const Exec&, int, int, int, int, int, int, T, const T*, long, T*, long);           \
+  template void dev_cgemm<T>(const Exec&, int, int, int, int, int, const T*, const T*, long, const T*, long,     \
+                             const T*, T*, long);
 XK_INST(double)
 XK_INST(float)
 #undef XK_INST

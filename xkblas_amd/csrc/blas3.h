@@ -62,6 +62,13 @@ template <typename T>
 void dev_trmm(const Exec& x, int side, int uplo, int transA, int diag, int m, int n, T alpha, const T* A, long lda,
               T* B, long ldb);
 
+// Complex C <- alpha op(A) op(B) + beta C (T = double: Z, float: C; complex
+// numbers interleaved (re, im), alpha/beta as (re, im) pairs, leading
+// dimensions in complex elements) as one real GEMM of order (2m, n, 2k).
+template <typename T>
+void dev_cgemm(const Exec& x, int transA, int transB, int m, int n, int k, const T* alpha, const T* A, long lda,
+               const T* B, long ldb, const T* beta, T* C, long ldc);
+
 // Solution of A X = B (A SPD, uplo triangle stored; B, X n x nrhs) from a
 // factor computed under `xl` (a reduced-precision split mode) and FP64
 // iterative refinement under `x`: LAPACK dsposv's scheme.  Returns the

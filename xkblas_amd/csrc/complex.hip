@@ -1,0 +1,124 @@
+// complex.hip — complex GEMM on the real MFMA kernel.
+//
+// With X = op(A) (m x k) and Y = op(B) (k x n) complex, the product P = X Y
+// is one real GEMM of twice the order:
+//     [Pr]   [Xr  -Xi] [Yr]
+//     [Pi] = [Xi   Xr] [Yi]      (2m x 2k) (2k x n) -> (2m x n)
+// (the "4M" form: four real products, here in a single launch).  These
+// kernels build the real operands from interleaved (re, im) storage — for a
+// transposed or conjugated operand they build the transpose of the block
+// matrix, which the GEMM then reads transposed — and fold alpha, beta back
+// into the interleaved C.  All are single HBM passes.
+#include "kernels.h"
+#include <algorithm>
+
+namespace xk {
+
+template <typename T>
+struct Cplx {
+  T re, im;
+};
+
+// Y (2 rows x 2 cols) <- [Ar, -s Ai; s Ai, Ar] for the rows x cols complex A.
+template <typename T>
+__global__ __launch_bounds__(256) void cplx_pack_a_kernel(int rows, int cols, int j0, T s, const Cplx<T>* __restrict__ A,
+                                                          long lda, T* __restrict__ Y, long ldy)
+{
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  const int j = j0 + (int)blockIdx.y;
+  if (i >= rows || j >= cols) return;
+  const Cplx<T> a = A[i + (long)j * lda];
+  Y[i + (long)j * ldy] = a.re;
+  Y[i + rows + (long)j * ldy] = s * a.im;
+  Y[i + (long)(j + cols) * ldy] = -s * a.im;
+  Y[i + rows + (long)(j + cols) * ldy] = a.re;
+}
+
+// vertical: Y (2 rows x cols) <- [Br; Bi]; else Y (rows x 2 cols) <- [Br, s Bi].
+template <typename T>
+__global__ __launch_bounds__(256) void cplx_pack_b_kernel(int vertical, int rows, int cols, int j0, T s,
+                                                          const Cplx<T>* __restrict__ B, long ldb, T* __restrict__ Y,
+                                                          long ldy)
+{
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  const int j = j0 + (int)blockIdx.y;
+  if (i >= rows || j >= cols) return;
+  const Cplx<T> b = B[i + (long)j * ldb];
+  Y[i + (long)j * ldy] = b.re;
+  if (vertical) Y[i + rows + (long)j * ldy] = b.im;
+  else Y[i + (long)(j + cols) * ldy] = s * b.im;
+}
+
+// C <- alpha (Pr + i Pi) + beta C, P = [Pr; Pi] real (2m x n); use_p = 0:
+// C <- beta C; beta = 0 never reads C.
+template <typename T>
+__global__ __launch_bounds__(256) void cplx_combine_kernel(int m, int n, int j0, int use_p, const T* __restrict__ P,
+                                                           long ldp, T ar, T ai, T br, T bi, Cplx<T>* __restrict__ C,
+                                                           long ldc)
+{
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  const int j = j0 + (int)blockIdx.y;
+  if (i >= m || j >= n) return;
+  T re = T(0), im = T(0);
+  if (use_p) {
+    const T pr = P[i + (long)j * ldp], pi = P[i + m + (long)j * ldp];
+    re = ar * pr - ai * pi;
+    im = ar * pi + ai * pr;
+  }
+  Cplx<T>* c = C + i + (long)j * ldc;
+  if (br != T(0) || bi != T(0)) {
+    const Cplx<T> v = *c;
+    re += br * v.re - bi * v.im;
+    im += br * v.im + bi * v.re;
+  }
+  *c = Cplx<T>{re, im};
+}
+
+template <typename F>
+static void over_columns(int rows, int cols, F launch)
+{
+  for (int j0 = 0; j0 < cols; j0 += 65535) launch(dim3((rows + 255) / 256, std::min(65535, cols - j0)), j0);
+}
+
+template <typename T>
+void cplx_pack_a(hipStream_t s, int sigma, int rows, int cols, const T* A, long lda, T* Y, long ldy)
+{
+  if (rows <= 0 || cols <= 0) return;
+  over_columns(rows, cols, [&](dim3 g, int j0) {
+    hipLaunchKernelGGL((cplx_pack_a_kernel<T>), g, dim3(256), 0, s, rows, cols, j0, T(sigma),
+                       (const Cplx<T>*)A, lda, Y, ldy);
+  });
+  XK_HIP(hipGetLastError());
+}
+
+template <typename T>
+void cplx_pack_b(hipStream_t s, int vertical, int sign, int rows, int cols, const T* B, long ldb, T* Y, long ldy)
+{
+  if (rows <= 0 || cols <= 0) return;
+  over_columns(rows, cols, [&](dim3 g, int j0) {
+    hipLaunchKernelGGL((cplx_pack_b_kernel<T>), g, dim3(256), 0, s, vertical, rows, cols, j0, T(sign),
+                       (const Cplx<T>*)B, ldb, Y, ldy);
+  });
+  XK_HIP(hipGetLastError());
+}
+
+template <typename T>
+void cplx_combine(hipStream_t s, int m, int n, const T* P, long ldp, const T* alpha, const T* beta, T* C, long ldc)
+{
+  if (m <= 0 || n <= 0) return;
+  over_columns(m, n, [&](dim3 g, int j0) {
+    hipLaunchKernelGGL((cplx_combine_kernel<T>), g, dim3(256), 0, s, m, n, j0, P != nullptr ? 1 : 0, P, ldp,
+                       alpha[0], alpha[1], beta[0], beta[1], (Cplx<T>*)C, ldc);
+  });
+  XK_HIP(hipGetLastError());
+}
+
+#define XK_INST(T)                                                                                         \
+  template void cplx_pack_a<T>(hipStream_t, int, int, int, const T*, long, T*, long);                      \
+  template void cplx_pack_b<T>(hipStream_t, int, int, int, int, const T*, long, T*, long);                 \
+  template void cplx_combine<T>(hipStream_t, int, int, const T*, long, const T*, const T*, T*, long);
+XK_INST(double)
+XK_INST(float)
+#undef XK_INST
+
+}  // namespace xk

@@ -63,6 +63,19 @@ void tri_expand(hipStream_t s, int uplo, int unit, int n, const T* A, long lda, 
 // for the n columns of the m x n block X.
 void col_reduce(hipStream_t s, int op, int m, int n, const double* X, long ldx, double* out);
 
+// Complex GEMM operands for one real GEMM of twice the order (complex
+// matrices interleaved (re, im), leading dimensions in complex elements):
+// Y (2 rows x 2 cols) <- [Ar, -sigma Ai; sigma Ai, Ar];
+// Y <- [Br; Bi] (vertical, 2 rows x cols) or [Br, sign Bi] (rows x 2 cols);
+// C <- alpha (Pr + i Pi) + beta C with P = [Pr; Pi] (2m x n; P null: C <-
+// beta C; beta 0 never reads C); alpha, beta as (re, im).
+template <typename T>
+void cplx_pack_a(hipStream_t s, int sigma, int rows, int cols, const T* A, long lda, T* Y, long ldy);
+template <typename T>
+void cplx_pack_b(hipStream_t s, int vertical, int sign, int rows, int cols, const T* B, long ldb, T* Y, long ldy);
+template <typename T>
+void cplx_combine(hipStream_t s, int m, int n, const T* P, long ldp, const T* alpha, const T* beta, T* C, long ldc);
+
 // X <- X + D (m x n blocks).
 void add2d(hipStream_t s, int m, int n, const double* D, long ldd, double* X, long ldx);
 

@@ -1185,9 +1185,66 @@ int trmm_async(const char* fn, int side, int uplo, int transA, int diag, int N, 
   return 0;
 }
 
+// Complex GEMM: T is the real type; alpha/beta point at (re, im) pairs.
+template <typename T>
+int cgemm_async(const char* fn, int transA, int transB, int M, int N, int K, const T* alpha, const T* A, int LDA,
+                const T* B, int LDB, const T* beta, T* C, int LDC)
+{
+  // argument checks and codes of blas/zgemm.c:131-180
+  if (transA < NoTrans || transA > ConjTrans) return arg_error(fn, "illegal value of transA", -1);
+  if (transB < NoTrans || transB > ConjTrans) return arg_error(fn, "illegal value of transB", -2);
+  const int Am = transA == NoTrans ? M : K, An = transA == NoTrans ? K : M;
+  const int Bm = transB == NoTrans ? K : N, Bn = transB == NoTrans ? N : K;
+  if (M < 0) return arg_error(fn, "illegal value of M", -3);
+  if (N < 0) return arg_error(fn, "illegal value of N", -4);
+  if (K < 0) return arg_error(fn, "illegal value of K", -5);
+  if (LDA < std::max(1, Am)) return arg_error(fn, "illegal value of LDA", -8);
+  if (LDB < std::max(1, Bm)) return arg_error(fn, "illegal value of LDB", -10);
+  if (LDC < std::max(1, M)) return arg_error(fn, "illegal value of LDC", -13);
+  const bool a0 = alpha[0] == T(0) && alpha[1] == T(0), b1 = beta[0] == T(1) && beta[1] == T(0);
+  if (M == 0 || N == 0 || ((a0 || K == 0) && b1)) return 0;
+
+  Lock l(g_mu);
+  ensure_init();
+  Runtime& r = rt();
+  const int nb = (int)xkblas_auto_tilesize(nullptr, KERN_GEMM, M, N, K);
+  const int elt = 2 * sizeof(T);
+  std::vector<Replica*> keep;
+  const bool use_ab = K > 0 && !a0;
+  Replica* rA = use_ab ? r.replica(A, Am, An, LDA, elt, nb, keep) : nullptr;
+  Replica* rB = use_ab ? r.replica(B, Bm, Bn, LDB, elt, nb, keep) : nullptr;
+  Replica* rC = r.replica(C, M, N, LDC, elt, nb, keep);
+  if (rA) r.fetch(rA, 0, 0, Am, An);
+  if (rB) r.fetch(rB, 0, 0, Bm, Bn);
+  if (beta[0] != T(0) || beta[1] != T(0)) r.fetch(rC, 0, 0, M, N);
+  r.before_compute();
+  Exec x = r.exec();
+  r.prof_begin(sizeof(T) == 8 ? "gemm_c128" : "gemm_c64", 8.0 * M * N * (double)K);
+  const T zero[2] = {T(0), T(0)};
+  dev_cgemm<T>(x, transA, transB, M, N, use_ab ? K : 0, use_ab ? alpha : zero, rA ? (const T*)rA->dev : nullptr,
+               rA ? rA->ldd : 1, rB ? (const T*)rB->dev : nullptr, rB ? rB->ldd : 1, beta, (T*)rC->dev, rC->ldd);
+  r.prof_end();
+  r.mark_written(rC, 0, 0, M, N, 0);
+  return 0;
+}
+
 }  // namespace
 
 extern "C" {
+
+int xkblas_zgemm_async(int transA, int transB, int M, int N, int K, const Complex64_t* alpha, const Complex64_t* A,
+                       int LDA, const Complex64_t* B, int LDB, const Complex64_t* beta, Complex64_t* C, int LDC)
+{
+  return cgemm_async<double>("xkblas_zgemm_async", transA, transB, M, N, K, (const double*)alpha,
+                             (const double*)A, LDA, (const double*)B, LDB, (const double*)beta, (double*)C, LDC);
+}
+
+int xkblas_cgemm_async(int transA, int transB, int M, int N, int K, const Complex32_t* alpha, const Complex32_t* A,
+                       int LDA, const Complex32_t* B, int LDB, const Complex32_t* beta, Complex32_t* C, int LDC)
+{
+  return cgemm_async<float>("xkblas_cgemm_async", transA, transB, M, N, K, (const float*)alpha, (const float*)A,
+                            LDA, (const float*)B, LDB, (const float*)beta, (float*)C, LDC);
+}
 
 #define XK_NEXT_OPS(P, T)                                                                                         \
   int xkblas_##P##gemmt_async(int uplo, int transA, int transB, int N, int K, const T* alpha, const T* A, int LDA, \
