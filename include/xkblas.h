@@ -51,6 +51,7 @@ extern "C" {
 typedef float complex Complex32_t;
 typedef double complex Complex64_t;
 #endif
+typedef double CFloat64_t;
 
 /* Kernel list (reference blas/xkblas.h:124-139). */
 typedef enum {
@@ -123,6 +124,24 @@ int xkblas_zgemm_async(int transA, int transB, int M, int N, int K,
                        const Complex64_t* B, int LDB,
                        const Complex64_t* beta, Complex64_t* C, int LDC);
 int xkblas_cgemm_async(int transA, int transB, int M, int N, int K,
+                       const Complex32_t* alpha, const Complex32_t* A, int LDA,
+                       const Complex32_t* B, int LDB,
+                       const Complex32_t* beta, Complex32_t* C, int LDC);
+
+/* Complex HERK and HEMM (reference blas/zxkblas.h:119,131, zherk.c, zhemm.c):
+ * same arguments and codes (HERK's alpha, beta are real; the diagonal of C is
+ * left real). */
+int xkblas_zherk_async(int uplo, int trans, int N, int K,
+                       const CFloat64_t* alpha, const Complex64_t* A, int LDA,
+                       const CFloat64_t* beta, Complex64_t* C, int LDC);
+int xkblas_cherk_async(int uplo, int trans, int N, int K,
+                       const float* alpha, const Complex32_t* A, int LDA,
+                       const float* beta, Complex32_t* C, int LDC);
+int xkblas_zhemm_async(int side, int uplo, int M, int N,
+                       const Complex64_t* alpha, const Complex64_t* A, int LDA,
+                       const Complex64_t* B, int LDB,
+                       const Complex64_t* beta, Complex64_t* C, int LDC);
+int xkblas_chemm_async(int side, int uplo, int M, int N,
                        const Complex32_t* alpha, const Complex32_t* A, int LDA,
                        const Complex32_t* B, int LDB,
                        const Complex32_t* beta, Complex32_t* C, int LDC);

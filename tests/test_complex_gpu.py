@@ -95,3 +95,67 @@ def test_cgemm(xk, ta, tb):
     xk.invalidate()
     ref = (0.5 + 2.0j) * (_op(A.astype(np.complex128), ta) @ _op(B.astype(np.complex128), tb)) + C0
     assert np.abs(C - ref).max() / np.abs(ref).max() < 2e-6
+
+
+@pytest.mark.parametrize("uplo", [121, 122])
+@pytest.mark.parametrize("trans", [111, 113])
+@pytest.mark.parametrize("NK", [(150, 70), (1, 5), (64, 200)])
+def test_zherk(xk, uplo, trans, NK):
+    N, K = NK
+    r = rng(N + K + uplo + trans)
+    A = _c(r, N, K) if trans == 111 else _c(r, K, N)
+    C0 = _c(r, N, N)
+    C = C0.copy(order="F")
+    xk.herk(uplo, trans, 0.75, A, -1.5, C)
+    xk.invalidate()
+    oA = A if trans == 111 else A.conj().T
+    ref = 0.75 * (oA @ oA.conj().T) - 1.5 * C0
+    np.fill_diagonal(ref, ref.diagonal().real)
+    mask = np.tril(np.ones((N, N), bool)) if uplo == 122 else np.triu(np.ones((N, N), bool))
+    R = _inf(np.where(mask, ref - C, 0))
+    assert R / ((_inf(oA) ** 2 + _inf(C0)) * max(N, 1) * EPS) <= 10.0
+    assert np.all(C.diagonal().imag == 0.0)
+    assert np.array_equal(C[~mask], C0[~mask])
+
+
+@pytest.mark.parametrize("side", [141, 142])
+@pytest.mark.parametrize("uplo", [121, 122])
+def test_zhemm(xk, side, uplo):
+    M, N = 140, 90
+    ka = M if side == 141 else N
+    r = rng(side + uplo)
+    H = _c(r, ka, ka)
+    H = H + H.conj().T
+    np.fill_diagonal(H, H.diagonal().real)
+    mask = np.tril(np.ones((ka, ka), bool)) if uplo == 122 else np.triu(np.ones((ka, ka), bool))
+    A = np.asfortranarray(np.where(mask, H, np.nan))
+    A[np.diag_indices(ka)] += 7j  # the imaginary part of the diagonal is not read
+    B, C0 = _c(r, M, N), _c(r, M, N)
+    alpha, beta = 0.5 + 1.0j, 1.0 - 0.5j
+    C = C0.copy(order="F")
+    xk.hemm(side, uplo, alpha, A, B, beta, C)
+    xk.invalidate()
+    ref = alpha * (H @ B if side == 141 else B @ H) + beta * C0
+    oA, oB = (H, B) if side == 141 else (B, H)
+    assert _bar(alpha, oA, oB, beta, C0, ref, C) <= 3.0
+
+
+def test_cherk_chemm_fp32(xk):
+    r = rng(79)
+    N, K = 120, 60
+    A = _c(r, N, K, np.complex64)
+    C0 = _c(r, N, N, np.complex64)
+    C = C0.copy(order="F")
+    xk.herk(122, 111, 1.0, A, 1.0, C)
+    xk.invalidate()
+    A64 = A.astype(np.complex128)
+    ref = A64 @ A64.conj().T + C0
+    np.fill_diagonal(ref, ref.diagonal().real)
+    assert np.abs(np.tril(C - ref)).max() / np.abs(ref).max() < 2e-6
+    H = (C0 + C0.conj().T).astype(np.complex64)
+    B = _c(r, N, 30, np.complex64)
+    Y = np.zeros((N, 30), dtype=np.complex64, order="F")
+    xk.hemm(141, 121, np.complex64(1.0), np.asfortranarray(H), B, np.complex64(0.0), Y)
+    xk.invalidate()
+    ref = H.astype(np.complex128) @ B.astype(np.complex128)
+    assert np.abs(Y - ref).max() / np.abs(ref).max() < 2e-6

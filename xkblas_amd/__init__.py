@@ -100,6 +100,10 @@ SIGNATURES = {
     "xkblas_set_potrf_pairing": (None, [_i]),
     "xkblas_zgemm_async": (_i, [_i, _i, _i, _i, _i, _p, _p, _i, _p, _i, _p, _p, _i]),
     "xkblas_cgemm_async": (_i, [_i, _i, _i, _i, _i, _p, _p, _i, _p, _i, _p, _p, _i]),
+    "xkblas_zherk_async": (_i, [_i, _i, _i, _i, _p, _p, _i, _p, _p, _i]),
+    "xkblas_cherk_async": (_i, [_i, _i, _i, _i, _p, _p, _i, _p, _p, _i]),
+    "xkblas_zhemm_async": (_i, [_i, _i, _i, _i, _p, _p, _i, _p, _i, _p, _p, _i]),
+    "xkblas_chemm_async": (_i, [_i, _i, _i, _i, _p, _p, _i, _p, _i, _p, _p, _i]),
     "xkblas_dgemmt_async": (_i, [_i, _i, _i, _i, _i, _p, _p, _i, _p, _i, _p, _p, _i]),
     "xkblas_sgemmt_async": (_i, [_i, _i, _i, _i, _i, _p, _p, _i, _p, _i, _p, _p, _i]),
     "xkblas_dsyr2k_async": (_i, [_i, _i, _i, _i, _p, _p, _i, _p, _i, _p, _p, _i]),
@@ -267,6 +271,36 @@ def trmm(side, uplo, ta, diag, alpha, A, B, sync=True):
     rc = fn(side, uplo, ta, diag, M, N, _ptr(al), _ptr(A), max(1, A.shape[0]), _ptr(B), max(1, M))
     if sync:
         coherent(B)
+    return rc
+
+
+def herk(uplo, trans, alpha, A, beta, C, sync=True):
+    """Complex C <- alpha op(A) op(A)^H + beta C on the uplo triangle (alpha,
+    beta real)."""
+    dt = C.dtype
+    for X in (A, C):
+        _check(X, dt)
+    N = C.shape[0]
+    K = A.shape[1] if trans == NoTrans else A.shape[0]
+    real = np.float64 if dt == np.complex128 else np.float32
+    fn = lib().xkblas_zherk_async if dt == np.complex128 else lib().xkblas_cherk_async
+    al, be = _scalar(alpha, real), _scalar(beta, real)
+    rc = fn(uplo, trans, N, K, _ptr(al), _ptr(A), max(1, A.shape[0]), _ptr(be), _ptr(C), max(1, N))
+    if sync:
+        coherent(C, uplo)
+    return rc
+
+
+def hemm(side, uplo, alpha, A, B, beta, C, sync=True):
+    dt = C.dtype
+    for X in (A, B, C):
+        _check(X, dt)
+    M, N = C.shape
+    fn = lib().xkblas_zhemm_async if dt == np.complex128 else lib().xkblas_chemm_async
+    al, be = _scalar(alpha, dt), _scalar(beta, dt)
+    rc = fn(side, uplo, M, N, _ptr(al), _ptr(A), max(1, A.shape[0]), _ptr(B), max(1, M), _ptr(be), _ptr(C), max(1, M))
+    if sync:
+        coherent(C)
     return rc
 
 

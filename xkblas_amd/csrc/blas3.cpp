@@ -506,13 +506,14 @@ void dev_trmm(const Exec& x, int side, int uplo, int transA, int diag, int m, in
 // its block matrix is built and read transposed.
 enum { WS_ZA = 20, WS_ZB = 21, WS_ZP = 22 };
 
+// uplo/herm: write only a triangle of the square C (HERK: real diagonal).
 template <typename T>
-void dev_cgemm(const Exec& x, int transA, int transB, int m, int n, int k, const T* alpha, const T* A, long lda,
-               const T* B, long ldb, const T* beta, T* C, long ldc)
+static void cgemm_impl(const Exec& x, int transA, int transB, int m, int n, int k, const T* alpha, const T* A, long lda,
+                       const T* B, long ldb, const T* beta, T* C, long ldc, int uplo, int herm)
 {
   if (m <= 0 || n <= 0) return;
   if (k == 0 || (alpha[0] == T(0) && alpha[1] == T(0))) {
-    cplx_combine<T>(x.s, m, n, nullptr, 0, alpha, beta, C, ldc);
+    cplx_combine<T>(x.s, m, n, nullptr, 0, alpha, beta, C, ldc, uplo, herm);
     return;
   }
   auto rup = [](long v) { return (v + 31) / 32 * 32; };
@@ -533,7 +534,39 @@ void dev_cgemm(const Exec& x, int transA, int transB, int m, int n, int k, const
   const long ldp = rup(2L * m);
   T* P = (T*)scratch(WS_ZP, sizeof(T) * (size_t)ldp * n);
   dev_gemm<T>(x, ta ? Trans : NoTrans, tb ? Trans : NoTrans, 2 * m, n, 2 * k, T(1), YA, ldya, YB, ldyb, T(0), P, ldp);
-  cplx_combine<T>(x.s, m, n, P, ldp, alpha, beta, C, ldc);
+  cplx_combine<T>(x.s, m, n, P, ldp, alpha, beta, C, ldc, uplo, herm);
+}
+
+template <typename T>
+void dev_cgemm(const Exec& x, int transA, int transB, int m, int n, int k, const T* alpha, const T* A, long lda,
+               const T* B, long ldb, const T* beta, T* C, long ldc)
+{
+  cgemm_impl<T>(x, transA, transB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, 0, 0);
+}
+
+template <typename T>
+void dev_cherk(const Exec& x, int uplo, int trans, int n, int k, T alpha, const T* A, long lda, T beta, T* C,
+               long ldc)
+{
+  // C <- alpha op(A) op(A)^H + beta C on the uplo triangle, real diagonal;
+  // the full product is formed (the real GEMM has no triangle mode for the
+  // 2n x n block output) and only the triangle is combined
+  const T al[2] = {alpha, T(0)}, be[2] = {beta, T(0)};
+  const int ta = trans == NoTrans ? NoTrans : ConjTrans, tb = trans == NoTrans ? ConjTrans : NoTrans;
+  cgemm_impl<T>(x, ta, tb, n, n, k, al, A, lda, A, lda, be, C, ldc, uplo, 1);
+}
+
+template <typename T>
+void dev_chemm(const Exec& x, int side, int uplo, int m, int n, const T* alpha, const T* A, long lda, const T* B,
+               long ldb, const T* beta, T* C, long ldc)
+{
+  // the full Hermitian operand, then the complex GEMM
+  const int ka = side == Left ? m : n;
+  const long ldf = (ka + 15) / 16 * 16;
+  T* F = (T*)scratch(WS_X0, 2 * sizeof(T) * (size_t)ldf * ka);
+  if (!(alpha[0] == T(0) && alpha[1] == T(0))) cplx_herm_expand<T>(x.s, uplo, ka, A, lda, F, ldf);
+  if (side == Left) cgemm_impl<T>(x, NoTrans, NoTrans, m, n, m, alpha, F, ldf, B, ldb, beta, C, ldc, 0, 0);
+  else cgemm_impl<T>(x, NoTrans, NoTrans, m, n, n, alpha, B, ldb, F, ldf, beta, C, ldc, 0, 0);
 }
 
 // ---------------------------------------------------------------- SPD solve
@@ -640,7 +673,10 @@ int dev_posv_ir(const Exec& x, const Exec& xl, const Exec& xf, int uplo, int n, 
   template void dev_symm<T>(const Exec&, int, int, int, int, T, const T*, long, const T*, long, T, T*, long);   \
   template void dev_trmm<T>(const Exec&, int, int, int, int, int, int, T, const T*, long, T*, long);           \
   template void dev_cgemm<T>(const Exec&, int, int, int, int, int, const T*, const T*, long, const T*, long,     \
-                             const T*, T*, long);
+                             const T*, T*, long);                                                               \
+  template void dev_cherk<T>(const Exec&, int, int, int, int, T, const T*, long, T, T*, long);                   \
+  template void dev_chemm<T>(const Exec&, int, int, int, int, const T*, const T*, long, const T*, long, const T*, \
+                             T*, long);
 XK_INST(double)
 XK_INST(float)
 #undef XK_INST

@@ -69,6 +69,18 @@ template <typename T>
 void dev_cgemm(const Exec& x, int transA, int transB, int m, int n, int k, const T* alpha, const T* A, long lda,
                const T* B, long ldb, const T* beta, T* C, long ldc);
 
+// Complex C <- alpha op(A) op(A)^H + beta C on the uplo triangle (alpha,
+// beta real; trans NoTrans: A n x k, else ConjTrans: A k x n), real diagonal.
+template <typename T>
+void dev_cherk(const Exec& x, int uplo, int trans, int n, int k, T alpha, const T* A, long lda, T beta, T* C,
+               long ldc);
+
+// Complex C <- alpha A B + beta C (Left) or alpha B A + beta C (Right), A
+// Hermitian with its uplo triangle stored.
+template <typename T>
+void dev_chemm(const Exec& x, int side, int uplo, int m, int n, const T* alpha, const T* A, long lda, const T* B,
+               long ldb, const T* beta, T* C, long ldc);
+
 // Solution of A X = B (A SPD, uplo triangle stored; B, X n x nrhs) from a
 // factor computed under `xl` (a reduced-precision split mode) and FP64
 // iterative refinement under `x`: LAPACK dsposv's scheme.  Returns the

@@ -50,15 +50,17 @@ __global__ __launch_bounds__(256) void cplx_pack_b_kernel(int vertical, int rows
 }
 
 // C <- alpha (Pr + i Pi) + beta C, P = [Pr; Pi] real (2m x n); use_p = 0:
-// C <- beta C; beta = 0 never reads C.
+// C <- beta C; beta = 0 never reads C.  uplo Lower / Upper: only that
+// triangle of the square C, with a real diagonal when herm (HERK).
 template <typename T>
 __global__ __launch_bounds__(256) void cplx_combine_kernel(int m, int n, int j0, int use_p, const T* __restrict__ P,
                                                            long ldp, T ar, T ai, T br, T bi, Cplx<T>* __restrict__ C,
-                                                           long ldc)
+                                                           long ldc, int uplo, int herm)
 {
   const int i = blockIdx.x * 256 + threadIdx.x;
   const int j = j0 + (int)blockIdx.y;
   if (i >= m || j >= n) return;
+  if ((uplo == Lower && i < j) || (uplo == Upper && i > j)) return;
   T re = T(0), im = T(0);
   if (use_p) {
     const T pr = P[i + (long)j * ldp], pi = P[i + m + (long)j * ldp];
@@ -71,7 +73,27 @@ __global__ __launch_bounds__(256) void cplx_combine_kernel(int m, int n, int j0,
     re += br * v.re - bi * v.im;
     im += br * v.im + bi * v.re;
   }
+  if (herm && i == j) im = T(0);
   *c = Cplx<T>{re, im};
+}
+
+// Y (n x n) <- the Hermitian matrix whose uplo triangle is stored in A (the
+// imaginary part of the diagonal is taken as zero).
+template <typename T>
+__global__ __launch_bounds__(256) void cplx_herm_expand_kernel(int uplo, int n, int j0, const Cplx<T>* __restrict__ A,
+                                                               long lda, Cplx<T>* __restrict__ Y, long ldy)
+{
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  const int j = j0 + (int)blockIdx.y;
+  if (i >= n || j >= n) return;
+  Cplx<T> v;
+  if (i == j) v = Cplx<T>{A[i + (long)i * lda].re, T(0)};
+  else if ((uplo == Lower) == (i > j)) v = A[i + (long)j * lda];
+  else {
+    const Cplx<T> a = A[j + (long)i * lda];
+    v = Cplx<T>{a.re, -a.im};
+  }
+  Y[i + (long)j * ldy] = v;
 }
 
 template <typename F>
@@ -103,12 +125,24 @@ void cplx_pack_b(hipStream_t s, int vertical, int sign, int rows, int cols, cons
 }
 
 template <typename T>
-void cplx_combine(hipStream_t s, int m, int n, const T* P, long ldp, const T* alpha, const T* beta, T* C, long ldc)
+void cplx_combine(hipStream_t s, int m, int n, const T* P, long ldp, const T* alpha, const T* beta, T* C, long ldc,
+                  int uplo, int herm)
 {
   if (m <= 0 || n <= 0) return;
   over_columns(m, n, [&](dim3 g, int j0) {
     hipLaunchKernelGGL((cplx_combine_kernel<T>), g, dim3(256), 0, s, m, n, j0, P != nullptr ? 1 : 0, P, ldp,
-                       alpha[0], alpha[1], beta[0], beta[1], (Cplx<T>*)C, ldc);
+                       alpha[0], alpha[1], beta[0], beta[1], (Cplx<T>*)C, ldc, uplo, herm);
+  });
+  XK_HIP(hipGetLastError());
+}
+
+template <typename T>
+void cplx_herm_expand(hipStream_t s, int uplo, int n, const T* A, long lda, T* Y, long ldy)
+{
+  if (n <= 0) return;
+  over_columns(n, n, [&](dim3 g, int j0) {
+    hipLaunchKernelGGL((cplx_herm_expand_kernel<T>), g, dim3(256), 0, s, uplo, n, j0, (const Cplx<T>*)A, lda,
+                       (Cplx<T>*)Y, ldy);
   });
   XK_HIP(hipGetLastError());
 }
@@ -116,7 +150,8 @@ void cplx_combine(hipStream_t s, int m, int n, const T* P, long ldp, const T* al
 #define XK_INST(T)                                                                                         \
   template void cplx_pack_a<T>(hipStream_t, int, int, int, const T*, long, T*, long);                      \
   template void cplx_pack_b<T>(hipStream_t, int, int, int, int, const T*, long, T*, long);                 \
-  template void cplx_combine<T>(hipStream_t, int, int, const T*, long, const T*, const T*, T*, long);
+  template void cplx_combine<T>(hipStream_t, int, int, const T*, long, const T*, const T*, T*, long, int, int);  \
+  template void cplx_herm_expand<T>(hipStream_t, int, int, const T*, long, T*, long);
 XK_INST(double)
 XK_INST(float)
 #undef XK_INST

@@ -73,8 +73,14 @@ template <typename T>
 void cplx_pack_a(hipStream_t s, int sigma, int rows, int cols, const T* A, long lda, T* Y, long ldy);
 template <typename T>
 void cplx_pack_b(hipStream_t s, int vertical, int sign, int rows, int cols, const T* B, long ldb, T* Y, long ldy);
+// uplo Lower / Upper: only that triangle of the square C is written, with a
+// real diagonal when herm (HERK); 0: all of C.
 template <typename T>
-void cplx_combine(hipStream_t s, int m, int n, const T* P, long ldp, const T* alpha, const T* beta, T* C, long ldc);
+void cplx_combine(hipStream_t s, int m, int n, const T* P, long ldp, const T* alpha, const T* beta, T* C, long ldc,
+                  int uplo = 0, int herm = 0);
+// Y (n x n complex) <- the Hermitian matrix whose uplo triangle is stored in A.
+template <typename T>
+void cplx_herm_expand(hipStream_t s, int uplo, int n, const T* A, long lda, T* Y, long ldy);
 
 // X <- X + D (m x n blocks).
 void add2d(hipStream_t s, int m, int n, const double* D, long ldd, double* X, long ldx);

@@ -1228,9 +1228,110 @@ int cgemm_async(const char* fn, int transA, int transB, int M, int N, int K, con
   return 0;
 }
 
+template <typename T>
+int cherk_async(const char* fn, int uplo, int trans, int N, int K, const T* alpha, const T* A, int LDA, const T* beta,
+                T* C, int LDC)
+{
+  // checks and codes of blas/zherk.c:104-137 (alpha, beta real)
+  if (uplo != Upper && uplo != Lower) return arg_error(fn, "illegal value of uplo", -1);
+  if (trans != NoTrans && trans != ConjTrans) return arg_error(fn, "illegal value of trans", -2);
+  const int Am = trans == NoTrans ? N : K, An = trans == NoTrans ? K : N;
+  if (N < 0) return arg_error(fn, "illegal value of N", -3);
+  if (K < 0) return arg_error(fn, "illegal value of K", -4);
+  if (LDA < std::max(1, Am)) return arg_error(fn, "illegal value of LDA", -7);
+  if (LDC < std::max(1, N)) return arg_error(fn, "illegal value of LDC", -10);
+  if (N == 0 || ((*alpha == T(0) || K == 0) && *beta == T(1))) return 0;
+
+  Lock l(g_mu);
+  ensure_init();
+  Runtime& r = rt();
+  const int nb = (int)xkblas_auto_tilesize(nullptr, KERN_HERK, N, N, K);
+  const int elt = 2 * sizeof(T);
+  std::vector<Replica*> keep;
+  const bool use_a = K > 0 && *alpha != T(0);
+  Replica* rA = use_a ? r.replica(A, Am, An, LDA, elt, nb, keep) : nullptr;
+  Replica* rC = r.replica(C, N, N, LDC, elt, nb, keep);
+  if (rA) r.fetch(rA, 0, 0, Am, An);
+  r.fetch(rC, 0, 0, N, N);  // the other triangle stays intact on the device
+  r.before_compute();
+  Exec x = r.exec();
+  r.prof_begin(sizeof(T) == 8 ? "herk_c128" : "herk_c64", 4.0 * K * N * (N + 1.0));
+  dev_cherk<T>(x, uplo, trans, N, use_a ? K : 0, use_a ? *alpha : T(0), rA ? (const T*)rA->dev : nullptr,
+               rA ? rA->ldd : 1, *beta, (T*)rC->dev, rC->ldd);
+  r.prof_end();
+  r.mark_written(rC, 0, 0, N, N, uplo);
+  return 0;
+}
+
+template <typename T>
+int chemm_async(const char* fn, int side, int uplo, int M, int N, const T* alpha, const T* A, int LDA, const T* B,
+                int LDB, const T* beta, T* C, int LDC)
+{
+  // checks and codes of blas/zhemm.c:113-147
+  if (side != Left && side != Right) return arg_error(fn, "illegal value of side", -1);
+  if (uplo != Lower && uplo != Upper) return arg_error(fn, "illegal value of uplo", -2);
+  const int Am = side == Left ? M : N;
+  if (M < 0) return arg_error(fn, "illegal value of M", -3);
+  if (N < 0) return arg_error(fn, "illegal value of N", -4);
+  if (LDA < std::max(1, Am)) return arg_error(fn, "illegal value of LDA", -7);
+  if (LDB < std::max(1, M)) return arg_error(fn, "illegal value of LDB", -9);
+  if (LDC < std::max(1, M)) return arg_error(fn, "illegal value of LDC", -12);
+  const bool a0 = alpha[0] == T(0) && alpha[1] == T(0), b1 = beta[0] == T(1) && beta[1] == T(0);
+  if (M == 0 || N == 0 || (a0 && b1)) return 0;
+
+  Lock l(g_mu);
+  ensure_init();
+  Runtime& r = rt();
+  const int nb = (int)xkblas_auto_tilesize(nullptr, KERN_HEMM, M, N, Am);
+  const int elt = 2 * sizeof(T);
+  std::vector<Replica*> keep;
+  Replica* rA = !a0 ? r.replica(A, Am, Am, LDA, elt, nb, keep) : nullptr;
+  Replica* rB = !a0 ? r.replica(B, M, N, LDB, elt, nb, keep) : nullptr;
+  Replica* rC = r.replica(C, M, N, LDC, elt, nb, keep);
+  if (rA) r.fetch(rA, 0, 0, Am, Am);
+  if (rB) r.fetch(rB, 0, 0, M, N);
+  if (beta[0] != T(0) || beta[1] != T(0)) r.fetch(rC, 0, 0, M, N);
+  r.before_compute();
+  Exec x = r.exec();
+  r.prof_begin(sizeof(T) == 8 ? "hemm_c128" : "hemm_c64", 8.0 * M * N * (double)Am);
+  dev_chemm<T>(x, side, uplo, M, N, alpha, rA ? (const T*)rA->dev : nullptr, rA ? rA->ldd : 1,
+               rB ? (const T*)rB->dev : nullptr, rB ? rB->ldd : 1, beta, (T*)rC->dev, rC->ldd);
+  r.prof_end();
+  r.mark_written(rC, 0, 0, M, N, 0);
+  return 0;
+}
+
 }  // namespace
 
 extern "C" {
+
+int xkblas_zherk_async(int uplo, int trans, int N, int K, const CFloat64_t* alpha, const Complex64_t* A, int LDA,
+                       const CFloat64_t* beta, Complex64_t* C, int LDC)
+{
+  return cherk_async<double>("xkblas_zherk_async", uplo, trans, N, K, alpha, (const double*)A, LDA, beta,
+                             (double*)C, LDC);
+}
+
+int xkblas_cherk_async(int uplo, int trans, int N, int K, const float* alpha, const Complex32_t* A, int LDA,
+                       const float* beta, Complex32_t* C, int LDC)
+{
+  return cherk_async<float>("xkblas_cherk_async", uplo, trans, N, K, alpha, (const float*)A, LDA, beta, (float*)C,
+                            LDC);
+}
+
+int xkblas_zhemm_async(int side, int uplo, int M, int N, const Complex64_t* alpha, const Complex64_t* A, int LDA,
+                       const Complex64_t* B, int LDB, const Complex64_t* beta, Complex64_t* C, int LDC)
+{
+  return chemm_async<double>("xkblas_zhemm_async", side, uplo, M, N, (const double*)alpha, (const double*)A, LDA,
+                             (const double*)B, LDB, (const double*)beta, (double*)C, LDC);
+}
+
+int xkblas_chemm_async(int side, int uplo, int M, int N, const Complex32_t* alpha, const Complex32_t* A, int LDA,
+                       const Complex32_t* B, int LDB, const Complex32_t* beta, Complex32_t* C, int LDC)
+{
+  return chemm_async<float>("xkblas_chemm_async", side, uplo, M, N, (const float*)alpha, (const float*)A, LDA,
+                            (const float*)B, LDB, (const float*)beta, (float*)C, LDC);
+}
 
 int xkblas_zgemm_async(int transA, int transB, int M, int N, int K, const Complex64_t* alpha, const Complex64_t* A,
                        int LDA, const Complex64_t* B, int LDB, const Complex64_t* beta, Complex64_t* C, int LDC)
