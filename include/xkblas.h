@@ -146,6 +146,34 @@ int xkblas_chemm_async(int side, int uplo, int M, int N,
                        const Complex32_t* B, int LDB,
                        const Complex32_t* beta, Complex32_t* C, int LDC);
 
+/* The other complex L3 routines (reference blas/zxkblas.h gen z / c:
+ * zsyrk.c, zsyr2k.c, zher2k.c, zgemmt.c, zsymm.c, ztrmm.c), same arguments
+ * and codes; HER2K's beta is real. */
+int xkblas_zsyrk_async(int uplo, int trans, int N, int K, const Complex64_t* alpha, const Complex64_t* A, int LDA,
+                       const Complex64_t* beta, Complex64_t* C, int LDC);   /* ref blas/zxkblas.h:97 */
+int xkblas_zsyr2k_async(int uplo, int trans, int N, int K, const Complex64_t* alpha, const Complex64_t* A, int LDA,
+                        const Complex64_t* B, int LDB, const Complex64_t* beta, Complex64_t* C, int LDC);   /* ref blas/zxkblas.h:107 */
+int xkblas_zher2k_async(int uplo, int trans, int N, int K, const Complex64_t* alpha, const Complex64_t* A, int LDA,
+                        const Complex64_t* B, int LDB, const double* beta, Complex64_t* C, int LDC);   /* ref blas/zxkblas.h:141 */
+int xkblas_zgemmt_async(int uplo, int transA, int transB, int N, int K, const Complex64_t* alpha,
+                        const Complex64_t* A, int LDA, const Complex64_t* B, int LDB, const Complex64_t* beta, Complex64_t* C, int LDC);   /* ref blas/zxkblas.h:57 */
+int xkblas_zsymm_async(int side, int uplo, int M, int N, const Complex64_t* alpha, const Complex64_t* A, int LDA,
+                       const Complex64_t* B, int LDB, const Complex64_t* beta, Complex64_t* C, int LDC);   /* ref blas/zxkblas.h:85 */
+int xkblas_ztrmm_async(int side, int uplo, int transA, int diag, int N, int NRHS, const Complex64_t* alpha,
+                       const Complex64_t* A, int LDA, Complex64_t* B, int LDB);   /* ref blas/zxkblas.h:77 */
+int xkblas_csyrk_async(int uplo, int trans, int N, int K, const Complex32_t* alpha, const Complex32_t* A, int LDA,
+                       const Complex32_t* beta, Complex32_t* C, int LDC);
+int xkblas_csyr2k_async(int uplo, int trans, int N, int K, const Complex32_t* alpha, const Complex32_t* A, int LDA,
+                        const Complex32_t* B, int LDB, const Complex32_t* beta, Complex32_t* C, int LDC);
+int xkblas_cher2k_async(int uplo, int trans, int N, int K, const Complex32_t* alpha, const Complex32_t* A, int LDA,
+                        const Complex32_t* B, int LDB, const float* beta, Complex32_t* C, int LDC);
+int xkblas_cgemmt_async(int uplo, int transA, int transB, int N, int K, const Complex32_t* alpha,
+                        const Complex32_t* A, int LDA, const Complex32_t* B, int LDB, const Complex32_t* beta, Complex32_t* C, int LDC);
+int xkblas_csymm_async(int side, int uplo, int M, int N, const Complex32_t* alpha, const Complex32_t* A, int LDA,
+                       const Complex32_t* B, int LDB, const Complex32_t* beta, Complex32_t* C, int LDC);
+int xkblas_ctrmm_async(int side, int uplo, int transA, int diag, int N, int NRHS, const Complex32_t* alpha,
+                       const Complex32_t* A, int LDA, Complex32_t* B, int LDB);
+
 /* The other L3 tile algorithms (reference blas/zgemmt.c, zsyr2k.c, zsymm.c,
  * ztrmm.c), same argument order and codes.  GEMMT writes only the uplo
  * triangle of C (all of C for uplo 0), as the host BLAS does. */

@@ -159,3 +159,127 @@ def test_cherk_chemm_fp32(xk):
     xk.invalidate()
     ref = H.astype(np.complex128) @ B.astype(np.complex128)
     assert np.abs(Y - ref).max() / np.abs(ref).max() < 2e-6
+
+
+def _tri(n, uplo):
+    return np.tril(np.ones((n, n), bool)) if uplo == 122 else np.triu(np.ones((n, n), bool))
+
+
+@pytest.mark.parametrize("uplo", [121, 122])
+@pytest.mark.parametrize("trans", [111, 113])
+def test_zher2k(xk, uplo, trans):
+    N, K = 130, 75
+    r = rng(uplo + trans + 3)
+    A = _c(r, N, K) if trans == 111 else _c(r, K, N)
+    B = _c(r, N, K) if trans == 111 else _c(r, K, N)
+    C0 = _c(r, N, N)
+    alpha = 0.5 - 1.5j
+    C = C0.copy(order="F")
+    xk.her2k(uplo, trans, alpha, A, B, 2.0, C)
+    xk.invalidate()
+    oA, oB = (A, B) if trans == 111 else (A.conj().T, B.conj().T)
+    ref = alpha * (oA @ oB.conj().T) + np.conj(alpha) * (oB @ oA.conj().T) + 2.0 * C0
+    np.fill_diagonal(ref, ref.diagonal().real)
+    m = _tri(N, uplo)
+    assert _inf(np.where(m, ref - C, 0)) / ((abs(alpha) * _inf(oA) * _inf(oB) + _inf(C0)) * N * EPS) <= 10.0
+    assert np.all(C.diagonal().imag == 0.0)
+    assert np.array_equal(C[~m], C0[~m])
+
+
+@pytest.mark.parametrize("uplo", [121, 122])
+@pytest.mark.parametrize("trans", [111, 112])
+def test_zsyrk_zsyr2k(xk, uplo, trans):
+    N, K = 120, 90
+    r = rng(uplo + 5 * trans)
+    A = _c(r, N, K) if trans == 111 else _c(r, K, N)
+    B = _c(r, N, K) if trans == 111 else _c(r, K, N)
+    C0 = _c(r, N, N)
+    oA, oB = (A, B) if trans == 111 else (A.T, B.T)
+    m = _tri(N, uplo)
+    alpha, beta = 1.0 + 0.5j, -0.25 + 1.0j
+    C = C0.copy(order="F")
+    xk.syrk(uplo, trans, alpha, A, beta, C)
+    xk.invalidate()
+    ref = alpha * (oA @ oA.T) + beta * C0
+    assert _inf(np.where(m, ref - C, 0)) / ((abs(alpha) * _inf(oA) ** 2 + abs(beta) * _inf(C0)) * N * EPS) <= 10.0
+    assert np.array_equal(C[~m], C0[~m])
+    C = C0.copy(order="F")
+    xk.syr2k(uplo, trans, alpha, A, B, beta, C)
+    xk.invalidate()
+    ref = alpha * (oA @ oB.T + oB @ oA.T) + beta * C0
+    assert _inf(np.where(m, ref - C, 0)) / ((abs(alpha) * _inf(oA) * _inf(oB) + abs(beta) * _inf(C0)) * N * EPS) <= 10.0
+    assert np.array_equal(C[~m], C0[~m])
+
+
+@pytest.mark.parametrize("uplo", [0, 121, 122])
+def test_zgemmt(xk, uplo):
+    N, K = 110, 140
+    r = rng(uplo + 11)
+    A, B, C0 = _c(r, N, K), _c(r, N, K), _c(r, N, N)
+    C = C0.copy(order="F")
+    xk.gemmt(uplo, 111, 113, 2.0j, A, B, 0.5, C)
+    xk.invalidate()
+    ref = 2.0j * (A @ B.conj().T) + 0.5 * C0
+    m = np.ones((N, N), bool) if uplo == 0 else _tri(N, uplo)
+    assert _bar(2.0j, A, B.conj().T, 0.5, C0, np.where(m, ref, 0), np.where(m, C, 0)) <= 3.0
+    assert np.array_equal(C[~m], C0[~m])
+
+
+@pytest.mark.parametrize("side", [141, 142])
+@pytest.mark.parametrize("uplo", [121, 122])
+def test_zsymm(xk, side, uplo):
+    M, N = 100, 70
+    ka = M if side == 141 else N
+    r = rng(side + 2 * uplo)
+    S = _c(r, ka, ka)
+    S = S + S.T
+    A = np.asfortranarray(np.where(_tri(ka, uplo), S, np.nan))
+    B, C0 = _c(r, M, N), _c(r, M, N)
+    C = C0.copy(order="F")
+    xk.symm(side, uplo, 1.0 - 1.0j, A, B, 0.5j, C)
+    xk.invalidate()
+    ref = (1.0 - 1.0j) * (S @ B if side == 141 else B @ S) + 0.5j * C0
+    oA, oB = (S, B) if side == 141 else (B, S)
+    assert _bar(1.0 - 1.0j, oA, oB, 0.5j, C0, ref, C) <= 3.0
+
+
+@pytest.mark.parametrize("side", [141, 142])
+@pytest.mark.parametrize("uplo", [121, 122])
+@pytest.mark.parametrize("ta", [111, 112, 113])
+@pytest.mark.parametrize("diag", [131, 132])
+def test_ztrmm(xk, side, uplo, ta, diag):
+    M, N = 90, 60
+    ka = M if side == 141 else N
+    r = rng(side + uplo + ta + diag)
+    T = np.where(_tri(ka, uplo), _c(r, ka, ka), 0)
+    A = np.asfortranarray(np.where(_tri(ka, uplo), T, np.nan))
+    if diag == 132:
+        np.fill_diagonal(T, 1.0)
+        np.fill_diagonal(A, np.nan)
+    B0 = _c(r, M, N)
+    B = B0.copy(order="F")
+    xk.trmm(side, uplo, ta, diag, 0.5 + 0.5j, A, B)
+    xk.invalidate()
+    oT = _op(T, ta)
+    ref = (0.5 + 0.5j) * (oT @ B0 if side == 141 else B0 @ oT)
+    oA, oB = (oT, B0) if side == 141 else (B0, oT)
+    assert _bar(0.5 + 0.5j, oA, oB, 0.0, B0, ref, B) <= 3.0
+
+
+def test_csyrk_ctrmm_fp32(xk):
+    r = rng(83)
+    N, K = 100, 50
+    A = _c(r, N, K, np.complex64)
+    C0 = _c(r, N, N, np.complex64)
+    C = C0.copy(order="F")
+    xk.syrk(121, 111, np.complex64(1.0j), A, np.complex64(1.0), C)
+    xk.invalidate()
+    A64 = A.astype(np.complex128)
+    ref = 1.0j * (A64 @ A64.T) + C0
+    assert np.abs(np.triu(C - ref)).max() / np.abs(ref).max() < 2e-6
+    T = np.asfortranarray(np.triu(_c(r, N, N, np.complex64)))
+    X = C0.copy(order="F")
+    xk.trmm(141, 121, 113, 131, np.complex64(1.0), T, X)
+    xk.invalidate()
+    ref = T.astype(np.complex128).conj().T @ C0.astype(np.complex128)
+    assert np.abs(X - ref).max() / np.abs(ref).max() < 2e-6

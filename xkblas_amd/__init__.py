@@ -100,6 +100,18 @@ SIGNATURES = {
     "xkblas_set_potrf_pairing": (None, [_i]),
     "xkblas_zgemm_async": (_i, [_i, _i, _i, _i, _i, _p, _p, _i, _p, _i, _p, _p, _i]),
     "xkblas_cgemm_async": (_i, [_i, _i, _i, _i, _i, _p, _p, _i, _p, _i, _p, _p, _i]),
+    "xkblas_zsyrk_async": (_i, [_i, _i, _i, _i, _p, _p, _i, _p, _p, _i]),
+    "xkblas_zsyr2k_async": (_i, [_i, _i, _i, _i, _p, _p, _i, _p, _i, _p, _p, _i]),
+    "xkblas_zher2k_async": (_i, [_i, _i, _i, _i, _p, _p, _i, _p, _i, _p, _p, _i]),
+    "xkblas_zgemmt_async": (_i, [_i, _i, _i, _i, _i, _p, _p, _i, _p, _i, _p, _p, _i]),
+    "xkblas_zsymm_async": (_i, [_i, _i, _i, _i, _p, _p, _i, _p, _i, _p, _p, _i]),
+    "xkblas_ztrmm_async": (_i, [_i, _i, _i, _i, _i, _i, _p, _p, _i, _p, _i]),
+    "xkblas_csyrk_async": (_i, [_i, _i, _i, _i, _p, _p, _i, _p, _p, _i]),
+    "xkblas_csyr2k_async": (_i, [_i, _i, _i, _i, _p, _p, _i, _p, _i, _p, _p, _i]),
+    "xkblas_cher2k_async": (_i, [_i, _i, _i, _i, _p, _p, _i, _p, _i, _p, _p, _i]),
+    "xkblas_cgemmt_async": (_i, [_i, _i, _i, _i, _i, _p, _p, _i, _p, _i, _p, _p, _i]),
+    "xkblas_csymm_async": (_i, [_i, _i, _i, _i, _p, _p, _i, _p, _i, _p, _p, _i]),
+    "xkblas_ctrmm_async": (_i, [_i, _i, _i, _i, _i, _i, _p, _p, _i, _p, _i]),
     "xkblas_zherk_async": (_i, [_i, _i, _i, _i, _p, _p, _i, _p, _p, _i]),
     "xkblas_cherk_async": (_i, [_i, _i, _i, _i, _p, _p, _i, _p, _p, _i]),
     "xkblas_zhemm_async": (_i, [_i, _i, _i, _i, _p, _p, _i, _p, _i, _p, _p, _i]),
@@ -156,6 +168,10 @@ def _check(a, dtype):
         raise ValueError("expected a Fortran-ordered %s array" % np.dtype(dtype).name)
 
 
+_PREFIX = {np.dtype(np.float64): "d", np.dtype(np.float32): "s", np.dtype(np.complex128): "z",
+           np.dtype(np.complex64): "c"}
+
+
 def _scalar(v, dtype):
     return np.array([v], dtype=dtype)
 
@@ -183,7 +199,7 @@ def syrk(uplo, trans, alpha, A, beta, C, sync=True):
         _check(X, dt)
     N = C.shape[0]
     K = A.shape[1] if trans == NoTrans else A.shape[0]
-    fn = lib().xkblas_dsyrk_async if dt == np.float64 else lib().xkblas_ssyrk_async
+    fn = getattr(lib(), "xkblas_%ssyrk_async" % _PREFIX[np.dtype(dt)])
     al, be = _scalar(alpha, dt), _scalar(beta, dt)
     rc = fn(uplo, trans, N, K, _ptr(al), _ptr(A), max(1, A.shape[0]), _ptr(be), _ptr(C), max(1, N))
     if sync:
@@ -224,7 +240,7 @@ def gemmt(uplo, ta, tb, alpha, A, B, beta, C, sync=True):
         _check(X, dt)
     N = C.shape[0]
     K = A.shape[1] if ta == NoTrans else A.shape[0]
-    fn = lib().xkblas_dgemmt_async if dt == np.float64 else lib().xkblas_sgemmt_async
+    fn = getattr(lib(), "xkblas_%sgemmt_async" % _PREFIX[np.dtype(dt)])
     al, be = _scalar(alpha, dt), _scalar(beta, dt)
     rc = fn(uplo, ta, tb, N, K, _ptr(al), _ptr(A), max(1, A.shape[0]), _ptr(B), max(1, B.shape[0]), _ptr(be), _ptr(C),
             max(1, N))
@@ -239,7 +255,7 @@ def syr2k(uplo, trans, alpha, A, B, beta, C, sync=True):
         _check(X, dt)
     N = C.shape[0]
     K = A.shape[1] if trans == NoTrans else A.shape[0]
-    fn = lib().xkblas_dsyr2k_async if dt == np.float64 else lib().xkblas_ssyr2k_async
+    fn = getattr(lib(), "xkblas_%ssyr2k_async" % _PREFIX[np.dtype(dt)])
     al, be = _scalar(alpha, dt), _scalar(beta, dt)
     rc = fn(uplo, trans, N, K, _ptr(al), _ptr(A), max(1, A.shape[0]), _ptr(B), max(1, B.shape[0]), _ptr(be), _ptr(C),
             max(1, N))
@@ -253,7 +269,7 @@ def symm(side, uplo, alpha, A, B, beta, C, sync=True):
     for X in (A, B, C):
         _check(X, dt)
     M, N = C.shape
-    fn = lib().xkblas_dsymm_async if dt == np.float64 else lib().xkblas_ssymm_async
+    fn = getattr(lib(), "xkblas_%ssymm_async" % _PREFIX[np.dtype(dt)])
     al, be = _scalar(alpha, dt), _scalar(beta, dt)
     rc = fn(side, uplo, M, N, _ptr(al), _ptr(A), max(1, A.shape[0]), _ptr(B), max(1, M), _ptr(be), _ptr(C), max(1, M))
     if sync:
@@ -266,7 +282,7 @@ def trmm(side, uplo, ta, diag, alpha, A, B, sync=True):
     for X in (A, B):
         _check(X, dt)
     M, N = B.shape
-    fn = lib().xkblas_dtrmm_async if dt == np.float64 else lib().xkblas_strmm_async
+    fn = getattr(lib(), "xkblas_%strmm_async" % _PREFIX[np.dtype(dt)])
     al = _scalar(alpha, dt)
     rc = fn(side, uplo, ta, diag, M, N, _ptr(al), _ptr(A), max(1, A.shape[0]), _ptr(B), max(1, M))
     if sync:
@@ -301,6 +317,24 @@ def hemm(side, uplo, alpha, A, B, beta, C, sync=True):
     rc = fn(side, uplo, M, N, _ptr(al), _ptr(A), max(1, A.shape[0]), _ptr(B), max(1, M), _ptr(be), _ptr(C), max(1, M))
     if sync:
         coherent(C)
+    return rc
+
+
+def her2k(uplo, trans, alpha, A, B, beta, C, sync=True):
+    """Complex C <- alpha op(A) op(B)^H + conj(alpha) op(B) op(A)^H + beta C
+    (beta real) on the uplo triangle."""
+    dt = C.dtype
+    for X in (A, B, C):
+        _check(X, dt)
+    N = C.shape[0]
+    K = A.shape[1] if trans == NoTrans else A.shape[0]
+    real = np.float64 if dt == np.complex128 else np.float32
+    fn = getattr(lib(), "xkblas_%sher2k_async" % _PREFIX[np.dtype(dt)])
+    al, be = _scalar(alpha, dt), _scalar(beta, real)
+    rc = fn(uplo, trans, N, K, _ptr(al), _ptr(A), max(1, A.shape[0]), _ptr(B), max(1, B.shape[0]), _ptr(be), _ptr(C),
+            max(1, N))
+    if sync:
+        coherent(C, uplo)
     return rc
 
 

@@ -557,16 +557,59 @@ void dev_cherk(const Exec& x, int uplo, int trans, int n, int k, T alpha, const 
 }
 
 template <typename T>
-void dev_chemm(const Exec& x, int side, int uplo, int m, int n, const T* alpha, const T* A, long lda, const T* B,
-               long ldb, const T* beta, T* C, long ldc)
+void dev_chemm(const Exec& x, int herm, int side, int uplo, int m, int n, const T* alpha, const T* A, long lda,
+               const T* B, long ldb, const T* beta, T* C, long ldc)
 {
+  const int herm_mode = herm ? CPLX_HERM : CPLX_SYM;
   // the full Hermitian operand, then the complex GEMM
   const int ka = side == Left ? m : n;
   const long ldf = (ka + 15) / 16 * 16;
   T* F = (T*)scratch(WS_X0, 2 * sizeof(T) * (size_t)ldf * ka);
-  if (!(alpha[0] == T(0) && alpha[1] == T(0))) cplx_herm_expand<T>(x.s, uplo, ka, A, lda, F, ldf);
+  if (!(alpha[0] == T(0) && alpha[1] == T(0))) cplx_expand<T>(x.s, herm_mode, uplo, ka, A, lda, F, ldf);
   if (side == Left) cgemm_impl<T>(x, NoTrans, NoTrans, m, n, m, alpha, F, ldf, B, ldb, beta, C, ldc, 0, 0);
   else cgemm_impl<T>(x, NoTrans, NoTrans, m, n, n, alpha, B, ldb, F, ldf, beta, C, ldc, 0, 0);
+}
+
+template <typename T>
+void dev_cgemmt(const Exec& x, int uplo, int transA, int transB, int n, int k, const T* alpha, const T* A, long lda,
+                const T* B, long ldb, const T* beta, T* C, long ldc)
+{
+  cgemm_impl<T>(x, transA, transB, n, n, k, alpha, A, lda, B, ldb, beta, C, ldc,
+                uplo == Lower || uplo == Upper ? uplo : 0, 0);
+}
+
+template <typename T>
+void dev_csyr2k(const Exec& x, int herm, int uplo, int trans, int n, int k, const T* alpha, const T* A, long lda,
+                const T* B, long ldb, const T* beta, T* C, long ldc)
+{
+  // symmetric: alpha (op(A) op(B)^T + op(B) op(A)^T) + beta C; Hermitian:
+  // alpha op(A) op(B)^H + conj(alpha) op(B) op(A)^H + beta C, real diagonal
+  const int t = herm ? ConjTrans : Trans;
+  const int ta = trans == NoTrans ? NoTrans : t, tb = trans == NoTrans ? t : NoTrans;
+  cgemm_impl<T>(x, ta, tb, n, n, k, alpha, A, lda, B, ldb, beta, C, ldc, uplo, herm);
+  if (k == 0 || (alpha[0] == T(0) && alpha[1] == T(0))) return;
+  const T a2[2] = {alpha[0], herm ? -alpha[1] : alpha[1]}, one[2] = {T(1), T(0)};
+  cgemm_impl<T>(x, ta, tb, n, n, k, a2, B, ldb, A, lda, one, C, ldc, uplo, herm);
+}
+
+template <typename T>
+void dev_ctrmm(const Exec& x, int side, int uplo, int transA, int diag, int m, int n, const T* alpha, const T* A,
+               long lda, T* B, long ldb)
+{
+  // B <- alpha op(A) B (Left) or alpha B op(A) (Right), A triangular
+  const T zero[2] = {T(0), T(0)};
+  if (alpha[0] == T(0) && alpha[1] == T(0)) {
+    cplx_combine<T>(x.s, m, n, nullptr, 0, zero, zero, B, ldb);
+    return;
+  }
+  const int ka = side == Left ? m : n;
+  const long ldf = (ka + 15) / 16 * 16, ldw = (m + 15) / 16 * 16;
+  T* F = (T*)scratch(WS_X0, 2 * sizeof(T) * (size_t)ldf * ka);
+  T* W = (T*)scratch(WS_X1, 2 * sizeof(T) * (size_t)ldw * n);
+  cplx_expand<T>(x.s, diag == Unit ? CPLX_UNIT : CPLX_TRI, uplo, ka, A, lda, F, ldf);
+  copy2d<T>(x.s, 2 * m, n, B, 2 * ldb, W, 2 * ldw);  // interleaved: 2 reals per element
+  if (side == Left) cgemm_impl<T>(x, transA, NoTrans, m, n, m, alpha, F, ldf, W, ldw, zero, B, ldb, 0, 0);
+  else cgemm_impl<T>(x, NoTrans, transA, m, n, n, alpha, W, ldw, F, ldf, zero, B, ldb, 0, 0);
 }
 
 // ---------------------------------------------------------------- SPD solve
@@ -675,8 +718,13 @@ int dev_posv_ir(const Exec& x, const Exec& xl, const Exec& xf, int uplo, int n, 
   template void dev_cgemm<T>(const Exec&, int, int, int, int, int, const T*, const T*, long, const T*, long,     \
                              const T*, T*, long);                                                               \
   template void dev_cherk<T>(const Exec&, int, int, int, int, T, const T*, long, T, T*, long);                   \
-  template void dev_chemm<T>(const Exec&, int, int, int, int, const T*, const T*, long, const T*, long, const T*, \
-                             T*, long);
+  template void dev_chemm<T>(const Exec&, int, int, int, int, int, const T*, const T*, long, const T*, long,     \
+                             const T*, T*, long);                                                               \
+  template void dev_cgemmt<T>(const Exec&, int, int, int, int, int, const T*, const T*, long, const T*, long,    \
+                              const T*, T*, long);                                                              \
+  template void dev_csyr2k<T>(const Exec&, int, int, int, int, int, const T*, const T*, long, const T*, long,    \
+                              const T*, T*, long);                                                              \
+  template void dev_ctrmm<T>(const Exec&, int, int, int, int, int, int, const T*, const T*, long, T*, long);
 XK_INST(double)
 XK_INST(float)
 #undef XK_INST

@@ -76,10 +76,27 @@ void dev_cherk(const Exec& x, int uplo, int trans, int n, int k, T alpha, const 
                long ldc);
 
 // Complex C <- alpha A B + beta C (Left) or alpha B A + beta C (Right), A
-// Hermitian with its uplo triangle stored.
+// Hermitian (herm) or symmetric with its uplo triangle stored.
 template <typename T>
-void dev_chemm(const Exec& x, int side, int uplo, int m, int n, const T* alpha, const T* A, long lda, const T* B,
-               long ldb, const T* beta, T* C, long ldc);
+void dev_chemm(const Exec& x, int herm, int side, int uplo, int m, int n, const T* alpha, const T* A, long lda,
+               const T* B, long ldb, const T* beta, T* C, long ldc);
+
+// Complex GEMM on the uplo triangle of the n x n C (0: all of C).
+template <typename T>
+void dev_cgemmt(const Exec& x, int uplo, int transA, int transB, int n, int k, const T* alpha, const T* A, long lda,
+                const T* B, long ldb, const T* beta, T* C, long ldc);
+
+// Complex SYR2K (herm 0: alpha (op(A) op(B)^T + op(B) op(A)^T), trans N/T) or
+// HER2K (herm 1: alpha op(A) op(B)^H + conj(alpha) op(B) op(A)^H, trans N/C,
+// real diagonal), plus beta C, on the uplo triangle.  Also SYRK with B = A.
+template <typename T>
+void dev_csyr2k(const Exec& x, int herm, int uplo, int trans, int n, int k, const T* alpha, const T* A, long lda,
+                const T* B, long ldb, const T* beta, T* C, long ldc);
+
+// Complex TRMM: B <- alpha op(A) B (Left) or alpha B op(A) (Right).
+template <typename T>
+void dev_ctrmm(const Exec& x, int side, int uplo, int transA, int diag, int m, int n, const T* alpha, const T* A,
+               long lda, T* B, long ldb);
 
 // Solution of A X = B (A SPD, uplo triangle stored; B, X n x nrhs) from a
 // factor computed under `xl` (a reduced-precision split mode) and FP64

@@ -77,21 +77,33 @@ __global__ __launch_bounds__(256) void cplx_combine_kernel(int m, int n, int j0,
   *c = Cplx<T>{re, im};
 }
 
-// Y (n x n) <- the Hermitian matrix whose uplo triangle is stored in A (the
-// imaginary part of the diagonal is taken as zero).
+// Y (n x n) <- the matrix whose uplo triangle is stored in A: mode 0
+// symmetric, 1 Hermitian (mirror conjugated, imaginary part of the diagonal
+// taken as zero), 2 triangular (zeros outside), 3 unit triangular (diagonal
+// of ones, not read).
 template <typename T>
-__global__ __launch_bounds__(256) void cplx_herm_expand_kernel(int uplo, int n, int j0, const Cplx<T>* __restrict__ A,
-                                                               long lda, Cplx<T>* __restrict__ Y, long ldy)
+__global__ __launch_bounds__(256) void cplx_expand_kernel(int mode, int uplo, int n, int j0,
+                                                          const Cplx<T>* __restrict__ A, long lda,
+                                                          Cplx<T>* __restrict__ Y, long ldy)
 {
   const int i = blockIdx.x * 256 + threadIdx.x;
   const int j = j0 + (int)blockIdx.y;
   if (i >= n || j >= n) return;
+  const bool stored = (uplo == Lower) == (i > j);
   Cplx<T> v;
-  if (i == j) v = Cplx<T>{A[i + (long)i * lda].re, T(0)};
-  else if ((uplo == Lower) == (i > j)) v = A[i + (long)j * lda];
-  else {
-    const Cplx<T> a = A[j + (long)i * lda];
-    v = Cplx<T>{a.re, -a.im};
+  if (i == j) {
+    if (mode == 3) v = Cplx<T>{T(1), T(0)};
+    else {
+      v = A[i + (long)i * lda];
+      if (mode == 1) v.im = T(0);
+    }
+  } else if (stored) {
+    v = A[i + (long)j * lda];
+  } else if (mode >= 2) {
+    v = Cplx<T>{T(0), T(0)};
+  } else {
+    v = A[j + (long)i * lda];
+    if (mode == 1) v.im = -v.im;
   }
   Y[i + (long)j * ldy] = v;
 }
@@ -137,11 +149,11 @@ void cplx_combine(hipStream_t s, int m, int n, const T* P, long ldp, const T* al
 }
 
 template <typename T>
-void cplx_herm_expand(hipStream_t s, int uplo, int n, const T* A, long lda, T* Y, long ldy)
+void cplx_expand(hipStream_t s, int mode, int uplo, int n, const T* A, long lda, T* Y, long ldy)
 {
   if (n <= 0) return;
   over_columns(n, n, [&](dim3 g, int j0) {
-    hipLaunchKernelGGL((cplx_herm_expand_kernel<T>), g, dim3(256), 0, s, uplo, n, j0, (const Cplx<T>*)A, lda,
+    hipLaunchKernelGGL((cplx_expand_kernel<T>), g, dim3(256), 0, s, mode, uplo, n, j0, (const Cplx<T>*)A, lda,
                        (Cplx<T>*)Y, ldy);
   });
   XK_HIP(hipGetLastError());
@@ -151,7 +163,7 @@ void cplx_herm_expand(hipStream_t s, int uplo, int n, const T* A, long lda, T* Y
   template void cplx_pack_a<T>(hipStream_t, int, int, int, const T*, long, T*, long);                      \
   template void cplx_pack_b<T>(hipStream_t, int, int, int, int, const T*, long, T*, long);                 \
   template void cplx_combine<T>(hipStream_t, int, int, const T*, long, const T*, const T*, T*, long, int, int);  \
-  template void cplx_herm_expand<T>(hipStream_t, int, int, const T*, long, T*, long);
+  template void cplx_expand<T>(hipStream_t, int, int, int, const T*, long, T*, long);
 XK_INST(double)
 XK_INST(float)
 #undef XK_INST

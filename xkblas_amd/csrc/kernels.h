@@ -78,9 +78,12 @@ void cplx_pack_b(hipStream_t s, int vertical, int sign, int rows, int cols, cons
 template <typename T>
 void cplx_combine(hipStream_t s, int m, int n, const T* P, long ldp, const T* alpha, const T* beta, T* C, long ldc,
                   int uplo = 0, int herm = 0);
-// Y (n x n complex) <- the Hermitian matrix whose uplo triangle is stored in A.
+// Y (n x n complex) <- the matrix whose uplo triangle is stored in A: mode
+// CPLX_SYM symmetric, CPLX_HERM Hermitian, CPLX_TRI triangular, CPLX_UNIT unit
+// triangular.
+enum : int { CPLX_SYM = 0, CPLX_HERM = 1, CPLX_TRI = 2, CPLX_UNIT = 3 };
 template <typename T>
-void cplx_herm_expand(hipStream_t s, int uplo, int n, const T* A, long lda, T* Y, long ldy);
+void cplx_expand(hipStream_t s, int mode, int uplo, int n, const T* A, long lda, T* Y, long ldy);
 
 // X <- X + D (m x n blocks).
 void add2d(hipStream_t s, int m, int n, const double* D, long ldd, double* X, long ldx);

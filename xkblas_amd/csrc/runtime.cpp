@@ -1264,10 +1264,10 @@ int cherk_async(const char* fn, int uplo, int trans, int N, int K, const T* alph
 }
 
 template <typename T>
-int chemm_async(const char* fn, int side, int uplo, int M, int N, const T* alpha, const T* A, int LDA, const T* B,
-                int LDB, const T* beta, T* C, int LDC)
+int chemm_async(const char* fn, int herm, int side, int uplo, int M, int N, const T* alpha, const T* A, int LDA,
+                const T* B, int LDB, const T* beta, T* C, int LDC)
 {
-  // checks and codes of blas/zhemm.c:113-147
+  // checks and codes of blas/zhemm.c:113-147 (zsymm.c:115-148 the same)
   if (side != Left && side != Right) return arg_error(fn, "illegal value of side", -1);
   if (uplo != Lower && uplo != Upper) return arg_error(fn, "illegal value of uplo", -2);
   const int Am = side == Left ? M : N;
@@ -1293,17 +1293,190 @@ int chemm_async(const char* fn, int side, int uplo, int M, int N, const T* alpha
   if (beta[0] != T(0) || beta[1] != T(0)) r.fetch(rC, 0, 0, M, N);
   r.before_compute();
   Exec x = r.exec();
-  r.prof_begin(sizeof(T) == 8 ? "hemm_c128" : "hemm_c64", 8.0 * M * N * (double)Am);
-  dev_chemm<T>(x, side, uplo, M, N, alpha, rA ? (const T*)rA->dev : nullptr, rA ? rA->ldd : 1,
+  r.prof_begin(herm ? (sizeof(T) == 8 ? "hemm_c128" : "hemm_c64") : (sizeof(T) == 8 ? "symm_c128" : "symm_c64"),
+               8.0 * M * N * (double)Am);
+  dev_chemm<T>(x, herm, side, uplo, M, N, alpha, rA ? (const T*)rA->dev : nullptr, rA ? rA->ldd : 1,
                rB ? (const T*)rB->dev : nullptr, rB ? rB->ldd : 1, beta, (T*)rC->dev, rC->ldd);
   r.prof_end();
   r.mark_written(rC, 0, 0, M, N, 0);
   return 0;
 }
 
+// complex SYRK / SYR2K / HER2K / GEMMT / TRMM: the real templates' checks
+// with complex scalars (HER2K's beta real, passed here as (beta, 0))
+template <typename T>
+int csyr2k_async(const char* fn, int herm, int two, int uplo, int trans, int N, int K, const T* alpha, const T* A,
+                 int LDA, const T* B, int LDB, const T* beta, T* C, int LDC)
+{
+  // blas/zsyrk.c:104-133, zsyr2k.c:119-156, zher2k.c:115-150
+  const int t = herm ? ConjTrans : Trans;
+  if (uplo != Upper && uplo != Lower) return arg_error(fn, "illegal value of uplo", -1);
+  if (trans != NoTrans && trans != t) return arg_error(fn, "illegal value of trans", -2);
+  const int Am = trans == NoTrans ? N : K, An = trans == NoTrans ? K : N;
+  if (N < 0) return arg_error(fn, "illegal value of N", -3);
+  if (K < 0) return arg_error(fn, "illegal value of K", -4);
+  if (LDA < std::max(1, Am)) return arg_error(fn, "illegal value of LDA", -7);
+  if (two && LDB < std::max(1, Am)) return arg_error(fn, "illegal value of LDB", -9);
+  if (LDC < std::max(1, N)) return arg_error(fn, "illegal value of LDC", two ? -12 : -10);
+  const bool a0 = alpha[0] == T(0) && alpha[1] == T(0), b1 = beta[0] == T(1) && beta[1] == T(0);
+  if (N == 0 || ((a0 || K == 0) && b1)) return 0;
+
+  Lock l(g_mu);
+  ensure_init();
+  Runtime& r = rt();
+  const int nb = (int)xkblas_auto_tilesize(nullptr, two ? KERN_SYR2K : KERN_SYRK, N, N, K);
+  const int elt = 2 * sizeof(T);
+  std::vector<Replica*> keep;
+  const bool use_ab = K > 0 && !a0;
+  Replica* rA = use_ab ? r.replica(A, Am, An, LDA, elt, nb, keep) : nullptr;
+  Replica* rB = use_ab && two ? r.replica(B, Am, An, LDB, elt, nb, keep) : nullptr;
+  Replica* rC = r.replica(C, N, N, LDC, elt, nb, keep);
+  if (rA) r.fetch(rA, 0, 0, Am, An);
+  if (rB) r.fetch(rB, 0, 0, Am, An);
+  r.fetch(rC, 0, 0, N, N);
+  r.before_compute();
+  Exec x = r.exec();
+  r.prof_begin(two ? (herm ? "her2k_c" : "syr2k_c") : "syrk_c", (two ? 8.0 : 4.0) * K * N * (N + 1.0));
+  const T zero[2] = {T(0), T(0)};
+  const T* dA = rA ? (const T*)rA->dev : nullptr;
+  const T* dB = two ? (rB ? (const T*)rB->dev : nullptr) : dA;
+  const long ldb = two ? (rB ? rB->ldd : 1) : (rA ? rA->ldd : 1);
+  if (two) {
+    dev_csyr2k<T>(x, herm, uplo, trans, N, use_ab ? K : 0, use_ab ? alpha : zero, dA, rA ? rA->ldd : 1, dB, ldb, beta,
+                  (T*)rC->dev, rC->ldd);
+  } else {
+    // SYRK: one product op(A) op(A)^T
+    const int ta = trans == NoTrans ? NoTrans : Trans, tb = trans == NoTrans ? Trans : NoTrans;
+    dev_cgemmt<T>(x, uplo, ta, tb, N, use_ab ? K : 0, use_ab ? alpha : zero, dA, rA ? rA->ldd : 1, dB, ldb, beta,
+                  (T*)rC->dev, rC->ldd);
+  }
+  r.prof_end();
+  r.mark_written(rC, 0, 0, N, N, uplo);
+  return 0;
+}
+
+template <typename T>
+int cgemmt_async(const char* fn, int uplo, int transA, int transB, int N, int K, const T* alpha, const T* A, int LDA,
+                 const T* B, int LDB, const T* beta, T* C, int LDC)
+{
+  // blas/zgemmt.c:123-165
+  if (uplo != 0 && uplo != Upper && uplo != Lower) return arg_error(fn, "illegal value of uplo", -1);
+  if (transA < NoTrans || transA > ConjTrans) return arg_error(fn, "illegal value of transA", -1);
+  if (transB < NoTrans || transB > ConjTrans) return arg_error(fn, "illegal value of transB", -2);
+  const int Am = transA == NoTrans ? N : K, An = transA == NoTrans ? K : N;
+  const int Bm = transB == NoTrans ? K : N, Bn = transB == NoTrans ? N : K;
+  if (N < 0) return arg_error(fn, "illegal value of N", -4);
+  if (K < 0) return arg_error(fn, "illegal value of K", -5);
+  if (LDA < std::max(1, Am)) return arg_error(fn, "illegal value of LDA", -8);
+  if (LDB < std::max(1, Bm)) return arg_error(fn, "illegal value of LDB", -10);
+  if (LDC < std::max(1, N)) return arg_error(fn, "illegal value of LDC", -13);
+  const bool a0 = alpha[0] == T(0) && alpha[1] == T(0), b1 = beta[0] == T(1) && beta[1] == T(0);
+  if (N == 0 || ((a0 || K == 0) && b1)) return 0;
+
+  Lock l(g_mu);
+  ensure_init();
+  Runtime& r = rt();
+  const int nb = (int)xkblas_auto_tilesize(nullptr, KERN_GEMMT, N, N, K);
+  const int elt = 2 * sizeof(T);
+  std::vector<Replica*> keep;
+  const bool use_ab = K > 0 && !a0;
+  Replica* rA = use_ab ? r.replica(A, Am, An, LDA, elt, nb, keep) : nullptr;
+  Replica* rB = use_ab ? r.replica(B, Bm, Bn, LDB, elt, nb, keep) : nullptr;
+  Replica* rC = r.replica(C, N, N, LDC, elt, nb, keep);
+  if (rA) r.fetch(rA, 0, 0, Am, An);
+  if (rB) r.fetch(rB, 0, 0, Bm, Bn);
+  r.fetch(rC, 0, 0, N, N);
+  r.before_compute();
+  Exec x = r.exec();
+  r.prof_begin("gemmt_c", 8.0 * N * N * (double)K);
+  const T zero[2] = {T(0), T(0)};
+  dev_cgemmt<T>(x, uplo, transA, transB, N, use_ab ? K : 0, use_ab ? alpha : zero, rA ? (const T*)rA->dev : nullptr,
+                rA ? rA->ldd : 1, rB ? (const T*)rB->dev : nullptr, rB ? rB->ldd : 1, beta, (T*)rC->dev, rC->ldd);
+  r.prof_end();
+  r.mark_written(rC, 0, 0, N, N, uplo == Upper || uplo == Lower ? uplo : 0);
+  return 0;
+}
+
+template <typename T>
+int ctrmm_async(const char* fn, int side, int uplo, int transA, int diag, int N, int NRHS, const T* alpha, const T* A,
+                int LDA, T* B, int LDB)
+{
+  // blas/ztrmm.c:116-153
+  const int NA = side == Left ? N : NRHS;
+  if (side != Left && side != Right) return arg_error(fn, "illegal value of side", -1);
+  if (uplo != Upper && uplo != Lower) return arg_error(fn, "illegal value of uplo", -2);
+  if (transA < NoTrans || transA > ConjTrans) return arg_error(fn, "illegal value of transA", -3);
+  if (diag != Unit && diag != NonUnit) return arg_error(fn, "illegal value of diag", -4);
+  if (N < 0) return arg_error(fn, "illegal value of N", -5);
+  if (NRHS < 0) return arg_error(fn, "illegal value of NRHS", -6);
+  if (LDA < std::max(1, NA)) return arg_error(fn, "illegal value of LDA", -8);
+  if (LDB < std::max(1, N)) return arg_error(fn, "illegal value of LDB", -10);
+  if (std::min(N, NRHS) == 0) return 0;
+
+  Lock l(g_mu);
+  ensure_init();
+  Runtime& r = rt();
+  const int nb = (int)xkblas_auto_tilesize(nullptr, KERN_TRMM, N, NRHS, NA);
+  const int elt = 2 * sizeof(T);
+  std::vector<Replica*> keep;
+  Replica* rA = r.replica(A, NA, NA, LDA, elt, nb, keep);
+  Replica* rB = r.replica(B, N, NRHS, LDB, elt, nb, keep);
+  r.fetch(rA, 0, 0, NA, NA);
+  r.fetch(rB, 0, 0, N, NRHS);
+  r.before_compute();
+  Exec x = r.exec();
+  r.prof_begin("trmm_c", 4.0 * (side == Left ? (double)N * N * NRHS : (double)N * NRHS * NRHS));
+  dev_ctrmm<T>(x, side, uplo, transA, diag, N, NRHS, alpha, (const T*)rA->dev, rA->ldd, (T*)rB->dev, rB->ldd);
+  r.prof_end();
+  r.mark_written(rB, 0, 0, N, NRHS, 0);
+  return 0;
+}
+
 }  // namespace
 
 extern "C" {
+
+#define XK_CPLX_OPS(P, C, R)                                                                                      \
+  int xkblas_##P##syrk_async(int uplo, int trans, int N, int K, const C* alpha, const C* A, int LDA, const C* beta, \
+                             C* Cm, int LDC)                                                                      \
+  {                                                                                                               \
+    return csyr2k_async<R>("xkblas_" #P "syrk_async", 0, 0, uplo, trans, N, K, (const R*)alpha, (const R*)A, LDA, \
+                           nullptr, 1, (const R*)beta, (R*)Cm, LDC);                                              \
+  }                                                                                                               \
+  int xkblas_##P##syr2k_async(int uplo, int trans, int N, int K, const C* alpha, const C* A, int LDA, const C* B, \
+                              int LDB, const C* beta, C* Cm, int LDC)                                             \
+  {                                                                                                               \
+    return csyr2k_async<R>("xkblas_" #P "syr2k_async", 0, 1, uplo, trans, N, K, (const R*)alpha, (const R*)A, LDA,\
+                           (const R*)B, LDB, (const R*)beta, (R*)Cm, LDC);                                        \
+  }                                                                                                               \
+  int xkblas_##P##her2k_async(int uplo, int trans, int N, int K, const C* alpha, const C* A, int LDA, const C* B, \
+                              int LDB, const R* beta, C* Cm, int LDC)                                             \
+  {                                                                                                               \
+    const R b2[2] = {*beta, R(0)};                                                                                \
+    return csyr2k_async<R>("xkblas_" #P "her2k_async", 1, 1, uplo, trans, N, K, (const R*)alpha, (const R*)A, LDA,\
+                           (const R*)B, LDB, b2, (R*)Cm, LDC);                                                    \
+  }                                                                                                               \
+  int xkblas_##P##gemmt_async(int uplo, int transA, int transB, int N, int K, const C* alpha, const C* A, int LDA, \
+                              const C* B, int LDB, const C* beta, C* Cm, int LDC)                                 \
+  {                                                                                                               \
+    return cgemmt_async<R>("xkblas_" #P "gemmt_async", uplo, transA, transB, N, K, (const R*)alpha, (const R*)A,   \
+                           LDA, (const R*)B, LDB, (const R*)beta, (R*)Cm, LDC);                                   \
+  }                                                                                                               \
+  int xkblas_##P##symm_async(int side, int uplo, int M, int N, const C* alpha, const C* A, int LDA, const C* B,    \
+                             int LDB, const C* beta, C* Cm, int LDC)                                              \
+  {                                                                                                               \
+    return chemm_async<R>("xkblas_" #P "symm_async", 0, side, uplo, M, N, (const R*)alpha, (const R*)A, LDA,      \
+                          (const R*)B, LDB, (const R*)beta, (R*)Cm, LDC);                                         \
+  }                                                                                                               \
+  int xkblas_##P##trmm_async(int side, int uplo, int transA, int diag, int N, int NRHS, const C* alpha,           \
+                             const C* A, int LDA, C* B, int LDB)                                                  \
+  {                                                                                                               \
+    return ctrmm_async<R>("xkblas_" #P "trmm_async", side, uplo, transA, diag, N, NRHS, (const R*)alpha,          \
+                          (const R*)A, LDA, (R*)B, LDB);                                                          \
+  }
+XK_CPLX_OPS(z, Complex64_t, double)
+XK_CPLX_OPS(c, Complex32_t, float)
+#undef XK_CPLX_OPS
 
 int xkblas_zherk_async(int uplo, int trans, int N, int K, const CFloat64_t* alpha, const Complex64_t* A, int LDA,
                        const CFloat64_t* beta, Complex64_t* C, int LDC)
@@ -1322,14 +1495,14 @@ int xkblas_cherk_async(int uplo, int trans, int N, int K, const float* alpha, co
 int xkblas_zhemm_async(int side, int uplo, int M, int N, const Complex64_t* alpha, const Complex64_t* A, int LDA,
                        const Complex64_t* B, int LDB, const Complex64_t* beta, Complex64_t* C, int LDC)
 {
-  return chemm_async<double>("xkblas_zhemm_async", side, uplo, M, N, (const double*)alpha, (const double*)A, LDA,
+  return chemm_async<double>("xkblas_zhemm_async", 1, side, uplo, M, N, (const double*)alpha, (const double*)A, LDA,
                              (const double*)B, LDB, (const double*)beta, (double*)C, LDC);
 }
 
 int xkblas_chemm_async(int side, int uplo, int M, int N, const Complex32_t* alpha, const Complex32_t* A, int LDA,
                        const Complex32_t* B, int LDB, const Complex32_t* beta, Complex32_t* C, int LDC)
 {
-  return chemm_async<float>("xkblas_chemm_async", side, uplo, M, N, (const float*)alpha, (const float*)A, LDA,
+  return chemm_async<float>("xkblas_chemm_async", 1, side, uplo, M, N, (const float*)alpha, (const float*)A, LDA,
                             (const float*)B, LDB, (const float*)beta, (float*)C, LDC);
 }
 
