@@ -149,6 +149,10 @@ int xkblas_chemm_async(int side, int uplo, int M, int N,
 /* The other complex L3 routines (reference blas/zxkblas.h gen z / c:
  * zsyrk.c, zsyr2k.c, zher2k.c, zgemmt.c, zsymm.c, ztrmm.c), same arguments
  * and codes; HER2K's beta is real. */
+int xkblas_ztrsm_async(int side, int uplo, int transA, int diag, int N, int NRHS, const Complex64_t* alpha,
+                       const Complex64_t* A, int LDA, Complex64_t* B, int LDB);   /* ref blas/zxkblas.h:69 */
+int xkblas_ctrsm_async(int side, int uplo, int transA, int diag, int N, int NRHS, const Complex32_t* alpha,
+                       const Complex32_t* A, int LDA, Complex32_t* B, int LDB);
 int xkblas_zsyrk_async(int uplo, int trans, int N, int K, const Complex64_t* alpha, const Complex64_t* A, int LDA,
                        const Complex64_t* beta, Complex64_t* C, int LDC);   /* ref blas/zxkblas.h:97 */
 int xkblas_zsyr2k_async(int uplo, int trans, int N, int K, const Complex64_t* alpha, const Complex64_t* A, int LDA,

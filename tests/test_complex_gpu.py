@@ -283,3 +283,46 @@ def test_csyrk_ctrmm_fp32(xk):
     xk.invalidate()
     ref = T.astype(np.complex128).conj().T @ C0.astype(np.complex128)
     assert np.abs(X - ref).max() / np.abs(ref).max() < 2e-6
+
+
+@pytest.mark.parametrize("side", [141, 142])
+@pytest.mark.parametrize("uplo", [121, 122])
+@pytest.mark.parametrize("ta", [111, 112, 113])
+@pytest.mark.parametrize("diag", [131, 132])
+def test_ztrsm(xk, side, uplo, ta, diag):
+    """The reference TRSM test matrix made complex (testing/ztesting_trsm.c:
+    129-134: entries / n, unit diagonal); bar <= 10 as there."""
+    M, N = 300, 170
+    na = M if side == 141 else N
+    r = rng(side * 7 + uplo + ta + diag)
+    T = (r.uniform(0.0, 1.0, (na, na)) + 1j * r.uniform(0.0, 1.0, (na, na))) / na
+    np.fill_diagonal(T, 1.0 + 0.5j)
+    T = np.where(_tri(na, uplo), T, 0)
+    A = np.asfortranarray(np.where(_tri(na, uplo), T, np.nan))
+    Tm = T.copy()
+    if diag == 132:
+        np.fill_diagonal(Tm, 1.0)
+    B0 = _c(r, M, N)
+    X = B0.copy(order="F")
+    alpha = 0.8 - 0.2j
+    xk.lib().xkblas_set_param(128, 0)
+    xk.trsm(side, uplo, ta, diag, alpha, A, X)
+    xk.invalidate()
+    oT = _op(Tm, ta)
+    Xref = np.linalg.solve(oT, alpha * B0) if side == 141 else np.linalg.solve(oT.T, (alpha * B0).T).T
+    err = _inf(X - Xref) / ((_inf(oT) + _inf(Xref)) * max(M, N) * EPS)
+    assert err <= 10.0, err
+
+
+def test_ctrsm_fp32(xk):
+    r = rng(89)
+    n, nrhs = 200, 40
+    T = np.tril((r.uniform(0, 1, (n, n)) + 1j * r.uniform(0, 1, (n, n))) / n)
+    np.fill_diagonal(T, 1.0)
+    T = np.asfortranarray(T.astype(np.complex64))
+    B0 = _c(r, n, nrhs, np.complex64)
+    X = B0.copy(order="F")
+    xk.trsm(141, 122, 113, 131, np.complex64(1.0), T, X)
+    xk.invalidate()
+    ref = np.linalg.solve(T.astype(np.complex128).conj().T, B0.astype(np.complex128))
+    assert np.abs(X - ref).max() / np.abs(ref).max() < 1e-5

@@ -112,6 +112,8 @@ SIGNATURES = {
     "xkblas_cgemmt_async": (_i, [_i, _i, _i, _i, _i, _p, _p, _i, _p, _i, _p, _p, _i]),
     "xkblas_csymm_async": (_i, [_i, _i, _i, _i, _p, _p, _i, _p, _i, _p, _p, _i]),
     "xkblas_ctrmm_async": (_i, [_i, _i, _i, _i, _i, _i, _p, _p, _i, _p, _i]),
+    "xkblas_ztrsm_async": (_i, [_i, _i, _i, _i, _i, _i, _p, _p, _i, _p, _i]),
+    "xkblas_ctrsm_async": (_i, [_i, _i, _i, _i, _i, _i, _p, _p, _i, _p, _i]),
     "xkblas_zherk_async": (_i, [_i, _i, _i, _i, _p, _p, _i, _p, _p, _i]),
     "xkblas_cherk_async": (_i, [_i, _i, _i, _i, _p, _p, _i, _p, _p, _i]),
     "xkblas_zhemm_async": (_i, [_i, _i, _i, _i, _p, _p, _i, _p, _i, _p, _p, _i]),
@@ -212,7 +214,7 @@ def trsm(side, uplo, ta, diag, alpha, A, B, sync=True):
     for X in (A, B):
         _check(X, dt)
     M, N = B.shape
-    fn = lib().xkblas_dtrsm_async if dt == np.float64 else lib().xkblas_strsm_async
+    fn = getattr(lib(), "xkblas_%strsm_async" % _PREFIX[np.dtype(dt)])
     al = _scalar(alpha, dt)
     rc = fn(side, uplo, ta, diag, M, N, _ptr(al), _ptr(A), max(1, A.shape[0]), _ptr(B), max(1, M))
     if sync:

@@ -612,6 +612,104 @@ void dev_ctrmm(const Exec& x, int side, int uplo, int transA, int diag, int m, i
   else cgemm_impl<T>(x, NoTrans, transA, m, n, n, alpha, W, ldw, F, ldf, zero, B, ldb, 0, 0);
 }
 
+// Complex triangular inverse (Tinv n x n, other triangle zeroed): the
+// recursion of dev_trtri with complex products; leaves of 64.
+enum { WS_ZW = 23 };
+
+template <typename T>
+static void dev_ctrtri(const Exec& x, int uplo, int unit, int n, const T* A, long lda, T* Tinv, long ldinv)
+{
+  if (n <= 64) {
+    cplx_trtri_leaf<T>(x.s, uplo, unit, n, A, lda, 0, Tinv, ldinv, 0, 1);
+    return;
+  }
+  const int n1 = half64(n), n2 = n - n1;
+  auto at = [](auto* base, long off) { return base + 2 * off; };  // complex element offsets
+  const T* A22 = at(A, n1 + (long)n1 * lda);
+  T* I22 = at(Tinv, n1 + (long)n1 * ldinv);
+  dev_ctrtri<T>(x, uplo, unit, n1, A, lda, Tinv, ldinv);
+  dev_ctrtri<T>(x, uplo, unit, n2, A22, lda, I22, ldinv);
+  T* W = (T*)scratch(WS_ZW, 2 * sizeof(T) * (size_t)n1 * n2);
+  const T one[2] = {T(1), T(0)}, mone[2] = {T(-1), T(0)}, zero[2] = {T(0), T(0)};
+  if (uplo == Lower) {
+    // I21 = -I22 A21 I11, upper block zeroed
+    cgemm_impl<T>(x, NoTrans, NoTrans, n2, n1, n1, one, at(A, n1), lda, Tinv, ldinv, zero, W, n2, 0, 0);
+    cgemm_impl<T>(x, NoTrans, NoTrans, n2, n1, n2, mone, I22, ldinv, W, n2, zero, at(Tinv, n1), ldinv, 0, 0);
+    cplx_combine<T>(x.s, n1, n2, nullptr, 0, zero, zero, at(Tinv, (long)n1 * ldinv), ldinv);
+  } else {
+    // I12 = -I11 A12 I22, lower block zeroed
+    cgemm_impl<T>(x, NoTrans, NoTrans, n1, n2, n2, one, at(A, (long)n1 * lda), lda, I22, ldinv, zero, W, n1, 0, 0);
+    cgemm_impl<T>(x, NoTrans, NoTrans, n1, n2, n1, mone, Tinv, ldinv, W, n1, zero, at(Tinv, (long)n1 * ldinv), ldinv, 0,
+                  0);
+    cplx_combine<T>(x.s, n2, n1, nullptr, 0, zero, zero, at(Tinv, n1), ldinv);
+  }
+}
+
+// Complex TRSM: the block sweep of dev_trsm (inverse products for the
+// diagonal blocks, updates by -1 with alpha folded into the first step) with
+// complex products.
+template <typename T>
+void dev_ctrsm(const Exec& x, int side, int uplo, int transA, int diag, int m, int n, const T* alpha, const T* A,
+               long lda, T* B, long ldb, int nb)
+{
+  if (m <= 0 || n <= 0) return;
+  const T zero[2] = {T(0), T(0)}, one[2] = {T(1), T(0)}, mone[2] = {T(-1), T(0)};
+  if (alpha[0] == T(0) && alpha[1] == T(0)) {
+    cplx_combine<T>(x.s, m, n, nullptr, 0, zero, zero, B, ldb);
+    return;
+  }
+  auto at = [](auto* base, long off) { return base + 2 * off; };
+  const bool tr = transA != NoTrans;
+  const bool lower_eff = (uplo == Lower) != tr;
+  const int na = side == Left ? m : n;
+  nb = trsm_block(nb, na);
+  const int nblk = (na + nb - 1) / nb;
+  auto bsz = [&](int i) { return i == nblk - 1 ? na - i * nb : nb; };
+  T* inv = (T*)scratch(WS_TRSM, 2 * sizeof(T) * (size_t)nblk * nb * nb);
+  for (int d = 0; d < nblk; ++d)
+    dev_ctrtri<T>(x, uplo, diag == Unit, bsz(d), at(A, (long)d * nb * (1 + lda)), lda, at(inv, (long)d * nb * nb), nb);
+  auto opblk = [&](int i, int j) -> const T* {
+    return tr ? at(A, (long)j * nb + (long)i * nb * lda) : at(A, (long)i * nb + (long)j * nb * lda);
+  };
+  if (side == Left) {
+    T* W = (T*)scratch(WS_TRSMW, 2 * sizeof(T) * (size_t)nb * n);
+    for (int s = 0; s < nblk; ++s) {
+      const int k = lower_eff ? s : nblk - 1 - s;
+      const int kk = bsz(k);
+      const T* la = s == 0 ? alpha : one;
+      T* Bk = at(B, (long)k * nb);
+      copy2d<T>(x.s, 2 * kk, n, Bk, 2 * ldb, W, 2L * kk);
+      cgemm_impl<T>(x, transA, NoTrans, kk, n, kk, la, at(inv, (long)k * nb * nb), nb, W, kk, zero, Bk, ldb, 0, 0);
+      if (s == nblk - 1) break;
+      if (lower_eff) {
+        const int r0 = (k + 1) * nb;
+        cgemm_impl<T>(x, transA, NoTrans, m - r0, n, kk, mone, opblk(k + 1, k), lda, Bk, ldb, la, at(B, r0), ldb, 0, 0);
+      } else {
+        cgemm_impl<T>(x, transA, NoTrans, k * nb, n, kk, mone, opblk(0, k), lda, Bk, ldb, la, B, ldb, 0, 0);
+      }
+    }
+  } else {
+    T* W = (T*)scratch(WS_TRSMW, 2 * sizeof(T) * (size_t)m * nb);
+    const bool forward = !lower_eff;
+    for (int s = 0; s < nblk; ++s) {
+      const int k = forward ? s : nblk - 1 - s;
+      const int kk = bsz(k);
+      const T* la = s == 0 ? alpha : one;
+      T* Bk = at(B, (long)k * nb * ldb);
+      copy2d<T>(x.s, 2 * m, kk, Bk, 2 * ldb, W, 2L * m);
+      cgemm_impl<T>(x, NoTrans, transA, m, kk, kk, la, W, m, at(inv, (long)k * nb * nb), nb, zero, Bk, ldb, 0, 0);
+      if (s == nblk - 1) break;
+      if (forward) {
+        const int c0 = (k + 1) * nb;
+        cgemm_impl<T>(x, NoTrans, transA, m, n - c0, kk, mone, Bk, ldb, opblk(k, k + 1), lda, la,
+                      at(B, (long)c0 * ldb), ldb, 0, 0);
+      } else {
+        cgemm_impl<T>(x, NoTrans, transA, m, k * nb, kk, mone, Bk, ldb, opblk(k, 0), lda, la, B, ldb, 0, 0);
+      }
+    }
+  }
+}
+
 // ---------------------------------------------------------------- SPD solve
 // LAPACK dsposv's mixed-precision scheme (factor in low precision, residuals
 // and updates in FP64, stop when every column satisfies
@@ -724,7 +822,8 @@ int dev_posv_ir(const Exec& x, const Exec& xl, const Exec& xf, int uplo, int n, 
                               const T*, T*, long);                                                              \
   template void dev_csyr2k<T>(const Exec&, int, int, int, int, int, const T*, const T*, long, const T*, long,    \
                               const T*, T*, long);                                                              \
-  template void dev_ctrmm<T>(const Exec&, int, int, int, int, int, int, const T*, const T*, long, T*, long);
+  template void dev_ctrmm<T>(const Exec&, int, int, int, int, int, int, const T*, const T*, long, T*, long);     \
+  template void dev_ctrsm<T>(const Exec&, int, int, int, int, int, int, const T*, const T*, long, T*, long, int);
 XK_INST(double)
 XK_INST(float)
 #undef XK_INST

@@ -93,6 +93,12 @@ template <typename T>
 void dev_csyr2k(const Exec& x, int herm, int uplo, int trans, int n, int k, const T* alpha, const T* A, long lda,
                 const T* B, long ldb, const T* beta, T* C, long ldc);
 
+// Complex TRSM: op(A) X = alpha B (Left) or X op(A) = alpha B (Right), B m x
+// n overwritten by X; nb = diagonal block order.
+template <typename T>
+void dev_ctrsm(const Exec& x, int side, int uplo, int transA, int diag, int m, int n, const T* alpha, const T* A,
+               long lda, T* B, long ldb, int nb);
+
 // Complex TRMM: B <- alpha op(A) B (Left) or alpha B op(A) (Right).
 template <typename T>
 void dev_ctrmm(const Exec& x, int side, int uplo, int transA, int diag, int m, int n, const T* alpha, const T* A,

@@ -108,6 +108,61 @@ __global__ __launch_bounds__(256) void cplx_expand_kernel(int mode, int uplo, in
   Y[i + (long)j * ldy] = v;
 }
 
+// Inverse of `batch` complex triangular blocks of order n <= 64 (block b at
+// A + b sA, inverse at Tinv + b sInv, complex elements).  One wave per block,
+// lane j solving column j of inv(L) by forward substitution in the lower view
+// (Upper: L = U^T, inv(U) = inv(L)^T); L is read through the cache (all lanes
+// read the same entry), the columns being built live in LDS.
+template <typename T>
+__global__ __launch_bounds__(64) void cplx_trtri_leaf_kernel(int uplo, int unit, int n, const Cplx<T>* __restrict__ A,
+                                                             long lda, long sA, Cplx<T>* __restrict__ Tinv, long ldinv,
+                                                             long sInv)
+{
+  __shared__ T Xr[64 * 65], Xi[64 * 65];
+  const int j = threadIdx.x;
+  A += (long)blockIdx.x * sA;
+  Tinv += (long)blockIdx.x * sInv;
+  auto L = [&](int i, int k) { return uplo == Lower ? A[i + (long)k * lda] : A[k + (long)i * lda]; };
+  for (int i = 0; i < n; ++i) {
+    // s = sum_{k < i} L(i,k) x_k; x_k = 0 for k < j, so all lanes run the same loop
+    T sr = T(0), si = T(0);
+    for (int k = 0; k < i; ++k) {
+      const Cplx<T> l = L(i, k);
+      const T xr = Xr[k * 65 + j], xi = Xi[k * 65 + j];
+      sr += l.re * xr - l.im * xi;
+      si += l.re * xi + l.im * xr;
+    }
+    T vr, vi;
+    if (i < j) {
+      vr = T(0);
+      vi = T(0);
+    } else {
+      // i == j: x = 1 / L(i,i); i > j: x = -s / L(i,i)
+      const T nr = i == j ? T(1) : -sr, ni = i == j ? T(0) : -si;
+      if (unit) {
+        vr = nr;
+        vi = ni;
+      } else {
+        const Cplx<T> d = L(i, i);
+        const T den = d.re * d.re + d.im * d.im;
+        vr = (nr * d.re + ni * d.im) / den;
+        vi = (ni * d.re - nr * d.im) / den;
+      }
+    }
+    if (j < n) {
+      Xr[i * 65 + j] = vr;
+      Xi[i * 65 + j] = vi;
+    }
+    __syncthreads();
+  }
+  if (j >= n) return;
+  for (int i = 0; i < n; ++i) {
+    const Cplx<T> v{Xr[i * 65 + j], Xi[i * 65 + j]};
+    if (uplo == Lower) Tinv[i + (long)j * ldinv] = v;
+    else Tinv[j + (long)i * ldinv] = v;
+  }
+}
+
 template <typename F>
 static void over_columns(int rows, int cols, F launch)
 {
@@ -159,7 +214,19 @@ void cplx_expand(hipStream_t s, int mode, int uplo, int n, const T* A, long lda,
   XK_HIP(hipGetLastError());
 }
 
+template <typename T>
+void cplx_trtri_leaf(hipStream_t s, int uplo, int unit, int n, const T* A, long lda, long sA, T* Tinv, long ldinv,
+                     long sInv, int batch)
+{
+  XK_CHECK(n >= 1 && n <= 64, "complex trtri leaf order must be in [1,64]");
+  if (batch <= 0) return;
+  hipLaunchKernelGGL((cplx_trtri_leaf_kernel<T>), dim3(batch), dim3(64), 0, s, uplo, unit, n, (const Cplx<T>*)A, lda,
+                     sA, (Cplx<T>*)Tinv, ldinv, sInv);
+  XK_HIP(hipGetLastError());
+}
+
 #define XK_INST(T)                                                                                         \
+  template void cplx_trtri_leaf<T>(hipStream_t, int, int, int, const T*, long, long, T*, long, long, int); \
   template void cplx_pack_a<T>(hipStream_t, int, int, int, const T*, long, T*, long);                      \
   template void cplx_pack_b<T>(hipStream_t, int, int, int, int, const T*, long, T*, long);                 \
   template void cplx_combine<T>(hipStream_t, int, int, const T*, long, const T*, const T*, T*, long, int, int);  \

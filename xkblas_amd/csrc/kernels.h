@@ -82,6 +82,11 @@ void cplx_combine(hipStream_t s, int m, int n, const T* P, long ldp, const T* al
 // CPLX_SYM symmetric, CPLX_HERM Hermitian, CPLX_TRI triangular, CPLX_UNIT unit
 // triangular.
 enum : int { CPLX_SYM = 0, CPLX_HERM = 1, CPLX_TRI = 2, CPLX_UNIT = 3 };
+// Inverse of `batch` complex triangular blocks of order n <= 64 (strides in
+// complex elements); the other triangle of each n x n inverse is zeroed.
+template <typename T>
+void cplx_trtri_leaf(hipStream_t s, int uplo, int unit, int n, const T* A, long lda, long sA, T* Tinv, long ldinv,
+                     long sInv, int batch);
 template <typename T>
 void cplx_expand(hipStream_t s, int mode, int uplo, int n, const T* A, long lda, T* Y, long ldy);
 

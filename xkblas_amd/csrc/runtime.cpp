@@ -1432,9 +1432,59 @@ int ctrmm_async(const char* fn, int side, int uplo, int transA, int diag, int N,
   return 0;
 }
 
+template <typename T>
+int ctrsm_async(const char* fn, int side, int uplo, int transA, int diag, int N, int NRHS, const T* alpha, const T* A,
+                int LDA, T* B, int LDB)
+{
+  // checks of blas/ztrsm.c:117-167
+  const int NA = side == Left ? N : NRHS;
+  if (side != Left && side != Right) return arg_error(fn, "illegal value of side", -1);
+  if (uplo != Upper && uplo != Lower) return arg_error(fn, "illegal value of uplo", -2);
+  if (transA < NoTrans || transA > ConjTrans) return arg_error(fn, "illegal value of transA", -3);
+  if (diag != Unit && diag != NonUnit) return arg_error(fn, "illegal value of diag", -4);
+  if (N < 0) return arg_error(fn, "illegal value of N", -5);
+  if (NRHS < 0) return arg_error(fn, "illegal value of NRHS", -6);
+  if (LDA < std::max(1, NA)) return arg_error(fn, "illegal value of LDA", -8);
+  if (LDB < std::max(1, N)) return arg_error(fn, "illegal value of LDB", -10);
+  if (std::min(N, NRHS) == 0) return 0;
+
+  Lock l(g_mu);
+  ensure_init();
+  Runtime& r = rt();
+  const int nb = (int)xkblas_auto_tilesize(nullptr, KERN_TRSM, N, NRHS, NA);
+  const int elt = 2 * sizeof(T);
+  std::vector<Replica*> keep;
+  Replica* rA = r.replica(A, NA, NA, LDA, elt, nb, keep);
+  Replica* rB = r.replica(B, N, NRHS, LDB, elt, nb, keep);
+  r.fetch(rA, 0, 0, NA, NA);
+  r.fetch(rB, 0, 0, N, NRHS);
+  r.before_compute();
+  Exec x = r.exec();
+  r.prof_begin("trsm_c", 4.0 * (side == Left ? (double)N * N * NRHS : (double)N * NRHS * NRHS));
+  dev_ctrsm<T>(x, side, uplo, transA, diag, N, NRHS, alpha, (const T*)rA->dev, rA->ldd, (T*)rB->dev, rB->ldd,
+               std::min(nb, 256));
+  r.prof_end();
+  r.mark_written(rB, 0, 0, N, NRHS, 0);
+  return 0;
+}
+
 }  // namespace
 
 extern "C" {
+
+int xkblas_ztrsm_async(int side, int uplo, int transA, int diag, int N, int NRHS, const Complex64_t* alpha,
+                       const Complex64_t* A, int LDA, Complex64_t* B, int LDB)
+{
+  return ctrsm_async<double>("xkblas_ztrsm_async", side, uplo, transA, diag, N, NRHS, (const double*)alpha,
+                             (const double*)A, LDA, (double*)B, LDB);
+}
+
+int xkblas_ctrsm_async(int side, int uplo, int transA, int diag, int N, int NRHS, const Complex32_t* alpha,
+                       const Complex32_t* A, int LDA, Complex32_t* B, int LDB)
+{
+  return ctrsm_async<float>("xkblas_ctrsm_async", side, uplo, transA, diag, N, NRHS, (const float*)alpha,
+                            (const float*)A, LDA, (float*)B, LDB);
+}
 
 #define XK_CPLX_OPS(P, C, R)                                                                                      \
   int xkblas_##P##syrk_async(int uplo, int trans, int N, int K, const C* alpha, const C* A, int LDA, const C* beta, \
