@@ -149,6 +149,10 @@ int xkblas_chemm_async(int side, int uplo, int M, int N,
 /* The other complex L3 routines (reference blas/zxkblas.h gen z / c:
  * zsyrk.c, zsyr2k.c, zher2k.c, zgemmt.c, zsymm.c, ztrmm.c), same arguments
  * and codes; HER2K's beta is real. */
+/* Complex Cholesky (A = L L^H or U^H U; reference blas/zpotrf.c:46-48, unbuilt
+ * there); the LAPACK info is read after xkblas_sync() with xkblas_potrf_info. */
+int xkblas_zpotrf_async(int uplo, int N, Complex64_t* A, int LDA);
+int xkblas_cpotrf_async(int uplo, int N, Complex32_t* A, int LDA);
 int xkblas_ztrsm_async(int side, int uplo, int transA, int diag, int N, int NRHS, const Complex64_t* alpha,
                        const Complex64_t* A, int LDA, Complex64_t* B, int LDB);   /* ref blas/zxkblas.h:69 */
 int xkblas_ctrsm_async(int side, int uplo, int transA, int diag, int N, int NRHS, const Complex32_t* alpha,

@@ -326,3 +326,50 @@ def test_ctrsm_fp32(xk):
     xk.invalidate()
     ref = np.linalg.solve(T.astype(np.complex128).conj().T, B0.astype(np.complex128))
     assert np.abs(X - ref).max() / np.abs(ref).max() < 1e-5
+
+
+@pytest.mark.parametrize("uplo", [121, 122])
+@pytest.mark.parametrize("n_nb", [(64, 64), (300, 128), (777, 256), (130, 512)])
+def test_zpotrf(xk, uplo, n_nb):
+    n, nb = n_nb
+    r = rng(n + uplo)
+    G = _c(r, n, n)
+    H = np.asfortranarray(G @ G.conj().T + n * np.eye(n))
+    A = H.copy(order="F")
+    xk.lib().xkblas_set_param(nb, 0)
+    info = xk.potrf(uplo, A)
+    xk.invalidate()
+    assert info == 0
+    F = np.tril(A) if uplo == 122 else np.triu(A)
+    R = H - (F @ F.conj().T if uplo == 122 else F.conj().T @ F)
+    assert _inf(R) / (n * _inf(H) * EPS) <= 10.0
+    assert np.all(np.abs(F.diagonal().imag) == 0.0) and np.all(F.diagonal().real > 0)
+    other = np.triu(np.ones((n, n), bool), 1) if uplo == 122 else np.tril(np.ones((n, n), bool), -1)
+    assert np.array_equal(A[other], H[other])
+    Lref = np.linalg.cholesky(H)
+    Fref = Lref if uplo == 122 else Lref.conj().T
+    assert np.abs(F - Fref).max() <= 1e-12 * np.abs(Fref).max()
+
+
+@pytest.mark.parametrize("bad", [0, 150, 299])
+def test_zpotrf_info(xk, bad):
+    n = 300
+    A = np.asfortranarray(np.diag(np.linspace(1.0, 2.0, n)).astype(np.complex128))
+    A[bad, bad] = -1.0
+    xk.lib().xkblas_set_param(128, 0)
+    info = xk.potrf(122, A)
+    xk.invalidate()
+    assert info == bad + 1
+
+
+def test_cpotrf_fp32(xk):
+    r = rng(97)
+    n = 200
+    G = _c(r, n, n, np.complex64)
+    H = (G @ G.conj().T + n * np.eye(n)).astype(np.complex64)
+    A = np.asfortranarray(H.copy())
+    xk.lib().xkblas_set_param(64, 0)
+    assert xk.potrf(122, A) == 0
+    xk.invalidate()
+    F = np.tril(A).astype(np.complex128)
+    assert np.abs(H - F @ F.conj().T).max() / np.abs(H).max() < 1e-5

@@ -112,6 +112,8 @@ SIGNATURES = {
     "xkblas_cgemmt_async": (_i, [_i, _i, _i, _i, _i, _p, _p, _i, _p, _i, _p, _p, _i]),
     "xkblas_csymm_async": (_i, [_i, _i, _i, _i, _p, _p, _i, _p, _i, _p, _p, _i]),
     "xkblas_ctrmm_async": (_i, [_i, _i, _i, _i, _i, _i, _p, _p, _i, _p, _i]),
+    "xkblas_zpotrf_async": (_i, [_i, _i, _p, _i]),
+    "xkblas_cpotrf_async": (_i, [_i, _i, _p, _i]),
     "xkblas_ztrsm_async": (_i, [_i, _i, _i, _i, _i, _i, _p, _p, _i, _p, _i]),
     "xkblas_ctrsm_async": (_i, [_i, _i, _i, _i, _i, _i, _p, _p, _i, _p, _i]),
     "xkblas_zherk_async": (_i, [_i, _i, _i, _i, _p, _p, _i, _p, _p, _i]),
@@ -227,7 +229,7 @@ def potrf(uplo, A, sync=True):
     dt = A.dtype
     _check(A, dt)
     N = A.shape[0]
-    fn = lib().xkblas_dpotrf_async if dt == np.float64 else lib().xkblas_spotrf_async
+    fn = getattr(lib(), "xkblas_%spotrf_async" % _PREFIX[np.dtype(dt)])
     rc = fn(uplo, N, _ptr(A), max(1, N))
     if rc != 0 or not sync:
         return rc

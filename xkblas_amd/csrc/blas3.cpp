@@ -710,6 +710,72 @@ void dev_ctrsm(const Exec& x, int side, int uplo, int transA, int diag, int m, i
   }
 }
 
+// Complex Cholesky (A = L L^H lower, U^H U upper): the right-looking blocked
+// algorithm of dev_potrf without lookahead; each diagonal block is factored
+// recursively (leaves of 64) and inverted, the panel is a product with the
+// inverse, the trailing update a Hermitian rank-nb product on the triangle.
+enum { WS_ZDI = 24, WS_ZPAN = 25, WS_ZINV = 26 };
+
+template <typename T>
+static void cpotrf_diag(const Exec& x, int uplo, int n, T* A, long lda, int* info, int base)
+{
+  if (n <= 64) {
+    cplx_potrf_leaf<T>(x.s, uplo, n, A, lda, info, base);
+    return;
+  }
+  auto at = [](auto* b, long off) { return b + 2 * off; };
+  const int n1 = half64(n), n2 = n - n1;
+  const T one[2] = {T(1), T(0)}, mone[2] = {T(-1), T(0)}, zero[2] = {T(0), T(0)};
+  cpotrf_diag<T>(x, uplo, n1, A, lda, info, base);
+  T* inv = (T*)scratch(WS_ZDI, 2 * sizeof(T) * (size_t)n1 * n1);
+  dev_ctrtri<T>(x, uplo, 0, n1, A, lda, inv, n1);
+  T* W = (T*)scratch(WS_ZPAN, 2 * sizeof(T) * (size_t)n1 * n2);
+  T* A22 = at(A, n1 + (long)n1 * lda);
+  if (uplo == Lower) {
+    T* A21 = at(A, n1);
+    copy2d<T>(x.s, 2 * n2, n1, A21, 2 * lda, W, 2L * n2);
+    cgemm_impl<T>(x, NoTrans, ConjTrans, n2, n1, n1, one, W, n2, inv, n1, zero, A21, lda, 0, 0);
+    cgemm_impl<T>(x, NoTrans, ConjTrans, n2, n2, n1, mone, A21, lda, A21, lda, one, A22, lda, Lower, 1);
+  } else {
+    T* A12 = at(A, (long)n1 * lda);
+    copy2d<T>(x.s, 2 * n1, n2, A12, 2 * lda, W, 2L * n1);
+    cgemm_impl<T>(x, ConjTrans, NoTrans, n1, n2, n1, one, inv, n1, W, n1, zero, A12, lda, 0, 0);
+    cgemm_impl<T>(x, ConjTrans, NoTrans, n2, n2, n1, mone, A12, lda, A12, lda, one, A22, lda, Upper, 1);
+  }
+  cpotrf_diag<T>(x, uplo, n2, A22, lda, info, base + n1);
+}
+
+template <typename T>
+void dev_cpotrf(const Exec& x, int uplo, int n, T* A, long lda, int nb, int* info)
+{
+  if (n <= 0) return;
+  auto at = [](auto* b, long off) { return b + 2 * off; };
+  nb = std::max(64, std::min(nb, n));
+  const int nt = (n + nb - 1) / nb;
+  const T one[2] = {T(1), T(0)}, mone[2] = {T(-1), T(0)}, zero[2] = {T(0), T(0)};
+  for (int k = 0; k < nt; ++k) {
+    const int k0 = k * nb, kk = std::min(nb, n - k0), rest = n - k0 - kk;
+    T* Akk = at(A, (long)k0 * (1 + lda));
+    cpotrf_diag<T>(x, uplo, kk, Akk, lda, info, k0);
+    if (rest == 0) break;
+    T* inv = (T*)scratch(WS_ZINV, 2 * sizeof(T) * (size_t)nb * nb);
+    dev_ctrtri<T>(x, uplo, 0, kk, Akk, lda, inv, nb);
+    T* W = (T*)scratch(WS_ZPAN, 2 * sizeof(T) * (size_t)kk * rest);
+    T* A22 = at(A, (long)(k0 + kk) * (1 + lda));
+    if (uplo == Lower) {
+      T* A21 = at(A, (long)(k0 + kk) + (long)k0 * lda);
+      copy2d<T>(x.s, 2 * rest, kk, A21, 2 * lda, W, 2L * rest);
+      cgemm_impl<T>(x, NoTrans, ConjTrans, rest, kk, kk, one, W, rest, inv, nb, zero, A21, lda, 0, 0);
+      cgemm_impl<T>(x, NoTrans, ConjTrans, rest, rest, kk, mone, A21, lda, A21, lda, one, A22, lda, Lower, 1);
+    } else {
+      T* A12 = at(A, (long)k0 + (long)(k0 + kk) * lda);
+      copy2d<T>(x.s, 2 * kk, rest, A12, 2 * lda, W, 2L * kk);
+      cgemm_impl<T>(x, ConjTrans, NoTrans, kk, rest, kk, one, inv, nb, W, kk, zero, A12, lda, 0, 0);
+      cgemm_impl<T>(x, ConjTrans, NoTrans, rest, rest, kk, mone, A12, lda, A12, lda, one, A22, lda, Upper, 1);
+    }
+  }
+}
+
 // ---------------------------------------------------------------- SPD solve
 // LAPACK dsposv's mixed-precision scheme (factor in low precision, residuals
 // and updates in FP64, stop when every column satisfies
@@ -823,7 +889,8 @@ int dev_posv_ir(const Exec& x, const Exec& xl, const Exec& xf, int uplo, int n, 
   template void dev_csyr2k<T>(const Exec&, int, int, int, int, int, const T*, const T*, long, const T*, long,    \
                               const T*, T*, long);                                                              \
   template void dev_ctrmm<T>(const Exec&, int, int, int, int, int, int, const T*, const T*, long, T*, long);     \
-  template void dev_ctrsm<T>(const Exec&, int, int, int, int, int, int, const T*, const T*, long, T*, long, int);
+  template void dev_ctrsm<T>(const Exec&, int, int, int, int, int, int, const T*, const T*, long, T*, long, int); \
+  template void dev_cpotrf<T>(const Exec&, int, int, T*, long, int, int*);
 XK_INST(double)
 XK_INST(float)
 #undef XK_INST

@@ -99,6 +99,11 @@ template <typename T>
 void dev_ctrsm(const Exec& x, int side, int uplo, int transA, int diag, int m, int n, const T* alpha, const T* A,
                long lda, T* B, long ldb, int nb);
 
+// Complex Cholesky in place (uplo triangle; A = L L^H or U^H U), block order
+// nb; info as for dev_potrf.
+template <typename T>
+void dev_cpotrf(const Exec& x, int uplo, int n, T* A, long lda, int nb, int* info);
+
 // Complex TRMM: B <- alpha op(A) B (Left) or alpha B op(A) (Right).
 template <typename T>
 void dev_ctrmm(const Exec& x, int side, int uplo, int transA, int diag, int m, int n, const T* alpha, const T* A,

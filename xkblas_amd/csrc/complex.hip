@@ -163,6 +163,59 @@ __global__ __launch_bounds__(64) void cplx_trtri_leaf_kernel(int uplo, int unit,
   }
 }
 
+// Cholesky of one Hermitian block of order n <= 64 in place, one wave, in the
+// lower view (Upper: L = U^H, so U = L^H is written back).  Right-looking by
+// columns in LDS; lane i owns row i of the trailing update.  A non-positive
+// (or NaN) pivot at column j stops the factorization and puts base + j + 1
+// into *info (atomic min).
+template <typename T>
+__global__ __launch_bounds__(64) void cplx_potrf_leaf_kernel(int uplo, int n, Cplx<T>* __restrict__ A, long lda,
+                                                             int* info, int base)
+{
+  __shared__ T Lr[64 * 65], Li[64 * 65];
+  const int i = threadIdx.x;
+  for (int k = 0; k < n; ++k)
+    if (i < n && k <= i) {
+      Cplx<T> v = uplo == Lower ? A[i + (long)k * lda] : A[k + (long)i * lda];
+      if (uplo != Lower) v.im = -v.im;
+      Lr[i + k * 65] = v.re;
+      Li[i + k * 65] = v.im;
+    }
+  __syncthreads();
+  for (int j = 0; j < n; ++j) {
+    const T d = Lr[j + j * 65];
+    if (!(d > T(0))) {  // the same value in every lane: a uniform exit
+      if (i == 0) atomicMin(info, base + j + 1);
+      break;
+    }
+    const T sd = sqrt(d);
+    __syncthreads();
+    if (i == j) {
+      Lr[j + j * 65] = sd;
+      Li[j + j * 65] = T(0);
+    } else if (i > j && i < n) {
+      Lr[i + j * 65] /= sd;
+      Li[i + j * 65] /= sd;
+    }
+    __syncthreads();
+    if (i > j && i < n) {
+      // row i of the trailing triangle: A(i,k) -= L(i,j) conj(L(k,j)), j < k <= i
+      const T lr = Lr[i + j * 65], li = Li[i + j * 65];
+      for (int k = j + 1; k <= i; ++k) {
+        const T kr = Lr[k + j * 65], ki = Li[k + j * 65];
+        Lr[i + k * 65] -= lr * kr + li * ki;
+        Li[i + k * 65] -= li * kr - lr * ki;
+      }
+    }
+    __syncthreads();
+  }
+  for (int k = 0; k < n; ++k)
+    if (i < n && k <= i) {
+      if (uplo == Lower) A[i + (long)k * lda] = Cplx<T>{Lr[i + k * 65], Li[i + k * 65]};
+      else A[k + (long)i * lda] = Cplx<T>{Lr[i + k * 65], -Li[i + k * 65]};
+    }
+}
+
 template <typename F>
 static void over_columns(int rows, int cols, F launch)
 {
@@ -225,7 +278,16 @@ void cplx_trtri_leaf(hipStream_t s, int uplo, int unit, int n, const T* A, long 
   XK_HIP(hipGetLastError());
 }
 
+template <typename T>
+void cplx_potrf_leaf(hipStream_t s, int uplo, int n, T* A, long lda, int* info, int base)
+{
+  XK_CHECK(n >= 1 && n <= 64, "complex potrf leaf order must be in [1,64]");
+  hipLaunchKernelGGL((cplx_potrf_leaf_kernel<T>), dim3(1), dim3(64), 0, s, uplo, n, (Cplx<T>*)A, lda, info, base);
+  XK_HIP(hipGetLastError());
+}
+
 #define XK_INST(T)                                                                                         \
+  template void cplx_potrf_leaf<T>(hipStream_t, int, int, T*, long, int*, int);                            \
   template void cplx_trtri_leaf<T>(hipStream_t, int, int, int, const T*, long, long, T*, long, long, int); \
   template void cplx_pack_a<T>(hipStream_t, int, int, int, const T*, long, T*, long);                      \
   template void cplx_pack_b<T>(hipStream_t, int, int, int, int, const T*, long, T*, long);                 \

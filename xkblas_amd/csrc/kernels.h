@@ -87,6 +87,11 @@ enum : int { CPLX_SYM = 0, CPLX_HERM = 1, CPLX_TRI = 2, CPLX_UNIT = 3 };
 template <typename T>
 void cplx_trtri_leaf(hipStream_t s, int uplo, int unit, int n, const T* A, long lda, long sA, T* Tinv, long ldinv,
                      long sInv, int batch);
+// Cholesky (A = L L^H, or U^H U for Upper) of one complex Hermitian block of
+// order n <= 64 in place; a failing pivot j puts base + j + 1 into *info
+// (atomic min).
+template <typename T>
+void cplx_potrf_leaf(hipStream_t s, int uplo, int n, T* A, long lda, int* info, int base);
 template <typename T>
 void cplx_expand(hipStream_t s, int mode, int uplo, int n, const T* A, long lda, T* Y, long ldy);
 

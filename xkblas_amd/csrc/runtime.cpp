@@ -1468,9 +1468,47 @@ int ctrsm_async(const char* fn, int side, int uplo, int transA, int diag, int N,
   return 0;
 }
 
+template <typename T>
+int cpotrf_async(const char* fn, int uplo, int N, T* A, int LDA)
+{
+  // checks of blas/zpotrf.c:57-68
+  if (uplo != Upper && uplo != Lower) return arg_error(fn, "illegal value of uplo", -2);
+  if (N < 0) return arg_error(fn, "illegal value of N", -5);
+  if (LDA < std::max(1, N)) return arg_error(fn, "illegal value of LDA", -8);
+  Lock l(g_mu);
+  ensure_init();
+  Runtime& r = rt();
+  std::vector<Replica*> keep;
+  if (N == 0) return 0;
+  const int nb = (int)xkblas_auto_tilesize(nullptr, KERN_POTRF, N, N, N);
+  Replica* rA = r.replica(A, N, N, LDA, 2 * sizeof(T), nb, keep);
+  if (rA->info_slot < 0) rA->info_slot = r.next_info++ % Runtime::INFO_SLOTS;
+  r.fetch(rA, 0, 0, N, N);
+  r.before_compute();
+  Exec x = r.exec();
+  int* dinfo = r.d_info + rA->info_slot;
+  fill_int(x.s, dinfo, 1, INT_MAX);
+  r.prof_begin("potrf_c", 4.0 * ((double)N * N * N / 3.0 + (double)N * N / 2.0));
+  dev_cpotrf<T>(x, uplo, N, (T*)rA->dev, rA->ldd, std::min(nb, 512), dinfo);
+  r.prof_end();
+  XK_HIP(hipMemcpyAsync(r.h_info + rA->info_slot, dinfo, sizeof(int), hipMemcpyDeviceToHost, x.s));
+  r.mark_written(rA, 0, 0, N, N, uplo);
+  return 0;
+}
+
 }  // namespace
 
 extern "C" {
+
+int xkblas_zpotrf_async(int uplo, int N, Complex64_t* A, int LDA)
+{
+  return cpotrf_async<double>("xkblas_zpotrf_async", uplo, N, (double*)A, LDA);
+}
+
+int xkblas_cpotrf_async(int uplo, int N, Complex32_t* A, int LDA)
+{
+  return cpotrf_async<float>("xkblas_cpotrf_async", uplo, N, (float*)A, LDA);
+}
 
 int xkblas_ztrsm_async(int side, int uplo, int transA, int diag, int N, int NRHS, const Complex64_t* alpha,
                        const Complex64_t* A, int LDA, Complex64_t* B, int LDB)

@@ -50,7 +50,7 @@ struct ProfStat {
 };
 
 struct Runtime {
-  static constexpr int NSCRATCH = 24;
+  static constexpr int NSCRATCH = 27;
   static constexpr int INFO_SLOTS = 1024;
 
   bool device_ready = false;
