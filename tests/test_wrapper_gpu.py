@@ -54,6 +54,34 @@ T = np.asfortranarray(np.triu(C0))
 Y = X.copy(order="F")
 W.dtrmm_(b"L", b"U", b"T", b"N", i(n), i(70), d(2.0), p(T), i(n), p(Y), i(n))
 err["trmm"] = nrm(Y - 2.0 * T.T @ X) / nrm(2.0 * T.T @ X)
+# complex and single precision entries against their textbook definitions
+zc = lambda m, n, dt=np.complex128: np.asfortranarray((r.uniform(-.5, .5, (m, n)) + 1j * r.uniform(-.5, .5, (m, n))).astype(dt))
+za = lambda v, dt=np.complex128: np.array([v], dtype=dt).ctypes.data_as(ctypes.c_void_p)
+ZA, ZB, ZC0 = zc(150, 90), zc(120, 90), zc(150, 120)
+ZC = ZC0.copy(order="F")
+W.zgemm_(b"N", b"C", i(150), i(120), i(90), za(1 - 1j), p(ZA), i(150), p(ZB), i(120), za(0.5j), p(ZC), i(150))
+err["zgemm"] = nrm(ZC - ((1 - 1j) * ZA @ ZB.conj().T + 0.5j * ZC0)) / nrm(ZC0)
+ZH0 = zc(150, 150)
+ZH = ZH0.copy(order="F")
+W.zherk_(b"L", b"N", i(150), i(90), d(1.0), p(ZA), i(150), d(0.5), p(ZH), i(150))
+ref = ZA @ ZA.conj().T + 0.5 * ZH0
+np.fill_diagonal(ref, ref.diagonal().real)
+err["zherk"] = nrm(np.tril(ZH - ref)) / nrm(ref)
+ZT = np.asfortranarray(np.triu(zc(150, 150)) + 3 * np.eye(150))
+ZX = ZA.copy(order="F")
+W.ztrsm_(b"L", b"U", b"C", b"N", i(150), i(90), za(2.0), p(ZT), i(150), p(ZX), i(150))
+err["ztrsm"] = nrm(ZT.conj().T @ ZX - 2.0 * ZA) / nrm(ZA)
+CA, CB, CC0 = zc(100, 80, np.complex64), zc(80, 60, np.complex64), zc(100, 60, np.complex64)
+CC = CC0.copy(order="F")
+W.cgemm_(b"N", b"N", i(100), i(60), i(80), za(1.0, np.complex64), p(CA), i(100), p(CB), i(80), za(1.0, np.complex64),
+         p(CC), i(100))
+err["cgemm_fp32"] = nrm(CC - (CA.astype(complex) @ CB.astype(complex) + CC0)) / nrm(CC0) * 1e-8
+SA = np.asfortranarray(r.uniform(-.5, .5, (100, 50)).astype(np.float32))
+SC0 = np.asfortranarray(r.uniform(-.5, .5, (100, 100)).astype(np.float32))
+SC = SC0.copy(order="F")
+fs = lambda v: ctypes.byref(ctypes.c_float(v))
+W.ssyrk_(b"U", b"N", i(100), i(50), fs(1.0), p(SA), i(100), fs(1.0), p(SC), i(100))
+err["ssyrk_fp32"] = nrm(np.triu(SC - (SA.astype(float) @ SA.T.astype(float) + SC0))) / nrm(SC0) * 1e-8
 print(json.dumps({"gemm": res_gemm, "potrf": res_potrf, "info": info.value, "more": err}))
 '''
 

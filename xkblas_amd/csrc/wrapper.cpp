@@ -13,6 +13,7 @@
 // follows LAPACK's interface.
 #include "../../include/xkblas.h"
 #include "../../include/xkblas_f77.h"
+#include <complex>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -92,6 +93,18 @@ double ai_trmm(bool left, double m, double n, double elt)
   const double f = left ? n * m * m : m * n * n;
   const double d = left ? (0.5 * m * m + 2.0 * m * n) : (0.5 * n * n + 2.0 * n * m);
   return f / (elt * d);
+}
+
+// zero / one tests of a scalar argument, real or complex
+template <typename T>
+bool is0(const T* v)
+{
+  return *v == T(0);
+}
+template <typename T>
+bool is1(const T* v)
+{
+  return *v == T(1);
 }
 
 template <typename T>
@@ -239,5 +252,221 @@ void dpotrf_(const char* uplo, const int* n, double* A, const int* lda, int* inf
   *info = xkblas_potrf_info(A);
   xkblas_memory_invalidate_caches();
 }
+
+// ---- the other precisions of blas/zlibxkblas_wrapper.c:46-400 (s, c, z):
+// same dispatch; the host branch calls the host BLAS routine of the same
+// name.  Complex arithmetic intensity uses 4 real flops per complex
+// multiply-add pair and 2x the element size.
+#define XKW_HOST(F, NAME, ...)                      \
+  do {                                              \
+    static F f_ = nullptr;                          \
+    if (!f_) xkblas_load_sym((void**)&f_, NAME);    \
+    f_(__VA_ARGS__);                                \
+  } while (0)
+
+#define XKW_THRESHOLD(KEY)                                                            \
+  static const double thr_ = [] {                                                     \
+    const char* v = getenv("XKBLAS_THRESHOLD_" KEY);                                  \
+    if (v) return atol(v) < 0 ? 0.0 : (double)atol(v);                                \
+    const char* g = getenv("XKBLAS_THRESHOLD");                                       \
+    return g ? (atol(g) < 0 ? 0.0 : (double)atol(g)) : 1600.0;                        \
+  }()
+
+#define XKW_FAMILY(P, KEYP, S, R, CPLX)                                                                          \
+  void P##gemm_(const char* ta, const char* tb, const int* m, const int* n, const int* k, const S* alpha,        \
+                const S* A, const int* lda, const S* B, const int* ldb, const S* beta, S* C, const int* ldc)     \
+  {                                                                                                             \
+    if (*m == 0 || *n == 0 || ((is0(alpha) || *k == 0) && is1(beta))) return;                                    \
+    XKW_THRESHOLD(KEYP "GEMM");                                                                                 \
+    if ((CPLX ? 4.0 : 1.0) * ai_gemm(*m, *n, *k, sizeof(S)) < thr_) {                                           \
+      typedef void (*f_t)(const char*, const char*, const int*, const int*, const int*, const S*, const S*,       \
+                          const int*, const S*, const int*, const S*, S*, const int*);                          \
+      XKW_HOST(f_t, #P "gemm_", ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);                          \
+      return;                                                                                                   \
+    }                                                                                                           \
+    gpu_ready();                                                                                                \
+    xkblas_##P##gemm_async(trans_of(ta), trans_of(tb), *m, *n, *k, alpha, A, *lda, B, *ldb, beta, C, *ldc);       \
+    coherent_sync<S>(0, *m, *n, C, *ldc);                                                                       \
+  }                                                                                                             \
+  void P##gemmt_(const char* uplo, const char* ta, const char* tb, const int* n, const int* k, const S* alpha,   \
+                 const S* A, const int* lda, const S* B, const int* ldb, const S* beta, S* C, const int* ldc)    \
+  {                                                                                                             \
+    if (*n == 0 || ((is0(alpha) || *k == 0) && is1(beta))) return;                                              \
+    XKW_THRESHOLD(KEYP "GEMMT");                                                                                \
+    if ((CPLX ? 4.0 : 1.0) * ai_gemmt(*n, *k, sizeof(S)) < thr_) {                                              \
+      typedef void (*f_t)(const char*, const char*, const char*, const int*, const int*, const S*, const S*,      \
+                          const int*, const S*, const int*, const S*, S*, const int*);                          \
+      XKW_HOST(f_t, #P "gemmt_", uplo, ta, tb, n, k, alpha, A, lda, B, ldb, beta, C, ldc);                      \
+      return;                                                                                                   \
+    }                                                                                                           \
+    gpu_ready();                                                                                                \
+    xkblas_##P##gemmt_async(uplo_of(uplo), trans_of(ta), trans_of(tb), *n, *k, alpha, A, *lda, B, *ldb, beta, C, \
+                            *ldc);                                                                              \
+    coherent_sync<S>(uplo_of(uplo), *n, *n, C, *ldc);                                                           \
+  }                                                                                                             \
+  void P##trsm_(const char* side, const char* uplo, const char* ta, const char* diag, const int* m, const int* n, \
+                const S* alpha, const S* A, const int* lda, S* B, const int* ldb)                                \
+  {                                                                                                             \
+    if (*m == 0 || *n == 0) return;                                                                             \
+    XKW_THRESHOLD(KEYP "TRSM");                                                                                 \
+    if ((CPLX ? 4.0 : 1.0) * ai_trsm(side_of(side) == CblasLeft, *m, *n, sizeof(S)) < thr_) {                   \
+      typedef void (*f_t)(const char*, const char*, const char*, const char*, const int*, const int*, const S*,   \
+                          const S*, const int*, S*, const int*);                                                \
+      XKW_HOST(f_t, #P "trsm_", side, uplo, ta, diag, m, n, alpha, A, lda, B, ldb);                             \
+      return;                                                                                                   \
+    }                                                                                                           \
+    gpu_ready();                                                                                                \
+    xkblas_##P##trsm_async(side_of(side), uplo_of(uplo), trans_of(ta), diag_of(diag), *m, *n, alpha, A, *lda, B,  \
+                           *ldb);                                                                               \
+    coherent_sync<S>(0, *m, *n, B, *ldb);                                                                       \
+  }                                                                                                             \
+  void P##trmm_(const char* side, const char* uplo, const char* ta, const char* diag, const int* m, const int* n, \
+                const S* alpha, const S* A, const int* lda, S* B, const int* ldb)                                \
+  {                                                                                                             \
+    if (*m == 0 || *n == 0) return;                                                                             \
+    XKW_THRESHOLD(KEYP "TRMM");                                                                                 \
+    if ((CPLX ? 4.0 : 1.0) * ai_trmm(side_of(side) == CblasLeft, *m, *n, sizeof(S)) < thr_) {                   \
+      typedef void (*f_t)(const char*, const char*, const char*, const char*, const int*, const int*, const S*,   \
+                          const S*, const int*, S*, const int*);                                                \
+      XKW_HOST(f_t, #P "trmm_", side, uplo, ta, diag, m, n, alpha, A, lda, B, ldb);                             \
+      return;                                                                                                   \
+    }                                                                                                           \
+    gpu_ready();                                                                                                \
+    xkblas_##P##trmm_async(side_of(side), uplo_of(uplo), trans_of(ta), diag_of(diag), *m, *n, alpha, A, *lda, B,  \
+                           *ldb);                                                                               \
+    coherent_sync<S>(0, *m, *n, B, *ldb);                                                                       \
+  }                                                                                                             \
+  void P##symm_(const char* side, const char* uplo, const int* m, const int* n, const S* alpha, const S* A,       \
+                const int* lda, const S* B, const int* ldb, const S* beta, S* C, const int* ldc)                 \
+  {                                                                                                             \
+    if (*m == 0 || *n == 0 || (is0(alpha) && is1(beta))) return;                                                \
+    XKW_THRESHOLD(KEYP "SYMM");                                                                                 \
+    if ((CPLX ? 4.0 : 1.0) * ai_symm(side_of(side) == CblasLeft, *m, *n, sizeof(S)) < thr_) {                   \
+      typedef void (*f_t)(const char*, const char*, const int*, const int*, const S*, const S*, const int*,       \
+                          const S*, const int*, const S*, S*, const int*);                                      \
+      XKW_HOST(f_t, #P "symm_", side, uplo, m, n, alpha, A, lda, B, ldb, beta, C, ldc);                         \
+      return;                                                                                                   \
+    }                                                                                                           \
+    gpu_ready();                                                                                                \
+    xkblas_##P##symm_async(side_of(side), uplo_of(uplo), *m, *n, alpha, A, *lda, B, *ldb, beta, C, *ldc);        \
+    coherent_sync<S>(0, *m, *n, C, *ldc);                                                                       \
+  }                                                                                                             \
+  void P##syrk_(const char* uplo, const char* trans, const int* n, const int* k, const S* alpha, const S* A,     \
+                const int* lda, const S* beta, S* C, const int* ldc)                                            \
+  {                                                                                                             \
+    if (*n == 0 || ((is0(alpha) || *k == 0) && is1(beta))) return;                                              \
+    XKW_THRESHOLD(KEYP "SYRK");                                                                                 \
+    if ((CPLX ? 4.0 : 1.0) * ai_syrk(*n, *k, sizeof(S)) < thr_) {                                               \
+      typedef void (*f_t)(const char*, const char*, const int*, const int*, const S*, const S*, const int*,       \
+                          const S*, S*, const int*);                                                            \
+      XKW_HOST(f_t, #P "syrk_", uplo, trans, n, k, alpha, A, lda, beta, C, ldc);                                \
+      return;                                                                                                   \
+    }                                                                                                           \
+    gpu_ready();                                                                                                \
+    xkblas_##P##syrk_async(uplo_of(uplo), trans_of(trans), *n, *k, alpha, A, *lda, beta, C, *ldc);              \
+    coherent_sync<S>(uplo_of(uplo), *n, *n, C, *ldc);                                                           \
+  }                                                                                                             \
+  void P##syr2k_(const char* uplo, const char* trans, const int* n, const int* k, const S* alpha, const S* A,    \
+                 const int* lda, const S* B, const int* ldb, const S* beta, S* C, const int* ldc)                \
+  {                                                                                                             \
+    if (*n == 0 || ((is0(alpha) || *k == 0) && is1(beta))) return;                                              \
+    XKW_THRESHOLD(KEYP "SYR2K");                                                                                \
+    if ((CPLX ? 4.0 : 1.0) * ai_syr2k(*n, *k, sizeof(S)) < thr_) {                                              \
+      typedef void (*f_t)(const char*, const char*, const int*, const int*, const S*, const S*, const int*,       \
+                          const S*, const int*, const S*, S*, const int*);                                      \
+      XKW_HOST(f_t, #P "syr2k_", uplo, trans, n, k, alpha, A, lda, B, ldb, beta, C, ldc);                       \
+      return;                                                                                                   \
+    }                                                                                                           \
+    gpu_ready();                                                                                                \
+    xkblas_##P##syr2k_async(uplo_of(uplo), trans_of(trans), *n, *k, alpha, A, *lda, B, *ldb, beta, C, *ldc);     \
+    coherent_sync<S>(uplo_of(uplo), *n, *n, C, *ldc);                                                           \
+  }
+
+#define XKW_HERM(P, KEYP, S, R)                                                                                  \
+  void P##hemm_(const char* side, const char* uplo, const int* m, const int* n, const S* alpha, const S* A,       \
+                const int* lda, const S* B, const int* ldb, const S* beta, S* C, const int* ldc)                 \
+  {                                                                                                             \
+    if (*m == 0 || *n == 0 || (is0(alpha) && is1(beta))) return;                                                \
+    XKW_THRESHOLD(KEYP "HEMM");                                                                                 \
+    if (4.0 * ai_symm(side_of(side) == CblasLeft, *m, *n, sizeof(S)) < thr_) {                                  \
+      typedef void (*f_t)(const char*, const char*, const int*, const int*, const S*, const S*, const int*,       \
+                          const S*, const int*, const S*, S*, const int*);                                      \
+      XKW_HOST(f_t, #P "hemm_", side, uplo, m, n, alpha, A, lda, B, ldb, beta, C, ldc);                         \
+      return;                                                                                                   \
+    }                                                                                                           \
+    gpu_ready();                                                                                                \
+    xkblas_##P##hemm_async(side_of(side), uplo_of(uplo), *m, *n, alpha, A, *lda, B, *ldb, beta, C, *ldc);        \
+    coherent_sync<S>(0, *m, *n, C, *ldc);                                                                       \
+  }                                                                                                             \
+  void P##herk_(const char* uplo, const char* trans, const int* n, const int* k, const R* alpha, const S* A,     \
+                const int* lda, const R* beta, S* C, const int* ldc)                                            \
+  {                                                                                                             \
+    if (*n == 0 || ((*alpha == R(0) || *k == 0) && *beta == R(1))) return;                                       \
+    XKW_THRESHOLD(KEYP "HERK");                                                                                 \
+    if (4.0 * ai_syrk(*n, *k, sizeof(S)) < thr_) {                                                              \
+      typedef void (*f_t)(const char*, const char*, const int*, const int*, const R*, const S*, const int*,       \
+                          const R*, S*, const int*);                                                            \
+      XKW_HOST(f_t, #P "herk_", uplo, trans, n, k, alpha, A, lda, beta, C, ldc);                                \
+      return;                                                                                                   \
+    }                                                                                                           \
+    gpu_ready();                                                                                                \
+    xkblas_##P##herk_async(uplo_of(uplo), trans_of(trans), *n, *k, alpha, A, *lda, beta, C, *ldc);              \
+    coherent_sync<S>(uplo_of(uplo), *n, *n, C, *ldc);                                                           \
+  }                                                                                                             \
+  void P##her2k_(const char* uplo, const char* trans, const int* n, const int* k, const S* alpha, const S* A,    \
+                 const int* lda, const S* B, const int* ldb, const R* beta, S* C, const int* ldc)                \
+  {                                                                                                             \
+    if (*n == 0 || ((is0(alpha) || *k == 0) && *beta == R(1))) return;                                          \
+    XKW_THRESHOLD(KEYP "HER2K");                                                                                \
+    if (4.0 * ai_syr2k(*n, *k, sizeof(S)) < thr_) {                                                             \
+      typedef void (*f_t)(const char*, const char*, const int*, const int*, const S*, const S*, const int*,       \
+                          const S*, const int*, const R*, S*, const int*);                                      \
+      XKW_HOST(f_t, #P "her2k_", uplo, trans, n, k, alpha, A, lda, B, ldb, beta, C, ldc);                       \
+      return;                                                                                                   \
+    }                                                                                                           \
+    gpu_ready();                                                                                                \
+    xkblas_##P##her2k_async(uplo_of(uplo), trans_of(trans), *n, *k, alpha, A, *lda, B, *ldb, beta, C, *ldc);     \
+    coherent_sync<S>(uplo_of(uplo), *n, *n, C, *ldc);                                                           \
+  }
+
+XKW_FAMILY(z, "Z", Complex64_t, double, true)
+XKW_FAMILY(c, "C", Complex32_t, float, true)
+XKW_HERM(z, "Z", Complex64_t, double)
+XKW_HERM(c, "C", Complex32_t, float)
+
+// single precision: the entries dgemm_ ... have no s twin above yet
+#define XKW_SREAL(P, KEYP, S)                                                                                    \
+  void P##syrk_(const char* uplo, const char* trans, const int* n, const int* k, const S* alpha, const S* A,     \
+                const int* lda, const S* beta, S* C, const int* ldc)                                            \
+  {                                                                                                             \
+    if (*n == 0 || ((is0(alpha) || *k == 0) && is1(beta))) return;                                              \
+    XKW_THRESHOLD(KEYP "SYRK");                                                                                 \
+    if (ai_syrk(*n, *k, sizeof(S)) < thr_) {                                                                    \
+      typedef void (*f_t)(const char*, const char*, const int*, const int*, const S*, const S*, const int*,       \
+                          const S*, S*, const int*);                                                            \
+      XKW_HOST(f_t, #P "syrk_", uplo, trans, n, k, alpha, A, lda, beta, C, ldc);                                \
+      return;                                                                                                   \
+    }                                                                                                           \
+    gpu_ready();                                                                                                \
+    xkblas_##P##syrk_async(uplo_of(uplo), trans_of(trans), *n, *k, alpha, A, *lda, beta, C, *ldc);              \
+    coherent_sync<S>(uplo_of(uplo), *n, *n, C, *ldc);                                                           \
+  }                                                                                                             \
+  void P##trsm_(const char* side, const char* uplo, const char* ta, const char* diag, const int* m, const int* n, \
+                const S* alpha, const S* A, const int* lda, S* B, const int* ldb)                                \
+  {                                                                                                             \
+    if (*m == 0 || *n == 0) return;                                                                             \
+    XKW_THRESHOLD(KEYP "TRSM");                                                                                 \
+    if (ai_trsm(side_of(side) == CblasLeft, *m, *n, sizeof(S)) < thr_) {                                        \
+      typedef void (*f_t)(const char*, const char*, const char*, const char*, const int*, const int*, const S*,   \
+                          const S*, const int*, S*, const int*);                                                \
+      XKW_HOST(f_t, #P "trsm_", side, uplo, ta, diag, m, n, alpha, A, lda, B, ldb);                             \
+      return;                                                                                                   \
+    }                                                                                                           \
+    gpu_ready();                                                                                                \
+    xkblas_##P##trsm_async(side_of(side), uplo_of(uplo), trans_of(ta), diag_of(diag), *m, *n, alpha, A, *lda, B,  \
+                           *ldb);                                                                               \
+    coherent_sync<S>(0, *m, *n, B, *ldb);                                                                       \
+  }
+XKW_SREAL(s, "S", float)
 
 }  // extern "C"
