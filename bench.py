@@ -364,8 +364,10 @@ def main():
     xk.invalidate()
     print("bench: dgemm 2048 tile 512 (the CPU-runnable configuration)", file=sys.stderr, flush=True)
     c1 = bench_c1(xk)
-    print("bench: dgemmt/dsyr2k/dsymm/dtrmm 8192", file=sys.stderr, flush=True)
+    print("bench: dgemmt/dsyr2k/dsymm/dtrmm/zgemm 8192", file=sys.stderr, flush=True)
     l3_more = bench_l3_more(xk)
+    print("bench: zherk/ztrsm/zpotrf 4096", file=sys.stderr, flush=True)
+    cplx = bench_complex(xk)
     cpu = None if args.no_cpu else cpu_baseline()
 
     value = split["tflops"]
@@ -393,6 +395,7 @@ def main():
         "split_fp8": split_fp8,
         "dgemm_2048_tile512": c1,
         "l3_more_fp64": l3_more,
+        "complex_fp64": cplx,
         "dpotrf": potrf,
         "batched": batched,
         "roofline": {"bound": "mfma", "achieved": round(bf16_achieved, 1), "peak": BF16_PEAK, "unit": "TFLOP/s",
@@ -635,6 +638,49 @@ def bench_l3_more(xk, n=8192):
             best = ms if best is None else min(best, ms)
         out[name] = {"ms": round(best, 3), "tflops": round(fl / best / 1e9, 3),
                      "frac_of_fp64_peak": round(fl / best / 1e9 / FP64_PEAK, 4)}
+    xk.invalidate()
+    return out
+
+
+def bench_complex(xk, n=4096):
+    """ZHERK (lower, N), ZTRSM (left, lower, N, n RHS) and ZPOTRF (lower) of
+    order n, operands resident (ZPOTRF: each factorization from a fresh host
+    copy, its transfer outside the event window): HIP-event time, best of two.
+    TFLOP/s from real flop counts, 4x those of the real routines."""
+    L = xk.lib()
+    L.xkblas_set_modemath(0)
+    L.xkblas_set_param(0, 0)
+    r = np.random.default_rng(19)
+    G = np.asfortranarray(r.uniform(-0.5, 0.5, (n, n)) + 1j * r.uniform(-0.5, 0.5, (n, n)))
+    H = np.asfortranarray(G @ G.conj().T + n * np.eye(n))
+    Tl = np.asfortranarray(np.tril(G) + n * np.eye(n))
+    X = np.asfortranarray(G.copy())
+    C = np.zeros((n, n), dtype=np.complex128, order="F")
+    out = {"n": n}
+    runs = {
+        "zherk": ("herk_c128", 4.0 * n * n * (n + 1.0), lambda: xk.herk(122, 111, 1.0, G, 0.0, C, sync=False)),
+        "ztrsm": ("trsm_c", 4.0 * n ** 3, lambda: xk.trsm(141, 122, 111, 131, 1.0, Tl, X, sync=False)),
+    }
+    for name, (cls, fl, call) in runs.items():
+        best = None
+        for _ in range(2):
+            L.xkblas_profile_reset()
+            call()
+            L.xkblas_sync()
+            ms = xk.profile(cls)[1]
+            best = ms if best is None else min(best, ms)
+        out[name] = {"ms": round(best, 3), "tflops": round(fl / best / 1e9, 3)}
+    best = None
+    for _ in range(2):
+        F = H.copy(order="F")
+        L.xkblas_profile_reset()
+        xk.potrf(122, F, sync=False)
+        L.xkblas_sync()
+        ms = xk.profile("potrf_c")[1]
+        xk.invalidate()
+        best = ms if best is None else min(best, ms)
+    fl = 4.0 * (n ** 3 / 3.0 + n * n / 2.0)
+    out["zpotrf"] = {"ms": round(best, 3), "tflops": round(fl / best / 1e9, 3)}
     xk.invalidate()
     return out
 
