@@ -214,6 +214,51 @@ int xkblas_dtrsm_native(int side, int uplo, int transA, int diag, int N, int NRH
                         const double* alpha, const double* A, int LDA,
                         double* B, int LDB);
 int xkblas_dpotrf_native(int uplo, int N, double* A, int LDA);
+/* complex host-BLAS forwarders (gen z, c of the reference's *_native) */
+int xkblas_zgemm_native(int transA, int transB, int M, int N, int K, const Complex64_t* alpha, const Complex64_t* A,
+                        int LDA, const Complex64_t* B, int LDB, const Complex64_t* beta, Complex64_t* C, int LDC);
+int xkblas_zgemmt_native(int uplo, int transA, int transB, int N, int K, const Complex64_t* alpha,
+                         const Complex64_t* A, int LDA, const Complex64_t* B, int LDB, const Complex64_t* beta,
+                         Complex64_t* C, int LDC);
+int xkblas_ztrsm_native(int side, int uplo, int transA, int diag, int N, int NRHS, const Complex64_t* alpha,
+                        const Complex64_t* A, int LDA, Complex64_t* B, int LDB);
+int xkblas_ztrmm_native(int side, int uplo, int transA, int diag, int N, int NRHS, const Complex64_t* alpha,
+                        const Complex64_t* A, int LDA, Complex64_t* B, int LDB);
+int xkblas_zsymm_native(int side, int uplo, int M, int N, const Complex64_t* alpha, const Complex64_t* A, int LDA,
+                        const Complex64_t* B, int LDB, const Complex64_t* beta, Complex64_t* C, int LDC);
+int xkblas_zhemm_native(int side, int uplo, int M, int N, const Complex64_t* alpha, const Complex64_t* A, int LDA,
+                        const Complex64_t* B, int LDB, const Complex64_t* beta, Complex64_t* C, int LDC);
+int xkblas_zsyrk_native(int uplo, int trans, int N, int K, const Complex64_t* alpha, const Complex64_t* A, int LDA,
+                        const Complex64_t* beta, Complex64_t* C, int LDC);
+int xkblas_zherk_native(int uplo, int trans, int N, int K, const double* alpha, const Complex64_t* A, int LDA,
+                        const double* beta, Complex64_t* C, int LDC);
+int xkblas_zsyr2k_native(int uplo, int trans, int N, int K, const Complex64_t* alpha, const Complex64_t* A, int LDA,
+                         const Complex64_t* B, int LDB, const Complex64_t* beta, Complex64_t* C, int LDC);
+int xkblas_zher2k_native(int uplo, int trans, int N, int K, const Complex64_t* alpha, const Complex64_t* A, int LDA,
+                         const Complex64_t* B, int LDB, const double* beta, Complex64_t* C, int LDC);
+int xkblas_zpotrf_native(int uplo, int N, Complex64_t* A, int LDA);
+int xkblas_cgemm_native(int transA, int transB, int M, int N, int K, const Complex32_t* alpha, const Complex32_t* A,
+                        int LDA, const Complex32_t* B, int LDB, const Complex32_t* beta, Complex32_t* C, int LDC);
+int xkblas_cgemmt_native(int uplo, int transA, int transB, int N, int K, const Complex32_t* alpha,
+                         const Complex32_t* A, int LDA, const Complex32_t* B, int LDB, const Complex32_t* beta,
+                         Complex32_t* C, int LDC);
+int xkblas_ctrsm_native(int side, int uplo, int transA, int diag, int N, int NRHS, const Complex32_t* alpha,
+                        const Complex32_t* A, int LDA, Complex32_t* B, int LDB);
+int xkblas_ctrmm_native(int side, int uplo, int transA, int diag, int N, int NRHS, const Complex32_t* alpha,
+                        const Complex32_t* A, int LDA, Complex32_t* B, int LDB);
+int xkblas_csymm_native(int side, int uplo, int M, int N, const Complex32_t* alpha, const Complex32_t* A, int LDA,
+                        const Complex32_t* B, int LDB, const Complex32_t* beta, Complex32_t* C, int LDC);
+int xkblas_chemm_native(int side, int uplo, int M, int N, const Complex32_t* alpha, const Complex32_t* A, int LDA,
+                        const Complex32_t* B, int LDB, const Complex32_t* beta, Complex32_t* C, int LDC);
+int xkblas_csyrk_native(int uplo, int trans, int N, int K, const Complex32_t* alpha, const Complex32_t* A, int LDA,
+                        const Complex32_t* beta, Complex32_t* C, int LDC);
+int xkblas_cherk_native(int uplo, int trans, int N, int K, const float* alpha, const Complex32_t* A, int LDA,
+                        const float* beta, Complex32_t* C, int LDC);
+int xkblas_csyr2k_native(int uplo, int trans, int N, int K, const Complex32_t* alpha, const Complex32_t* A, int LDA,
+                         const Complex32_t* B, int LDB, const Complex32_t* beta, Complex32_t* C, int LDC);
+int xkblas_cher2k_native(int uplo, int trans, int N, int K, const Complex32_t* alpha, const Complex32_t* A, int LDA,
+                         const Complex32_t* B, int LDB, const float* beta, Complex32_t* C, int LDC);
+int xkblas_cpotrf_native(int uplo, int N, Complex32_t* A, int LDA);
 int xkblas_dgemmt_native(int uplo, int transA, int transB, int N, int K,
                          const double* alpha, const double* A, int LDA,
                          const double* B, int LDB,

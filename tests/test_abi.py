@@ -229,3 +229,40 @@ def test_native_forwarders_of_gemmt_syr2k_symm_trmm():
     X = A.copy(order="F")
     L.xkblas_dtrmm_native(141, 121, 112, 131, n, k, p(one), p(T), n, p(X), n)
     assert np.allclose(X, T.T @ A, atol=1e-13)
+
+
+def test_complex_native_forwarders():
+    """xkblas_{z,c}*_native run the host BLAS (no GPU): checked against their
+    textbook definitions."""
+    import numpy as np
+    import xkblas_amd
+    L = xkblas_amd.lib()
+    r = np.random.default_rng(7)
+    n, k = 60, 30
+    c = lambda m, q, dt=np.complex128: np.asfortranarray((r.uniform(-1, 1, (m, q)) + 1j * r.uniform(-1, 1, (m, q))).astype(dt))
+    p = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    A, B, C0 = c(n, k), c(k, n), c(n, n)
+    al, be = np.array([1 - 2j]), np.array([0.5j])
+    C = C0.copy(order="F")
+    L.xkblas_zgemm_native(111, 111, n, n, k, p(al), p(A), n, p(B), k, p(be), p(C), n)
+    assert np.allclose(C, (1 - 2j) * A @ B + 0.5j * C0, atol=1e-12)
+    C = C0.copy(order="F")
+    L.xkblas_zherk_native(122, 111, n, k, p(np.array([2.0])), p(A), n, p(np.array([1.0])), p(C), n)
+    ref = 2.0 * A @ A.conj().T + C0
+    np.fill_diagonal(ref, ref.diagonal().real)
+    assert np.allclose(np.tril(C), np.tril(ref), atol=1e-12)
+    T = np.asfortranarray(np.triu(c(n, n)) + 4 * np.eye(n))
+    X = c(n, k)
+    X0 = X.copy()
+    L.xkblas_ztrsm_native(141, 121, 113, 131, n, k, p(np.array([1.0 + 0j])), p(T), n, p(X), n)
+    assert np.allclose(T.conj().T @ X, X0, atol=1e-12)
+    G = c(n, n)
+    H = np.asfortranarray(G @ G.conj().T + n * np.eye(n))
+    F = H.copy(order="F")
+    assert L.xkblas_zpotrf_native(122, n, p(F), n) == 0
+    assert np.allclose(np.tril(F) @ np.tril(F).conj().T, H, atol=1e-10)
+    A32, B32 = c(n, k, np.complex64), c(k, n, np.complex64)
+    C32 = np.zeros((n, n), dtype=np.complex64, order="F")
+    L.xkblas_cgemm_native(111, 111, n, n, k, p(np.array([1], np.complex64)), p(A32), n, p(B32), k,
+                          p(np.array([0], np.complex64)), p(C32), n)
+    assert np.allclose(C32, A32 @ B32, atol=1e-4)

@@ -56,6 +56,28 @@ SIGNATURES = {
     "xkblas_dsyrk_native": (_i, [_i, _i, _i, _i, _p, _p, _i, _p, _p, _i]),
     "xkblas_dtrsm_native": (_i, [_i, _i, _i, _i, _i, _i, _p, _p, _i, _p, _i]),
     "xkblas_dpotrf_native": (_i, [_i, _i, _p, _i]),
+    "xkblas_zgemm_native": (_i, [_i]*5 + [_p, _p, _i, _p, _i, _p, _p, _i]),
+    "xkblas_zgemmt_native": (_i, [_i]*5 + [_p, _p, _i, _p, _i, _p, _p, _i]),
+    "xkblas_ztrsm_native": (_i, [_i]*6 + [_p, _p, _i, _p, _i]),
+    "xkblas_ztrmm_native": (_i, [_i]*6 + [_p, _p, _i, _p, _i]),
+    "xkblas_zsymm_native": (_i, [_i]*4 + [_p, _p, _i, _p, _i, _p, _p, _i]),
+    "xkblas_zhemm_native": (_i, [_i]*4 + [_p, _p, _i, _p, _i, _p, _p, _i]),
+    "xkblas_zsyrk_native": (_i, [_i]*4 + [_p, _p, _i, _p, _p, _i]),
+    "xkblas_zherk_native": (_i, [_i]*4 + [_p, _p, _i, _p, _p, _i]),
+    "xkblas_zsyr2k_native": (_i, [_i]*4 + [_p, _p, _i, _p, _i, _p, _p, _i]),
+    "xkblas_zher2k_native": (_i, [_i]*4 + [_p, _p, _i, _p, _i, _p, _p, _i]),
+    "xkblas_zpotrf_native": (_i, [_i, _i, _p, _i]),
+    "xkblas_cgemm_native": (_i, [_i]*5 + [_p, _p, _i, _p, _i, _p, _p, _i]),
+    "xkblas_cgemmt_native": (_i, [_i]*5 + [_p, _p, _i, _p, _i, _p, _p, _i]),
+    "xkblas_ctrsm_native": (_i, [_i]*6 + [_p, _p, _i, _p, _i]),
+    "xkblas_ctrmm_native": (_i, [_i]*6 + [_p, _p, _i, _p, _i]),
+    "xkblas_csymm_native": (_i, [_i]*4 + [_p, _p, _i, _p, _i, _p, _p, _i]),
+    "xkblas_chemm_native": (_i, [_i]*4 + [_p, _p, _i, _p, _i, _p, _p, _i]),
+    "xkblas_csyrk_native": (_i, [_i]*4 + [_p, _p, _i, _p, _p, _i]),
+    "xkblas_cherk_native": (_i, [_i]*4 + [_p, _p, _i, _p, _p, _i]),
+    "xkblas_csyr2k_native": (_i, [_i]*4 + [_p, _p, _i, _p, _i, _p, _p, _i]),
+    "xkblas_cher2k_native": (_i, [_i]*4 + [_p, _p, _i, _p, _i, _p, _p, _i]),
+    "xkblas_cpotrf_native": (_i, [_i, _i, _p, _i]),
     "xkblas_dgemmt_native": (_i, [_i, _i, _i, _i, _i, _p, _p, _i, _p, _i, _p, _p, _i]),
     "xkblas_dsyr2k_native": (_i, [_i, _i, _i, _i, _p, _p, _i, _p, _i, _p, _p, _i]),
     "xkblas_dsymm_native": (_i, [_i, _i, _i, _i, _p, _p, _i, _p, _i, _p, _p, _i]),
@@ -136,6 +158,28 @@ SIGNATURES = {
     "xkblas_dsyrk_native_": (None, [_p] * 10),
     "xkblas_dtrsm_native_": (None, [_p] * 11),
     "xkblas_dpotrf_native_": (None, [_p] * 5),
+    "xkblas_zgemm_native_": (None, [_p] * 13),
+    "xkblas_zgemmt_native_": (None, [_p] * 13),
+    "xkblas_ztrsm_native_": (None, [_p] * 11),
+    "xkblas_ztrmm_native_": (None, [_p] * 11),
+    "xkblas_zsymm_native_": (None, [_p] * 12),
+    "xkblas_zhemm_native_": (None, [_p] * 12),
+    "xkblas_zsyrk_native_": (None, [_p] * 10),
+    "xkblas_zherk_native_": (None, [_p] * 10),
+    "xkblas_zsyr2k_native_": (None, [_p] * 12),
+    "xkblas_zher2k_native_": (None, [_p] * 12),
+    "xkblas_zpotrf_native_": (None, [_p] * 5),
+    "xkblas_cgemm_native_": (None, [_p] * 13),
+    "xkblas_cgemmt_native_": (None, [_p] * 13),
+    "xkblas_ctrsm_native_": (None, [_p] * 11),
+    "xkblas_ctrmm_native_": (None, [_p] * 11),
+    "xkblas_csymm_native_": (None, [_p] * 12),
+    "xkblas_chemm_native_": (None, [_p] * 12),
+    "xkblas_csyrk_native_": (None, [_p] * 10),
+    "xkblas_cherk_native_": (None, [_p] * 10),
+    "xkblas_csyr2k_native_": (None, [_p] * 12),
+    "xkblas_cher2k_native_": (None, [_p] * 12),
+    "xkblas_cpotrf_native_": (None, [_p] * 5),
     "xkblas_dgemmt_native_": (None, [_p] * 13),
     "xkblas_dsyr2k_native_": (None, [_p] * 12),
     "xkblas_dsymm_native_": (None, [_p] * 12),

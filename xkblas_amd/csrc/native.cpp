@@ -215,4 +215,423 @@ int xkblas_dpotrf_native(int uplo, int N, double* A, int LDA)
   return info;
 }
 
+// ---- complex forwarders (reference blas/z*.c *_native / *_native_, gen z, c)
+void xkblas_zgemm_native_(const char* ta, const char* tb, const int* m, const int* n, const int* k,
+                          const Complex64_t* alpha, const Complex64_t* A, const int* lda, const Complex64_t* B,
+                          const int* ldb, const Complex64_t* beta, Complex64_t* C, const int* ldc)
+{
+  typedef void (*f_t)(const char*, const char*, const int*, const int*, const int*, const Complex64_t*,
+                const Complex64_t*, const int*, const Complex64_t*, const int*, const Complex64_t*, Complex64_t*,
+                const int*);
+  static f_t f = nullptr;
+  if (!f) xkblas_load_sym((void**)&f, "zgemm_");
+  f(ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+
+int xkblas_zgemm_native(int transA, int transB, int M, int N, int K, const Complex64_t* alpha, const Complex64_t* A,
+                        int LDA, const Complex64_t* B, int LDB, const Complex64_t* beta, Complex64_t* C, int LDC)
+{
+  char a = trans_char(transA), b = trans_char(transB);
+  xkblas_zgemm_native_(&a, &b, &M, &N, &K, alpha, A, &LDA, B, &LDB, beta, C, &LDC);
+  return 0;
+}
+
+void xkblas_zgemmt_native_(const char* uplo, const char* ta, const char* tb, const int* n, const int* k,
+                           const Complex64_t* alpha, const Complex64_t* A, const int* lda, const Complex64_t* B,
+                           const int* ldb, const Complex64_t* beta, Complex64_t* C, const int* ldc)
+{
+  typedef void (*f_t)(const char*, const char*, const char*, const int*, const int*, const Complex64_t*,
+                const Complex64_t*, const int*, const Complex64_t*, const int*, const Complex64_t*, Complex64_t*,
+                const int*);
+  static f_t f = nullptr;
+  if (!f) xkblas_load_sym((void**)&f, "zgemmt_");
+  f(uplo, ta, tb, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+
+int xkblas_zgemmt_native(int uplo, int transA, int transB, int N, int K, const Complex64_t* alpha,
+                         const Complex64_t* A, int LDA, const Complex64_t* B, int LDB, const Complex64_t* beta,
+                         Complex64_t* C, int LDC)
+{
+  char u = uplo_char(uplo), a = trans_char(transA), b = trans_char(transB);
+  xkblas_zgemmt_native_(&u, &a, &b, &N, &K, alpha, A, &LDA, B, &LDB, beta, C, &LDC);
+  return 0;
+}
+
+void xkblas_ztrsm_native_(const char* side, const char* uplo, const char* ta, const char* diag, const int* m,
+                          const int* n, const Complex64_t* alpha, const Complex64_t* A, const int* lda, Complex64_t* B,
+                          const int* ldb)
+{
+  typedef void (*f_t)(const char*, const char*, const char*, const char*, const int*, const int*, const Complex64_t*,
+                const Complex64_t*, const int*, Complex64_t*, const int*);
+  static f_t f = nullptr;
+  if (!f) xkblas_load_sym((void**)&f, "ztrsm_");
+  f(side, uplo, ta, diag, m, n, alpha, A, lda, B, ldb);
+}
+
+int xkblas_ztrsm_native(int side, int uplo, int transA, int diag, int N, int NRHS, const Complex64_t* alpha,
+                        const Complex64_t* A, int LDA, Complex64_t* B, int LDB)
+{
+  char s = side == CblasLeft ? 'L' : 'R', u = uplo_char(uplo), a = trans_char(transA),
+                                                        d = diag == CblasUnit ? 'U' : 'N';
+  xkblas_ztrsm_native_(&s, &u, &a, &d, &N, &NRHS, alpha, A, &LDA, B, &LDB);
+  return 0;
+}
+
+void xkblas_ztrmm_native_(const char* side, const char* uplo, const char* ta, const char* diag, const int* m,
+                          const int* n, const Complex64_t* alpha, const Complex64_t* A, const int* lda, Complex64_t* B,
+                          const int* ldb)
+{
+  typedef void (*f_t)(const char*, const char*, const char*, const char*, const int*, const int*, const Complex64_t*,
+                const Complex64_t*, const int*, Complex64_t*, const int*);
+  static f_t f = nullptr;
+  if (!f) xkblas_load_sym((void**)&f, "ztrmm_");
+  f(side, uplo, ta, diag, m, n, alpha, A, lda, B, ldb);
+}
+
+int xkblas_ztrmm_native(int side, int uplo, int transA, int diag, int N, int NRHS, const Complex64_t* alpha,
+                        const Complex64_t* A, int LDA, Complex64_t* B, int LDB)
+{
+  char s = side == CblasLeft ? 'L' : 'R', u = uplo_char(uplo), a = trans_char(transA),
+                                                        d = diag == CblasUnit ? 'U' : 'N';
+  xkblas_ztrmm_native_(&s, &u, &a, &d, &N, &NRHS, alpha, A, &LDA, B, &LDB);
+  return 0;
+}
+
+void xkblas_zsymm_native_(const char* side, const char* uplo, const int* m, const int* n, const Complex64_t* alpha,
+                          const Complex64_t* A, const int* lda, const Complex64_t* B, const int* ldb,
+                          const Complex64_t* beta, Complex64_t* C, const int* ldc)
+{
+  typedef void (*f_t)(const char*, const char*, const int*, const int*, const Complex64_t*, const Complex64_t*,
+                const int*, const Complex64_t*, const int*, const Complex64_t*, Complex64_t*, const int*);
+  static f_t f = nullptr;
+  if (!f) xkblas_load_sym((void**)&f, "zsymm_");
+  f(side, uplo, m, n, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+
+int xkblas_zsymm_native(int side, int uplo, int M, int N, const Complex64_t* alpha, const Complex64_t* A, int LDA,
+                        const Complex64_t* B, int LDB, const Complex64_t* beta, Complex64_t* C, int LDC)
+{
+  char s = side == CblasLeft ? 'L' : 'R', u = uplo_char(uplo);
+  xkblas_zsymm_native_(&s, &u, &M, &N, alpha, A, &LDA, B, &LDB, beta, C, &LDC);
+  return 0;
+}
+
+void xkblas_zhemm_native_(const char* side, const char* uplo, const int* m, const int* n, const Complex64_t* alpha,
+                          const Complex64_t* A, const int* lda, const Complex64_t* B, const int* ldb,
+                          const Complex64_t* beta, Complex64_t* C, const int* ldc)
+{
+  typedef void (*f_t)(const char*, const char*, const int*, const int*, const Complex64_t*, const Complex64_t*,
+                const int*, const Complex64_t*, const int*, const Complex64_t*, Complex64_t*, const int*);
+  static f_t f = nullptr;
+  if (!f) xkblas_load_sym((void**)&f, "zhemm_");
+  f(side, uplo, m, n, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+
+int xkblas_zhemm_native(int side, int uplo, int M, int N, const Complex64_t* alpha, const Complex64_t* A, int LDA,
+                        const Complex64_t* B, int LDB, const Complex64_t* beta, Complex64_t* C, int LDC)
+{
+  char s = side == CblasLeft ? 'L' : 'R', u = uplo_char(uplo);
+  xkblas_zhemm_native_(&s, &u, &M, &N, alpha, A, &LDA, B, &LDB, beta, C, &LDC);
+  return 0;
+}
+
+void xkblas_zsyrk_native_(const char* uplo, const char* trans, const int* n, const int* k, const Complex64_t* alpha,
+                          const Complex64_t* A, const int* lda, const Complex64_t* beta, Complex64_t* C,
+                          const int* ldc)
+{
+  typedef void (*f_t)(const char*, const char*, const int*, const int*, const Complex64_t*, const Complex64_t*,
+                const int*, const Complex64_t*, Complex64_t*, const int*);
+  static f_t f = nullptr;
+  if (!f) xkblas_load_sym((void**)&f, "zsyrk_");
+  f(uplo, trans, n, k, alpha, A, lda, beta, C, ldc);
+}
+
+int xkblas_zsyrk_native(int uplo, int trans, int N, int K, const Complex64_t* alpha, const Complex64_t* A, int LDA,
+                        const Complex64_t* beta, Complex64_t* C, int LDC)
+{
+  char u = uplo_char(uplo), t = trans_char(trans);
+  xkblas_zsyrk_native_(&u, &t, &N, &K, alpha, A, &LDA, beta, C, &LDC);
+  return 0;
+}
+
+void xkblas_zherk_native_(const char* uplo, const char* trans, const int* n, const int* k, const double* alpha,
+                          const Complex64_t* A, const int* lda, const double* beta, Complex64_t* C, const int* ldc)
+{
+  typedef void (*f_t)(const char*, const char*, const int*, const int*, const double*, const Complex64_t*, const int*,
+                const double*, Complex64_t*, const int*);
+  static f_t f = nullptr;
+  if (!f) xkblas_load_sym((void**)&f, "zherk_");
+  f(uplo, trans, n, k, alpha, A, lda, beta, C, ldc);
+}
+
+int xkblas_zherk_native(int uplo, int trans, int N, int K, const double* alpha, const Complex64_t* A, int LDA,
+                        const double* beta, Complex64_t* C, int LDC)
+{
+  char u = uplo_char(uplo), t = trans_char(trans);
+  xkblas_zherk_native_(&u, &t, &N, &K, alpha, A, &LDA, beta, C, &LDC);
+  return 0;
+}
+
+void xkblas_zsyr2k_native_(const char* uplo, const char* trans, const int* n, const int* k, const Complex64_t* alpha,
+                           const Complex64_t* A, const int* lda, const Complex64_t* B, const int* ldb,
+                           const Complex64_t* beta, Complex64_t* C, const int* ldc)
+{
+  typedef void (*f_t)(const char*, const char*, const int*, const int*, const Complex64_t*, const Complex64_t*,
+                const int*, const Complex64_t*, const int*, const Complex64_t*, Complex64_t*, const int*);
+  static f_t f = nullptr;
+  if (!f) xkblas_load_sym((void**)&f, "zsyr2k_");
+  f(uplo, trans, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+
+int xkblas_zsyr2k_native(int uplo, int trans, int N, int K, const Complex64_t* alpha, const Complex64_t* A, int LDA,
+                         const Complex64_t* B, int LDB, const Complex64_t* beta, Complex64_t* C, int LDC)
+{
+  char u = uplo_char(uplo), t = trans_char(trans);
+  xkblas_zsyr2k_native_(&u, &t, &N, &K, alpha, A, &LDA, B, &LDB, beta, C, &LDC);
+  return 0;
+}
+
+void xkblas_zher2k_native_(const char* uplo, const char* trans, const int* n, const int* k, const Complex64_t* alpha,
+                           const Complex64_t* A, const int* lda, const Complex64_t* B, const int* ldb,
+                           const double* beta, Complex64_t* C, const int* ldc)
+{
+  typedef void (*f_t)(const char*, const char*, const int*, const int*, const Complex64_t*, const Complex64_t*,
+                const int*, const Complex64_t*, const int*, const double*, Complex64_t*, const int*);
+  static f_t f = nullptr;
+  if (!f) xkblas_load_sym((void**)&f, "zher2k_");
+  f(uplo, trans, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+
+int xkblas_zher2k_native(int uplo, int trans, int N, int K, const Complex64_t* alpha, const Complex64_t* A, int LDA,
+                         const Complex64_t* B, int LDB, const double* beta, Complex64_t* C, int LDC)
+{
+  char u = uplo_char(uplo), t = trans_char(trans);
+  xkblas_zher2k_native_(&u, &t, &N, &K, alpha, A, &LDA, B, &LDB, beta, C, &LDC);
+  return 0;
+}
+
+void xkblas_zpotrf_native_(const char* uplo, const int* n, Complex64_t* A, const int* lda, int* info)
+{
+  typedef void (*f_t)(const char*, const int*, Complex64_t*, const int*, int*);
+  static f_t f = nullptr;
+  if (!f) xkblas_load_sym((void**)&f, "zpotrf_");
+  f(uplo, n, A, lda, info);
+}
+
+int xkblas_zpotrf_native(int uplo, int N, Complex64_t* A, int LDA)
+{
+  char u = uplo_char(uplo); int info = 0;
+  xkblas_zpotrf_native_(&u, &N, A, &LDA, &info);
+  return info;
+}
+
+void xkblas_cgemm_native_(const char* ta, const char* tb, const int* m, const int* n, const int* k,
+                          const Complex32_t* alpha, const Complex32_t* A, const int* lda, const Complex32_t* B,
+                          const int* ldb, const Complex32_t* beta, Complex32_t* C, const int* ldc)
+{
+  typedef void (*f_t)(const char*, const char*, const int*, const int*, const int*, const Complex32_t*,
+                const Complex32_t*, const int*, const Complex32_t*, const int*, const Complex32_t*, Complex32_t*,
+                const int*);
+  static f_t f = nullptr;
+  if (!f) xkblas_load_sym((void**)&f, "cgemm_");
+  f(ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+
+int xkblas_cgemm_native(int transA, int transB, int M, int N, int K, const Complex32_t* alpha, const Complex32_t* A,
+                        int LDA, const Complex32_t* B, int LDB, const Complex32_t* beta, Complex32_t* C, int LDC)
+{
+  char a = trans_char(transA), b = trans_char(transB);
+  xkblas_cgemm_native_(&a, &b, &M, &N, &K, alpha, A, &LDA, B, &LDB, beta, C, &LDC);
+  return 0;
+}
+
+void xkblas_cgemmt_native_(const char* uplo, const char* ta, const char* tb, const int* n, const int* k,
+                           const Complex32_t* alpha, const Complex32_t* A, const int* lda, const Complex32_t* B,
+                           const int* ldb, const Complex32_t* beta, Complex32_t* C, const int* ldc)
+{
+  typedef void (*f_t)(const char*, const char*, const char*, const int*, const int*, const Complex32_t*,
+                const Complex32_t*, const int*, const Complex32_t*, const int*, const Complex32_t*, Complex32_t*,
+                const int*);
+  static f_t f = nullptr;
+  if (!f) xkblas_load_sym((void**)&f, "cgemmt_");
+  f(uplo, ta, tb, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+
+int xkblas_cgemmt_native(int uplo, int transA, int transB, int N, int K, const Complex32_t* alpha,
+                         const Complex32_t* A, int LDA, const Complex32_t* B, int LDB, const Complex32_t* beta,
+                         Complex32_t* C, int LDC)
+{
+  char u = uplo_char(uplo), a = trans_char(transA), b = trans_char(transB);
+  xkblas_cgemmt_native_(&u, &a, &b, &N, &K, alpha, A, &LDA, B, &LDB, beta, C, &LDC);
+  return 0;
+}
+
+void xkblas_ctrsm_native_(const char* side, const char* uplo, const char* ta, const char* diag, const int* m,
+                          const int* n, const Complex32_t* alpha, const Complex32_t* A, const int* lda, Complex32_t* B,
+                          const int* ldb)
+{
+  typedef void (*f_t)(const char*, const char*, const char*, const char*, const int*, const int*, const Complex32_t*,
+                const Complex32_t*, const int*, Complex32_t*, const int*);
+  static f_t f = nullptr;
+  if (!f) xkblas_load_sym((void**)&f, "ctrsm_");
+  f(side, uplo, ta, diag, m, n, alpha, A, lda, B, ldb);
+}
+
+int xkblas_ctrsm_native(int side, int uplo, int transA, int diag, int N, int NRHS, const Complex32_t* alpha,
+                        const Complex32_t* A, int LDA, Complex32_t* B, int LDB)
+{
+  char s = side == CblasLeft ? 'L' : 'R', u = uplo_char(uplo), a = trans_char(transA),
+                                                        d = diag == CblasUnit ? 'U' : 'N';
+  xkblas_ctrsm_native_(&s, &u, &a, &d, &N, &NRHS, alpha, A, &LDA, B, &LDB);
+  return 0;
+}
+
+void xkblas_ctrmm_native_(const char* side, const char* uplo, const char* ta, const char* diag, const int* m,
+                          const int* n, const Complex32_t* alpha, const Complex32_t* A, const int* lda, Complex32_t* B,
+                          const int* ldb)
+{
+  typedef void (*f_t)(const char*, const char*, const char*, const char*, const int*, const int*, const Complex32_t*,
+                const Complex32_t*, const int*, Complex32_t*, const int*);
+  static f_t f = nullptr;
+  if (!f) xkblas_load_sym((void**)&f, "ctrmm_");
+  f(side, uplo, ta, diag, m, n, alpha, A, lda, B, ldb);
+}
+
+int xkblas_ctrmm_native(int side, int uplo, int transA, int diag, int N, int NRHS, const Complex32_t* alpha,
+                        const Complex32_t* A, int LDA, Complex32_t* B, int LDB)
+{
+  char s = side == CblasLeft ? 'L' : 'R', u = uplo_char(uplo), a = trans_char(transA),
+                                                        d = diag == CblasUnit ? 'U' : 'N';
+  xkblas_ctrmm_native_(&s, &u, &a, &d, &N, &NRHS, alpha, A, &LDA, B, &LDB);
+  return 0;
+}
+
+void xkblas_csymm_native_(const char* side, const char* uplo, const int* m, const int* n, const Complex32_t* alpha,
+                          const Complex32_t* A, const int* lda, const Complex32_t* B, const int* ldb,
+                          const Complex32_t* beta, Complex32_t* C, const int* ldc)
+{
+  typedef void (*f_t)(const char*, const char*, const int*, const int*, const Complex32_t*, const Complex32_t*,
+                const int*, const Complex32_t*, const int*, const Complex32_t*, Complex32_t*, const int*);
+  static f_t f = nullptr;
+  if (!f) xkblas_load_sym((void**)&f, "csymm_");
+  f(side, uplo, m, n, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+
+int xkblas_csymm_native(int side, int uplo, int M, int N, const Complex32_t* alpha, const Complex32_t* A, int LDA,
+                        const Complex32_t* B, int LDB, const Complex32_t* beta, Complex32_t* C, int LDC)
+{
+  char s = side == CblasLeft ? 'L' : 'R', u = uplo_char(uplo);
+  xkblas_csymm_native_(&s, &u, &M, &N, alpha, A, &LDA, B, &LDB, beta, C, &LDC);
+  return 0;
+}
+
+void xkblas_chemm_native_(const char* side, const char* uplo, const int* m, const int* n, const Complex32_t* alpha,
+                          const Complex32_t* A, const int* lda, const Complex32_t* B, const int* ldb,
+                          const Complex32_t* beta, Complex32_t* C, const int* ldc)
+{
+  typedef void (*f_t)(const char*, const char*, const int*, const int*, const Complex32_t*, const Complex32_t*,
+                const int*, const Complex32_t*, const int*, const Complex32_t*, Complex32_t*, const int*);
+  static f_t f = nullptr;
+  if (!f) xkblas_load_sym((void**)&f, "chemm_");
+  f(side, uplo, m, n, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+
+int xkblas_chemm_native(int side, int uplo, int M, int N, const Complex32_t* alpha, const Complex32_t* A, int LDA,
+                        const Complex32_t* B, int LDB, const Complex32_t* beta, Complex32_t* C, int LDC)
+{
+  char s = side == CblasLeft ? 'L' : 'R', u = uplo_char(uplo);
+  xkblas_chemm_native_(&s, &u, &M, &N, alpha, A, &LDA, B, &LDB, beta, C, &LDC);
+  return 0;
+}
+
+void xkblas_csyrk_native_(const char* uplo, const char* trans, const int* n, const int* k, const Complex32_t* alpha,
+                          const Complex32_t* A, const int* lda, const Complex32_t* beta, Complex32_t* C,
+                          const int* ldc)
+{
+  typedef void (*f_t)(const char*, const char*, const int*, const int*, const Complex32_t*, const Complex32_t*,
+                const int*, const Complex32_t*, Complex32_t*, const int*);
+  static f_t f = nullptr;
+  if (!f) xkblas_load_sym((void**)&f, "csyrk_");
+  f(uplo, trans, n, k, alpha, A, lda, beta, C, ldc);
+}
+
+int xkblas_csyrk_native(int uplo, int trans, int N, int K, const Complex32_t* alpha, const Complex32_t* A, int LDA,
+                        const Complex32_t* beta, Complex32_t* C, int LDC)
+{
+  char u = uplo_char(uplo), t = trans_char(trans);
+  xkblas_csyrk_native_(&u, &t, &N, &K, alpha, A, &LDA, beta, C, &LDC);
+  return 0;
+}
+
+void xkblas_cherk_native_(const char* uplo, const char* trans, const int* n, const int* k, const float* alpha,
+                          const Complex32_t* A, const int* lda, const float* beta, Complex32_t* C, const int* ldc)
+{
+  typedef void (*f_t)(const char*, const char*, const int*, const int*, const float*, const Complex32_t*, const int*,
+                const float*, Complex32_t*, const int*);
+  static f_t f = nullptr;
+  if (!f) xkblas_load_sym((void**)&f, "cherk_");
+  f(uplo, trans, n, k, alpha, A, lda, beta, C, ldc);
+}
+
+int xkblas_cherk_native(int uplo, int trans, int N, int K, const float* alpha, const Complex32_t* A, int LDA,
+                        const float* beta, Complex32_t* C, int LDC)
+{
+  char u = uplo_char(uplo), t = trans_char(trans);
+  xkblas_cherk_native_(&u, &t, &N, &K, alpha, A, &LDA, beta, C, &LDC);
+  return 0;
+}
+
+void xkblas_csyr2k_native_(const char* uplo, const char* trans, const int* n, const int* k, const Complex32_t* alpha,
+                           const Complex32_t* A, const int* lda, const Complex32_t* B, const int* ldb,
+                           const Complex32_t* beta, Complex32_t* C, const int* ldc)
+{
+  typedef void (*f_t)(const char*, const char*, const int*, const int*, const Complex32_t*, const Complex32_t*,
+                const int*, const Complex32_t*, const int*, const Complex32_t*, Complex32_t*, const int*);
+  static f_t f = nullptr;
+  if (!f) xkblas_load_sym((void**)&f, "csyr2k_");
+  f(uplo, trans, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+
+int xkblas_csyr2k_native(int uplo, int trans, int N, int K, const Complex32_t* alpha, const Complex32_t* A, int LDA,
+                         const Complex32_t* B, int LDB, const Complex32_t* beta, Complex32_t* C, int LDC)
+{
+  char u = uplo_char(uplo), t = trans_char(trans);
+  xkblas_csyr2k_native_(&u, &t, &N, &K, alpha, A, &LDA, B, &LDB, beta, C, &LDC);
+  return 0;
+}
+
+void xkblas_cher2k_native_(const char* uplo, const char* trans, const int* n, const int* k, const Complex32_t* alpha,
+                           const Complex32_t* A, const int* lda, const Complex32_t* B, const int* ldb,
+                           const float* beta, Complex32_t* C, const int* ldc)
+{
+  typedef void (*f_t)(const char*, const char*, const int*, const int*, const Complex32_t*, const Complex32_t*,
+                const int*, const Complex32_t*, const int*, const float*, Complex32_t*, const int*);
+  static f_t f = nullptr;
+  if (!f) xkblas_load_sym((void**)&f, "cher2k_");
+  f(uplo, trans, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+
+int xkblas_cher2k_native(int uplo, int trans, int N, int K, const Complex32_t* alpha, const Complex32_t* A, int LDA,
+                         const Complex32_t* B, int LDB, const float* beta, Complex32_t* C, int LDC)
+{
+  char u = uplo_char(uplo), t = trans_char(trans);
+  xkblas_cher2k_native_(&u, &t, &N, &K, alpha, A, &LDA, B, &LDB, beta, C, &LDC);
+  return 0;
+}
+
+void xkblas_cpotrf_native_(const char* uplo, const int* n, Complex32_t* A, const int* lda, int* info)
+{
+  typedef void (*f_t)(const char*, const int*, Complex32_t*, const int*, int*);
+  static f_t f = nullptr;
+  if (!f) xkblas_load_sym((void**)&f, "cpotrf_");
+  f(uplo, n, A, lda, info);
+}
+
+int xkblas_cpotrf_native(int uplo, int N, Complex32_t* A, int LDA)
+{
+  char u = uplo_char(uplo); int info = 0;
+  xkblas_cpotrf_native_(&u, &N, A, &LDA, &info);
+  return info;
+}
+
 }  // extern "C"
