@@ -87,7 +87,8 @@ const char* xkblas_get_configuration(void);    /* ref blas/xkblas.h:273 */
 void   xkblas_load_sym(void** ptr, const char* name); /* ref blas/xkblas.h:278, xkblas.c:2678-2695 */
 uint64_t xkblas_elapsedns(void);               /* ref blas/xkblas.h:492 */
 double xkblas_elapsedtime(void);               /* ref blas/xkblas.h:498 */
-size_t xkblas_auto_tilesize(void* xkctxt, xkblas_kernel_t kernel, size_t M, size_t N, size_t K); /* ref blas/xkblas.h:310, xkblas.c:2239 */
+/* ref blas/xkblas.h:310, xkblas.c:2239 */
+size_t xkblas_auto_tilesize(void* xkctxt, xkblas_kernel_t kernel, size_t M, size_t N, size_t K);
 
 /* ---- memory and coherence ------------------------------------------------- */
 void*  xkblas_malloc(size_t sz);               /* ref blas/xkblas.h:318, xkblas.c:622 (pinned) */
@@ -159,14 +160,19 @@ int xkblas_ctrsm_async(int side, int uplo, int transA, int diag, int N, int NRHS
                        const Complex32_t* A, int LDA, Complex32_t* B, int LDB);
 int xkblas_zsyrk_async(int uplo, int trans, int N, int K, const Complex64_t* alpha, const Complex64_t* A, int LDA,
                        const Complex64_t* beta, Complex64_t* C, int LDC);   /* ref blas/zxkblas.h:97 */
+/* ref blas/zxkblas.h:107 */
 int xkblas_zsyr2k_async(int uplo, int trans, int N, int K, const Complex64_t* alpha, const Complex64_t* A, int LDA,
-                        const Complex64_t* B, int LDB, const Complex64_t* beta, Complex64_t* C, int LDC);   /* ref blas/zxkblas.h:107 */
+                        const Complex64_t* B, int LDB, const Complex64_t* beta, Complex64_t* C, int LDC);
+/* ref blas/zxkblas.h:141 */
 int xkblas_zher2k_async(int uplo, int trans, int N, int K, const Complex64_t* alpha, const Complex64_t* A, int LDA,
-                        const Complex64_t* B, int LDB, const double* beta, Complex64_t* C, int LDC);   /* ref blas/zxkblas.h:141 */
+                        const Complex64_t* B, int LDB, const double* beta, Complex64_t* C, int LDC);
+/* ref blas/zxkblas.h:57 */
 int xkblas_zgemmt_async(int uplo, int transA, int transB, int N, int K, const Complex64_t* alpha,
-                        const Complex64_t* A, int LDA, const Complex64_t* B, int LDB, const Complex64_t* beta, Complex64_t* C, int LDC);   /* ref blas/zxkblas.h:57 */
+                        const Complex64_t* A, int LDA, const Complex64_t* B, int LDB, const Complex64_t* beta,
+                        Complex64_t* C, int LDC);
+/* ref blas/zxkblas.h:85 */
 int xkblas_zsymm_async(int side, int uplo, int M, int N, const Complex64_t* alpha, const Complex64_t* A, int LDA,
-                       const Complex64_t* B, int LDB, const Complex64_t* beta, Complex64_t* C, int LDC);   /* ref blas/zxkblas.h:85 */
+                       const Complex64_t* B, int LDB, const Complex64_t* beta, Complex64_t* C, int LDC);
 int xkblas_ztrmm_async(int side, int uplo, int transA, int diag, int N, int NRHS, const Complex64_t* alpha,
                        const Complex64_t* A, int LDA, Complex64_t* B, int LDB);   /* ref blas/zxkblas.h:77 */
 int xkblas_csyrk_async(int uplo, int trans, int N, int K, const Complex32_t* alpha, const Complex32_t* A, int LDA,
@@ -176,7 +182,8 @@ int xkblas_csyr2k_async(int uplo, int trans, int N, int K, const Complex32_t* al
 int xkblas_cher2k_async(int uplo, int trans, int N, int K, const Complex32_t* alpha, const Complex32_t* A, int LDA,
                         const Complex32_t* B, int LDB, const float* beta, Complex32_t* C, int LDC);
 int xkblas_cgemmt_async(int uplo, int transA, int transB, int N, int K, const Complex32_t* alpha,
-                        const Complex32_t* A, int LDA, const Complex32_t* B, int LDB, const Complex32_t* beta, Complex32_t* C, int LDC);
+                        const Complex32_t* A, int LDA, const Complex32_t* B, int LDB, const Complex32_t* beta,
+                        Complex32_t* C, int LDC);
 int xkblas_csymm_async(int side, int uplo, int M, int N, const Complex32_t* alpha, const Complex32_t* A, int LDA,
                        const Complex32_t* B, int LDB, const Complex32_t* beta, Complex32_t* C, int LDC);
 int xkblas_ctrmm_async(int side, int uplo, int transA, int diag, int N, int NRHS, const Complex32_t* alpha,
