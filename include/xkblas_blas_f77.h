@@ -39,52 +39,62 @@ void dtrmm_(const char* side, const char* uplo, const char* transa, const char* 
             const double* alpha, const double* A, const int* lda, double* B, const int* ldb);
 
 /* the other precisions (reference blas/zlibxkblas_wrapper.c:46-400, gen s, c, z) */
-void zgemm_(const char* transa, const char* transb, const int* m, const int* n, const int* k, const Complex64_t* alpha,
-           const Complex64_t* A, const int* lda, const Complex64_t* B, const int* ldb, const Complex64_t* beta, Complex64_t* C, const int* ldc);
+void zgemm_(const char* transa, const char* transb, const int* m, const int* n, const int* k,
+            const Complex64_t* alpha, const Complex64_t* A, const int* lda, const Complex64_t* B, const int* ldb,
+            const Complex64_t* beta, Complex64_t* C, const int* ldc);
 void zgemmt_(const char* uplo, const char* transa, const char* transb, const int* n, const int* k,
-            const Complex64_t* alpha, const Complex64_t* A, const int* lda, const Complex64_t* B, const int* ldb, const Complex64_t* beta, Complex64_t* C,
-            const int* ldc);
+             const Complex64_t* alpha, const Complex64_t* A, const int* lda, const Complex64_t* B, const int* ldb,
+             const Complex64_t* beta, Complex64_t* C, const int* ldc);
 void ztrsm_(const char* side, const char* uplo, const char* transa, const char* diag, const int* m, const int* n,
-           const Complex64_t* alpha, const Complex64_t* A, const int* lda, Complex64_t* B, const int* ldb);
+            const Complex64_t* alpha, const Complex64_t* A, const int* lda, Complex64_t* B, const int* ldb);
 void ztrmm_(const char* side, const char* uplo, const char* transa, const char* diag, const int* m, const int* n,
-           const Complex64_t* alpha, const Complex64_t* A, const int* lda, Complex64_t* B, const int* ldb);
-void zsymm_(const char* side, const char* uplo, const int* m, const int* n, const Complex64_t* alpha, const Complex64_t* A,
-           const int* lda, const Complex64_t* B, const int* ldb, const Complex64_t* beta, Complex64_t* C, const int* ldc);
-void zsyrk_(const char* uplo, const char* trans, const int* n, const int* k, const Complex64_t* alpha, const Complex64_t* A,
-           const int* lda, const Complex64_t* beta, Complex64_t* C, const int* ldc);
-void zsyr2k_(const char* uplo, const char* trans, const int* n, const int* k, const Complex64_t* alpha, const Complex64_t* A,
-            const int* lda, const Complex64_t* B, const int* ldb, const Complex64_t* beta, Complex64_t* C, const int* ldc);
-void zhemm_(const char* side, const char* uplo, const int* m, const int* n, const Complex64_t* alpha, const Complex64_t* A,
-           const int* lda, const Complex64_t* B, const int* ldb, const Complex64_t* beta, Complex64_t* C, const int* ldc);
-void zherk_(const char* uplo, const char* trans, const int* n, const int* k, const double* alpha, const Complex64_t* A,
-           const int* lda, const double* beta, Complex64_t* C, const int* ldc);
-void zher2k_(const char* uplo, const char* trans, const int* n, const int* k, const Complex64_t* alpha, const Complex64_t* A,
-            const int* lda, const Complex64_t* B, const int* ldb, const double* beta, Complex64_t* C, const int* ldc);
-void cgemm_(const char* transa, const char* transb, const int* m, const int* n, const int* k, const Complex32_t* alpha,
-           const Complex32_t* A, const int* lda, const Complex32_t* B, const int* ldb, const Complex32_t* beta, Complex32_t* C, const int* ldc);
+            const Complex64_t* alpha, const Complex64_t* A, const int* lda, Complex64_t* B, const int* ldb);
+void zsymm_(const char* side, const char* uplo, const int* m, const int* n, const Complex64_t* alpha,
+            const Complex64_t* A, const int* lda, const Complex64_t* B, const int* ldb, const Complex64_t* beta,
+            Complex64_t* C, const int* ldc);
+void zsyrk_(const char* uplo, const char* trans, const int* n, const int* k, const Complex64_t* alpha,
+            const Complex64_t* A, const int* lda, const Complex64_t* beta, Complex64_t* C, const int* ldc);
+void zsyr2k_(const char* uplo, const char* trans, const int* n, const int* k, const Complex64_t* alpha,
+             const Complex64_t* A, const int* lda, const Complex64_t* B, const int* ldb, const Complex64_t* beta,
+             Complex64_t* C, const int* ldc);
+void zhemm_(const char* side, const char* uplo, const int* m, const int* n, const Complex64_t* alpha,
+            const Complex64_t* A, const int* lda, const Complex64_t* B, const int* ldb, const Complex64_t* beta,
+            Complex64_t* C, const int* ldc);
+void zherk_(const char* uplo, const char* trans, const int* n, const int* k, const double* alpha,
+            const Complex64_t* A, const int* lda, const double* beta, Complex64_t* C, const int* ldc);
+void zher2k_(const char* uplo, const char* trans, const int* n, const int* k, const Complex64_t* alpha,
+             const Complex64_t* A, const int* lda, const Complex64_t* B, const int* ldb, const double* beta,
+             Complex64_t* C, const int* ldc);
+void cgemm_(const char* transa, const char* transb, const int* m, const int* n, const int* k,
+            const Complex32_t* alpha, const Complex32_t* A, const int* lda, const Complex32_t* B, const int* ldb,
+            const Complex32_t* beta, Complex32_t* C, const int* ldc);
 void cgemmt_(const char* uplo, const char* transa, const char* transb, const int* n, const int* k,
-            const Complex32_t* alpha, const Complex32_t* A, const int* lda, const Complex32_t* B, const int* ldb, const Complex32_t* beta, Complex32_t* C,
-            const int* ldc);
+             const Complex32_t* alpha, const Complex32_t* A, const int* lda, const Complex32_t* B, const int* ldb,
+             const Complex32_t* beta, Complex32_t* C, const int* ldc);
 void ctrsm_(const char* side, const char* uplo, const char* transa, const char* diag, const int* m, const int* n,
-           const Complex32_t* alpha, const Complex32_t* A, const int* lda, Complex32_t* B, const int* ldb);
+            const Complex32_t* alpha, const Complex32_t* A, const int* lda, Complex32_t* B, const int* ldb);
 void ctrmm_(const char* side, const char* uplo, const char* transa, const char* diag, const int* m, const int* n,
-           const Complex32_t* alpha, const Complex32_t* A, const int* lda, Complex32_t* B, const int* ldb);
-void csymm_(const char* side, const char* uplo, const int* m, const int* n, const Complex32_t* alpha, const Complex32_t* A,
-           const int* lda, const Complex32_t* B, const int* ldb, const Complex32_t* beta, Complex32_t* C, const int* ldc);
-void csyrk_(const char* uplo, const char* trans, const int* n, const int* k, const Complex32_t* alpha, const Complex32_t* A,
-           const int* lda, const Complex32_t* beta, Complex32_t* C, const int* ldc);
-void csyr2k_(const char* uplo, const char* trans, const int* n, const int* k, const Complex32_t* alpha, const Complex32_t* A,
-            const int* lda, const Complex32_t* B, const int* ldb, const Complex32_t* beta, Complex32_t* C, const int* ldc);
-void chemm_(const char* side, const char* uplo, const int* m, const int* n, const Complex32_t* alpha, const Complex32_t* A,
-           const int* lda, const Complex32_t* B, const int* ldb, const Complex32_t* beta, Complex32_t* C, const int* ldc);
-void cherk_(const char* uplo, const char* trans, const int* n, const int* k, const float* alpha, const Complex32_t* A,
-           const int* lda, const float* beta, Complex32_t* C, const int* ldc);
-void cher2k_(const char* uplo, const char* trans, const int* n, const int* k, const Complex32_t* alpha, const Complex32_t* A,
-            const int* lda, const Complex32_t* B, const int* ldb, const float* beta, Complex32_t* C, const int* ldc);
+            const Complex32_t* alpha, const Complex32_t* A, const int* lda, Complex32_t* B, const int* ldb);
+void csymm_(const char* side, const char* uplo, const int* m, const int* n, const Complex32_t* alpha,
+            const Complex32_t* A, const int* lda, const Complex32_t* B, const int* ldb, const Complex32_t* beta,
+            Complex32_t* C, const int* ldc);
+void csyrk_(const char* uplo, const char* trans, const int* n, const int* k, const Complex32_t* alpha,
+            const Complex32_t* A, const int* lda, const Complex32_t* beta, Complex32_t* C, const int* ldc);
+void csyr2k_(const char* uplo, const char* trans, const int* n, const int* k, const Complex32_t* alpha,
+             const Complex32_t* A, const int* lda, const Complex32_t* B, const int* ldb, const Complex32_t* beta,
+             Complex32_t* C, const int* ldc);
+void chemm_(const char* side, const char* uplo, const int* m, const int* n, const Complex32_t* alpha,
+            const Complex32_t* A, const int* lda, const Complex32_t* B, const int* ldb, const Complex32_t* beta,
+            Complex32_t* C, const int* ldc);
+void cherk_(const char* uplo, const char* trans, const int* n, const int* k, const float* alpha,
+            const Complex32_t* A, const int* lda, const float* beta, Complex32_t* C, const int* ldc);
+void cher2k_(const char* uplo, const char* trans, const int* n, const int* k, const Complex32_t* alpha,
+             const Complex32_t* A, const int* lda, const Complex32_t* B, const int* ldb, const float* beta,
+             Complex32_t* C, const int* ldc);
 void ssyrk_(const char* uplo, const char* trans, const int* n, const int* k, const float* alpha, const float* A,
-           const int* lda, const float* beta, float* C, const int* ldc);
+            const int* lda, const float* beta, float* C, const int* ldc);
 void strsm_(const char* side, const char* uplo, const char* transa, const char* diag, const int* m, const int* n,
-           const float* alpha, const float* A, const int* lda, float* B, const int* ldb);
+            const float* alpha, const float* A, const int* lda, float* B, const int* ldb);
 
 #ifdef __cplusplus
 }
