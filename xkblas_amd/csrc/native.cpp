@@ -221,7 +221,7 @@ void xkblas_zgemm_native_(const char* ta, const char* tb, const int* m, const in
                           const int* ldb, const Complex64_t* beta, Complex64_t* C, const int* ldc)
 {
   typedef void (*f_t)(const char*, const char*, const int*, const int*, const int*, const Complex64_t*,
-                const Complex64_t*, const int*, const Complex64_t*, const int*, const Complex64_t*, Complex64_t*,
+                      const Complex64_t*, const int*, const Complex64_t*, const int*, const Complex64_t*, Complex64_t*,
                 const int*);
   static f_t f = nullptr;
   if (!f) xkblas_load_sym((void**)&f, "zgemm_");
@@ -241,7 +241,7 @@ void xkblas_zgemmt_native_(const char* uplo, const char* ta, const char* tb, con
                            const int* ldb, const Complex64_t* beta, Complex64_t* C, const int* ldc)
 {
   typedef void (*f_t)(const char*, const char*, const char*, const int*, const int*, const Complex64_t*,
-                const Complex64_t*, const int*, const Complex64_t*, const int*, const Complex64_t*, Complex64_t*,
+                      const Complex64_t*, const int*, const Complex64_t*, const int*, const Complex64_t*, Complex64_t*,
                 const int*);
   static f_t f = nullptr;
   if (!f) xkblas_load_sym((void**)&f, "zgemmt_");
@@ -262,7 +262,7 @@ void xkblas_ztrsm_native_(const char* side, const char* uplo, const char* ta, co
                           const int* ldb)
 {
   typedef void (*f_t)(const char*, const char*, const char*, const char*, const int*, const int*, const Complex64_t*,
-                const Complex64_t*, const int*, Complex64_t*, const int*);
+                      const Complex64_t*, const int*, Complex64_t*, const int*);
   static f_t f = nullptr;
   if (!f) xkblas_load_sym((void**)&f, "ztrsm_");
   f(side, uplo, ta, diag, m, n, alpha, A, lda, B, ldb);
@@ -282,7 +282,7 @@ void xkblas_ztrmm_native_(const char* side, const char* uplo, const char* ta, co
                           const int* ldb)
 {
   typedef void (*f_t)(const char*, const char*, const char*, const char*, const int*, const int*, const Complex64_t*,
-                const Complex64_t*, const int*, Complex64_t*, const int*);
+                      const Complex64_t*, const int*, Complex64_t*, const int*);
   static f_t f = nullptr;
   if (!f) xkblas_load_sym((void**)&f, "ztrmm_");
   f(side, uplo, ta, diag, m, n, alpha, A, lda, B, ldb);
@@ -302,7 +302,7 @@ void xkblas_zsymm_native_(const char* side, const char* uplo, const int* m, cons
                           const Complex64_t* beta, Complex64_t* C, const int* ldc)
 {
   typedef void (*f_t)(const char*, const char*, const int*, const int*, const Complex64_t*, const Complex64_t*,
-                const int*, const Complex64_t*, const int*, const Complex64_t*, Complex64_t*, const int*);
+                      const int*, const Complex64_t*, const int*, const Complex64_t*, Complex64_t*, const int*);
   static f_t f = nullptr;
   if (!f) xkblas_load_sym((void**)&f, "zsymm_");
   f(side, uplo, m, n, alpha, A, lda, B, ldb, beta, C, ldc);
@@ -321,7 +321,7 @@ void xkblas_zhemm_native_(const char* side, const char* uplo, const int* m, cons
                           const Complex64_t* beta, Complex64_t* C, const int* ldc)
 {
   typedef void (*f_t)(const char*, const char*, const int*, const int*, const Complex64_t*, const Complex64_t*,
-                const int*, const Complex64_t*, const int*, const Complex64_t*, Complex64_t*, const int*);
+                      const int*, const Complex64_t*, const int*, const Complex64_t*, Complex64_t*, const int*);
   static f_t f = nullptr;
   if (!f) xkblas_load_sym((void**)&f, "zhemm_");
   f(side, uplo, m, n, alpha, A, lda, B, ldb, beta, C, ldc);
@@ -340,7 +340,7 @@ void xkblas_zsyrk_native_(const char* uplo, const char* trans, const int* n, con
                           const int* ldc)
 {
   typedef void (*f_t)(const char*, const char*, const int*, const int*, const Complex64_t*, const Complex64_t*,
-                const int*, const Complex64_t*, Complex64_t*, const int*);
+                      const int*, const Complex64_t*, Complex64_t*, const int*);
   static f_t f = nullptr;
   if (!f) xkblas_load_sym((void**)&f, "zsyrk_");
   f(uplo, trans, n, k, alpha, A, lda, beta, C, ldc);
@@ -358,7 +358,7 @@ void xkblas_zherk_native_(const char* uplo, const char* trans, const int* n, con
                           const Complex64_t* A, const int* lda, const double* beta, Complex64_t* C, const int* ldc)
 {
   typedef void (*f_t)(const char*, const char*, const int*, const int*, const double*, const Complex64_t*, const int*,
-                const double*, Complex64_t*, const int*);
+                      const double*, Complex64_t*, const int*);
   static f_t f = nullptr;
   if (!f) xkblas_load_sym((void**)&f, "zherk_");
   f(uplo, trans, n, k, alpha, A, lda, beta, C, ldc);
@@ -377,7 +377,7 @@ void xkblas_zsyr2k_native_(const char* uplo, const char* trans, const int* n, co
                            const Complex64_t* beta, Complex64_t* C, const int* ldc)
 {
   typedef void (*f_t)(const char*, const char*, const int*, const int*, const Complex64_t*, const Complex64_t*,
-                const int*, const Complex64_t*, const int*, const Complex64_t*, Complex64_t*, const int*);
+                      const int*, const Complex64_t*, const int*, const Complex64_t*, Complex64_t*, const int*);
   static f_t f = nullptr;
   if (!f) xkblas_load_sym((void**)&f, "zsyr2k_");
   f(uplo, trans, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
@@ -396,7 +396,7 @@ void xkblas_zher2k_native_(const char* uplo, const char* trans, const int* n, co
                            const double* beta, Complex64_t* C, const int* ldc)
 {
   typedef void (*f_t)(const char*, const char*, const int*, const int*, const Complex64_t*, const Complex64_t*,
-                const int*, const Complex64_t*, const int*, const double*, Complex64_t*, const int*);
+                      const int*, const Complex64_t*, const int*, const double*, Complex64_t*, const int*);
   static f_t f = nullptr;
   if (!f) xkblas_load_sym((void**)&f, "zher2k_");
   f(uplo, trans, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
@@ -430,7 +430,7 @@ void xkblas_cgemm_native_(const char* ta, const char* tb, const int* m, const in
                           const int* ldb, const Complex32_t* beta, Complex32_t* C, const int* ldc)
 {
   typedef void (*f_t)(const char*, const char*, const int*, const int*, const int*, const Complex32_t*,
-                const Complex32_t*, const int*, const Complex32_t*, const int*, const Complex32_t*, Complex32_t*,
+                      const Complex32_t*, const int*, const Complex32_t*, const int*, const Complex32_t*, Complex32_t*,
                 const int*);
   static f_t f = nullptr;
   if (!f) xkblas_load_sym((void**)&f, "cgemm_");
@@ -450,7 +450,7 @@ void xkblas_cgemmt_native_(const char* uplo, const char* ta, const char* tb, con
                            const int* ldb, const Complex32_t* beta, Complex32_t* C, const int* ldc)
 {
   typedef void (*f_t)(const char*, const char*, const char*, const int*, const int*, const Complex32_t*,
-                const Complex32_t*, const int*, const Complex32_t*, const int*, const Complex32_t*, Complex32_t*,
+                      const Complex32_t*, const int*, const Complex32_t*, const int*, const Complex32_t*, Complex32_t*,
                 const int*);
   static f_t f = nullptr;
   if (!f) xkblas_load_sym((void**)&f, "cgemmt_");
@@ -471,7 +471,7 @@ void xkblas_ctrsm_native_(const char* side, const char* uplo, const char* ta, co
                           const int* ldb)
 {
   typedef void (*f_t)(const char*, const char*, const char*, const char*, const int*, const int*, const Complex32_t*,
-                const Complex32_t*, const int*, Complex32_t*, const int*);
+                      const Complex32_t*, const int*, Complex32_t*, const int*);
   static f_t f = nullptr;
   if (!f) xkblas_load_sym((void**)&f, "ctrsm_");
   f(side, uplo, ta, diag, m, n, alpha, A, lda, B, ldb);
@@ -491,7 +491,7 @@ void xkblas_ctrmm_native_(const char* side, const char* uplo, const char* ta, co
                           const int* ldb)
 {
   typedef void (*f_t)(const char*, const char*, const char*, const char*, const int*, const int*, const Complex32_t*,
-                const Complex32_t*, const int*, Complex32_t*, const int*);
+                      const Complex32_t*, const int*, Complex32_t*, const int*);
   static f_t f = nullptr;
   if (!f) xkblas_load_sym((void**)&f, "ctrmm_");
   f(side, uplo, ta, diag, m, n, alpha, A, lda, B, ldb);
@@ -511,7 +511,7 @@ void xkblas_csymm_native_(const char* side, const char* uplo, const int* m, cons
                           const Complex32_t* beta, Complex32_t* C, const int* ldc)
 {
   typedef void (*f_t)(const char*, const char*, const int*, const int*, const Complex32_t*, const Complex32_t*,
-                const int*, const Complex32_t*, const int*, const Complex32_t*, Complex32_t*, const int*);
+                      const int*, const Complex32_t*, const int*, const Complex32_t*, Complex32_t*, const int*);
   static f_t f = nullptr;
   if (!f) xkblas_load_sym((void**)&f, "csymm_");
   f(side, uplo, m, n, alpha, A, lda, B, ldb, beta, C, ldc);
@@ -530,7 +530,7 @@ void xkblas_chemm_native_(const char* side, const char* uplo, const int* m, cons
                           const Complex32_t* beta, Complex32_t* C, const int* ldc)
 {
   typedef void (*f_t)(const char*, const char*, const int*, const int*, const Complex32_t*, const Complex32_t*,
-                const int*, const Complex32_t*, const int*, const Complex32_t*, Complex32_t*, const int*);
+                      const int*, const Complex32_t*, const int*, const Complex32_t*, Complex32_t*, const int*);
   static f_t f = nullptr;
   if (!f) xkblas_load_sym((void**)&f, "chemm_");
   f(side, uplo, m, n, alpha, A, lda, B, ldb, beta, C, ldc);
@@ -549,7 +549,7 @@ void xkblas_csyrk_native_(const char* uplo, const char* trans, const int* n, con
                           const int* ldc)
 {
   typedef void (*f_t)(const char*, const char*, const int*, const int*, const Complex32_t*, const Complex32_t*,
-                const int*, const Complex32_t*, Complex32_t*, const int*);
+                      const int*, const Complex32_t*, Complex32_t*, const int*);
   static f_t f = nullptr;
   if (!f) xkblas_load_sym((void**)&f, "csyrk_");
   f(uplo, trans, n, k, alpha, A, lda, beta, C, ldc);
@@ -567,7 +567,7 @@ void xkblas_cherk_native_(const char* uplo, const char* trans, const int* n, con
                           const Complex32_t* A, const int* lda, const float* beta, Complex32_t* C, const int* ldc)
 {
   typedef void (*f_t)(const char*, const char*, const int*, const int*, const float*, const Complex32_t*, const int*,
-                const float*, Complex32_t*, const int*);
+                      const float*, Complex32_t*, const int*);
   static f_t f = nullptr;
   if (!f) xkblas_load_sym((void**)&f, "cherk_");
   f(uplo, trans, n, k, alpha, A, lda, beta, C, ldc);
@@ -586,7 +586,7 @@ void xkblas_csyr2k_native_(const char* uplo, const char* trans, const int* n, co
                            const Complex32_t* beta, Complex32_t* C, const int* ldc)
 {
   typedef void (*f_t)(const char*, const char*, const int*, const int*, const Complex32_t*, const Complex32_t*,
-                const int*, const Complex32_t*, const int*, const Complex32_t*, Complex32_t*, const int*);
+                      const int*, const Complex32_t*, const int*, const Complex32_t*, Complex32_t*, const int*);
   static f_t f = nullptr;
   if (!f) xkblas_load_sym((void**)&f, "csyr2k_");
   f(uplo, trans, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
@@ -605,7 +605,7 @@ void xkblas_cher2k_native_(const char* uplo, const char* trans, const int* n, co
                            const float* beta, Complex32_t* C, const int* ldc)
 {
   typedef void (*f_t)(const char*, const char*, const int*, const int*, const Complex32_t*, const Complex32_t*,
-                const int*, const Complex32_t*, const int*, const float*, Complex32_t*, const int*);
+                      const int*, const Complex32_t*, const int*, const float*, Complex32_t*, const int*);
   static f_t f = nullptr;
   if (!f) xkblas_load_sym((void**)&f, "cher2k_");
   f(uplo, trans, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
