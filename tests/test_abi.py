@@ -5,6 +5,7 @@ import glob
 import os
 import re
 import subprocess
+import sys
 
 import pytest
 
@@ -266,3 +267,12 @@ def test_complex_native_forwarders():
     L.xkblas_cgemm_native(111, 111, n, n, k, p(np.array([1], np.complex64)), p(A32), n, p(B32), k,
                           p(np.array([0], np.complex64)), p(C32), n)
     assert np.allclose(C32, A32 @ B32, atol=1e-4)
+
+
+def test_bench_refuses_several_gpus():
+    """bench.py measures one GPU: --gpus N > 1 exits with a message, before any
+    device use."""
+    out = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                         text=True, timeout=120)
+    assert out.returncode == 2
+    assert "one GPU" in out.stderr
