@@ -90,6 +90,36 @@ double xkblas_elapsedtime(void);               /* ref blas/xkblas.h:498 */
 /* ref blas/xkblas.h:310, xkblas.c:2239 */
 size_t xkblas_auto_tilesize(void* xkctxt, xkblas_kernel_t kernel, size_t M, size_t N, size_t K);
 
+/* Device properties (ref blas/xkblas.h:221-256, xkblas.c:1443).  The
+ * reference's architecture codes are kept; MI355X, unknown to the reference
+ * (its HIP plugin names MI250 and MI300A only, kaapi/kaapi_plugin_hip.c:
+ * 1931-1937), gets the extension code XKBLAS_ARCH_MI355X.  has_unified is 0:
+ * no unified memory is used.  Returns 0, or EINVAL (22) for a null pointer. */
+typedef struct xkblas_dev_prop_t {
+  int arch;
+  int has_unified;
+  int device_count;
+} xkblas_dev_prop_t;
+typedef enum xkblas_dev_arch_t {
+  XKBLAS_ARCH_UNDEF = -1,
+  XKBLAS_ARCH_UNKNOWN = 0,
+  XKBLAS_ARCH_V100 = 1,
+  XKBLAS_ARCH_A100 = 2,
+  XKBLAS_ARCH_RESERVED0 = 3,
+  XKBLAS_ARCH_GRACEHOPPER = 4,
+  XKBLAS_ARCH_MI50 = 101,
+  XKBLAS_ARCH_RESERVED1 = 102,
+  XKBLAS_ARCH_RESERVED2 = 103,
+  XKBLAS_ARCH_MI250 = 104,
+  XKBLAS_ARCH_RESERVED3 = 105,
+  XKBLAS_ARCH_MI300A = 106,
+  XKBLAS_ARCH_MI355X = 107 /* extension */
+} xkblas_dev_arch_t;
+int    xkblas_get_device_properties(xkblas_dev_prop_t* prop); /* ref blas/xkblas.h:256 */
+/* ref blas/xkblas.h:267: unified memory is not used, so EINVAL (22), the
+ * reference's answer for devices without it (kaapi/kaapi_offload.c:631-640) */
+int    xkblas_force_set_unified(void);
+
 /* ---- memory and coherence ------------------------------------------------- */
 void*  xkblas_malloc(size_t sz);               /* ref blas/xkblas.h:318, xkblas.c:622 (pinned) */
 void   xkblas_free(void* ptr, size_t sz);      /* ref blas/xkblas.h:322 */
@@ -99,6 +129,17 @@ int    xkblas_memory_syncall(void);            /* ref blas/xkblas.h:452 */
 int    xkblas_memory_invalidate(const void* A);/* ref blas/xkblas.h:458 */
 int    xkblas_memory_invalidate_caches(void);  /* ref blas/xkblas.h:462, xkblas.c:1855 */
 int    xkblas_memory_free(void);               /* ref blas/xkblas.h:466 */
+/* Host registration (ref blas/xkblas.h:323-366, xkblas.c:668-774).  Copies
+ * from pageable memory already go through a ring of pinned staging buffers,
+ * so registering existing memory has nothing to do: every request completes
+ * at once with status 0 (test returns 1), handles are distinct and non-zero. */
+uint64_t xkblas_register_memory_async(void* ptr, size_t sz);   /* ref blas/xkblas.h:328 */
+uint64_t xkblas_unregister_memory_async(void* ptr, size_t sz); /* ref blas/xkblas.h:336 */
+int    xkblas_register_memory_test(uint64_t handle);           /* ref blas/xkblas.h:345 */
+int    xkblas_register_memory_wait(uint64_t handle);           /* ref blas/xkblas.h:351 */
+int    xkblas_register_memory_waitall(void);                   /* ref blas/xkblas.h:355 */
+int    xkblas_register_memory(void* ptr, size_t sz);           /* ref blas/xkblas.h:361 */
+int    xkblas_unregister_memory(void* ptr, size_t sz);         /* ref blas/xkblas.h:365 */
 
 /* ---- FP64 compute (gen d of reference blas/zxkblas.h) --------------------- */
 int xkblas_dgemm_async(int transA, int transB, int M, int N, int K,

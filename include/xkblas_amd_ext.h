@@ -113,8 +113,13 @@ void xkblas_set_potrf_pairing(int mode);
  * in the reference; argument order, codes and *iter follow LAPACK dsposv:
  * the Cholesky factor is computed in split-BF16 mode with a reduced number of
  * CRT moduli (diagonal blocks in FP64), the residual B - A X and the updates
- * are FP64, and refinement stops when every column has
- * ||r||_inf < sqrt(N) ||x||_inf ||A||_inf eps.  *iter = corrections applied;
+ * are FP64.  Refinement stops when every column passes dsposv's test on the
+ * residual the last correction d was computed from,
+ * ||r||_inf <= sqrt(N) ||x||_inf ||A||_inf eps, and a forward test: ||d||_inf
+ * <= 2^-48 ||x||_inf, or (from the second correction) rho / (1 - rho) ||d||_inf
+ * <= 2^-48 ||x||_inf with rho = ||d|| / ||previous d||, or rho > 1/2 (no
+ * further progress).  dsposv's test alone allows a relative error of about
+ * sqrt(N) eps cond(A).  *iter = corrections applied;
  * -3 if the reduced factor failed, -(itermax + 1) after itermax corrections:
  * then X comes from the factor of the current mode (as dsposv falls back to
  * dpotrf).  Returns 0, > 0 (LAPACK info: A not positive definite) or < 0

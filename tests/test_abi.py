@@ -76,6 +76,52 @@ def test_query_calls_without_device():
     assert L.xkblas_auto_tilesize(None, 1, 2048, 2048, 2048) == 2048
 
 
+# (kernel id, M, expected NB), each worked by hand from the reference's
+# one-GPU rule (blas/xkblas.c:2437-2521): kernel ids of xkblas_kernel_t.
+TILE_CASES = [
+    # GEMM (1), GEMMT (2): M, M/2 above 4096, at least 512, 64-aligned
+    (1, 300, 512), (1, 4096, 4096), (1, 5000, 2560), (2, 16384, 8192),
+    # TRMM (3), TRSM (4): M/2 64-aligned, at least 2048
+    (3, 3000, 2048), (4, 10000, 5056),
+    # SYMM (5), HEMM (8): M down to a multiple of 1024, at least 1024,
+    # doubled below 4096
+    (5, 3000, 4096), (8, 16384, 16384), (5, 500, 2048),
+    # SYRK (6), HERK (9): M/2 masked to 2048 multiples; M/2 <= 1024 -> M
+    # (masked too); above 4096 -> M/4
+    (6, 2048, 2048), (6, 2049, 2048), (9, 1000, 1000), (6, 8192, 4096), (6, 20000, 5000), (6, 700, 700),
+    # SYR2K (7), HER2K (10): M/4, M/3, M/2 (64-aligned) until above 2048
+    (7, 16384, 4096), (7, 8192, 2752), (10, 4096, 2048), (7, 1000, 512), (7, 40000, 5000),
+    # default branch (SWAP 11, COPYSCALE 12, POTRF 13): M, M/2 from 4096,
+    # at least 1024, 64-aligned
+    (11, 3000, 3008), (13, 16384, 8192), (12, 500, 1024),
+]
+
+
+@pytest.mark.parametrize("kernel,m,nb", TILE_CASES)
+def test_auto_tilesize_rule(kernel, m, nb):
+    import xkblas_amd
+    L = xkblas_amd.lib()
+    L.xkblas_set_param(0, 0)
+    assert L.xkblas_auto_tilesize(None, kernel, m, m // 2, 64) == nb
+
+
+def test_device_properties_and_registration_without_device():
+    import ctypes
+    import xkblas_amd
+    L = xkblas_amd.lib()
+    assert L.xkblas_get_device_properties(None) == 22
+    assert L.xkblas_force_set_unified() == 22
+    h1 = L.xkblas_register_memory_async(None, 0)
+    h2 = L.xkblas_unregister_memory_async(None, 0)
+    assert h1 and h2 and h1 != h2
+    assert L.xkblas_register_memory_test(h1) == 1
+    assert L.xkblas_register_memory_wait(h1) == 0
+    assert L.xkblas_register_memory_waitall() == 0
+    buf = (ctypes.c_char * 64)()
+    assert L.xkblas_register_memory(buf, 64) == 0
+    assert L.xkblas_unregister_memory(buf, 64) == 0
+
+
 def test_split_plans():
     """Moduli and residue-GEMM counts of the split modes (host arithmetic).
     The bits per side t satisfy k 2^(2t) < P / 2 for P the product of the

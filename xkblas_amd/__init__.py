@@ -44,6 +44,15 @@ SIGNATURES = {
     "xkblas_malloc": (_p, [_sz]),
     "xkblas_free": (None, [_p, _sz]),
     "xkblas_memory_coherent_async": (_i, [_i, _i, _sz, _sz, _p, _sz, _sz]),
+    "xkblas_get_device_properties": (_i, [_p]),
+    "xkblas_force_set_unified": (_i, []),
+    "xkblas_register_memory_async": (_u64, [_p, _sz]),
+    "xkblas_unregister_memory_async": (_u64, [_p, _sz]),
+    "xkblas_register_memory_test": (_i, [_u64]),
+    "xkblas_register_memory_wait": (_i, [_u64]),
+    "xkblas_register_memory_waitall": (_i, []),
+    "xkblas_register_memory": (_i, [_p, _sz]),
+    "xkblas_unregister_memory": (_i, [_p, _sz]),
     "xkblas_memory_syncall": (_i, []),
     "xkblas_memory_invalidate": (_i, [_p]),
     "xkblas_memory_invalidate_caches": (_i, []),

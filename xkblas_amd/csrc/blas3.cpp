@@ -778,13 +778,34 @@ void dev_cpotrf(const Exec& x, int uplo, int n, T* A, long lda, int nb, int* inf
 
 // ---------------------------------------------------------------- SPD solve
 // LAPACK dsposv's mixed-precision scheme (factor in low precision, residuals
-// and updates in FP64, stop when every column satisfies
-// ||r||_inf < sqrt(n) ||x||_inf ||A||_inf eps, fall back to the accurate
-// factor after itermax corrections or a failed low-precision factor), with
-// the low-precision factor a reduced-moduli split-BF16 Cholesky whose
-// diagonal blocks stay FP64.  A's full symmetric image is kept for the
-// residual GEMM (B - A X); the solves reuse one set of diagonal-block
-// inverses of the factor.
+// and updates in FP64, fall back to the accurate factor after itermax
+// corrections or a failed low-precision factor), with the low-precision
+// factor a reduced-moduli split-BF16 Cholesky whose diagonal blocks stay FP64.
+// A's full symmetric image is kept for the residual GEMM (B - A X); the
+// solves reuse one set of diagonal-block inverses of the factor.
+//
+// Stopping rule, per column j of x_i = x_{i-1} + d_i (d_i = F^-1 r_{i-1}):
+//  * dsposv's backward test on the residual r_{i-1} of the solution the
+//    correction was computed from: ||r||_inf <= sqrt(n) ||x||_inf ||A||_inf eps;
+//  * and a forward test, because dsposv's sqrt(n) allows a relative error of
+//    ~ sqrt(n) eps kappa (4e-14 at n = 32768): the correction is already
+//    below IR_TAU ||x||_inf, or, from the second correction on, the observed
+//    contraction rho = ||d_i|| / ||d_{i-1}|| predicts the error left in x_i,
+//    rho / (1 - rho) ||d_i||, below IR_TAU ||x||_inf, or rho > 1/2 (no further
+//    progress: the FP64 residual's own rounding is reached).
+// At the correction limit only the backward test is applied (dsposv's).
+static constexpr double IR_TAU = 3.552713678800501e-15;  // 2^-48
+
+static bool ir_column_done(bool res_ok, int corr, double dn, double dprev, double xn)
+{
+  if (!res_ok) return false;
+  if (dn <= IR_TAU * xn) return true;
+  if (corr < 2 || !(dprev > 0.0)) return false;
+  const double rho = dn / dprev;
+  if (rho > 0.5) return true;
+  return rho / (1.0 - rho) * dn <= IR_TAU * xn;
+}
+
 enum { WS_IR_A = 16, WS_IR_F = 17, WS_IR_R = 18, WS_IR_V = 19 };
 
 int dev_posv_ir(const Exec& x, const Exec& xl, const Exec& xf, int uplo, int n, int nrhs, const double* A, long lda,
@@ -803,10 +824,11 @@ int dev_posv_ir(const Exec& x, const Exec& xl, const Exec& xf, int uplo, int n, 
   // R (n x nrhs) then the diagonal-block inverses of the factor
   double* R = (double*)scratch(WS_IR_R, sizeof(double) * ((size_t)ldf * nrhs + (size_t)nblk * tnb * tnb));
   double* inv = R + (size_t)ldf * nrhs;
-  // n column sums of |A|, nrhs residual norms, nrhs solution norms, info
-  double* V = (double*)scratch(WS_IR_V, sizeof(double) * ((size_t)n + 2 * nrhs + 1));
-  int* dinfo = (int*)(V + n + 2 * nrhs);
-  std::vector<double> hv((size_t)n + 2 * nrhs);
+  // n column sums of |A|, then per column: residual, solution and correction
+  // norms; then the info word
+  double* V = (double*)scratch(WS_IR_V, sizeof(double) * ((size_t)n + 3 * nrhs + 1));
+  int* dinfo = (int*)(V + n + 3 * nrhs);
+  std::vector<double> hv((size_t)n + 3 * nrhs), dprev(nrhs, 0.0);
   int hinfo = 0;
 
   const int tA = uplo == Lower ? NoTrans : Trans;  // op of the first solve's factor
@@ -821,6 +843,19 @@ int dev_posv_ir(const Exec& x, const Exec& xl, const Exec& xf, int uplo, int n, 
     XK_HIP(hipStreamSynchronize(x.s));
     return hinfo == INT_MAX ? 0 : hinfo;
   };
+  double* rn = V + n;            // ||r_j||_inf
+  double* xn = V + n + nrhs;     // ||x_j||_inf
+  double* dn = V + n + 2 * nrhs; // ||d_j||_inf
+  auto residual = [&]() {        // R <- B - A X, then the norms of R and X
+    copy2d<double>(x.s, n, nrhs, B, ldb, R, ldf);
+    dev_gemm<double>(x, NoTrans, NoTrans, n, nrhs, n, -1.0, Af, ldf, X, ldx, 1.0, R, ldf);
+    col_reduce(x.s, 0, n, nrhs, R, ldf, rn);
+    col_reduce(x.s, 0, n, nrhs, X, ldx, xn);
+  };
+  auto to_host = [&](int count) {  // per-column norms from rn on
+    XK_HIP(hipMemcpyAsync(hv.data() + n, rn, sizeof(double) * count * nrhs, hipMemcpyDeviceToHost, x.s));
+    XK_HIP(hipStreamSynchronize(x.s));
+  };
 
   symm_expand(x.s, uplo, n, A, lda, Af, ldf, F, ldf);
   col_reduce(x.s, 1, n, n, Af, ldf, V);  // row sums of |A| = column sums (symmetric)
@@ -829,6 +864,8 @@ int dev_posv_ir(const Exec& x, const Exec& xl, const Exec& xf, int uplo, int n, 
   double anrm = 0.0;
   for (int j = 0; j < n; ++j) anrm = (hv[j] != hv[j] || hv[j] > anrm) ? hv[j] : anrm;
   const double cte = anrm * 1.1102230246251565e-16 * std::sqrt((double)n);  // eps = 2^-53
+  // dsposv's test (continue while rnrm > xnrm cte; NaN never passes)
+  auto res_ok = [&](int j) { return hv[n + j] <= hv[n + nrhs + j] * cte; };
 
   int fallback = 0;
   if (factor(xl) != 0) {
@@ -838,21 +875,31 @@ int dev_posv_ir(const Exec& x, const Exec& xl, const Exec& xf, int uplo, int n, 
     copy2d<double>(x.s, n, nrhs, B, ldb, X, ldx);
     solve(X, ldx);
     for (int it = 0;; ++it) {
-      copy2d<double>(x.s, n, nrhs, B, ldb, R, ldf);
-      dev_gemm<double>(x, NoTrans, NoTrans, n, nrhs, n, -1.0, Af, ldf, X, ldx, 1.0, R, ldf);
-      col_reduce(x.s, 0, n, nrhs, R, ldf, V + n);
-      col_reduce(x.s, 0, n, nrhs, X, ldx, V + n + nrhs);
-      XK_HIP(hipMemcpyAsync(hv.data() + n, V + n, sizeof(double) * 2 * nrhs, hipMemcpyDeviceToHost, x.s));
-      XK_HIP(hipStreamSynchronize(x.s));
+      residual();
+      if (it == itermax) {
+        to_host(2);
+        bool done = true;
+        for (int j = 0; j < nrhs && done; ++j) done = res_ok(j);
+        if (done) {
+          *iter = it;
+          return 0;
+        }
+        break;
+      }
+      solve(R, ldf);  // the correction d
+      col_reduce(x.s, 0, n, nrhs, R, ldf, dn);
+      add2d(x.s, n, nrhs, R, ldf, X, ldx);
+      to_host(3);
       bool done = true;
-      for (int j = 0; j < nrhs && done; ++j) done = hv[n + j] < hv[n + nrhs + j] * cte;
+      for (int j = 0; j < nrhs; ++j) {
+        const double d = hv[n + 2 * nrhs + j];
+        done = done && ir_column_done(res_ok(j), it + 1, d, dprev[j], hv[n + nrhs + j]);
+        dprev[j] = d;
+      }
       if (done) {
-        *iter = it;
+        *iter = it + 1;
         return 0;
       }
-      if (it == itermax) break;
-      solve(R, ldf);
-      add2d(x.s, n, nrhs, R, ldf, X, ldx);
     }
     fallback = -(itermax + 1);
   }

@@ -24,6 +24,7 @@
 #include "../../include/xkblas.h"
 
 #include <algorithm>
+#include <atomic>
 #include <climits>
 #include <cmath>
 #include <condition_variable>
@@ -69,11 +70,16 @@ void* scratch(int slot, size_t bytes)
       // kernels already enqueued may still use the old buffer
       XK_HIP(hipDeviceSynchronize());
       XK_HIP(hipFree(r.scratch_ptr[slot]));
+      r.cache_used -= r.scratch_size[slot];
       r.scratch_ptr[slot] = nullptr;
       r.scratch_size[slot] = 0;
     }
-    XK_HIP(hipMalloc(&r.scratch_ptr[slot], sz));
+    // scratch counts against the cache limit: mirrors outside the running
+    // call make room first (best effort: the call's own operands stay)
+    r.evict_for(sz, r.call_keep, false);
+    r.scratch_ptr[slot] = r.device_alloc(sz, r.call_keep);
     r.scratch_size[slot] = sz;
+    r.cache_used += sz;
   }
   return r.scratch_ptr[slot];
 }
@@ -85,7 +91,16 @@ static long pad_ld(long m, size_t elt)
   return std::max(q, (m + q - 1) / q * q);
 }
 
-void Runtime::evict_for(size_t need, const std::vector<Replica*>& keep)
+std::vector<Replica*>& Runtime::begin_call()
+{
+  call_keep.clear();
+  return call_keep;
+}
+
+// Evict least-recently-used mirrors outside `keep` (dirty tiles written back
+// first) until `need` more bytes fit the limit.  strict: an operand set that
+// cannot fit is fatal; otherwise returns false.
+bool Runtime::evict_for(size_t need, const std::vector<Replica*>& keep, bool strict)
 {
   while (cache_used + need > cache_limit) {
     Replica* victim = nullptr;
@@ -95,6 +110,7 @@ void Runtime::evict_for(size_t need, const std::vector<Replica*>& keep)
       if (!victim || r->lru < victim->lru) victim = r;
     }
     if (!victim) {
+      if (!strict) return false;
       char msg[256];
       snprintf(msg, sizeof msg, "operands need %zu bytes of HBM beyond the cache limit of %zu bytes", need,
                cache_limit);
@@ -106,6 +122,36 @@ void Runtime::evict_for(size_t need, const std::vector<Replica*>& keep)
     drop(victim);
     stats.evictions++;
   }
+  return true;
+}
+
+// hipMalloc; when the device is out of memory, every mirror outside `keep`
+// is evicted and the allocation tried once more.
+void* Runtime::device_alloc(size_t bytes, const std::vector<Replica*>& keep)
+{
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, bytes);
+  if (e == hipErrorOutOfMemory) {
+    (void)hipGetLastError();
+    sync_all();
+    std::vector<Replica*> victims;
+    for (auto& kv : map)
+      if (std::find(keep.begin(), keep.end(), kv.second) == keep.end()) victims.push_back(kv.second);
+    for (Replica* v : victims) {
+      writeback(v, 0, 0, v->M, v->N, 0);
+      XK_HIP(hipStreamSynchronize(s_d2h));
+      drain_stages();
+      drop(v);
+      stats.evictions++;
+    }
+    e = hipMalloc(&p, bytes);
+  }
+  if (e != hipSuccess) {
+    char msg[160];
+    snprintf(msg, sizeof msg, "hipMalloc of %zu bytes: %s", bytes, hipGetErrorString(e));
+    fatal("device memory", __FILE__, __LINE__, msg);
+  }
+  return p;
 }
 
 void Runtime::drop(Replica* r)
@@ -116,7 +162,46 @@ void Runtime::drop(Replica* r)
     XK_HIP(hipFree(r->dev));
   }
   cache_used -= r->bytes;
+  if (r->info_slot >= 0) free_info.push_back(r->info_slot);
   delete r;
+}
+
+// Retired mirrors, at a public synchronisation point (no call is running and
+// every stream has drained).
+void Runtime::free_retired()
+{
+  for (Replica* r : retired) {
+    if (r->dev) XK_HIP(hipFree(r->dev));
+    cache_used -= r->bytes;
+    if (r->info_slot >= 0) free_info.push_back(r->info_slot);
+    delete r;
+  }
+  retired.clear();
+}
+
+int Runtime::take_info_slot()
+{
+  if (!free_info.empty()) {
+    const int slot = free_info.back();
+    free_info.pop_back();
+    return slot;
+  }
+  if (next_info == info_slots) {
+    // every slot is taken: grow both arrays (pending copies into the old host
+    // array and kernels writing the old device array finish first)
+    sync_all();
+    const int n2 = info_slots * 2;
+    int *d2 = nullptr, *h2 = nullptr;
+    XK_HIP(hipMalloc(&d2, sizeof(int) * n2));
+    XK_HIP(hipHostMalloc(&h2, sizeof(int) * n2, hipHostMallocDefault));
+    for (int i = 0; i < n2; ++i) h2[i] = i < info_slots ? h_info[i] : INT_MAX;
+    XK_HIP(hipFree(d_info));
+    XK_HIP(hipHostFree(h_info));
+    d_info = d2;
+    h_info = h2;
+    info_slots = n2;
+  }
+  return next_info++;
 }
 
 Replica* Runtime::replica(const void* host, int M, int N, long ld, int elt, int nb, std::vector<Replica*>& keep)
@@ -130,11 +215,20 @@ Replica* Runtime::replica(const void* host, int M, int N, long ld, int elt, int 
       return r;
     }
     // The reference keeps the stale descriptor (blas/xkblas.c:416-421); here
-    // a pointer reused with new dimensions gets a fresh mirror.
+    // a pointer reused with new dimensions gets a fresh mirror, read from
+    // host memory once the old one's dirty tiles are back there.
     writeback(r, 0, 0, r->M, r->N, 0);
     XK_HIP(hipStreamSynchronize(s_d2h));
     drain_stages();
-    drop(r);
+    if (std::find(keep.begin(), keep.end(), r) != keep.end()) {
+      // an operand of this call with other dimensions (e.g. dgemm with B
+      // aliasing A): the old mirror stays alive, unkeyed, until the call's
+      // work has drained
+      map.erase(host);
+      retired.push_back(r);
+    } else {
+      drop(r);
+    }
   }
   Replica* r = new Replica();
   r->host = host;
@@ -146,7 +240,7 @@ Replica* Runtime::replica(const void* host, int M, int N, long ld, int elt, int 
   r->bytes = (size_t)r->ldd * (size_t)std::max(N, 1) * elt;
   r->state.assign((size_t)r->mt * r->nt, HOST_VALID);
   evict_for(r->bytes, keep);
-  XK_HIP(hipMalloc(&r->dev, r->bytes));
+  r->dev = (char*)device_alloc(r->bytes, keep);
   cache_used += r->bytes;
   r->lru = ++clock;
   map[host] = r;
@@ -436,9 +530,10 @@ void Runtime::init_device()
   XK_HIP(hipEventCreateWithFlags(&ev_d2h, hipEventDisableTiming));
   for (int b = 0; b < NSTAGE; ++b) XK_HIP(hipEventCreateWithFlags(&stage_ev[b], hipEventDisableTiming));
   staging = !(getenv("XKBLAS_STAGING") && atoi(getenv("XKBLAS_STAGING")) == 0);
-  XK_HIP(hipMalloc(&d_info, sizeof(int) * INFO_SLOTS));
-  XK_HIP(hipHostMalloc(&h_info, sizeof(int) * INFO_SLOTS, hipHostMallocDefault));
-  for (int i = 0; i < INFO_SLOTS; ++i) h_info[i] = INT_MAX;
+  info_slots = INFO_SLOTS;
+  XK_HIP(hipMalloc(&d_info, sizeof(int) * info_slots));
+  XK_HIP(hipHostMalloc(&h_info, sizeof(int) * info_slots, hipHostMallocDefault));
+  for (int i = 0; i < info_slots; ++i) h_info[i] = INT_MAX;
   size_t freeb = 0, total = 0;
   XK_HIP(hipMemGetInfo(&freeb, &total));
   // reference default: 98 % of free memory minus a reserve
@@ -664,6 +759,7 @@ int xkblas_finalize(void)
   if (r.refcount > 0) r.refcount--;
   if (r.refcount == 0 && r.device_ready) {
     r.sync_all();
+    r.free_retired();
     r.trace_stop();
     if (getenv("XKBLAS_VERBOSE") && atoi(getenv("XKBLAS_VERBOSE")) > 0) {
       fprintf(stderr, "[xkblas_amd] h2d %.3f GB  d2h %.3f GB  evictions %ld  mirrors %zu (%.3f GB)\n",
@@ -683,7 +779,7 @@ int xkblas_sync(void)
   ensure_init();
   Runtime& r = rt();
   r.sync_all();
-  // publish POTRF info values copied at the end of each factorization
+  r.free_retired();
   return 0;
 }
 
@@ -738,7 +834,9 @@ uint64_t xkblas_elapsedns(void)
 double xkblas_elapsedtime(void) { return (double)xkblas_elapsedns() * 1e-9; }
 
 // Tile-size choice (blas/xkblas.c:2239-2530): the set NB wins; otherwise the
-// reference's per-kernel heuristic for one GPU.
+// reference's per-kernel rule for one GPU (its default-mapping branch,
+// blas/xkblas.c:2437-2521, with ngpus = 1).  POTRF has no kernel id there and
+// takes the default branch.
 size_t xkblas_auto_tilesize(void* ctx, xkblas_kernel_t kernel, size_t M, size_t N, size_t K)
 {
   (void)ctx;
@@ -746,34 +844,132 @@ size_t xkblas_auto_tilesize(void* ctx, xkblas_kernel_t kernel, size_t M, size_t 
   size_t nb = rt().nb;
   if (nb) return nb;
   if (N > M) M = N;
+  const size_t a64 = ~(size_t)63;
   switch (kernel) {
     case KERN_SYRK:
     case KERN_HERK: {
+      // M / 2 rounded down to a multiple of 2048 when that large; M itself
+      // when M / 2 is at most 1024 (the rule repeated with factor 1)
       size_t fact = 2;
-      nb = M / fact;
-      if (nb >= 2048) nb &= ~(size_t)2047;
-      if (nb <= 1024) nb = M;
+      for (;;) {
+        nb = M / fact;
+        if (nb >= 2048) nb &= ~(size_t)2047;
+        if (nb <= 1024 && fact == 2) {
+          fact = 1;
+          continue;
+        }
+        break;
+      }
       if (nb > 4096) nb = M / (2 * fact);
       if (nb < 512) nb = 512;
       break;
     }
+    case KERN_SYR2K:
+    case KERN_HER2K: {
+      // M / 4, M / 3, M / 2 (64-aligned): the first factor whose block is
+      // above 2048, down to factor 2
+      size_t fact = 4;
+      for (;;) {
+        nb = (M / fact + 63) & a64;
+        if (nb <= 2048 && fact > 2) {
+          --fact;
+          continue;
+        }
+        break;
+      }
+      if (nb > 4096) nb = M / (2 * fact);
+      if (nb < 512) nb = 512;
+      break;
+    }
+    case KERN_SYMM:
+    case KERN_HEMM:
+      nb = M & ~(size_t)1023;
+      if (nb < 1024) nb = 1024;
+      if (nb < 4096) nb *= 2;
+      break;
     case KERN_TRSM:
     case KERN_TRMM:
-      nb = M / 2;
-      nb = (nb + 63) & ~(size_t)63;
+      nb = (M / 2 + 63) & a64;
       if (nb < 2048) nb = 2048;
       break;
     case KERN_GEMM:
     case KERN_GEMMT:
-    default:
       nb = M;
-      if (M > 16384 && nb < 2048) nb = 2048;
+      if (M > 1 && M > 16384 && nb < 2048) nb = 2048;
       if (nb > 4096) nb = M / 2;
       if (nb < 512) nb = 512;
-      nb = (nb + 63) & ~(size_t)63;
+      nb = (nb + 63) & a64;
+      break;
+    default:  // SWAP, COPYSCALE and this build's POTRF
+      nb = M;
+      if (nb >= 4096) nb = M / 2;
+      if (nb < 1024) nb = 1024;
+      nb = (nb + 63) & a64;
       break;
   }
   return nb;
+}
+
+// Device properties (blas/xkblas.h:221-256, kaapi/kaapi_offload.c:575-611):
+// the reference's HIP plugin knows MI250/MI300A by name and reports any other
+// device as XKBLAS_ARCH_UNKNOWN; this build names the MI355X (gfx950)
+// XKBLAS_ARCH_MI355X.  No unified memory is used (has_unified 0), as in a
+// reference build without KAAPI_UNIFIED.
+int xkblas_get_device_properties(xkblas_dev_prop_t* prop)
+{
+  if (!prop) return 22; /* EINVAL */
+  prop->arch = XKBLAS_ARCH_UNKNOWN;
+  prop->has_unified = 0;
+  prop->device_count = xkblas_get_devicecount();
+  if (prop->device_count > 0) {
+    hipDeviceProp_t dp;
+    if (hipGetDeviceProperties(&dp, 0) == hipSuccess && strncmp(dp.gcnArchName, "gfx950", 6) == 0)
+      prop->arch = XKBLAS_ARCH_MI355X;
+  }
+  return 0;
+}
+
+// Unified memory is not used by this build: the reference returns EINVAL
+// when the devices cannot use it (kaapi/kaapi_offload.c:631-640).
+int xkblas_force_set_unified(void) { return 22; /* EINVAL */ }
+
+// Host registration (blas/xkblas.c:668-774).  The reference pins existing
+// host memory through the GPU driver; here copies from pageable memory go
+// through the runtime's ring of pinned staging buffers at pinned-copy rates,
+// so registration has nothing to do: requests complete at once, with status
+// 0, and handles are distinct non-zero numbers.
+static std::atomic<uint64_t> g_reg_handle{0};
+uint64_t xkblas_register_memory_async(void* ptr, size_t sz)
+{
+  (void)ptr;
+  (void)sz;
+  return ++g_reg_handle;
+}
+uint64_t xkblas_unregister_memory_async(void* ptr, size_t sz)
+{
+  (void)ptr;
+  (void)sz;
+  return ++g_reg_handle;
+}
+int xkblas_register_memory_test(uint64_t handle)
+{
+  (void)handle;
+  return 1;
+}
+int xkblas_register_memory_wait(uint64_t handle)
+{
+  (void)handle;
+  return 0;
+}
+int xkblas_register_memory_waitall(void) { return 0; }
+int xkblas_register_memory(void* ptr, size_t sz)
+{
+  return xkblas_register_memory_wait(xkblas_register_memory_async(ptr, sz));
+}
+int xkblas_unregister_memory(void* ptr, size_t sz)
+{
+  xkblas_unregister_memory_async(ptr, sz);
+  return xkblas_register_memory_waitall();
 }
 
 void* xkblas_malloc(size_t sz)
@@ -846,6 +1042,7 @@ int xkblas_memory_invalidate(const void* A)
   auto it = r.map.find(A);
   if (it == r.map.end()) return 0;
   r.sync_all();
+  r.free_retired();
   r.drop(it->second);
   return 0;
 }
@@ -856,6 +1053,7 @@ int xkblas_memory_invalidate_caches(void)
   Runtime& r = rt();
   if (!r.device_ready) return 0;
   r.sync_all();
+  r.free_retired();
   std::vector<Replica*> all;
   for (auto& kv : r.map) all.push_back(kv.second);
   for (Replica* rep : all) r.drop(rep);
@@ -912,7 +1110,7 @@ int gemm_async(const char* fn, int transA, int transB, int M, int N, int K, cons
   ensure_init();
   Runtime& r = rt();
   const int nb = (int)xkblas_auto_tilesize(nullptr, KERN_GEMM, M, N, K);
-  std::vector<Replica*> keep;
+  std::vector<Replica*>& keep = r.begin_call();
   const bool use_ab = K > 0 && *alpha != T(0);
   Replica* rA = use_ab ? r.replica(A, Am, An, LDA, sizeof(T), nb, keep) : nullptr;
   Replica* rB = use_ab ? r.replica(B, Bm, Bn, LDB, sizeof(T), nb, keep) : nullptr;
@@ -951,7 +1149,7 @@ int syrk_async(const char* fn, int uplo, int trans, int N, int K, const T* alpha
   ensure_init();
   Runtime& r = rt();
   const int nb = (int)xkblas_auto_tilesize(nullptr, KERN_SYRK, N, N, K);
-  std::vector<Replica*> keep;
+  std::vector<Replica*>& keep = r.begin_call();
   const bool use_a = K > 0 && *alpha != T(0);
   Replica* rA = use_a ? r.replica(A, Am, An, LDA, sizeof(T), nb, keep) : nullptr;
   Replica* rC = r.replica(C, N, N, LDC, sizeof(T), nb, keep);
@@ -987,7 +1185,7 @@ int trsm_async(const char* fn, int side, int uplo, int transA, int diag, int N, 
   ensure_init();
   Runtime& r = rt();
   const int nb = (int)xkblas_auto_tilesize(nullptr, KERN_TRSM, N, NRHS, NA);
-  std::vector<Replica*> keep;
+  std::vector<Replica*>& keep = r.begin_call();
   Replica* rA = r.replica(A, NA, NA, LDA, sizeof(T), nb, keep);
   Replica* rB = r.replica(B, N, NRHS, LDB, sizeof(T), nb, keep);
   r.fetch(rA, 0, 0, NA, NA);
@@ -1014,11 +1212,11 @@ int potrf_async(const char* fn, int uplo, int N, T* A, int LDA)
   Lock l(g_mu);
   ensure_init();
   Runtime& r = rt();
-  std::vector<Replica*> keep;
+  std::vector<Replica*>& keep = r.begin_call();
   if (N == 0) return 0;
   const int nb = (int)xkblas_auto_tilesize(nullptr, KERN_POTRF, N, N, N);
   Replica* rA = r.replica(A, N, N, LDA, sizeof(T), nb, keep);
-  if (rA->info_slot < 0) rA->info_slot = r.next_info++ % Runtime::INFO_SLOTS;
+  if (rA->info_slot < 0) rA->info_slot = r.take_info_slot();
   r.fetch(rA, 0, 0, N, N);
   r.before_compute();
   Exec x = r.exec();
@@ -1054,7 +1252,7 @@ int gemmt_async(const char* fn, int uplo, int transA, int transB, int N, int K, 
   ensure_init();
   Runtime& r = rt();
   const int nb = (int)xkblas_auto_tilesize(nullptr, KERN_GEMMT, N, N, K);
-  std::vector<Replica*> keep;
+  std::vector<Replica*>& keep = r.begin_call();
   const bool use_ab = K > 0 && *alpha != T(0);
   Replica* rA = use_ab ? r.replica(A, Am, An, LDA, sizeof(T), nb, keep) : nullptr;
   Replica* rB = use_ab ? r.replica(B, Bm, Bn, LDB, sizeof(T), nb, keep) : nullptr;
@@ -1093,7 +1291,7 @@ int syr2k_async(const char* fn, int uplo, int trans, int N, int K, const T* alph
   ensure_init();
   Runtime& r = rt();
   const int nb = (int)xkblas_auto_tilesize(nullptr, KERN_SYR2K, N, N, K);
-  std::vector<Replica*> keep;
+  std::vector<Replica*>& keep = r.begin_call();
   const bool use_ab = K > 0 && *alpha != T(0);
   Replica* rA = use_ab ? r.replica(A, Am, An, LDA, sizeof(T), nb, keep) : nullptr;
   Replica* rB = use_ab ? r.replica(B, Am, An, LDB, sizeof(T), nb, keep) : nullptr;
@@ -1131,7 +1329,7 @@ int symm_async(const char* fn, int side, int uplo, int M, int N, const T* alpha,
   ensure_init();
   Runtime& r = rt();
   const int nb = (int)xkblas_auto_tilesize(nullptr, KERN_SYMM, M, N, Am);
-  std::vector<Replica*> keep;
+  std::vector<Replica*>& keep = r.begin_call();
   const bool use_ab = *alpha != T(0);
   Replica* rA = use_ab ? r.replica(A, Am, Am, LDA, sizeof(T), nb, keep) : nullptr;
   Replica* rB = use_ab ? r.replica(B, M, N, LDB, sizeof(T), nb, keep) : nullptr;
@@ -1170,7 +1368,7 @@ int trmm_async(const char* fn, int side, int uplo, int transA, int diag, int N, 
   ensure_init();
   Runtime& r = rt();
   const int nb = (int)xkblas_auto_tilesize(nullptr, KERN_TRMM, N, NRHS, NA);
-  std::vector<Replica*> keep;
+  std::vector<Replica*>& keep = r.begin_call();
   Replica* rA = r.replica(A, NA, NA, LDA, sizeof(T), nb, keep);
   Replica* rB = r.replica(B, N, NRHS, LDB, sizeof(T), nb, keep);
   r.fetch(rA, 0, 0, NA, NA);
@@ -1209,7 +1407,7 @@ int cgemm_async(const char* fn, int transA, int transB, int M, int N, int K, con
   Runtime& r = rt();
   const int nb = (int)xkblas_auto_tilesize(nullptr, KERN_GEMM, M, N, K);
   const int elt = 2 * sizeof(T);
-  std::vector<Replica*> keep;
+  std::vector<Replica*>& keep = r.begin_call();
   const bool use_ab = K > 0 && !a0;
   Replica* rA = use_ab ? r.replica(A, Am, An, LDA, elt, nb, keep) : nullptr;
   Replica* rB = use_ab ? r.replica(B, Bm, Bn, LDB, elt, nb, keep) : nullptr;
@@ -1247,7 +1445,7 @@ int cherk_async(const char* fn, int uplo, int trans, int N, int K, const T* alph
   Runtime& r = rt();
   const int nb = (int)xkblas_auto_tilesize(nullptr, KERN_HERK, N, N, K);
   const int elt = 2 * sizeof(T);
-  std::vector<Replica*> keep;
+  std::vector<Replica*>& keep = r.begin_call();
   const bool use_a = K > 0 && *alpha != T(0);
   Replica* rA = use_a ? r.replica(A, Am, An, LDA, elt, nb, keep) : nullptr;
   Replica* rC = r.replica(C, N, N, LDC, elt, nb, keep);
@@ -1284,7 +1482,7 @@ int chemm_async(const char* fn, int herm, int side, int uplo, int M, int N, cons
   Runtime& r = rt();
   const int nb = (int)xkblas_auto_tilesize(nullptr, KERN_HEMM, M, N, Am);
   const int elt = 2 * sizeof(T);
-  std::vector<Replica*> keep;
+  std::vector<Replica*>& keep = r.begin_call();
   Replica* rA = !a0 ? r.replica(A, Am, Am, LDA, elt, nb, keep) : nullptr;
   Replica* rB = !a0 ? r.replica(B, M, N, LDB, elt, nb, keep) : nullptr;
   Replica* rC = r.replica(C, M, N, LDC, elt, nb, keep);
@@ -1326,7 +1524,7 @@ int csyr2k_async(const char* fn, int herm, int two, int uplo, int trans, int N, 
   Runtime& r = rt();
   const int nb = (int)xkblas_auto_tilesize(nullptr, two ? KERN_SYR2K : KERN_SYRK, N, N, K);
   const int elt = 2 * sizeof(T);
-  std::vector<Replica*> keep;
+  std::vector<Replica*>& keep = r.begin_call();
   const bool use_ab = K > 0 && !a0;
   Replica* rA = use_ab ? r.replica(A, Am, An, LDA, elt, nb, keep) : nullptr;
   Replica* rB = use_ab && two ? r.replica(B, Am, An, LDB, elt, nb, keep) : nullptr;
@@ -1378,7 +1576,7 @@ int cgemmt_async(const char* fn, int uplo, int transA, int transB, int N, int K,
   Runtime& r = rt();
   const int nb = (int)xkblas_auto_tilesize(nullptr, KERN_GEMMT, N, N, K);
   const int elt = 2 * sizeof(T);
-  std::vector<Replica*> keep;
+  std::vector<Replica*>& keep = r.begin_call();
   const bool use_ab = K > 0 && !a0;
   Replica* rA = use_ab ? r.replica(A, Am, An, LDA, elt, nb, keep) : nullptr;
   Replica* rB = use_ab ? r.replica(B, Bm, Bn, LDB, elt, nb, keep) : nullptr;
@@ -1418,7 +1616,7 @@ int ctrmm_async(const char* fn, int side, int uplo, int transA, int diag, int N,
   Runtime& r = rt();
   const int nb = (int)xkblas_auto_tilesize(nullptr, KERN_TRMM, N, NRHS, NA);
   const int elt = 2 * sizeof(T);
-  std::vector<Replica*> keep;
+  std::vector<Replica*>& keep = r.begin_call();
   Replica* rA = r.replica(A, NA, NA, LDA, elt, nb, keep);
   Replica* rB = r.replica(B, N, NRHS, LDB, elt, nb, keep);
   r.fetch(rA, 0, 0, NA, NA);
@@ -1453,7 +1651,7 @@ int ctrsm_async(const char* fn, int side, int uplo, int transA, int diag, int N,
   Runtime& r = rt();
   const int nb = (int)xkblas_auto_tilesize(nullptr, KERN_TRSM, N, NRHS, NA);
   const int elt = 2 * sizeof(T);
-  std::vector<Replica*> keep;
+  std::vector<Replica*>& keep = r.begin_call();
   Replica* rA = r.replica(A, NA, NA, LDA, elt, nb, keep);
   Replica* rB = r.replica(B, N, NRHS, LDB, elt, nb, keep);
   r.fetch(rA, 0, 0, NA, NA);
@@ -1478,11 +1676,11 @@ int cpotrf_async(const char* fn, int uplo, int N, T* A, int LDA)
   Lock l(g_mu);
   ensure_init();
   Runtime& r = rt();
-  std::vector<Replica*> keep;
+  std::vector<Replica*>& keep = r.begin_call();
   if (N == 0) return 0;
   const int nb = (int)xkblas_auto_tilesize(nullptr, KERN_POTRF, N, N, N);
   Replica* rA = r.replica(A, N, N, LDA, 2 * sizeof(T), nb, keep);
-  if (rA->info_slot < 0) rA->info_slot = r.next_info++ % Runtime::INFO_SLOTS;
+  if (rA->info_slot < 0) rA->info_slot = r.take_info_slot();
   r.fetch(rA, 0, 0, N, N);
   r.before_compute();
   Exec x = r.exec();
@@ -1709,7 +1907,7 @@ int xkblas_dposv_ir(int uplo, int N, int NRHS, const double* A, int LDA, const d
   ensure_init();
   Runtime& r = rt();
   const int nb = (int)xkblas_auto_tilesize(nullptr, KERN_POTRF, N, N, N);
-  std::vector<Replica*> keep;
+  std::vector<Replica*>& keep = r.begin_call();
   Replica* rA = r.replica(A, N, N, LDA, sizeof(double), nb, keep);
   Replica* rB = r.replica(B, N, NRHS, LDB, sizeof(double), nb, keep);
   Replica* rX = r.replica(X, N, NRHS, LDX, sizeof(double), nb, keep);

@@ -72,9 +72,20 @@ struct Runtime {
   void* scratch_ptr[NSCRATCH] = {};
   size_t scratch_size[NSCRATCH] = {};
 
+  // POTRF info words, one slot per factored mirror (freed with it; the
+  // arrays grow when every slot is taken)
   int* d_info = nullptr;
   int* h_info = nullptr;
-  int next_info = 0;
+  int info_slots = 0, next_info = 0;
+  std::vector<int> free_info;
+
+  // The replicas of the current call (never evicted or dropped while it
+  // runs): an API entry clears it, replica() appends.
+  std::vector<Replica*> call_keep;
+  // Mirrors that an operand of the running call still uses but that lost
+  // their pointer key (an aliased operand of other dimensions): freed once
+  // every stream has drained (sync_all).
+  std::vector<Replica*> retired;
 
   // Pageable host memory goes through a ring of pinned staging buffers: a
   // host-to-device chunk is packed by the copy threads, then sent; a
@@ -117,8 +128,12 @@ struct Runtime {
   void fetch(Replica* r, int r0, int c0, int m, int n);
   void mark_written(Replica* r, int r0, int c0, int m, int n, int uplo);
   void writeback(Replica* r, int r0, int c0, int m, int n, int uplo);
-  void evict_for(size_t need, const std::vector<Replica*>& keep);
+  std::vector<Replica*>& begin_call();
+  bool evict_for(size_t need, const std::vector<Replica*>& keep, bool strict = true);
+  void* device_alloc(size_t bytes, const std::vector<Replica*>& keep);
   void drop(Replica* r);
+  void free_retired();
+  int take_info_slot();
   void before_compute();
   bool host_pinned(const void* p, size_t bytes) const;
   void copy_h2d(char* dst, size_t dpitch, const char* src, size_t spitch, size_t width, size_t height);

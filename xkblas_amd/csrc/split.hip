@@ -1334,6 +1334,9 @@ template <>
 void split_gemm<double>(const Exec& x, int transA, int transB, int m, int n, int k, double alpha, const double* A,
                         long lda, const double* B, long ldb, double beta, double* C, long ldc, int tri)
 {
+  // real operands: ConjTrans is Trans (the SYRK shortcut below compares them)
+  if (transA == ConjTrans) transA = Trans;
+  if (transB == ConjTrans) transB = Trans;
   if (x.mode != 3 || k <= F8_KMAX) {
     split_gemm_chunk(x, transA, transB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, tri);
     return;
@@ -1361,7 +1364,7 @@ static void split_gemm_chunk(const Exec& x, int transA, int transB, int m, int n
   const long mpad = rup(m, MG_PAD), npad = rup(n, MG_PAD), kpad = rup(k, 512);
   static_assert(MG_PAD % DM_BM == 0 && MG_PAD % DM_BN == 0, "padding must cover the tiles");
   XK_CHECK(!fp8 || kpad <= F8_KMAX, "FP8 residue product beyond its exact inner dimension");
-  const bool same = (A == B) && (lda == ldb) && (transA != transB) && m == n;  // SYRK: op(B) = op(A)^T
+  const bool same = (A == B) && (lda == ldb) && ((transA == NoTrans) != (transB == NoTrans)) && m == n;  // op(B) = op(A)^T
   const size_t esz = fp8 ? 1 : sizeof(__bf16);
 
   const size_t rsz = fp8 ? sizeof(int8_t) : sizeof(int16_t);  // residue element
