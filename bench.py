@@ -29,10 +29,24 @@ sys.path.insert(0, ROOT)
 FP64_PEAK = 78.6  # TFLOP/s, MI355X dense FP64 MFMA (AMD datasheet)
 BF16_PEAK = 2500.0
 FP8_PEAK = 5000.0
-# HBM bytes per launch of the BF16 residue GEMM at 16384^3 (13 planes):
-# rocprofv3 FETCH_SIZE x 2 + WRITE_SIZE x 1 (factors from a 1 GiB device copy),
-# profiles/split_dgemm16384_hbm_traffic.md
-BF16_GEMM_TRAFFIC_16384 = 167.5e9 + 6.98e9
+
+
+def profiled_traffic(section="split-BF16"):
+    """HBM bytes per launch of the residue GEMM at 16384^3, from the committed
+    rocprofv3 summary (FETCH_SIZE x 2 + WRITE_SIZE x 1, factors from a 1 GiB
+    device copy): the 'Residue GEMM per launch: **R GB read + W GB written'
+    line of the given section; None when absent."""
+    import re
+    path = os.path.join(ROOT, "profiles", "split_dgemm16384_hbm_traffic.md")
+    try:
+        text = open(path).read()
+    except OSError:
+        return None
+    part = text.split("### " + section, 1)
+    if len(part) < 2:
+        return None
+    m = re.search(r"Residue GEMM per launch: \*\*([0-9.]+) GB read \+ ([0-9.]+) GB written", part[1])
+    return None if m is None else (float(m.group(1)) + float(m.group(2))) * 1e9
 
 
 def make_inputs(n, seed):
@@ -168,8 +182,23 @@ def kernel_stats(xk, cls):
     return cnt, ms, fl
 
 
-def cpu_baseline(budget_s=12.0):
-    """Host BLAS dgemm_ (MKL) on a 4096^3 sample, all host cores."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(budget_s=10.0):
+    """The reference's CPU path: the host BLAS (MKL) through the oracle's
+    restatement of the reference's dlopen path, all host cores.  `value` is
+    dgemm_ NN 4096^3 (median of the runs that fit `budget_s`); beside it
+    dpotrf_ (lower, order 8192, A = G G^T + n I), dsyrk_ (lower, n = k =
+    4096) and dtrsm_ (left, lower, non-unit, 4096 x 4096), median of 3 each."""
     import oracle
     path = oracle.hostblas()
     if path is None:
@@ -187,10 +216,45 @@ def cpu_baseline(budget_s=12.0):
         oracle.host_dgemm(111, 111, 1.5, A, B, -0.5, C)
         times.append(time.perf_counter() - t0)
     med = float(np.median(times))
+
+    def med3(fn):
+        ts = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t0)
+        return float(np.median(ts))
+
+    others = {}
+    npo = 8192
+    G = r.uniform(-1, 1, (npo, npo))
+    S = np.asfortranarray(G @ G.T + npo * np.eye(npo))
+    del G
+    F = S.copy(order="F")
+
+    def potrf():
+        F[:] = S
+        assert oracle.host_dpotrf(122, F) == 0
+    t_copy = med3(lambda: F.__setitem__(slice(None), S))
+    t = max(med3(potrf) - t_copy, 1e-9)
+    others["dpotrf_8192"] = round((npo**3 / 3.0 + npo**2 / 2.0 + npo / 6.0) / t / 1e12, 4)
+    del S, F
+    t = med3(lambda: oracle.host_dsyrk(122, 111, 1.0, A, 0.0, C))
+    others["dsyrk_4096"] = round(float(n) * n * (n + 1) / t / 1e12, 4)
+    T = np.asfortranarray(np.tril(r.uniform(0, 1, (n, n))) / n + np.eye(n))
+    X = B.copy(order="F")
+
+    def trsm():
+        X[:] = B
+        oracle.host_dtrsm(141, 122, 111, 131, 1.0, T, X)
+    t = max(med3(trsm) - med3(lambda: X.__setitem__(slice(None), B)), 1e-9)
+    others["dtrsm_4096"] = round(float(n) ** 3 / t / 1e12, 4)
     cores = int(os.environ.get("MKL_NUM_THREADS", len(os.sched_getaffinity(0))))
     return {"value": round(2.0 * n**3 / med / 1e12, 4), "unit": "TFLOP/s", "cores": cores, "kind": "port",
-            "sample": "host BLAS dgemm_ (%s) NN 4096^3, median of %d runs, through the oracle's restatement of "
-                      "the reference's dlopen host-BLAS path" % (os.path.basename(path), len(times))}
+            "sample": "host BLAS dgemm_ NN 4096^3, median of %d runs, through the oracle's restatement of the "
+                      "reference's dlopen host-BLAS path" % len(times),
+            "cpu": cpu_model(), "host_blas": oracle.hostblas_version(path), "library": os.path.basename(path),
+            "other_routines_tflops": others}
 
 
 def copy_probe():
@@ -221,6 +285,8 @@ def main():
                     help="DPOTRF size for the split-BF16 vs FP64 comparison (0: skip)")
     ap.add_argument("--only-split", type=int, default=0, metavar="MODE",
                     help="profile helper: two split DGEMMs in math mode MODE (2 BF16, 3 FP8), no JSON")
+    ap.add_argument("--only-fp64", action="store_true",
+                    help="profile helper: two plain FP64-MFMA DGEMMs of the headline workload, no JSON")
     ap.add_argument("--only-potrf", type=int, default=-1, metavar="MODE",
                     help="profile helper: DPOTRF of order --potrf-split-n (block order --potrf-nb) in math mode "
                          "MODE, twice, no JSON")
@@ -276,6 +342,10 @@ def main():
 
     if args.only_split:
         time_dgemm(xk, A, B, C, 1, 1, args.only_split, args.nb)
+        L.xkblas_finalize()
+        return
+    if args.only_fp64:
+        time_dgemm(xk, A, B, C, 1, 1, 0, args.nb)
         L.xkblas_finalize()
         return
 
@@ -400,7 +470,8 @@ def main():
         "batched": batched,
         "roofline": {"bound": "mfma", "achieved": round(bf16_achieved, 1), "peak": BF16_PEAK, "unit": "TFLOP/s",
                      "frac": round(bf16_achieved / BF16_PEAK, 4),
-                     "traffic": BF16_GEMM_TRAFFIC_16384 if n == 16384 else None,
+                     "traffic": profiled_traffic() if n == 16384 else None,
+                     "traffic_source": "profiles/split_dgemm16384_hbm_traffic.md (rocprofv3 FETCH_SIZE/WRITE_SIZE)",
                      # A and B planes read once (2 bytes each), int16 residues written
                      "traffic_algorithmic": split["residue_gemms"] * (2 * 2 * n * n + 2 * n * n),
                      "kernel": "mod_gemm_w8_kernel (BF16 residue GEMM, 256x256 tiles, 8 waves, all moduli in one "
@@ -408,6 +479,13 @@ def main():
                      "kernel_ms": round(bf16_ms, 3),
                      "fp64_equivalent_frac_of_fp64_peak": round(flops / t_split / 1e12 / FP64_PEAK, 4)},
         "cpu_baseline": cpu,
+        "accuracy_target": {
+            "bound": 1e-14,
+            "dgemm": "split_bf16.rel_err.normwise_sampled: ||C - C_dd||_2 / ||C_dd||_2 over sampled entries, "
+                     "C_dd from double-double dot products",
+            "dpotrf_factor": "dpotrf.large_split_bf16.max_rel_diff_vs_fp64_factor: max|L - L_fp64| / max|L_fp64|",
+            "dpotrf_solve": "dpotrf.posv_ir.max_rel_diff_vs_fp64_route: max|X - X_fp64| / max|X_fp64|, X_fp64 from "
+                            "the FP64 factor and two DTRSM"},
         "library": cfg,
     }
     L.xkblas_finalize()

@@ -36,6 +36,14 @@ int  xkblas_trace_stop(void);
  * trailing update runs (default on). */
 void xkblas_set_lookahead(int on);
 
+/* A DGEMM/SGEMM whose op(A) or C still has tiles in host memory runs by row
+ * panels of C (one tile row each): panel p waits only for B and its own rows
+ * of op(A) and C, so the host-to-device copies of panel p+1 overlap the
+ * product of panel p, and xkblas_memory_coherent_async copies each panel's
+ * rows back as soon as they are final.  Same result bits as one launch.
+ * Default on; 0 turns it off (environment XKBLAS_PIPELINE). */
+void xkblas_set_pipeline(int on);
+
 /* Tile-cache introspection and capacity (bytes of HBM; the reference's
  * XKBLAS_CACHE_LIMIT, kaapi/kaapi_impl.c:311-317). */
 long xkblas_cache_bytes(void);

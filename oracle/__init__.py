@@ -134,6 +134,27 @@ def hostblas():
     return None
 
 
+def hostblas_version(path):
+    """Vendor and version string of the host BLAS at `path` (MKL's
+    MKL_Get_Version_String, OpenBLAS's openblas_get_config), or its file name."""
+    try:
+        L = ctypes.CDLL(path)
+    except OSError:
+        return os.path.basename(path)
+    f = getattr(L, "MKL_Get_Version_String", None)
+    if f is not None:
+        buf = ctypes.create_string_buffer(256)
+        f.argtypes = [ctypes.c_char_p, ctypes.c_int]
+        f.restype = None
+        f(buf, 256)
+        return buf.value.decode(errors="replace").strip()
+    f = getattr(L, "openblas_get_config", None)
+    if f is not None:
+        f.restype = ctypes.c_char_p
+        return "OpenBLAS " + f().decode(errors="replace").strip()
+    return os.path.basename(path)
+
+
 def host_dgemm(ta, tb, alpha, A, B, beta, C):
     M, N = C.shape
     K = A.shape[1] if ta == NoTrans else A.shape[0]

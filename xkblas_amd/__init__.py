@@ -104,6 +104,7 @@ SIGNATURES = {
     "xkblas_trace_stop": (_i, []),
     "xkblas_profile_get": (_i, [ctypes.c_char_p, _p, _p, _p]),
     "xkblas_set_lookahead": (None, [_i]),
+    "xkblas_set_pipeline": (None, [_i]),
     "xkblas_cache_bytes": (_l, []),
     "xkblas_cache_evictions": (_l, []),
     "xkblas_set_cache_limit": (None, [_sz]),

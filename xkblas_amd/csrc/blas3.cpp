@@ -144,7 +144,18 @@ static void potrf_diag(hipStream_t s, int uplo, int n, T* A, long lda, T* Tinv, 
   }
 }
 
-enum { WS_DIAG0 = 0, WS_DIAG1 = 1, WS_PANEL = 2, WS_INV = 3, WS_TRSM = 4, WS_TRSMW = 5 };
+enum { WS_TRSM = 4, WS_TRSMW = 5 };
+
+void potrf_scratch_caps(int n, int nb, size_t elt, size_t cap[4])
+{
+  nb = std::max(64, std::min(nb, std::max(n, 1)));
+  // the largest recursion node of a diagonal block of order <= nb is the top
+  // one: W holds its n1 x n2 block
+  const size_t h = nb > 64 ? (size_t)half64(nb) * (size_t)(nb - half64(nb)) : 0;
+  cap[WS_DIAG0] = cap[WS_DIAG1] = elt * std::max<size_t>(h, 1);
+  cap[WS_PANEL] = elt * std::max<size_t>((size_t)std::max(n - nb, 0) * nb, 1);
+  cap[WS_INV] = elt * 2 * (size_t)nb * nb;
+}
 
 // Trailing updates of POTRF: 1 (default) paired, 0 one per panel.
 // XKBLAS_POTRF_PAIR sets the initial value.
@@ -160,9 +171,12 @@ void dev_potrf(const Exec& x, int uplo, int n, T* A, long lda, int nb, int* info
   const int nt = (n + nb - 1) / nb;
   // inverses of two consecutive diagonal blocks (ping-pong for lookahead)
   T* inv = (T*)scratch(WS_INV, sizeof(T) * (size_t)2 * nb * nb);
-  hipEvent_t ev_diag, ev_look;
-  XK_HIP(hipEventCreateWithFlags(&ev_diag, hipEventDisableTiming));
-  XK_HIP(hipEventCreateWithFlags(&ev_look, hipEventDisableTiming));
+  hipEvent_t ev_diag = x.ev0, ev_look = x.ev1;
+  const bool own_events = !ev_diag || !ev_look;
+  if (own_events) {
+    XK_HIP(hipEventCreateWithFlags(&ev_diag, hipEventDisableTiming));
+    XK_HIP(hipEventCreateWithFlags(&ev_look, hipEventDisableTiming));
+  }
   const bool lookahead = x.s_diag != nullptr && nt > 1;
   hipStream_t sd = lookahead ? x.s_diag : x.s;
   // trailing updates of two panels at once (inner dimension 2 nb): longer
@@ -285,8 +299,10 @@ void dev_potrf(const Exec& x, int uplo, int n, T* A, long lda, int nb, int* info
                     info, (k + 1) * nb, WS_DIAG0, true);
   }
   if (split) split_cache_clear();
-  XK_HIP(hipEventDestroy(ev_diag));
-  XK_HIP(hipEventDestroy(ev_look));
+  if (own_events) {
+    XK_HIP(hipEventDestroy(ev_diag));
+    XK_HIP(hipEventDestroy(ev_look));
+  }
 }
 
 // Diagonal block order of the TRSM sweep for a triangle of order na.

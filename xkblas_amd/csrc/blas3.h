@@ -10,12 +10,23 @@ namespace xk {
 // Device scratch handed out by the runtime: slot-indexed, grow-only buffers.
 void* scratch(int slot, size_t bytes);
 
+// While an arena is set, scratch(slot, bytes) returns base + off[slot]
+// (bytes must not exceed cap[slot]) instead of a runtime buffer: the
+// allocation-free entry points carve their workspace out of a fixed array.
+void scratch_arena_set(char* base, const size_t* off, const size_t* cap, int nslot);
+void scratch_arena_clear();
+
 struct Exec {
   hipStream_t s;      // main compute stream
   hipStream_t s_diag; // side stream for latency-bound diagonal-block work
   int mode;           // xkblas_mode_math_t
   int nsplit;         // CRT moduli for split modes (0 = automatic)
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;  // POTRF's stream-ordering events (created per call when null)
 };
+
+// Scratch slots of dev_potrf, and the bytes each needs for order n, block nb.
+enum { WS_DIAG0 = 0, WS_DIAG1 = 1, WS_PANEL = 2, WS_INV = 3 };
+void potrf_scratch_caps(int n, int nb, size_t elt, size_t cap[4]);
 
 template <typename T>
 void dev_gemm(const Exec& x, int transA, int transB, int m, int n, int k, T alpha, const T* A, long lda,

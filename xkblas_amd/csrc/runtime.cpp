@@ -20,6 +20,7 @@
 //     312-316) unless XKBLAS_ERROR_RETURN=1, HIP errors abort.
 #include "blas3.h"
 #include "kernels.h"
+#include "split.h"
 #include "runtime.h"
 #include "../../include/xkblas.h"
 
@@ -59,10 +60,40 @@ static size_t env_size(const char* name, size_t dflt)
   return (size_t)strtoull(v, nullptr, 10);
 }
 
+// Workspace of an allocation-free entry point (a fixed device array carved
+// into per-slot regions); empty outside such a call.
+static struct {
+  char* base = nullptr;
+  size_t off[Runtime::NSCRATCH] = {}, cap[Runtime::NSCRATCH] = {};
+  int nslot = 0;
+} g_arena;
+
+void scratch_arena_set(char* base, const size_t* off, const size_t* cap, int nslot)
+{
+  XK_CHECK(nslot > 0 && nslot <= Runtime::NSCRATCH, "bad arena");
+  g_arena.base = base;
+  g_arena.nslot = nslot;
+  for (int i = 0; i < nslot; ++i) {
+    g_arena.off[i] = off[i];
+    g_arena.cap[i] = cap[i];
+  }
+}
+
+void scratch_arena_clear()
+{
+  g_arena.base = nullptr;
+  g_arena.nslot = 0;
+}
+
 void* scratch(int slot, size_t bytes)
 {
   Runtime& r = rt();
   XK_CHECK(slot >= 0 && slot < Runtime::NSCRATCH, "bad scratch slot");
+  if (g_arena.base) {
+    // an allocation-free call: its workspace layout covers every request
+    XK_CHECK(slot < g_arena.nslot && bytes <= g_arena.cap[slot], "workspace request beyond the arena's layout");
+    return g_arena.base + g_arena.off[slot];
+  }
   if (bytes == 0) bytes = 256;
   if (r.scratch_size[slot] < bytes) {
     size_t sz = std::max(bytes + bytes / 4, (size_t)16 << 20);
@@ -157,6 +188,7 @@ void* Runtime::device_alloc(size_t bytes, const std::vector<Replica*>& keep)
 void Runtime::drop(Replica* r)
 {
   map.erase(r->host);
+  clear_panels(r);
   if (r->dev) {
     XK_HIP(hipDeviceSynchronize());
     XK_HIP(hipFree(r->dev));
@@ -171,6 +203,7 @@ void Runtime::drop(Replica* r)
 void Runtime::free_retired()
 {
   for (Replica* r : retired) {
+    clear_panels(r);
     if (r->dev) XK_HIP(hipFree(r->dev));
     cache_used -= r->bytes;
     if (r->info_slot >= 0) free_info.push_back(r->info_slot);
@@ -275,10 +308,29 @@ void Runtime::fetch(Replica* r, int r0, int c0, int m, int n)
   }
 }
 
-// Mark tiles as written on the device (host copy stale).
+bool Runtime::all_valid(const Replica* r, int r0, int c0, int m, int n) const
+{
+  if (m <= 0 || n <= 0) return true;
+  const int ti0 = r0 / r->nb, ti1 = (r0 + m - 1) / r->nb;
+  const int tj0 = c0 / r->nb, tj1 = (c0 + n - 1) / r->nb;
+  for (int tj = tj0; tj <= tj1; ++tj)
+    for (int ti = ti0; ti <= ti1; ++ti)
+      if (!(r->state[tj * r->mt + ti] & DEV_VALID)) return false;
+  return true;
+}
+
+void Runtime::clear_panels(Replica* r)
+{
+  for (auto& pe : r->panels) XK_HIP(hipEventDestroy(pe.ev));  // released once complete
+  r->panels.clear();
+}
+
+// Mark tiles as written on the device (host copy stale).  Any later write
+// orders after the whole previous computation, so per-panel events lapse.
 void Runtime::mark_written(Replica* r, int r0, int c0, int m, int n, int uplo)
 {
   if (m <= 0 || n <= 0) return;
+  clear_panels(r);
   const int ti0 = r0 / r->nb, ti1 = (r0 + m - 1) / r->nb;
   const int tj0 = c0 / r->nb, tj1 = (c0 + n - 1) / r->nb;
   for (int tj = tj0; tj <= tj1; ++tj)
@@ -294,8 +346,35 @@ void Runtime::mark_written(Replica* r, int r0, int c0, int m, int n, int uplo)
 void Runtime::writeback(Replica* r, int r0, int c0, int m, int n, int uplo)
 {
   if (m <= 0 || n <= 0) return;
+  if (!r->panels.empty()) {
+    // a product pipelined by row panels: each panel's rows go back once
+    // that panel is computed, under the computation of the next ones
+    int lo = 0;
+    const std::vector<Replica::PanelEvent> pv = r->panels;
+    for (const auto& pe : pv) {
+      const int a = std::max(lo, r0), b = std::min(pe.end, r0 + m);
+      if (a < b) {
+        XK_HIP(hipStreamWaitEvent(s_d2h, pe.ev, 0));
+        writeback_rows(r, a, c0, b - a, n, uplo);
+      }
+      lo = pe.end;
+    }
+    if (lo < r0 + m) {
+      XK_HIP(hipEventRecord(ev_comp, s_comp));
+      XK_HIP(hipStreamWaitEvent(s_d2h, ev_comp, 0));
+      writeback_rows(r, std::max(lo, r0), c0, r0 + m - std::max(lo, r0), n, uplo);
+    }
+    return;
+  }
   XK_HIP(hipEventRecord(ev_comp, s_comp));
   XK_HIP(hipStreamWaitEvent(s_d2h, ev_comp, 0));
+  writeback_rows(r, r0, c0, m, n, uplo);
+}
+
+// The D2H copies of writeback (the stream's ordering is set by the caller).
+void Runtime::writeback_rows(Replica* r, int r0, int c0, int m, int n, int uplo)
+{
+  if (m <= 0 || n <= 0) return;
   const int ti0 = r0 / r->nb, ti1 = (r0 + m - 1) / r->nb;
   const int tj0 = c0 / r->nb, tj1 = (c0 + n - 1) / r->nb;
   bool any = false;
@@ -512,6 +591,8 @@ Exec Runtime::exec() const
   x.s_diag = lookahead ? s_diag : nullptr;
   x.mode = mode;
   x.nsplit = nsplit;
+  x.ev0 = ev_pot0;
+  x.ev1 = ev_pot1;
   return x;
 }
 
@@ -528,6 +609,8 @@ void Runtime::init_device()
   XK_HIP(hipEventCreateWithFlags(&ev_comp, hipEventDisableTiming));
   XK_HIP(hipEventCreateWithFlags(&ev_h2d, hipEventDisableTiming));
   XK_HIP(hipEventCreateWithFlags(&ev_d2h, hipEventDisableTiming));
+  XK_HIP(hipEventCreateWithFlags(&ev_pot0, hipEventDisableTiming));
+  XK_HIP(hipEventCreateWithFlags(&ev_pot1, hipEventDisableTiming));
   for (int b = 0; b < NSTAGE; ++b) XK_HIP(hipEventCreateWithFlags(&stage_ev[b], hipEventDisableTiming));
   staging = !(getenv("XKBLAS_STAGING") && atoi(getenv("XKBLAS_STAGING")) == 0);
   info_slots = INFO_SLOTS;
@@ -1089,6 +1172,18 @@ int xkblas_potrf_info(const void* A)
 // ---------------------------------------------------------------- compute
 namespace {
 
+// Row-panel pipelining of products whose operands are still in host memory
+// (XKBLAS_PIPELINE=0 turns it off; xkblas_set_pipeline).
+int g_pipeline = -1;
+bool pipeline_enabled()
+{
+  if (g_pipeline < 0) {
+    const char* e = getenv("XKBLAS_PIPELINE");
+    g_pipeline = (e && *e) ? (atoi(e) != 0) : 1;
+  }
+  return g_pipeline != 0;
+}
+
 template <typename T>
 int gemm_async(const char* fn, int transA, int transB, int M, int N, int K, const T* alpha, const T* A, int LDA,
                const T* B, int LDB, const T* beta, T* C, int LDC)
@@ -1115,13 +1210,51 @@ int gemm_async(const char* fn, int transA, int transB, int M, int N, int K, cons
   Replica* rA = use_ab ? r.replica(A, Am, An, LDA, sizeof(T), nb, keep) : nullptr;
   Replica* rB = use_ab ? r.replica(B, Bm, Bn, LDB, sizeof(T), nb, keep) : nullptr;
   Replica* rC = r.replica(C, M, N, LDC, sizeof(T), nb, keep);
-  if (rA) r.fetch(rA, 0, 0, Am, An);
-  if (rB) r.fetch(rB, 0, 0, Bm, Bn);
-  // C is write-only when beta == 0 (reference access mode W, ztask_gemm.c:133-134)
-  if (*beta != T(0)) r.fetch(rC, 0, 0, M, N);
-  r.before_compute();
+  const bool readC = *beta != T(0);  // C is write-only when beta == 0 (ztask_gemm.c:133-134)
   Exec x = r.exec();
   const double flops = 2.0 * M * N * (double)K;
+  // Operands still in host memory: the product runs by row panels of C (one
+  // tile row each), and panel p waits only for B and its own rows of op(A)
+  // and C, so the copies of panel p+1 run under the product of panel p and
+  // each panel's rows go back (xkblas_memory_coherent_async) as soon as they
+  // are final.  Every element sees the same operands and inner order as in
+  // one launch (in the split modes the row scalings span the whole inner
+  // dimension and B's residue planes are computed once), so the result is
+  // the same bit for bit.
+  const int panels = (M + nb - 1) / nb;
+  const bool rowsA = transA == NoTrans;
+  const bool pipe = use_ab && rA != rC && rB != rC && panels > 1 && pipeline_enabled() &&
+                    (!r.all_valid(rA, 0, 0, Am, An) || (readC && !r.all_valid(rC, 0, 0, M, N)));
+  if (pipe) {
+    r.fetch(rB, 0, 0, Bm, Bn);
+    const bool split = sizeof(T) == 8 && split_enabled<T>(x);
+    if (split) split_cache_register((const double*)rB->dev, rB->ldd, N, K, transB != NoTrans);
+    std::vector<Replica::PanelEvent> pev;
+    r.prof_begin(sizeof(T) == 8 ? "gemm_f64" : "gemm_f32", flops);
+    for (int p = 0; p < panels; ++p) {
+      const int r0 = p * nb, rows = std::min(nb, M - r0);
+      if (rowsA) r.fetch(rA, r0, 0, rows, K);
+      else r.fetch(rA, 0, r0, K, rows);
+      if (readC) r.fetch(rC, r0, 0, rows, N);
+      r.before_compute();
+      const T* Ap = (const T*)rA->dev + (rowsA ? (long)r0 : (long)r0 * rA->ldd);
+      dev_gemm<T>(x, transA, transB, rows, N, K, *alpha, Ap, rA->ldd, (const T*)rB->dev, rB->ldd, *beta,
+                  (T*)rC->dev + r0, rC->ldd);
+      hipEvent_t ev;
+      XK_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+      XK_HIP(hipEventRecord(ev, x.s));
+      pev.push_back(Replica::PanelEvent{r0 + rows, ev});
+    }
+    r.prof_end();
+    if (split) split_cache_clear();
+    r.mark_written(rC, 0, 0, M, N, 0);
+    rC->panels = pev;
+    return 0;
+  }
+  if (rA) r.fetch(rA, 0, 0, Am, An);
+  if (rB) r.fetch(rB, 0, 0, Bm, Bn);
+  if (readC) r.fetch(rC, 0, 0, M, N);
+  r.before_compute();
   r.prof_begin(sizeof(T) == 8 ? "gemm_f64" : "gemm_f32", flops);
   dev_gemm<T>(x, transA, transB, M, N, use_ab ? K : 0, use_ab ? *alpha : T(0), rA ? (const T*)rA->dev : (const T*)rC->dev,
               rA ? rA->ldd : rC->ldd, rB ? (const T*)rB->dev : (const T*)rC->dev, rB ? rB->ldd : rC->ldd, *beta,
@@ -1972,6 +2105,12 @@ int xkblas_profile_get(const char* cls, long* launches, double* total_ms, double
   if (total_ms) *total_ms = it->second.ms;
   if (flops) *flops = it->second.flops;
   return 0;
+}
+
+void xkblas_set_pipeline(int on)
+{
+  Lock l(g_mu);
+  g_pipeline = on != 0;
 }
 
 void xkblas_set_lookahead(int on)

@@ -27,6 +27,14 @@ struct Replica {
   std::vector<uint8_t> state;  // per tile (column-major tile order)
   uint64_t lru = 0;
   int info_slot = -1;
+  // Rows [.., end) of the mirror are final once `ev` completes on the
+  // compute stream (a product pipelined by row panels): the copy back of a
+  // row panel waits only for its own event.  Empty otherwise.
+  struct PanelEvent {
+    int end;
+    hipEvent_t ev;
+  };
+  std::vector<PanelEvent> panels;
 };
 
 struct ProfEntry {
@@ -63,6 +71,7 @@ struct Runtime {
 
   hipStream_t s_comp = nullptr, s_diag = nullptr, s_h2d = nullptr, s_d2h = nullptr;
   hipEvent_t ev_comp = nullptr, ev_h2d = nullptr, ev_d2h = nullptr;
+  hipEvent_t ev_pot0 = nullptr, ev_pot1 = nullptr;  // POTRF's lookahead ordering
   bool pending_h2d = false, pending_d2h = false;
 
   std::unordered_map<const void*, Replica*> map;
@@ -128,6 +137,9 @@ struct Runtime {
   void fetch(Replica* r, int r0, int c0, int m, int n);
   void mark_written(Replica* r, int r0, int c0, int m, int n, int uplo);
   void writeback(Replica* r, int r0, int c0, int m, int n, int uplo);
+  void writeback_rows(Replica* r, int r0, int c0, int m, int n, int uplo);
+  bool all_valid(const Replica* r, int r0, int c0, int m, int n) const;
+  void clear_panels(Replica* r);
   std::vector<Replica*>& begin_call();
   bool evict_for(size_t need, const std::vector<Replica*>& keep, bool strict = true);
   void* device_alloc(size_t bytes, const std::vector<Replica*>& keep);
