@@ -484,8 +484,10 @@ def main():
             "dgemm": "split_bf16.rel_err.normwise_sampled: ||C - C_dd||_2 / ||C_dd||_2 over sampled entries, "
                      "C_dd from double-double dot products",
             "dpotrf_factor": "dpotrf.large_split_bf16.max_rel_diff_vs_fp64_factor: max|L - L_fp64| / max|L_fp64|",
-            "dpotrf_solve": "dpotrf.posv_ir.max_rel_diff_vs_fp64_route: max|X - X_fp64| / max|X_fp64|, X_fp64 from "
-                            "the FP64 factor and two DTRSM"},
+            "dpotrf_solve": "dpotrf.posv_ir.rel_err_vs_refined_solution: max|x - x*| / max|x*| per sampled "
+                            "right-hand side, x* the FP64 route's solution refined with an extended-precision "
+                            "residual (the FP64 route's own error beside it; max_rel_diff_vs_fp64_route = "
+                            "max|X - X_fp64| / max|X_fp64| mixes both routes' errors)"},
         "library": cfg,
     }
     L.xkblas_finalize()
@@ -778,6 +780,26 @@ def sampled_backward_error(S, X, B, samples=96, seed=11):
     return worst
 
 
+def refined_columns(S, F, B, Y, cols, block=1024):
+    """Reference solutions of A x = b for the columns `cols` of B: Y's
+    columns refined once with the residual b - A y evaluated in 80-bit long
+    double (error ~ 2^-64 sqrt(n) |A| |y|) and the correction solved with the
+    FP64 factor F (lower).  A is symmetric with its lower triangle in S."""
+    from scipy.linalg import solve_triangular
+    n = S.shape[0]
+    y = Y[:, cols].astype(np.longdouble)
+    r = B[:, cols].astype(np.longdouble)
+    for i0 in range(0, n, block):
+        i1 = min(n, i0 + block)
+        ii = np.arange(i0, i1)[:, None]
+        jj = np.arange(n)[None, :]
+        rows = np.where(jj <= ii, S[i0:i1, :], S[:, i0:i1].T)  # A[i0:i1, :]
+        r[i0:i1] -= rows.astype(np.longdouble) @ y
+    d = solve_triangular(F, np.asarray(r, dtype=np.float64), lower=True, check_finite=False)
+    d = solve_triangular(F, d, lower=True, trans=1, check_finite=False)
+    return np.asarray(y + d.astype(np.longdouble), dtype=np.float64)
+
+
 def bench_posv_ir(xk, S, F64, nb, p64_ms, psp_ms, nrhs=64, moduli=7):
     """xkblas_dposv_ir (split-BF16 factor with `moduli` CRT moduli, FP64
     residuals and corrections) on A = S (lower), B ~ U[-0.5,0.5] n x nrhs,
@@ -810,6 +832,11 @@ def bench_posv_ir(xk, S, F64, nb, p64_ms, psp_ms, nrhs=64, moduli=7):
     trsm_ms = xk.profile("trsm_f64")[1]
     xk.coherent(Y)
     xk.invalidate()
+    # forward errors of both routes on two sampled right-hand sides, against
+    # the FP64 route's solution refined in extended precision
+    cols = np.random.default_rng(17).choice(nrhs, 2, replace=False)
+    Xs = refined_columns(S, F64, B, Y, cols)
+    fwd = lambda Z: float(max(np.abs(Z[:, c] - Xs[:, q]).max() / np.abs(Xs[:, q]).max() for q, c in enumerate(cols)))
     fl = n**3 / 3.0 + 2.0 * n * n * nrhs
     return {"n": n, "nrhs": nrhs, "tile": nb, "factor": "split_bf16, %d moduli" % moduli, "info": info,
             "iterations": it, "ms": round(best, 3), "tflops_equivalent": round(fl / best / 1e9, 3),
@@ -818,7 +845,10 @@ def bench_posv_ir(xk, S, F64, nb, p64_ms, psp_ms, nrhs=64, moduli=7):
             "speedup_vs_split_accurate_route": round((psp_ms + trsm_ms) / best, 3),
             "sampled_backward_error": sampled_backward_error(S, X, B),
             "fp64_route_sampled_backward_error": sampled_backward_error(S, Y, B),
-            "max_rel_diff_vs_fp64_route": float(np.abs(X - Y).max() / np.abs(Y).max())}
+            "max_rel_diff_vs_fp64_route": float(np.abs(X - Y).max() / np.abs(Y).max()),
+            "rel_err_vs_refined_solution": fwd(X), "fp64_route_rel_err_vs_refined_solution": fwd(Y),
+            "refined_solution": "2 sampled right-hand sides: the FP64 route's x plus one correction from the "
+                                "residual b - A x in 80-bit long double on the host"}
 
 
 if __name__ == "__main__":

@@ -106,8 +106,13 @@ void xkblas_set_split_kernel(int variant);  /* BF16 residue GEMM: 0 = 8 waves of
                                              1 = 4 waves of 128x128 */
 int  xkblas_split_side_bits(int k, int nsplit);  /* split-BF16 bits per side t */
 /* split mode (XKBLAS_SPLIT_BF16_MATH / XKBLAS_SPLIT_FP8_MATH), inner dimension
- * k and requested count (0 = automatic): returns the bits per side t, stores
- * the number of CRT moduli and of residue GEMMs ("planes"); -1 for another mode */
+ * k and requested count (0 = automatic: the smallest count with t >= 46):
+ * returns the bits per side t, stores the number of CRT moduli and of residue
+ * GEMMs ("planes"); -1 for another mode.  Rows of op(A) and columns of op(B)
+ * are scaled by powers of two so that their 2-norms stay within 2^T, 2^(2T) <
+ * P / 2 (P the product of the moduli); t = floor(T - log2(k) / 2) is what a
+ * row of k equal-magnitude entries keeps of its largest entry (its rounding
+ * error is below 2^-t of it); rows with spread magnitudes keep more. */
 int  xkblas_split_info(int mode, int k, int nsplit, int* nmod, int* nplane);
 /* DPOTRF/SPOTRF trailing updates: 1 = paired (default; -1 selects it too):
  * every other step applies two panels as one rank-2NB update of the trailing

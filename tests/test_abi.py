@@ -124,35 +124,39 @@ def test_device_properties_and_registration_without_device():
 
 def test_split_plans():
     """Moduli and residue-GEMM counts of the split modes (host arithmetic).
-    The bits per side t satisfy k 2^(2t) < P / 2 for P the product of the
-    moduli, recomputed here from the moduli sets."""
+    The 2-norm budget per side is T = (log2 P - 1) / 2 for P the product of
+    the moduli (||A'_i|| ||B'_j|| <= 2^(2T) < P / 2 bounds every entry of
+    the integer product); a row of k equal-magnitude entries keeps
+    t = floor(T - log2(k) / 2) bits of its largest entry, and the default
+    count is the smallest with t >= 46.  All recomputed here from the sets."""
     import ctypes
     import math
     import xkblas_amd
     L = xkblas_amd.lib()
-    bf16 = [481, 479, 477, 475, 473, 469, 467, 463, 461, 457, 449, 443, 439]
+    bf16 = [511, 509, 507, 505, 503, 499, 493, 491, 487, 479, 473, 467, 463]
     fp8 = [32, 31, 29, 27, 25, 23, 19, 17, 13, 11, 7, 251, 241, 239, 233, 229, 227, 223, 211, 199]
     for mods in (bf16, fp8):  # pairwise coprime, product below 2^123
         assert all(math.gcd(a, b) == 1 for i, a in enumerate(mods) for b in mods[i + 1:])
         assert math.log2(math.prod(mods)) < 123
+    assert all(m % 2 == 1 for m in bf16) and max(bf16) <= 511
     assert all(m <= 33 for m in fp8[:11]) and all(m <= 255 for m in fp8)
     nm, npl = ctypes.c_int(), ctypes.c_int()
-    # residues within [-240, 240]: exact BF16 values, and 272 products (the
-    # longest run between two folds) plus a remainder stay below 2^24
-    assert max(bf16) <= 481 and 272 * 240**2 + 240 < 2**24
+    # residues within [-255, 255]: exact BF16 values, and 256 products (the
+    # run between two folds) plus a remainder stay below 2^24
+    assert 256 * 255**2 + 255 < 2**24
     for mode, mods, direct in ((2, bf16, 13), (3, fp8, 11)):
         for k in (64, 1000, 16384, 32768):
             t = L.xkblas_split_info(mode, k, 0, ctypes.byref(nm), ctypes.byref(npl))
             n = nm.value
             assert npl.value == min(n, direct) + 3 * max(0, n - direct)
-            bits = lambda c: math.log2(math.prod(mods[:c]))
-            want = lambda c: min(52, math.floor((bits(c) - 1 - math.ceil(math.log2(k))) / 2))
-            assert t == want(n) >= 48
-            assert n == 8 or want(n - 1) < 48  # smallest count reaching 48 bits
-    assert L.xkblas_split_info(3, 16384, 0, ctypes.byref(nm), ctypes.byref(npl)) == 51
-    assert (nm.value, npl.value) == (20, 38)
-    assert L.xkblas_split_info(2, 16384, 0, ctypes.byref(nm), ctypes.byref(npl)) == 50
-    assert (nm.value, npl.value) == (13, 13)
+            T = lambda c: (math.log2(math.prod(mods[:c])) - 1) / 2
+            want = lambda c: min(52, math.floor(T(c) - 1e-9 - math.log2(k) / 2))
+            assert t == want(n) >= 46
+            assert n == 8 or want(n - 1) < 46  # smallest count reaching 46 bits
+    assert L.xkblas_split_info(3, 16384, 0, ctypes.byref(nm), ctypes.byref(npl)) == 47
+    assert (nm.value, npl.value) == (19, 35)
+    assert L.xkblas_split_info(2, 16384, 0, ctypes.byref(nm), ctypes.byref(npl)) == 46
+    assert (nm.value, npl.value) == (12, 12)
     assert L.xkblas_split_info(3, 512, 12, ctypes.byref(nm), ctypes.byref(npl)) >= 8
     assert (nm.value, npl.value) == (12, 14)
     assert L.xkblas_split_info(0, 512, 0, None, None) == -1

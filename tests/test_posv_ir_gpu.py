@@ -3,19 +3,21 @@
 The stopping rule combines LAPACK dsposv's backward test (every column of
 r = B - A X has ||r||_inf <= sqrt(n) ||x||_inf ||A||_inf eps, eps = 2^-53)
 with a forward test on the corrections (restated and pinned on the CPU in
-oracle/posv_ir.py), so that X is within 1e-14 (normwise, max-abs relative) of
-the FP64 solution.  The tests check dsposv's test on the host from the
-returned X (FP64 NumPy residual, a factor 2 of slack for the different
-summation order), compare X with LAPACK's FP64 solve (numpy.linalg.solve) and
-with the oracle's tiled Cholesky + two tiled triangular solves at 1e-14, check
-that only the uplo triangle of A is read (the other one holds NaN), and force
-both fallbacks of dsposv: the correction limit (itermax 0) and a matrix that
-is not positive definite.
+oracle/posv_ir.py); the residuals sum short product runs without error, so X
+converges to within 1e-14 (max-abs relative) of the exact solution.  The
+tests check dsposv's test on the host from the returned X (FP64 NumPy
+residual, a factor 2 of slack for the different summation order), compare X
+at 1e-14 with the oracle's tiled Cholesky + two tiled triangular solves
+refined once with a long-double residual (the oracle's solve alone is only
+accurate to ~1e-14 at n = 4096), and with LAPACK's FP64 solve
+(numpy.linalg.solve) for the fallbacks, check that only the uplo triangle of
+A is read (the other one holds NaN), and force both fallbacks of dsposv: the
+correction limit (itermax 0) and a matrix that is not positive definite.
 """
 import numpy as np
 import pytest
 
-from helpers import rng, spd, uniform
+from helpers import rng, spd, uniform, refined_solution
 
 pytestmark = pytest.mark.gpu
 
@@ -76,14 +78,20 @@ def test_dposv_ir_converges(xk, orc, uplo, n, nrhs, nb):
         # first correction is large, the forward test needs a second
         assert 2 <= it <= 10, it
     assert np.all(_criterion(A, B, X) < 2.0)
+    # forward error against the oracle's solution refined once with an
+    # extended-precision residual (the oracle's tiled solve alone is itself
+    # only accurate to ~1e-14 at n = 4096)
     Xr = _oracle_solve(orc, uplo, A, B, nb)
-    assert np.abs(X - Xr).max() / np.abs(Xr).max() <= 1e-14
+    Xs = refined_solution(A, np.linalg.cholesky(A), B, Xr)
+    assert np.abs(Xr - Xs).max() / np.abs(Xs).max() <= 1e-13
+    assert np.abs(X - Xs).max() / np.abs(Xs).max() <= 1e-14
 
 
 @pytest.mark.parametrize("mode", [0, 2])
 def test_dposv_ir_fallback_after_itermax(xk, mode):
     """itermax 0: no correction allowed, so X comes from the factor of the
-    current mode (FP64 MFMA or 13-moduli split-BF16) and *iter = -1."""
+    current mode (FP64 MFMA or split-BF16 at its default count) and
+    *iter = -1."""
     L = xk.lib()
     r = rng(91 + mode)
     n, nrhs = 700, 9

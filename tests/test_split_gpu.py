@@ -76,6 +76,31 @@ def test_split_accuracy_vs_dd(xk, orc, split_mode, shape):
     assert np.all(err <= 2.0**-45 * bound)
 
 
+@pytest.mark.parametrize("ta", [111, 112])
+def test_split_extreme_row_scales(xk, orc, split_mode, ta):
+    """Row/column shifts at the edges of the 2-norm rule: rows of op(A) at
+    2^600 (squares overflow) and 2^-700 (squares underflow) take their shift
+    from the row maximum instead; rows of equal-magnitude entries are the
+    2-norm bound's worst case.  beta = 0 keeps every entry at its own scale
+    (2^-1000 .. 2^900); componentwise error within 2^-44 of |A| |B|."""
+    r = rng(404 + ta)
+    M, N, K = 96, 64, 3000
+    X = r.uniform(-0.5, 0.5, size=(M, K))
+    X[:12] = np.where(X[:12] < 0, -0.25, 0.25)  # equal magnitudes
+    X *= np.exp2(r.choice([-700.0, 0.0, 600.0], size=M))[:, None]
+    B = r.uniform(-0.5, 0.5, size=(K, N))
+    B[:, :6] = np.where(B[:, :6] < 0, -1.0, 1.0)
+    B *= np.exp2(r.choice([-300.0, 0.0, 300.0], size=N))[None, :]
+    A = np.asfortranarray(X if ta == 111 else X.T)
+    B = np.asfortranarray(B)
+    C = np.zeros((M, N), order="F")
+    xk.gemm(ta, 111, 1.0, A, B, 0.0, C)
+    hi, lo = orc.dgemm_dd(ta, 111, 1.0, A, B, 0.0, np.zeros((M, N), order="F"))
+    bound = np.abs(X) @ np.abs(B)
+    assert np.all(np.isfinite(C))
+    assert np.all(np.abs(C - hi - lo) <= 2.0**-44 * bound)
+
+
 def test_split_count_controls_accuracy(xk, orc, split_mode):
     r = rng(77)
     M = N = K = 512

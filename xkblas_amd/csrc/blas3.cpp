@@ -822,7 +822,7 @@ static bool ir_column_done(bool res_ok, int corr, double dn, double dprev, doubl
   return rho / (1.0 - rho) * dn <= IR_TAU * xn;
 }
 
-enum { WS_IR_A = 16, WS_IR_F = 17, WS_IR_R = 18, WS_IR_V = 19 };
+enum { WS_IR_A = 16, WS_IR_F = 17, WS_IR_R = 18, WS_IR_V = 19, WS_IR_P = 28 };
 
 int dev_posv_ir(const Exec& x, const Exec& xl, const Exec& xf, int uplo, int n, int nrhs, const double* A, long lda,
                 const double* B, long ldb, double* X, long ldx, int nb, int itermax, int* iter)
@@ -862,9 +862,11 @@ int dev_posv_ir(const Exec& x, const Exec& xl, const Exec& xf, int uplo, int n, 
   double* rn = V + n;            // ||r_j||_inf
   double* xn = V + n + nrhs;     // ||x_j||_inf
   double* dn = V + n + 2 * nrhs; // ||d_j||_inf
-  auto residual = [&]() {        // R <- B - A X, then the norms of R and X
-    copy2d<double>(x.s, n, nrhs, B, ldb, R, ldf);
-    dev_gemm<double>(x, NoTrans, NoTrans, n, nrhs, n, -1.0, Af, ldf, X, ldx, 1.0, R, ldf);
+  // R <- B - A X with error-free sums of short product runs (the residual's
+  // own rounding is what refinement converges to), then the norms of R and X
+  void* Pw = scratch(WS_IR_P, resid_scratch_bytes(n, nrhs));
+  auto residual = [&]() {
+    resid_accurate(x.s, n, nrhs, Af, ldf, X, ldx, B, ldb, R, ldf, Pw);
     col_reduce(x.s, 0, n, nrhs, R, ldf, rn);
     col_reduce(x.s, 0, n, nrhs, X, ldx, xn);
   };

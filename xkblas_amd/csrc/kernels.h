@@ -98,4 +98,12 @@ void cplx_expand(hipStream_t s, int mode, int uplo, int n, const T* A, long lda,
 // X <- X + D (m x n blocks).
 void add2d(hipStream_t s, int m, int n, const double* D, long ldd, double* X, long ldx);
 
+// R <- B - A X (A n x n, X and B n x nrhs) with the k products summed in
+// short MFMA runs and the run sums added without rounding error (TwoSum):
+// the refinement residual, about sqrt(n / 64) times more accurate than one
+// FP64 GEMM.  work: resid_scratch_bytes(n, nrhs) bytes of device memory.
+size_t resid_scratch_bytes(int n, int nrhs);
+void resid_accurate(hipStream_t s, int n, int nrhs, const double* A, long lda, const double* X, long ldx,
+                    const double* B, long ldb, double* R, long ldr, void* work);
+
 }  // namespace xk

@@ -98,6 +98,129 @@ __global__ __launch_bounds__(256) void add2d_kernel(int m, int n, const double* 
   if (i < m && j < n) X[i + (long)j * ldx] += D[i + (long)j * ldd];
 }
 
+// ---- accurate residual R = B - A X for refinement
+// A FP64 GEMM sums its k products in one chain per element, so its rounding
+// error grows with k (typically sqrt(k) u |A| |X|) and refinement stalls at
+// that level.  Here the k products of each element are summed by MFMA in
+// runs of RES_SUB = 64 only; the run sums are added without error
+// (TwoSum, the rounding errors kept in a second word) per workgroup over a
+// k-range of RES_KR, and the ranges' (sum, error) pairs are added to B in
+// the same way by the reduction kernel.  The remaining error is that of the
+// 64-term runs and one final rounding: about sqrt(k / 64) times smaller.
+constexpr int RES_SUB = 64, RES_KR = 1024;
+typedef double rd4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ void two_sum(double a, double b, double& s, double& e)
+{
+  s = a + b;
+  const double bp = s - a;
+  e = (a - (s - bp)) + (b - bp);
+}
+
+// Partial sums over the k-range of blockIdx.z: P[z * sp + i + j ldp] =
+// (sum, error) of sum_k A(i, k) X(k, j).  Workgroup: 64 rows x 64 columns,
+// wave w the rows 16 w .. 16 w + 15, four 16 x 16 blocks of
+// v_mfma_f64_16x16x4f64 (operands loaded straight from memory: A is read
+// once, the k-range of X is shared by every row block and stays in L2).
+__global__ __launch_bounds__(256) void resid_partial_kernel(int n, int nrhs, const double* __restrict__ A, long lda,
+                                                            const double* __restrict__ X, long ldx,
+                                                            double2* __restrict__ P, long ldp, long sp)
+{
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int li = lane & 15, lk = lane >> 4;
+  const int i0 = blockIdx.x * 64 + wave * 16, j0 = blockIdx.y * 64;
+  const int k0 = blockIdx.z * RES_KR, k1 = min(n, k0 + RES_KR);
+  const int ia = i0 + li;
+  const bool rowok = ia < n;
+  double hi[4][4], lo[4][4];
+#pragma unroll
+  for (int b = 0; b < 4; ++b)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) hi[b][r] = lo[b][r] = 0.0;
+  for (int ks = k0; ks < k1; ks += RES_SUB) {
+    rd4 acc[4];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[b] = rd4{0.0, 0.0, 0.0, 0.0};
+    const int ke = min(k1, ks + RES_SUB);
+#pragma unroll 4
+    for (int k = ks; k < ke; k += 4) {
+      const int kk = k + lk;
+      const bool kok = kk < ke;
+      const double a = (rowok && kok) ? A[ia + (long)kk * lda] : 0.0;
+      double x[4];
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        const int j = j0 + 16 * b + li;
+        x[b] = (kok && j < nrhs) ? X[kk + (long)j * ldx] : 0.0;
+      }
+      // (x first: lane holds element (row li, column lk + 4 r) of the block)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) acc[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(x[b], a, acc[b], 0, 0, 0);
+    }
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        double t, e;
+        two_sum(hi[b][r], acc[b][r], t, e);
+        hi[b][r] = t;
+        lo[b][r] += e;
+      }
+  }
+  if (!rowok) return;
+  double2* out = P + blockIdx.z * sp;
+#pragma unroll
+  for (int b = 0; b < 4; ++b)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int j = j0 + 16 * b + lk + 4 * r;
+      if (j < nrhs) out[ia + (long)j * ldp] = make_double2(hi[b][r], lo[b][r]);
+    }
+}
+
+// R(i, j) = B(i, j) - sum_z P[z](i, j), the pairs added without error and
+// the errors summed apart.
+__global__ __launch_bounds__(256) void resid_reduce_kernel(int n, int nz, const double* __restrict__ B, long ldb,
+                                                           const double2* __restrict__ P, long ldp, long sp,
+                                                           double* __restrict__ R, long ldr)
+{
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  const int j = blockIdx.y;
+  if (i >= n) return;
+  double s = B[i + (long)j * ldb], c = 0.0;
+  for (int z = 0; z < nz; ++z) {
+    const double2 q = P[z * sp + i + (long)j * ldp];
+    double t, e;
+    two_sum(s, -q.x, t, e);
+    s = t;
+    c += e - q.y;
+  }
+  R[i + (long)j * ldr] = s + c;
+}
+
+size_t resid_scratch_bytes(int n, int nrhs)
+{
+  const long ldp = (n + 63) / 64 * 64;
+  const long nz = (n + RES_KR - 1) / RES_KR;
+  return sizeof(double2) * (size_t)nz * ldp * nrhs;
+}
+
+void resid_accurate(hipStream_t s, int n, int nrhs, const double* A, long lda, const double* X, long ldx,
+                    const double* B, long ldb, double* R, long ldr, void* work)
+{
+  if (n <= 0 || nrhs <= 0) return;
+  const long ldp = (n + 63) / 64 * 64;
+  const int nz = (n + RES_KR - 1) / RES_KR;
+  const long sp = ldp * nrhs;
+  double2* P = static_cast<double2*>(work);
+  XK_CHECK((nrhs + 63) / 64 <= 65535 && nz <= 65535 && nrhs <= 65535, "residual grid");
+  hipLaunchKernelGGL(resid_partial_kernel, dim3((n + 63) / 64, (nrhs + 63) / 64, nz), dim3(256), 0, s, n, nrhs, A,
+                     lda, X, ldx, P, ldp, sp);
+  hipLaunchKernelGGL(resid_reduce_kernel, dim3((n + 255) / 256, nrhs), dim3(256), 0, s, n, nz, B, ldb, P, ldp, sp,
+                     R, ldr);
+  XK_HIP(hipGetLastError());
+}
+
 template <typename T>
 void symm_expand(hipStream_t s, int uplo, int n, const T* A, long lda, T* Y, long ldy, T* Z, long ldz)
 {

@@ -141,7 +141,11 @@ bool Runtime::evict_for(size_t need, const std::vector<Replica*>& keep, bool str
       if (!victim || r->lru < victim->lru) victim = r;
     }
     if (!victim) {
-      if (!strict) return false;
+      // scratch counts against the limit, but only the operands themselves
+      // beyond it are fatal: scratch may overshoot the limit
+      size_t scratch_bytes = 0;
+      for (int i = 0; i < NSCRATCH; ++i) scratch_bytes += scratch_size[i];
+      if (!strict || cache_used - std::min(cache_used, scratch_bytes) + need <= cache_limit) return false;
       char msg[256];
       snprintf(msg, sizeof msg, "operands need %zu bytes of HBM beyond the cache limit of %zu bytes", need,
                cache_limit);
@@ -264,6 +268,7 @@ Replica* Runtime::replica(const void* host, int M, int N, long ld, int elt, int 
     }
   }
   Replica* r = new Replica();
+  r->owner = std::this_thread::get_id();
   r->host = host;
   r->M = M; r->N = N; r->ld = ld; r->elt = elt;
   r->nb = nb;
@@ -1137,9 +1142,13 @@ int xkblas_memory_invalidate_caches(void)
   if (!r.device_ready) return 0;
   r.sync_all();
   r.free_retired();
-  std::vector<Replica*> all;
-  for (auto& kv : r.map) all.push_back(kv.second);
-  for (Replica* rep : all) r.drop(rep);
+  // the calling thread's mirrors: another thread's results may still wait
+  // for its coherent_async
+  const std::thread::id me = std::this_thread::get_id();
+  std::vector<Replica*> mine;
+  for (auto& kv : r.map)
+    if (kv.second->owner == me) mine.push_back(kv.second);
+  for (Replica* rep : mine) r.drop(rep);
   return 0;
 }
 

@@ -6,6 +6,7 @@
 #include <mutex>
 #include <string>
 #include <unordered_map>
+#include <thread>
 #include <vector>
 
 namespace xk {
@@ -27,6 +28,10 @@ struct Replica {
   std::vector<uint8_t> state;  // per tile (column-major tile order)
   uint64_t lru = 0;
   int info_slot = -1;
+  // the thread whose call created the mirror: xkblas_memory_invalidate_caches
+  // drops the mirrors of the calling thread only (each thread's descriptor
+  // list in the reference, blas/xkblas.c:1855-1908)
+  std::thread::id owner;
   // Rows [.., end) of the mirror are final once `ev` completes on the
   // compute stream (a product pipelined by row panels): the copy back of a
   // row panel waits only for its own event.  Empty otherwise.
@@ -58,7 +63,7 @@ struct ProfStat {
 };
 
 struct Runtime {
-  static constexpr int NSCRATCH = 27;
+  static constexpr int NSCRATCH = 29;
   static constexpr int INFO_SLOTS = 1024;
 
   bool device_ready = false;

@@ -15,9 +15,11 @@ void split_gemm(const Exec& x, int transA, int transB, int m, int n, int k, T al
                 const T* B, long ldb, T beta, T* C, long ldc, int tri);
 
 // Number of CRT moduli for inner dimension k (requested >= 2 wins; else the
-// smallest count giving >= 48 bits per side), the resulting bits per side,
-// and the number of residue GEMMs (planes) it takes; fp8 selects the FP8
-// moduli set (single planes for moduli <= 33, Karatsuba triples above).
+// smallest count keeping >= 46 bits of every row's largest entry), those
+// guaranteed bits (a row of k equal-magnitude entries; rows with spread
+// magnitudes keep more), and the number of residue GEMMs (planes) it takes;
+// fp8 selects the FP8 moduli set (single planes for moduli <= 33, Karatsuba
+// triples above).
 int split_choose_count(int k, int requested, bool fp8);
 int split_side_bits(int k, int nsplit, bool fp8);
 int split_plane_count(int k, int requested, bool fp8);

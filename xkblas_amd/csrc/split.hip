@@ -1,15 +1,19 @@
 // split.hip — FP64-accurate GEMM from BF16 or FP8 residue products (gfx950).
 //
 // Method (CRT emulation of an FP64 product, Ozaki-scheme-II style):
-//  1. scale: every row i of op(A) by 2^sA_i and column j of op(B) by 2^sB_j,
-//     so that |entries| < 2^t, and round to integers A', B' (exact in FP64);
-//     t is chosen so that k * 2^(2t) < P/2, P = p_1 ... p_N (pairwise coprime
-//     moduli <= 256), which makes the integer product C' = A' B' recoverable
-//     from its residues;
+//  1. scale: every row i of op(A) by 2^sA_i and column j of op(B) by 2^sB_j
+//     and round to integers A', B' (exact in FP64).  The shifts bound the
+//     2-norms: ||A'_i||_2 <= 2^T and ||B'_j||_2 <= 2^T with 2^(2T) < P/2,
+//     P = p_1 ... p_N (pairwise coprime moduli), so by Cauchy-Schwarz every
+//     |C'_ij| = |A'_i . B'_j| < P/2 and the integer product C' = A' B' is
+//     recoverable from its residues.  A row whose entries all have the same
+//     magnitude keeps T - log2(k)/2 bits of its largest entry; rows with
+//     spread magnitudes keep more (about 0.8 bits more for uniform entries);
+//     shifts are also capped so |A'| < 2^52;
 //  2. split: the residues A' mod p_l, B' mod p_l are written as "planes" of
 //     k-contiguous rows, padded to the GEMM tile, all in one pass:
-//       BF16: one plane per modulus <= 481, residues in [-240, 240] are exact
-//       BF16 values (8 significant bits);
+//       BF16: one plane per odd modulus <= 511, residues in [-255, 255] are
+//       exact BF16 values (8 significant bits);
 //       FP8 (E4M3 holds the integers -16..16 exactly): a modulus <= 33 is one
 //       plane of its residue; a larger modulus p <= 255 is three planes of
 //       base-16 digits r = 16 h + l (h, l in [-8, 8]) and their sum s = h + l,
@@ -17,10 +21,10 @@
 //       + l_a l_b  (Karatsuba) needs three FP8 products instead of four;
 //  3. multiply: one MFMA GEMM per plane (v_mfma_f32_32x32x16_bf16, or
 //     v_mfma_f32_32x32x64_f8f6f4 for FP8) accumulates in FP32.  Products are
-//     < 2^16 (BF16) or <= 2^8 (FP8) in magnitude, so FP32 sums stay exact;
-//     the BF16 accumulator is reduced mod p (exactly, by an FMA) at least
-//     every 272 k, before it could pass 2^24; FP8 products run over at most 32768 k per
-//     call (longer inner dimensions are summed in FP64 over chunks).  The
+//     <= 255^2 < 2^16 (BF16) or <= 2^8 (FP8) in magnitude, so FP32 sums stay
+//     exact; the BF16 accumulator is reduced mod p (exactly, by an FMA) every
+//     256 k, before it could pass 2^24; FP8 products run over at most 32768 k
+//     per call (longer inner dimensions are summed in FP64 over chunks).  The
 //     output is the plane's sum mod p as int16 (BF16) or int8 (FP8);
 //  4. recombine: the digit planes of a Karatsuba modulus are first combined
 //     into its residue (exact in FP32), then C' = sum_l r_l W_l mod P,
@@ -40,12 +44,13 @@
 
 namespace xk {
 
-// BF16: pairwise coprime odd moduli <= 481 (prime factors 13.37, 479,
-// 3^2.53, 5^2.19, 11.43, 7.67, 467, 463, 461, 457, 449, 443, 439), residues
-// fit [-240, 240] (exact BF16 values: 8 significant bits), so a product is
-// <= 57600 and 272 of them plus a folded remainder stay below 2^24.
+// BF16: the 13 largest pairwise coprime odd moduli <= 511 (511 = 7.73,
+// 509, 507 = 3.13^2, 505 = 5.101, 503, 499, 493 = 17.29, 491, 487, 479,
+// 473 = 11.43, 467, 463); residues fit [-255, 255] (exact BF16 values: 8
+// significant bits), so a product is <= 65025 and 256 of them plus a folded
+// remainder stay below 2^24.
 constexpr int BF16_MAXMOD = 13;
-static const int kModuliBF16[BF16_MAXMOD] = {481, 479, 477, 475, 473, 469, 467, 463, 461, 457, 449, 443, 439};
+static const int kModuliBF16[BF16_MAXMOD] = {511, 509, 507, 505, 503, 499, 493, 491, 487, 479, 473, 467, 463};
 // FP8: the first FP8_DIRECT moduli (<= 33, residues in [-16, 16]) are single
 // planes; the primes after them are Karatsuba digit triples.  Together they
 // are pairwise coprime and P < 2^123 (the top limb of W stays below 2^41).
@@ -132,22 +137,39 @@ static Moduli make_moduli(int n, bool fp8)
   return M;
 }
 
-// bits per side: k * 2^(2t) < P/2
+// 2-norm budget per side: ||A'_i||_2, ||B'_j||_2 <= 2^T with 2^(2T) < P/2
+// (the margin covers the rounding of log2P, which is summed in doubles)
+static double norm_budget(const Moduli& M) { return 0.5 * (M.log2P - 1.0) - 1e-9; }
+
+// Exponent budget handed to the shift kernel: with 2^s ||x||_2 <= 2^Teff,
+// the rounded row x' = rint(2^s x) has ||x'||_2 <= 2^Teff + sqrt(k)/2 <= 2^T.
+static double shift_budget(const Moduli& M, int k)
+{
+  const double T = norm_budget(M);
+  const double room = std::exp2(T) - 0.5 * std::sqrt((double)std::max(k, 1));
+  return room > 1.0 ? std::log2(room) - 1e-9 : 0.0;
+}
+
+// Guaranteed bits of a row's largest entry: a row of k entries of equal
+// magnitude has ||x|| = sqrt(k) max|x|, so its largest entry is scaled to at
+// least 2^(T - log2(k)/2 - 1) and rounded with an error below 2^-(T -
+// log2(k)/2) of it; rows with spread magnitudes keep more.
 static int side_bits(const Moduli& M, int k)
 {
-  const double lk = std::ceil(std::log2((double)std::max(k, 2)));
-  int t = (int)std::floor((M.log2P - 1.0 - lk - 1e-9) / 2.0);
-  return std::min(t, 52);
+  const double t = norm_budget(M) - 0.5 * std::log2((double)std::max(k, 2));
+  return (int)std::min(52.0, std::floor(t));
 }
 
 int split_choose_count(int k, int requested, bool fp8)
 {
   const int maxn = fp8 ? MAXMOD : BF16_MAXMOD;
   if (requested >= 2) return std::min(requested, maxn);
-  // smallest count giving >= 48 bits per side (error ~ 2^-49 of the row max)
+  // smallest count keeping >= 46 bits of every row's largest entry: an
+  // input rounding error below 2^-46 of the row maximum, the size of the
+  // rounding error an FP64 dot product of 2^10 or more terms accumulates
   for (int n = 8; n <= maxn; ++n) {
     Moduli M = make_moduli(n, fp8);
-    if (side_bits(M, k) >= 48) return n;
+    if (side_bits(M, k) >= 46) return n;
   }
   return maxn;
 }
@@ -158,37 +180,52 @@ int split_plane_count(int k, int requested, bool fp8)
 }
 
 // ---------------------------------------------------------------- kernels
-// Row maxima of |X| for X viewed as rows x kdim (row_contig: X[i + k*ld],
-// else X[k + i*ld]); bits of non-negative doubles order like integers, so
-// the maxima combine with integer atomicMax.
+// Row statistics of X viewed as rows x kdim (row_contig: X[i + k*ld], else
+// X[k + i*ld]): the maxima of |X| (bits of non-negative doubles order like
+// integers, so they combine with integer atomicMax) and, per chunk c of
+// AM_K k, the partial sums of squares part[c * pld + i], each summed in a
+// fixed order (the shifts, hence the result bits, do not depend on timing).
 //  row_contig: a block of 256 threads owns 256 consecutive rows and a chunk
-//  of AM_K k; thread t keeps the running max of row i0 + t (consecutive
-//  lanes read consecutive addresses);
+//  of AM_K k; thread t walks row i0 + t (consecutive lanes read consecutive
+//  addresses);
 //  else: each wave owns one row and a chunk of AM_K k (64 lanes x 8 bytes
 //  contiguous per load), reduced across the wave at the end.
 constexpr int AM_K = 256;
-__global__ __launch_bounds__(256) void amax_kernel(const double* __restrict__ X, long ld, int rows, int kdim,
-                                                   int row_contig, unsigned long long* __restrict__ bits)
+__global__ __launch_bounds__(256) void rowstat_kernel(const double* __restrict__ X, long ld, int rows, int kdim,
+                                                      int row_contig, unsigned long long* __restrict__ bits,
+                                                      double* __restrict__ part, long pld)
 {
   const int t = threadIdx.x;
   const int k0 = blockIdx.x * AM_K, k1 = min(kdim, k0 + AM_K);
-  double m = 0.0;
+  double m = 0.0, q = 0.0;
   int row;
   if (row_contig) {
     row = blockIdx.y * 256 + t;
     if (row >= rows) return;
     const double* x = X + row + (long)k0 * ld;
 #pragma unroll 8
-    for (int k = k0; k < k1; ++k, x += ld) m = fmax(m, fabs(*x));
+    for (int k = k0; k < k1; ++k, x += ld) {
+      const double v = *x;
+      m = fmax(m, fabs(v));
+      q = fma(v, v, q);
+    }
   } else {
     row = blockIdx.y * 4 + (t >> 6);
     if (row >= rows) return;
     const double* x = X + (long)row * ld;
-    for (int k = k0 + (t & 63); k < k1; k += 64) m = fmax(m, fabs(x[k]));
+    for (int k = k0 + (t & 63); k < k1; k += 64) {
+      const double v = x[k];
+      m = fmax(m, fabs(v));
+      q = fma(v, v, q);
+    }
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
+    for (int o = 32; o > 0; o >>= 1) {
+      m = fmax(m, __shfl_xor(m, o));
+      q += __shfl_xor(q, o);
+    }
     if ((t & 63) != 0) return;
   }
+  part[(long)blockIdx.x * pld + row] = q;
   if (m > 0.0) atomicMax(bits + row, (unsigned long long)__double_as_longlong(m));
 }
 
@@ -198,15 +235,29 @@ __global__ void fill_u64_kernel(unsigned long long* p, int n)
   if (i < n) p[i] = 0ull;
 }
 
-// shift_i = t - E_i with max_i = f 2^E_i, f in [0.5, 1); zero rows get 0.
-__global__ void shift_kernel(const unsigned long long* __restrict__ bits, int rows, int t, int* __restrict__ shift)
+// Row shifts from the statistics: the 2-norm bound 2^s ||x||_2 <= 2^teff
+// (the sum of squares taken up by the relative bound (k + 2) 2^-52 of its
+// rounding) and the range bound 2^s max|x| < 2^52; zero rows get 0.  Where
+// squares could overflow, or underflow enough to matter, or x holds a NaN,
+// ||x||_2 <= sqrt(k) max|x| gives the shift instead (tsafe = teff -
+// log2(k)/2).
+__global__ void shift_kernel(const unsigned long long* __restrict__ bits, const double* __restrict__ part, long pld,
+                             int nchunk, int rows, double teff, double tsafe, double grow, int* __restrict__ shift)
 {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= rows) return;
   const double m = __longlong_as_double((long long)bits[i]);
+  if (!(m > 0.0)) {
+    shift[i] = 0;
+    return;
+  }
+  double q = 0.0;
+  for (int c = 0; c < nchunk; ++c) q += part[(long)c * pld + i];
   int e = 0;
-  if (m > 0.0) frexp(m, &e);
-  shift[i] = m > 0.0 ? t - e : 0;
+  frexp(m, &e);
+  const int s_norm = (q >= 0x1p-960 && q <= 0x1p+1000) ? (int)floor(teff - 0.5 * log2(q * grow))
+                                                       : (int)floor(tsafe) - e;
+  shift[i] = min(s_norm, 52 - e);
 }
 
 struct SplitParams {
@@ -364,14 +415,33 @@ constexpr int DM_BM = 256, DM_BN = 256, DM_BK = 32, DM_STAGES = XK_DM_STAGES;
 
 typedef __attribute__((address_space(3))) void lds_void;
 
+// s_waitcnt with only vmcnt constrained (expcnt, lgkmcnt at maximum); vmcnt
+// is 6 bits: low 4 in [3:0], high 2 in [15:14]
+#define XK_VMCNT(n) (((n) & 0xf) | (0x7 << 4) | (0xf << 8) | (((n) >> 4) << 14))
+
+// Waits until at most n vector-memory operations of this wave are in flight
+// (n rounded down to a multiple of 4; the stage loads come in groups of 4 or
+// 8 per wave, so the steady-state counts are exact).
 __device__ __forceinline__ void wait_vmcnt_le(int n)
 {
-  // s_waitcnt with only vmcnt constrained (expcnt, lgkmcnt at maximum);
-  // vmcnt is 6 bits: low 4 in [3:0], high 2 in [15:14]
-  if (n >= 24) __builtin_amdgcn_s_waitcnt((24 & 0xf) | (0x7 << 4) | (0xf << 8) | ((24 >> 4) << 14));
-  else if (n >= 16) __builtin_amdgcn_s_waitcnt((16 & 0xf) | (0x7 << 4) | (0xf << 8) | ((16 >> 4) << 14));
-  else if (n >= 8) __builtin_amdgcn_s_waitcnt((8 & 0xf) | (0x7 << 4) | (0xf << 8));
-  else __builtin_amdgcn_s_waitcnt((0x7 << 4) | (0xf << 8));
+  if (n >= 28) __builtin_amdgcn_s_waitcnt(XK_VMCNT(28));
+  else if (n >= 24) __builtin_amdgcn_s_waitcnt(XK_VMCNT(24));
+  else if (n >= 20) __builtin_amdgcn_s_waitcnt(XK_VMCNT(20));
+  else if (n >= 16) __builtin_amdgcn_s_waitcnt(XK_VMCNT(16));
+  else if (n >= 12) __builtin_amdgcn_s_waitcnt(XK_VMCNT(12));
+  else if (n >= 8) __builtin_amdgcn_s_waitcnt(XK_VMCNT(8));
+  else if (n >= 4) __builtin_amdgcn_s_waitcnt(XK_VMCNT(4));
+  else __builtin_amdgcn_s_waitcnt(XK_VMCNT(0));
+}
+
+// Waits until at most `pend` stages of PER loads each are in flight (the
+// stages still allowed to land later; pend <= 2 for rings of up to 5).
+template <int PER>
+__device__ __forceinline__ void wait_stages(int pend)
+{
+  if (pend >= 2) __builtin_amdgcn_s_waitcnt(XK_VMCNT(2 * PER));
+  else if (pend == 1) __builtin_amdgcn_s_waitcnt(XK_VMCNT(PER));
+  else __builtin_amdgcn_s_waitcnt(XK_VMCNT(0));
 }
 
 __device__ __forceinline__ void wait_lgkm0()
@@ -546,12 +616,13 @@ __global__ __launch_bounds__(256) void mod_gemm_dma_kernel(ModGemmParams prm)
 
   // The k loop runs in groups of 16 stages (kpad is a multiple of 512).
   // Folds are spread over the stages, two accumulator blocks per stage:
-  // stage u folds block f = u - 1 during half 0 and block f + 8 during
-  // half 1; each half issues its 16 MFMAs starting after the folded block
-  // and ending with it, so the fold's VALU work hides under 15 independent MFMAs and waits
-  // for no result (the folded block's previous MFMA was issued 8 or 9
-  // MFMAs earlier).  A block is then folded at most 8.5 stages = 272 k
-  // apart: 272 products <= 240^2 plus a folded remainder stay below 2^24.
+  // stage u folds block f = u mod 8 before its half-0 MFMAs and block f + 8
+  // before its half-1 MFMAs; each half issues its 16 MFMAs starting after
+  // the folded block and ending with it, so the fold's VALU work hides under
+  // independent MFMAs and waits for no result (at least 7 MFMAs were issued
+  // after the folded block's previous one).  Each block is folded in the same half
+  // every 8 stages = 256 k: 256 products <= 255^2 plus a folded remainder
+  // stay below 2^24.
   auto group = [&](int kt0) {
 #pragma unroll
     for (int u = 0; u < 16; ++u) {
@@ -563,9 +634,11 @@ __global__ __launch_bounds__(256) void mod_gemm_dma_kernel(ModGemmParams prm)
       for (int i = 0; i < 4; ++i) a1[i] = *reinterpret_cast<const bf16x8*>(st + rdA1 + i * 32 * DM_BK);
 #pragma unroll
       for (int j = 0; j < 4; ++j) b1[j] = *reinterpret_cast<const bf16x8*>(st + rdB1 + j * 32 * DM_BK);
-      // blocks folded in this stage: f in half 0, f + 8 in half 1; each half
-      // issues its MFMAs starting after its folded block and ending with it
-      const int f = (u + 15) & 15, f8 = (u + 7) & 15;
+      // blocks folded in this stage: f = u mod 8 in half 0 and f + 8 in
+      // half 1 (each block always in the same half, so every 8 stages); each
+      // half issues its MFMAs starting after its folded block and ending
+      // with it
+      const int f = u & 7, f8 = 8 + (u & 7);
 #pragma unroll
       for (int r = 0; r < 16; ++r) acc[f >> 2][f & 3][r] = fold(acc[f >> 2][f & 3][r], p, ip);
 #pragma unroll
@@ -586,7 +659,7 @@ __global__ __launch_bounds__(256) void mod_gemm_dma_kernel(ModGemmParams prm)
       if (kt + 1 < nk) {
         // stage kt+1 landed: only the loads of later, already issued stages
         // may be pending (8 per stage, completed in issue order)
-        wait_vmcnt_le(8 * (min(nk - 1, kt + DM_STAGES - 2) - (kt + 1)));
+        wait_stages<8>(min(nk - 1, kt + DM_STAGES - 2) - (kt + 1));
         __builtin_amdgcn_s_barrier();
         if (kt + DM_STAGES - 1 < nk) issue((kt + DM_STAGES - 1) % DM_STAGES, kt + DM_STAGES - 1);
       }
@@ -768,7 +841,7 @@ __global__ __launch_bounds__(512) void mod_gemm_w8_kernel(ModGemmParams prm)
       }
       wait_lgkm0();
       if (kt + 1 < nk) {
-        wait_vmcnt_le(4 * (min(nk - 1, kt + DM_STAGES - 2) - (kt + 1)));
+        wait_stages<4>(min(nk - 1, kt + DM_STAGES - 2) - (kt + 1));
         __builtin_amdgcn_s_barrier();
         if (kt + DM_STAGES - 1 < nk) issue((kt + DM_STAGES - 1) % DM_STAGES, kt + DM_STAGES - 1);
       }
@@ -922,7 +995,7 @@ __global__ __launch_bounds__(256) void mod_gemm_fp8_kernel(ModGemmParams prm)
     }
     wait_lgkm0();
     if (kt + 1 < nk) {
-      wait_vmcnt_le(8 * (min(nk - 1, kt + F8_STAGES - 2) - (kt + 1)));
+      wait_stages<8>(min(nk - 1, kt + F8_STAGES - 2) - (kt + 1));
       __builtin_amdgcn_s_barrier();
       if (kt + F8_STAGES - 1 < nk) issue((u + F8_STAGES - 1) % F8_STAGES, kt + F8_STAGES - 1);
     }
@@ -1034,7 +1107,7 @@ __global__ __launch_bounds__(512) void mod_gemm_fp8_w8_kernel(ModGemmParams prm)
       for (int j = 0; j < 2; ++j) acc[i][j] = mfma_fp8(a[i], b[j], acc[i][j]);
     wait_lgkm0();
     if (kt + 1 < nk) {
-      wait_vmcnt_le(4 * (min(nk - 1, kt + F8_STAGES - 2) - (kt + 1)));
+      wait_stages<4>(min(nk - 1, kt + F8_STAGES - 2) - (kt + 1));
       __builtin_amdgcn_s_barrier();
       if (kt + F8_STAGES - 1 < nk) issue((u + F8_STAGES - 1) % F8_STAGES, kt + F8_STAGES - 1);
     }
@@ -1231,7 +1304,7 @@ bool split_enabled(const Exec& x)
 
 static inline long rup(long v, long q) { return (v + q - 1) / q * q; }
 
-enum { WS_SA = 8, WS_SB = 9, WS_SC = 10, WS_SE = 11 };
+enum { WS_SA = 8, WS_SB = 9, WS_SC = 10, WS_SE = 11, WS_SN = 27 };
 
 // residue GEMMs (BF16 and FP8): 0 = 8 waves of 128 x 64, 1 = 4 waves of
 // 128 x 128
@@ -1381,17 +1454,21 @@ static void split_gemm_chunk(const Exec& x, int transA, int transB, int m, int n
   for (int l = 0; l < nmod; ++l) { sp.pd[l] = M.pd[l]; sp.invpd[l] = M.invpd[l]; }
   sp.kdim = k;
   sp.kb = kb;
-  auto amax = [&](const double* X, long ld, int rows, int rc, unsigned long long* out) {
-    XK_CHECK((rows + 3) / 4 <= 65535, "row-max grid");
-    const dim3 grid((unsigned)((k + AM_K - 1) / AM_K), (unsigned)(rc ? (rows + 255) / 256 : (rows + 3) / 4));
-    hipLaunchKernelGGL(amax_kernel, grid, dim3(256), 0, s, X, ld, rows, k, rc, out);
-  };
-  // bits[rpad] are the row maxima, zeroed first; writes planes[rpad] and shift[rows]
+  // row statistics: maxima into bits[rpad] (zeroed first), partial sums of
+  // squares into one scratch slot shared by every prep of the stream
+  const int nchunk = (k + AM_K - 1) / AM_K;
+  const double teff = shift_budget(M, k);
+  const double tsafe = teff - 0.5 * std::log2((double)std::max(k, 1));
+  const double grow = 1.0 + (k + 2.0) * 0x1p-52;
   auto prep = [&](const double* X, long ld, int rows, int rc, long rpad, unsigned long long* bits, int* shift,
                   void* planes) {
+    XK_CHECK((rows + 3) / 4 <= 65535, "row-statistics grid");
+    double* part = (double*)scratch(WS_SN, sizeof(double) * (size_t)nchunk * rpad);
     hipLaunchKernelGGL(fill_u64_kernel, dim3((int)((rpad + 255) / 256)), dim3(256), 0, s, bits, (int)rpad);
-    amax(X, ld, rows, rc, bits);
-    hipLaunchKernelGGL(shift_kernel, dim3((rows + 255) / 256), dim3(256), 0, s, bits, rows, tb, shift);
+    const dim3 sgrid((unsigned)nchunk, (unsigned)(rc ? (rows + 255) / 256 : (rows + 3) / 4));
+    hipLaunchKernelGGL(rowstat_kernel, sgrid, dim3(256), 0, s, X, ld, rows, k, rc, bits, part, rpad);
+    hipLaunchKernelGGL(shift_kernel, dim3((rows + 255) / 256), dim3(256), 0, s, bits, part, rpad, nchunk, rows,
+                       teff, tsafe, grow, shift);
     sp.X = X; sp.ld = ld; sp.rows = rows; sp.row_contig = rc; sp.shift = shift;
     sp.R = planes; sp.rpad = rpad; sp.rstride = rpad * kpad;
     const dim3 grid((unsigned)(kpad / 64), (unsigned)(rpad / 64));

@@ -41,11 +41,13 @@ MAXCOUNT = {2: 13, 3: 20}
 
 
 def split_counts(mode, k):
-    """Split counts that keep >= 48 bits per side for inner dimension k
-    (the accuracy contract of the split modes): the smallest and one more."""
+    """Split counts the model may choose for inner dimension k: the
+    library's default (the smallest count keeping >= 46 bits of every row's
+    largest entry, the accuracy contract of the split modes) and one more."""
     L = xk.lib()
-    ok = [c for c in range(8, MAXCOUNT[mode] + 1) if L.xkblas_split_info(mode, max(k, 2), c, None, None) >= 48]
-    return ok[:2] if ok else [MAXCOUNT[mode]]
+    nm = ctypes.c_int()
+    L.xkblas_split_info(mode, max(k, 2), 0, ctypes.byref(nm), None)
+    return [c for c in (nm.value, nm.value + 1) if c <= MAXCOUNT[mode]]
 
 
 def candidates(op, dtype, mode, m, n, k):
