@@ -33,7 +33,8 @@ def refined_solution(A, L, B, X):
     """X + A^-1 (B - A X) with the residual evaluated in 80-bit long double
     (its error ~ 2^-64 sqrt(n) |A| |X|) and the correction solved with the
     lower Cholesky factor L in FP64: a reference solution of A X = B whose
-    error is far below FP64 rounding for a well-conditioned A."""
+    error is far below FP64 rounding for a well-conditioned A.  Long-double
+    products are slow: pass only the columns to be checked."""
     from scipy.linalg import solve_triangular
     R = B.astype(np.longdouble) - A.astype(np.longdouble) @ X.astype(np.longdouble)
     D = solve_triangular(L, np.asarray(R, dtype=np.float64), lower=True, check_finite=False)

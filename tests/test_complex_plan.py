@@ -44,3 +44,54 @@ def test_block_plan_matches_complex_product(ta, tb):
     op = lambda X, t: X if t == N_ else (X.T if t == T_ else X.conj().T)
     ref = (0.5 - 2j) * (op(A, ta) @ op(B, tb)) + (1 + 1j) * C
     np.testing.assert_allclose(plan(ta, tb, A, B, 0.5 - 2j, 1 + 1j, C), ref, atol=1e-12)
+
+
+def pack2(B, vertical, swap, sr, si):
+    """[sr Br, si Bi] side by side, or stacked when vertical; swap puts
+    si Bi first (cplx_pack2_kernel)."""
+    first, second = (si * B.imag, sr * B.real) if swap else (sr * B.real, si * B.imag)
+    return np.vstack([first, second]) if vertical else np.hstack([first, second])
+
+
+def triangle_plan(ta, tb, A, B, n, k):
+    """The triangle path of cgemm_impl: U = [Xr Xi], V1 = [Zr -Zi],
+    V2 = [Zi Zr] with X = op(A), Z = op(B)^T, packed from the stored
+    matrices with the same flags; returns (Pr, Pi, same)."""
+    ta_t = ta != N_
+    tb_t = tb == N_
+    zi = -1 if tb == C_ else 1
+    U = pack2(A, ta_t, 0, 1, -1 if ta == C_ else 1)
+    V1 = pack2(B, tb_t, 0, 1, -zi)
+    V2 = pack2(B, tb_t, 1, 1, zi)
+    same = A is B and ta_t == tb_t and (-1 if ta == C_ else 1) == -zi
+    opu = U.T if ta_t else U
+    opv = lambda V: V if tb_t else V.T
+    return opu @ opv(V1), opu @ opv(V2), same, np.array_equal(U, V1)
+
+
+@pytest.mark.parametrize("ta,tb", list(itertools.product([N_, T_, C_], repeat=2)))
+def test_triangle_plan_matches_complex_product(ta, tb):
+    r = np.random.default_rng(ta * 5 + tb)
+    n, k = 6, 4
+    c = lambda p, q: r.standard_normal((p, q)) + 1j * r.standard_normal((p, q))
+    A = c(n, k) if ta == N_ else c(k, n)
+    B = c(k, n) if tb == N_ else c(n, k)
+    op = lambda X, t: X if t == N_ else (X.T if t == T_ else X.conj().T)
+    P = op(A, ta) @ op(B, tb)
+    Pr, Pi, _, _ = triangle_plan(ta, tb, A, B, n, k)
+    np.testing.assert_allclose(Pr, P.real, atol=1e-12)
+    np.testing.assert_allclose(Pi, P.imag, atol=1e-12)
+
+
+@pytest.mark.parametrize("trans", [N_, C_])
+def test_triangle_plan_herk_real_part_is_a_syrk(trans):
+    """HERK (B = A, op(B) = op(A)^H): the real part's operands are one packed
+    matrix, so the flag that reuses U for V1 is exactly when they are equal."""
+    r = np.random.default_rng(trans)
+    n, k = 5, 3
+    A = r.standard_normal((n, k) if trans == N_ else (k, n)) + 1j * r.standard_normal((n, k) if trans == N_ else (k, n))
+    ta, tb = (N_, C_) if trans == N_ else (C_, N_)
+    Pr, Pi, same, equal = triangle_plan(ta, tb, A, A, n, k)
+    assert same and equal
+    X = A if trans == N_ else A.conj().T
+    np.testing.assert_allclose(Pr + 1j * Pi, X @ X.conj().T, atol=1e-12)

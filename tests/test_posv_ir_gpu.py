@@ -7,10 +7,10 @@ oracle/posv_ir.py); the residuals sum short product runs without error, so X
 converges to within 1e-14 (max-abs relative) of the exact solution.  The
 tests check dsposv's test on the host from the returned X (FP64 NumPy
 residual, a factor 2 of slack for the different summation order), compare X
-at 1e-14 with the oracle's tiled Cholesky + two tiled triangular solves
-refined once with a long-double residual (the oracle's solve alone is only
-accurate to ~1e-14 at n = 4096), and with LAPACK's FP64 solve
-(numpy.linalg.solve) for the fallbacks, check that only the uplo triangle of
+at 1e-14 with LAPACK's FP64 solve refined once with a long-double residual
+(an FP64 solve alone is only accurate to ~1e-14 at n = 4096), and up to
+n = 2048 with the oracle's tiled Cholesky + two tiled triangular solves,
+and with LAPACK's FP64 solve for the fallbacks, check that only the uplo triangle of
 A is read (the other one holds NaN), and force both fallbacks of dsposv: the
 correction limit (itermax 0) and a matrix that is not positive definite.
 """
@@ -78,13 +78,18 @@ def test_dposv_ir_converges(xk, orc, uplo, n, nrhs, nb):
         # first correction is large, the forward test needs a second
         assert 2 <= it <= 10, it
     assert np.all(_criterion(A, B, X) < 2.0)
-    # forward error against the oracle's solution refined once with an
-    # extended-precision residual (the oracle's tiled solve alone is itself
-    # only accurate to ~1e-14 at n = 4096)
-    Xr = _oracle_solve(orc, uplo, A, B, nb)
-    Xs = refined_solution(A, np.linalg.cholesky(A), B, Xr)
-    assert np.abs(Xr - Xs).max() / np.abs(Xs).max() <= 1e-13
-    assert np.abs(X - Xs).max() / np.abs(Xs).max() <= 1e-14
+    # forward error on up to 4 columns against LAPACK's FP64 solution refined
+    # once with an extended-precision residual (an FP64 solve alone, the
+    # oracle's tiled one included, is only accurate to ~1e-14 at n = 4096)
+    Lc = np.linalg.cholesky(A)
+    cols = np.arange(min(nrhs, 4))
+    Xl = np.linalg.solve(A, B[:, cols])
+    Xs = refined_solution(A, Lc, B[:, cols], Xl)
+    assert np.abs(X[:, cols] - Xs).max() / np.abs(Xs).max() <= 1e-14
+    if n <= 2048:
+        Xr = _oracle_solve(orc, uplo, A, B, nb)
+        assert np.abs(Xr[:, cols] - Xs).max() / np.abs(Xs).max() <= 1e-13
+        assert np.abs(X - Xr).max() / np.abs(Xr).max() <= 2e-14
 
 
 @pytest.mark.parametrize("mode", [0, 2])

@@ -533,6 +533,39 @@ static void cgemm_impl(const Exec& x, int transA, int transB, int m, int n, int 
     return;
   }
   auto rup = [](long v) { return (v + 31) / 32 * 32; };
+  if ((uplo == Lower || uplo == Upper) && m == n) {
+    // Only a triangle of C: with X = op(A), Y = op(B) and Z = Y^T,
+    //   Pr = Xr Yr - Xi Yi = [Xr Xi] [Zr -Zi]^T,  Pi = Xi Yr + Xr Yi = [Xr Xi] [Zi Zr]^T,
+    // two real triangle-masked products of inner dimension 2k: half the
+    // flops of the full block product.  Each real operand is packed from the
+    // stored matrix without transposing it (a stored k x n matrix gives the
+    // stacked 2k x n transpose, which the product reads transposed).
+    const bool ta = transA != NoTrans;  // A stored k x n
+    const bool tb = transB == NoTrans;  // B stored k x n, so Z = B^T is stored transposed
+    const int ar = ta ? k : n, ac = ta ? n : k;
+    const int br = tb ? k : n, bc = tb ? n : k;
+    const int zi = transB == ConjTrans ? -1 : 1;  // Zi = zi Bi (stored orientation)
+    const long ldu = ta ? rup(2L * k) : rup(n), ldv = tb ? rup(2L * k) : rup(n);
+    const size_t usz = (size_t)ldu * (ta ? n : 2 * k), vsz = (size_t)ldv * (tb ? n : 2 * k);
+    T* U = (T*)scratch(WS_ZA, sizeof(T) * usz);
+    T* V = (T*)scratch(WS_ZB, sizeof(T) * 2 * vsz);
+    T* V1 = V;
+    T* V2 = V + vsz;
+    cplx_pack2<T>(x.s, ta, 0, 1, transA == ConjTrans ? -1 : 1, ar, ac, A, lda, U, ldu);
+    // [Zr -Zi] equals U when B is A with the conjugate operation (HERK):
+    // the real part is then a SYRK of U
+    const bool same = A == B && lda == ldb && ta == tb && (transA == ConjTrans ? -1 : 1) == -zi;
+    if (same) V1 = U;
+    else cplx_pack2<T>(x.s, tb, 0, 1, -zi, br, bc, B, ldb, V1, ldv);
+    cplx_pack2<T>(x.s, tb, 1, 1, zi, br, bc, B, ldb, V2, ldv);
+    const long ldp = rup(2L * n);
+    T* P = (T*)scratch(WS_ZP, sizeof(T) * (size_t)ldp * n);
+    const int opu = ta ? Trans : NoTrans, opv = tb ? NoTrans : Trans;
+    dev_gemmt<T>(x, uplo, opu, opv, n, 2 * k, T(1), U, ldu, V1, same ? ldu : ldv, T(0), P, ldp);
+    dev_gemmt<T>(x, uplo, opu, opv, n, 2 * k, T(1), U, ldu, V2, ldv, T(0), P + n, ldp);
+    cplx_combine<T>(x.s, m, n, P, ldp, alpha, beta, C, ldc, uplo, herm);
+    return;
+  }
   // op(A): NoTrans m x k -> [Ar -Ai; Ai Ar]; Trans (k x m stored) -> [Ar Ai;
   // -Ai Ar] read transposed; ConjTrans -> [Ar -Ai; Ai Ar] read transposed
   const bool ta = transA != NoTrans;
@@ -564,9 +597,9 @@ template <typename T>
 void dev_cherk(const Exec& x, int uplo, int trans, int n, int k, T alpha, const T* A, long lda, T beta, T* C,
                long ldc)
 {
-  // C <- alpha op(A) op(A)^H + beta C on the uplo triangle, real diagonal;
-  // the full product is formed (the real GEMM has no triangle mode for the
-  // 2n x n block output) and only the triangle is combined
+  // C <- alpha op(A) op(A)^H + beta C on the uplo triangle, real diagonal
+  // (the real part a SYRK of [Ar Ai], the imaginary part one triangle-masked
+  // product)
   const T al[2] = {alpha, T(0)}, be[2] = {beta, T(0)};
   const int ta = trans == NoTrans ? NoTrans : ConjTrans, tb = trans == NoTrans ? ConjTrans : NoTrans;
   cgemm_impl<T>(x, ta, tb, n, n, k, al, A, lda, A, lda, be, C, ldc, uplo, 1);

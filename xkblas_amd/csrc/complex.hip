@@ -49,6 +49,24 @@ __global__ __launch_bounds__(256) void cplx_pack_b_kernel(int vertical, int rows
   else Y[i + (long)(j + cols) * ldy] = s * b.im;
 }
 
+// Two real blocks of the rows x cols complex B side by side (vertical = 0:
+// Y rows x 2 cols) or stacked (vertical: Y 2 rows x cols): [sr Br, si Bi],
+// or with swap [si Bi, sr Br].
+template <typename T>
+__global__ __launch_bounds__(256) void cplx_pack2_kernel(int vertical, int swap, T sr, T si, int rows, int cols,
+                                                         int j0, const Cplx<T>* __restrict__ B, long ldb,
+                                                         T* __restrict__ Y, long ldy)
+{
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  const int j = j0 + (int)blockIdx.y;
+  if (i >= rows || j >= cols) return;
+  const Cplx<T> b = B[i + (long)j * ldb];
+  const T first = swap ? si * b.im : sr * b.re, second = swap ? sr * b.re : si * b.im;
+  Y[i + (long)j * ldy] = first;
+  if (vertical) Y[i + rows + (long)j * ldy] = second;
+  else Y[i + (long)(j + cols) * ldy] = second;
+}
+
 // C <- alpha (Pr + i Pi) + beta C, P = [Pr; Pi] real (2m x n); use_p = 0:
 // C <- beta C; beta = 0 never reads C.  uplo Lower / Upper: only that
 // triangle of the square C, with a real diagonal when herm (HERK).
@@ -245,6 +263,18 @@ void cplx_pack_b(hipStream_t s, int vertical, int sign, int rows, int cols, cons
 }
 
 template <typename T>
+void cplx_pack2(hipStream_t s, int vertical, int swap, int sr, int si, int rows, int cols, const T* B, long ldb, T* Y,
+                long ldy)
+{
+  if (rows <= 0 || cols <= 0) return;
+  over_columns(rows, cols, [&](dim3 g, int j0) {
+    hipLaunchKernelGGL((cplx_pack2_kernel<T>), g, dim3(256), 0, s, vertical, swap, T(sr), T(si), rows, cols, j0,
+                       (const Cplx<T>*)B, ldb, Y, ldy);
+  });
+  XK_HIP(hipGetLastError());
+}
+
+template <typename T>
 void cplx_combine(hipStream_t s, int m, int n, const T* P, long ldp, const T* alpha, const T* beta, T* C, long ldc,
                   int uplo, int herm)
 {
@@ -291,6 +321,7 @@ void cplx_potrf_leaf(hipStream_t s, int uplo, int n, T* A, long lda, int* info, 
   template void cplx_trtri_leaf<T>(hipStream_t, int, int, int, const T*, long, long, T*, long, long, int); \
   template void cplx_pack_a<T>(hipStream_t, int, int, int, const T*, long, T*, long);                      \
   template void cplx_pack_b<T>(hipStream_t, int, int, int, int, const T*, long, T*, long);                 \
+  template void cplx_pack2<T>(hipStream_t, int, int, int, int, int, int, const T*, long, T*, long);        \
   template void cplx_combine<T>(hipStream_t, int, int, const T*, long, const T*, const T*, T*, long, int, int);  \
   template void cplx_expand<T>(hipStream_t, int, int, int, const T*, long, T*, long);
 XK_INST(double)

@@ -73,6 +73,11 @@ template <typename T>
 void cplx_pack_a(hipStream_t s, int sigma, int rows, int cols, const T* A, long lda, T* Y, long ldy);
 template <typename T>
 void cplx_pack_b(hipStream_t s, int vertical, int sign, int rows, int cols, const T* B, long ldb, T* Y, long ldy);
+// Y <- [sr Br, si Bi] (rows x 2 cols) or, vertical, [sr Br; si Bi] (2 rows x
+// cols); swap puts the si Bi block first.
+template <typename T>
+void cplx_pack2(hipStream_t s, int vertical, int swap, int sr, int si, int rows, int cols, const T* B, long ldb, T* Y,
+                long ldy);
 // uplo Lower / Upper: only that triangle of the square C is written, with a
 // real diagonal when herm (HERK); 0: all of C.
 template <typename T>
