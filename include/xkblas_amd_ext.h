@@ -79,6 +79,21 @@ int   xkblas_batch_run(void* plan);
 int   xkblas_batch_use_graph(void* plan, int on);
 int   xkblas_batch_info(void* plan, int* nitems, int* grid, double* flops, double* bytes);
 void  xkblas_batch_destroy(void* plan);
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under hipGraph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Runs the problems like xkblas_batch_create +
+ * xkblas_batch_run + xkblas_batch_destroy, with the same kernels and results,
+ * but allocates nothing: the work counter, problem list and work list go to a
+ * fixed device array of 3.9 MB per GPU (held until the process ends) and the
+ * host-side lists to fixed host arrays.  At most 8192 problems and 196608
+ * work items (64 x 64 GEMM tiles plus 64-column TRSM panels): beyond that it
+ * prints the limit and returns -2 (problems) or -3 (items), before touching
+ * the GPU.  -1: bad arguments; -4: xkblas_init not called.  Asynchronous like
+ * xkblas_batch_run (xkblas_sync waits); the problem array may be reused as
+ * soon as the call returns.  Runs share the array: they are ordered on the
+ * library's compute stream, so calls from several threads serialise, but work
+ * that overlaps a run on the same GPU from another stream would race. */
+int   xkblas_batch_run_noalloc(const xkblas_batch_problem_t* problems, int count);
 
 /* ---- tile-cost model: 3-layer MLP (in -> 64 -> 64 -> 1, ReLU, FP32) ------
  * trained on the GPU by the library's own forward / backward / Adam kernels.

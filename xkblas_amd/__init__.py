@@ -110,6 +110,7 @@ SIGNATURES = {
     "xkblas_set_cache_limit": (None, [_sz]),
     "xkblas_batch_create": (_p, [_p, _i]),
     "xkblas_batch_run": (_i, [_p]),
+    "xkblas_batch_run_noalloc": (_i, [_p, _i]),
     "xkblas_batch_use_graph": (_i, [_p, _i]),
     "xkblas_batch_info": (_i, [_p, _p, _p, _p, _p]),
     "xkblas_batch_destroy": (None, [_p]),

@@ -24,6 +24,7 @@
 #include "runtime.h"
 #include "../../include/xkblas.h"
 #include <algorithm>
+#include <cstdio>
 #include <vector>
 
 namespace xk {
@@ -357,6 +358,124 @@ static void enqueue_run(BatchPlan* pl, hipStream_t s)
   XK_HIP(hipGetLastError());
 }
 
+// Work list of a batch: one item per 64 x 64 GEMM tile and per 64-column
+// TRSM panel, sorted by estimated cost, longest first (ties keep problem
+// order).  batch_count_items counts the items; batch_build fills hp[count]
+// and work[], returns the item count and adds the batch's flops and bytes.
+struct BatchCost {
+  double c;
+  long seq;
+  BatchItem w;
+};
+
+static long batch_count_items(const xkblas_batch_problem_t* probs, int count)
+{
+  long n = 0;
+  for (int i = 0; i < count; ++i) {
+    const xkblas_batch_problem_t& q = probs[i];
+    if (q.m <= 0 || q.n <= 0) continue;
+    n += q.kind == XKBLAS_BATCH_GEMM ? (long)((q.m + 63) / 64) * ((q.n + 63) / 64) : (long)((q.n + 63) / 64);
+  }
+  return n;
+}
+
+static long batch_build(const xkblas_batch_problem_t* probs, int count, BatchProblem* hp, BatchCost* work,
+                        double* flops, double* bytes)
+{
+  long nw = 0;
+  for (int i = 0; i < count; ++i) {
+    const xkblas_batch_problem_t& q = probs[i];
+    BatchProblem& p = hp[i];
+    p.kind = q.kind; p.dtype = q.dtype; p.ta = q.transA; p.tb = q.transB;
+    p.m = q.m; p.n = q.n; p.k = q.k; p.alpha = q.alpha; p.beta = q.beta;
+    p.A = q.A; p.lda = q.lda; p.B = q.B; p.ldb = q.ldb; p.C = q.C; p.ldc = q.ldc;
+    const double elt = q.dtype == 0 ? 8.0 : 4.0;
+    XK_CHECK(q.kind == XKBLAS_BATCH_GEMM || q.kind == XKBLAS_BATCH_TRSM, "batch: unknown problem kind");
+    XK_CHECK(q.dtype == 0 || q.dtype == 1, "batch: dtype must be 0 (FP64) or 1 (FP32)");
+    if (q.m <= 0 || q.n <= 0) continue;
+    if (q.kind == XKBLAS_BATCH_GEMM) {
+      const int tmx = (q.m + 63) / 64, tnx = (q.n + 63) / 64;
+      for (int a = 0; a < tmx; ++a)
+        for (int b = 0; b < tnx; ++b) {
+          // 64 x 64 tile: k-loop work plus a fixed overhead
+          work[nw] = BatchCost{2.0 * 64 * 64 * std::max(q.k, 1) + 64 * 64 * 8.0, nw, BatchItem{i, a, b, 0}};
+          ++nw;
+        }
+      *flops += 2.0 * q.m * q.n * (double)q.k;
+      *bytes += elt * ((double)q.m * q.k + (double)q.k * q.n + 2.0 * q.m * q.n);
+    } else {
+      const int np = (q.n + 63) / 64;
+      // left-looking blocks: one tile of k = 64 kb each plus the block's
+      // solve, priced like a tile of k = 128
+      const double nbk = (q.m + 63) / 64;
+      const double c = 2.0 * 64 * 64 * 64 * nbk * (nbk - 1) / 2 + nbk * 2.0 * 64 * 64 * 128;
+      for (int b = 0; b < np; ++b) {
+        work[nw] = BatchCost{c, nw, BatchItem{i, 0, b, 0}};
+        ++nw;
+      }
+      *flops += (double)q.m * q.m * q.n;
+      *bytes += elt * (0.5 * (double)q.m * q.m + 2.0 * q.m * q.n);
+    }
+  }
+  std::sort(work, work + nw,
+            [](const BatchCost& a, const BatchCost& b) { return a.c > b.c || (a.c == b.c && a.seq < b.seq); });
+  return nw;
+}
+
+// workgroups of the persistent kernel: as many as can be resident at once
+static int batch_grid(long nitems)
+{
+  int dev = 0, cus = 0, per_cu = 0;
+  XK_HIP(hipGetDevice(&dev));
+  XK_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  XK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_kernel, 256, 0));
+  return (int)std::max(1L, std::min((long)cus * std::max(per_cu, 1), std::max(nitems, 1L)));
+}
+
+// ---- allocation-free runs
+// xkblas_batch_run_noalloc serves callers that cannot allocate device
+// memory: its counter, problem list and work list live in one fixed device
+// array (`scratch`, below; one copy per GPU, held until the process ends),
+// whose device address hipGetSymbolAddress gives at initialisation.  The
+// host-side lists are built in fixed host arrays, so the call allocates
+// nothing on the host either.  Calls share the array: they are ordered on
+// the library's compute stream, and calls that overlap on one GPU from
+// other streams would race.
+constexpr int NOALLOC_MAX_PROBS = 8192;
+constexpr long NOALLOC_MAX_ITEMS = 196608;
+constexpr size_t NOALLOC_PROBS_OFF = 256;  // the work counter comes first
+constexpr size_t NOALLOC_ITEMS_OFF = NOALLOC_PROBS_OFF + sizeof(BatchProblem) * NOALLOC_MAX_PROBS;
+constexpr size_t NOALLOC_BYTES = NOALLOC_ITEMS_OFF + sizeof(BatchItem) * NOALLOC_MAX_ITEMS;
+constexpr int NOALLOC_MAX_DEVICES = 64;
+
+static char* g_noalloc_dev[NOALLOC_MAX_DEVICES];  // device address of `scratch`, per GPU
+static BatchProblem g_noalloc_probs[NOALLOC_MAX_PROBS];
+static BatchCost g_noalloc_work[NOALLOC_MAX_ITEMS];
+static BatchItem g_noalloc_items[NOALLOC_MAX_ITEMS];
+static hipEvent_t g_noalloc_staged = nullptr;  // the last run's copies have read the host lists
+
+}  // namespace xk
+
+// 3.9 MB: counter, NOALLOC_MAX_PROBS problems, NOALLOC_MAX_ITEMS work items.
+// In host code this name is only the symbol's host-side handle: kernels get
+// the address hipGetSymbolAddress returns.
+static __device__ __align__(256) char scratch[xk::NOALLOC_BYTES];
+
+namespace xk {
+
+void batch_noalloc_init()
+{
+  int dev = 0;
+  XK_HIP(hipGetDevice(&dev));
+  XK_CHECK(dev >= 0 && dev < NOALLOC_MAX_DEVICES, "device ordinal beyond the allocation-free table");
+  if (g_noalloc_dev[dev]) return;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(::scratch));
+  if (e != hipSuccess || !p) fatal("hipGetSymbolAddress(scratch)", __FILE__, __LINE__, hipGetErrorString(e));
+  g_noalloc_dev[dev] = static_cast<char*>(p);
+  if (!g_noalloc_staged) XK_HIP(hipEventCreateWithFlags(&g_noalloc_staged, hipEventDisableTiming));
+}
+
 }  // namespace xk
 
 using namespace xk;
@@ -368,67 +487,19 @@ void* xkblas_batch_create(const xkblas_batch_problem_t* probs, int count)
   if (count < 0 || (count > 0 && !probs)) return nullptr;
   xkblas_init();
   BatchPlan* pl = new BatchPlan();
-  std::vector<BatchProblem> hp(count);
-  struct Cost {
-    double c;
-    BatchItem w;
-  };
-  std::vector<Cost> work;
-  for (int i = 0; i < count; ++i) {
-    const xkblas_batch_problem_t& q = probs[i];
-    BatchProblem& p = hp[i];
-    p.kind = q.kind; p.dtype = q.dtype; p.ta = q.transA; p.tb = q.transB;
-    p.m = q.m; p.n = q.n; p.k = q.k; p.alpha = q.alpha; p.beta = q.beta;
-    p.A = q.A; p.lda = q.lda; p.B = q.B; p.ldb = q.ldb; p.C = q.C; p.ldc = q.ldc;
-    const double elt = q.dtype == 0 ? 8.0 : 4.0;
-    XK_CHECK(q.kind == XKBLAS_BATCH_GEMM || q.kind == XKBLAS_BATCH_TRSM, "batch: unknown problem kind");
-    XK_CHECK(q.dtype == 0 || q.dtype == 1, "batch: dtype must be 0 (FP64) or 1 (FP32)");
-    if (q.kind == XKBLAS_BATCH_GEMM) {
-      if (q.m <= 0 || q.n <= 0) continue;
-      const int tmx = (q.m + 63) / 64, tnx = (q.n + 63) / 64;
-      for (int a = 0; a < tmx; ++a)
-        for (int b = 0; b < tnx; ++b) {
-          Cost c;
-          c.w = BatchItem{i, a, b, 0};
-          // 64 x 64 tile: k-loop work plus a fixed overhead
-          c.c = 2.0 * 64 * 64 * std::max(q.k, 1) + 64 * 64 * 8.0;
-          work.push_back(c);
-        }
-      pl->flops += 2.0 * q.m * q.n * (double)q.k;
-      pl->bytes += elt * ((double)q.m * q.k + (double)q.k * q.n + 2.0 * q.m * q.n);
-    } else {
-      if (q.m <= 0 || q.n <= 0) continue;
-      const int np = (q.n + 63) / 64;
-      for (int b = 0; b < np; ++b) {
-        Cost c;
-        c.w = BatchItem{i, 0, b, 0};
-        // left-looking blocks: one tile of k = 64 kb each plus the block's
-        // solve, priced like a tile of k = 128
-        const double nbk = (q.m + 63) / 64;
-        c.c = 2.0 * 64 * 64 * 64 * nbk * (nbk - 1) / 2 + nbk * 2.0 * 64 * 64 * 128;
-        work.push_back(c);
-      }
-      pl->flops += (double)q.m * q.m * q.n;
-      pl->bytes += elt * (0.5 * (double)q.m * q.m + 2.0 * q.m * q.n);
-    }
-  }
-  std::stable_sort(work.begin(), work.end(), [](const Cost& a, const Cost& b) { return a.c > b.c; });
-  std::vector<BatchItem> items(work.size());
-  for (size_t i = 0; i < work.size(); ++i) items[i] = work[i].w;
-  pl->nitems = (int)items.size();
+  std::vector<BatchProblem> hp(std::max(count, 1));
+  std::vector<BatchCost> work(std::max(batch_count_items(probs, count), 1L));
+  const long nw = batch_build(probs, count, hp.data(), work.data(), &pl->flops, &pl->bytes);
+  std::vector<BatchItem> items(std::max(nw, 1L));
+  for (long i = 0; i < nw; ++i) items[i] = work[i].w;
+  pl->nitems = (int)nw;
   pl->nprobs = count;
   XK_HIP(hipMalloc(&pl->d_probs, sizeof(BatchProblem) * std::max(count, 1)));
-  XK_HIP(hipMalloc(&pl->d_items, sizeof(BatchItem) * std::max((int)items.size(), 1)));
+  XK_HIP(hipMalloc(&pl->d_items, sizeof(BatchItem) * items.size()));
   XK_HIP(hipMalloc(&pl->d_counter, sizeof(int)));
   if (count) XK_HIP(hipMemcpy(pl->d_probs, hp.data(), sizeof(BatchProblem) * count, hipMemcpyHostToDevice));
-  if (!items.empty())
-    XK_HIP(hipMemcpy(pl->d_items, items.data(), sizeof(BatchItem) * items.size(), hipMemcpyHostToDevice));
-  // as many workgroups as can be resident at once
-  int dev = 0, cus = 0, per_cu = 0;
-  XK_HIP(hipGetDevice(&dev));
-  XK_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-  XK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_kernel, 256, 0));
-  pl->grid = std::max(1, std::min(cus * std::max(per_cu, 1), std::max(pl->nitems, 1)));
+  if (nw) XK_HIP(hipMemcpy(pl->d_items, items.data(), sizeof(BatchItem) * nw, hipMemcpyHostToDevice));
+  pl->grid = batch_grid(nw);
   return pl;
 }
 
@@ -443,6 +514,50 @@ int xkblas_batch_run(void* plan)
   if (pl->exec) XK_HIP(hipGraphLaunch(pl->exec, r.s_comp));
   else enqueue_run(pl, r.s_comp);
   r.prof_end();
+  return 0;
+}
+
+int xkblas_batch_run_noalloc(const xkblas_batch_problem_t* problems, int count)
+{
+  if (count < 0 || (count > 0 && !problems)) return -1;
+  if (count > NOALLOC_MAX_PROBS) {
+    fprintf(stderr, "[xkblas_amd] xkblas_batch_run_noalloc: %d problems exceed the maximum of %d\n", count,
+            NOALLOC_MAX_PROBS);
+    return -2;
+  }
+  const long nw = batch_count_items(problems, count);
+  if (nw > NOALLOC_MAX_ITEMS) {
+    fprintf(stderr, "[xkblas_amd] xkblas_batch_run_noalloc: %ld work items exceed the maximum of %ld\n", nw,
+            NOALLOC_MAX_ITEMS);
+    return -3;
+  }
+  std::lock_guard<std::mutex> lock(api_mutex());
+  Runtime& r = rt();
+  if (!r.device_ready) {
+    fprintf(stderr, "[xkblas_amd] xkblas_batch_run_noalloc: call xkblas_init first\n");
+    return -4;
+  }
+  int dev = 0;
+  XK_HIP(hipGetDevice(&dev));
+  XK_CHECK(dev >= 0 && dev < NOALLOC_MAX_DEVICES && g_noalloc_dev[dev], "allocation-free array not set up");
+  char* base = g_noalloc_dev[dev];
+  // the previous run's copies must have read the host lists before they change
+  XK_HIP(hipEventSynchronize(g_noalloc_staged));
+  double flops = 0.0, bytes = 0.0;
+  batch_build(problems, count, g_noalloc_probs, g_noalloc_work, &flops, &bytes);
+  for (long i = 0; i < nw; ++i) g_noalloc_items[i] = g_noalloc_work[i].w;
+  BatchProblem* d_probs = reinterpret_cast<BatchProblem*>(base + NOALLOC_PROBS_OFF);
+  BatchItem* d_items = reinterpret_cast<BatchItem*>(base + NOALLOC_ITEMS_OFF);
+  int* d_counter = reinterpret_cast<int*>(base);
+  r.before_compute();
+  if (count)
+    XK_HIP(hipMemcpyAsync(d_probs, g_noalloc_probs, sizeof(BatchProblem) * count, hipMemcpyHostToDevice, r.s_comp));
+  if (nw) XK_HIP(hipMemcpyAsync(d_items, g_noalloc_items, sizeof(BatchItem) * nw, hipMemcpyHostToDevice, r.s_comp));
+  XK_HIP(hipEventRecord(g_noalloc_staged, r.s_comp));
+  fill_int(r.s_comp, d_counter, 1, 0);
+  hipLaunchKernelGGL(batch_kernel, dim3(batch_grid(nw)), dim3(256), 0, r.s_comp, d_probs, d_items, (int)nw,
+                     d_counter);
+  XK_HIP(hipGetLastError());
   return 0;
 }
 

@@ -48,6 +48,9 @@ void zero_triangle(hipStream_t s, int uplo, int n, T* A, long lda);
 
 // Fill an int array with a value (stream-ordered, graph-capturable).
 void fill_int(hipStream_t s, int* p, int n, int v);
+// device address of the allocation-free batched run's fixed array, for the
+// current device (called once per device at initialisation)
+void batch_noalloc_init();
 
 // Y (and Z when non-null) <- the full symmetric n x n matrix whose uplo
 // triangle is stored in A.

@@ -817,6 +817,7 @@ static void ensure_init()
   Runtime& r = rt();
   if (!r.device_ready) {
     r.init_device();
+    batch_noalloc_init();
     const char* ts = getenv("XKBLAS_TILE_SIZE");
     if (ts && *ts) r.nb = (size_t)atol(ts);
     const char* dm = getenv("XKBLAS_DEFAULT_MATH");
