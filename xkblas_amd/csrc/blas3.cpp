@@ -208,6 +208,8 @@ void dev_potrf(const Exec& x, int uplo, int n, T* A, long lda, int nb, int* info
     XK_HIP(hipEventRecord(ev_look, x.s));
     XK_HIP(hipStreamWaitEvent(sd, ev_look, 0));
   }
+  // the chain's products issue ahead of the trailing update's waves
+  if (lookahead) gemm_set_priority_stream(sd);
   // the inverses' blocks above the diagonal are never written with nonzeros:
   // one fill here replaces a fill per recursion node
   fill_zero<T>(sd, nb, 2 * nb, inv, nb);
@@ -299,6 +301,7 @@ void dev_potrf(const Exec& x, int uplo, int n, T* A, long lda, int nb, int* info
                     info, (k + 1) * nb, WS_DIAG0, true);
   }
   if (split) split_cache_clear();
+  if (lookahead) gemm_set_priority_stream(nullptr);
   if (own_events) {
     XK_HIP(hipEventDestroy(ev_diag));
     XK_HIP(hipEventDestroy(ev_look));

@@ -68,34 +68,48 @@ __device__ __forceinline__ void lower_inverse_to(const T* Ls, int lane, bool uni
 }
 
 // Cholesky factor and its inverse for one block of order <= 64, blocked by
-// 16 and spread over four waves.  The block and its inverse live in LDS
-// (column-major, stride LLD).  For each 16-column block jb:
+// 16 and spread over four waves.
+// LDS: the lower triangle packed by rows, P[i (i + 1) / 2 + k] = L(i, k),
+// and one 64 x 16 block column of the inverse: 24.3 KB in FP64.  A
+// trailing-update GEMM that fills a CU (four 33 KB workgroups) leaves 28 KB
+// of LDS free, so a leaf launched beside it starts at once in that gap
+// instead of waiting for a CU to drain (a leaf holding the block and its
+// inverse unpacked, 72 KB, waited up to milliseconds).
+// For each 16-column block jb:
 //   wave 0 factors the 16 x 16 diagonal block in registers (lane i holds row
-//   i, cross-lane reads for the broadcasts: at most 15 per step) and forms
-//   its inverse (lane c, column c), which is also the diagonal block of
-//   inv(L);
+//   i, cross-lane reads for the broadcasts: at most 15 per step), writes it
+//   to A and puts its inverse (lane c, column c) in its place in P;
 //   all threads solve the panel below, L21 = A21 inv(L11)^T, and update the
 //   trailing triangle, A22 -= L21 L21^T.
-// The off-diagonal blocks of inv(L) follow block row by block row:
-//   inv(L)[ib][jb] = -inv(L_ib,ib) sum_{t=jb}^{ib-1} L[ib][t] inv(L)[t][jb].
-// Rows/columns beyond n are the identity.  info receives the first order
-// whose pivot is not positive.
+// Then P holds L below the diagonal blocks and the diagonal blocks'
+// inverses; inv(L) follows by block columns jb: X_jb = inv(L_jb,jb), X_ib =
+// -inv(L_ib,ib) sum_{t=jb}^{ib-1} L[ib][t] X_t.  Rows/columns beyond n are the
+// identity.  info receives the first order whose pivot is not positive.
 constexpr int LB = 16;  // inner block of the leaf
+
+__device__ __forceinline__ int pk(int i, int k) { return (i * (i + 1) >> 1) + k; }
 
 template <typename T>
 __global__ __launch_bounds__(256) void potrf_leaf_kernel(int uplo, int n, T* A, long lda, long sA, T* Tinv,
                                                          long ldinv, long sInv, int* info, int base)
 {
-  __shared__ T Ls[LEAF * LLD];  // the factor
-  __shared__ T Is[LEAF * LLD];  // its inverse
-  __shared__ T Ts[3 * LB * LB]; // block-row partial products of the inverse
+  __shared__ T P[LEAF * (LEAF + 1) / 2];
+  __shared__ T X[LEAF * LB];  // X[i * LB + c]: rows of one block column of inv(L)
+  // the leaf is the latency-bound step of the diagonal chain and shares its
+  // CU with update GEMM waves: its waves issue first
+  __builtin_amdgcn_s_setprio(3);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   A += (long)blockIdx.x * sA;
   Tinv += (long)blockIdx.x * sInv;
+  auto put = [&](int i, int k, T v) {  // factor entry (i, k), k <= i, to the stored triangle
+    if (i < n && k < n) {
+      if (uplo == Lower) A[i + (long)k * lda] = v;
+      else A[k + (long)i * lda] = v;
+    }
+  };
   for (int e = tid; e < LEAF * LEAF; e += 256) {
     const int i = e & 63, k = e >> 6;
-    Ls[i + k * LLD] = lower_view(A, lda, uplo, n, i, k);
-    Is[i + k * LLD] = T(0);
+    if (k <= i) P[pk(i, k)] = lower_view(A, lda, uplo, n, i, k);
   }
   __syncthreads();
 
@@ -106,7 +120,7 @@ __global__ __launch_bounds__(256) void potrf_leaf_kernel(int uplo, int n, T* A, 
       const int i = lane & (LB - 1);
       T d[LB];
 #pragma unroll
-      for (int k = 0; k < LB; ++k) d[k] = Ls[(c0 + i) + (c0 + k) * LLD];
+      for (int k = 0; k < LB; ++k) d[k] = k <= i ? P[pk(c0 + i, c0 + k)] : T(0);
 #pragma unroll
       for (int j = 0; j < LB; ++j) {
         const T p = bcast(d[j], j);
@@ -129,9 +143,12 @@ __global__ __launch_bounds__(256) void potrf_leaf_kernel(int uplo, int n, T* A, 
       }
       if (lane < LB) {
 #pragma unroll
-        for (int k = 0; k < LB; ++k) Ls[(c0 + lane) + (c0 + k) * LLD] = k <= lane ? d[k] : T(0);
+        for (int k = 0; k < LB; ++k)
+          if (k <= lane) put(c0 + lane, c0 + k, d[k]);
+        // (the whole block was read into d before any lane writes)
 #pragma unroll
-        for (int r = 0; r < LB; ++r) Is[(c0 + r) + (c0 + lane) * LLD] = y[r];
+        for (int r = 0; r < LB; ++r)
+          if (r >= lane) P[pk(c0 + r, c0 + lane)] = y[r];
       }
     }
     __syncthreads();
@@ -145,11 +162,10 @@ __global__ __launch_bounds__(256) void potrf_leaf_kernel(int uplo, int n, T* A, 
         pv[u] = T(0);
         if (e < m * LB) {
           const int r = e % m, c = e / m;
-          // all 16 terms: inv(L11)[c][t] is 0 for t > c
+          const T* arow = P + pk(c0 + LB + r, c0);
+          const T* irow = P + pk(c0 + c, c0);
           T acc = T(0);
-#pragma unroll
-          for (int t = 0; t < LB; ++t)
-            acc = fma(Ls[(c0 + LB + r) + (c0 + t) * LLD], Is[(c0 + c) + (c0 + t) * LLD], acc);
+          for (int t = 0; t <= c; ++t) acc = fma(arow[t], irow[t], acc);
           pv[u] = acc;
         }
       }
@@ -157,73 +173,59 @@ __global__ __launch_bounds__(256) void potrf_leaf_kernel(int uplo, int n, T* A, 
 #pragma unroll
       for (int u = 0; u < 3; ++u) {
         const int e = tid + 256 * u;
-        if (e < m * LB) Ls[(c0 + LB + e % m) + (c0 + e / m) * LLD] = pv[u];
+        if (e < m * LB) P[pk(c0 + LB + e % m, c0 + e / m)] = pv[u];
       }
       __syncthreads();
       // trailing triangle: L[c0+LB+r][c0+LB+q] -= sum_t L21[r][t] L21[q][t], r >= q
       for (int e = tid; e < m * m; e += 256) {
         const int r = e % m, q = e / m;
         if (r >= q) {
-          T acc = Ls[(c0 + LB + r) + (c0 + LB + q) * LLD];
+          const T* rr = P + pk(c0 + LB + r, c0);
+          const T* rq = P + pk(c0 + LB + q, c0);
+          T acc = P[pk(c0 + LB + r, c0 + LB + q)];
 #pragma unroll
-          for (int t = 0; t < LB; ++t)
-            acc = fma(-Ls[(c0 + LB + r) + (c0 + t) * LLD], Ls[(c0 + LB + q) + (c0 + t) * LLD], acc);
-          Ls[(c0 + LB + r) + (c0 + LB + q) * LLD] = acc;
+          for (int t = 0; t < LB; ++t) acc = fma(-rr[t], rq[t], acc);
+          P[pk(c0 + LB + r, c0 + LB + q)] = acc;
         }
       }
       __syncthreads();
     }
   }
-  // factor back to the stored triangle
+  // the factor below the diagonal blocks to the stored triangle
   for (int e = tid; e < LEAF * LEAF; e += 256) {
     const int i = e & 63, k = e >> 6;
-    if (i < n && k < n && k <= i) {
-      if (uplo == Lower) A[i + (long)k * lda] = Ls[i + k * LLD];
-      else A[k + (long)i * lda] = Ls[i + k * LLD];
-    }
+    if (k < (i & ~(LB - 1))) put(i, k, P[pk(i, k)]);
   }
   if (!Tinv) return;
-  for (int ib = 1; ib < LEAF / LB; ++ib) {
-    // Ts[jb] = sum_{t=jb}^{ib-1} L[ib][t] inv(L)[t][jb], jb < ib
-#pragma unroll
-    for (int u = 0; u < 3; ++u) {
-      const int e = tid + 256 * u;
-      if (e < ib * LB * LB) {
-        const int jb = e >> 8, r = e & (LB - 1), c = (e >> 4) & (LB - 1);
-        T acc = T(0);
-        for (int tb = jb; tb < ib; ++tb) {
-#pragma unroll
-          for (int t = 0; t < LB; ++t) {
-            const int s = tb * LB + t;
-            acc = fma(Ls[(ib * LB + r) + s * LLD], Is[s + (jb * LB + c) * LLD], acc);
-          }
-        }
-        Ts[e] = acc;
-      }
-    }
-    __syncthreads();
-    // inv(L)[ib][jb] = -inv(L_ib,ib) Ts[jb]
-#pragma unroll
-    for (int u = 0; u < 3; ++u) {
-      const int e = tid + 256 * u;
-      if (e < ib * LB * LB) {
-        const int jb = e >> 8, r = e & (LB - 1), c = (e >> 4) & (LB - 1);
-        T acc = T(0);  // all 16 terms: the diagonal block's inverse is 0 above its diagonal
-#pragma unroll
-        for (int s = 0; s < LB; ++s)
-          acc = fma(Is[(ib * LB + r) + (ib * LB + s) * LLD], Ts[(jb << 8) + s + (c << 4)], acc);
-        Is[(ib * LB + r) + (jb * LB + c) * LLD] = -acc;
-      }
-    }
-    __syncthreads();
-  }
-  // column j of inv(L) (zeros above the diagonal).  Lower: Tinv[i + j*ld];
-  // Upper (inv(U) = inv(L)^T): Tinv[j + i*ld].
-  for (int e = tid; e < LEAF * LEAF; e += 256) {
-    const int i = e & 63, j = e >> 6;
+  // inv(L) by block columns; thread (r, c) owns entry (r, c) of a 16 x 16 block
+  const int r = tid & (LB - 1), c = tid >> 4;
+  auto out = [&](int i, int j, T v) {  // inv(L)(i, j) to Tinv (Upper: inv(U) = inv(L)^T)
     if (i < n && j < n) {
-      if (uplo == Lower) Tinv[i + (long)j * ldinv] = Is[i + j * LLD];
-      else Tinv[j + (long)i * ldinv] = Is[i + j * LLD];
+      if (uplo == Lower) Tinv[i + (long)j * ldinv] = v;
+      else Tinv[j + (long)i * ldinv] = v;
+    }
+  };
+  for (int jb = 0; jb < LEAF / LB; ++jb) {
+    const int j0 = jb * LB;
+    const T xd = c <= r ? P[pk(j0 + r, j0 + c)] : T(0);  // inv(L_jb,jb)
+    X[(j0 + r) * LB + c] = xd;
+    out(j0 + r, j0 + c, xd);
+    for (int ib = 0; ib < jb; ++ib) out(ib * LB + r, j0 + c, T(0));
+    __syncthreads();
+    for (int ib = jb + 1; ib < LEAF / LB; ++ib) {
+      const int i0 = ib * LB;
+      T acc = T(0);
+      const T* lrow = P + pk(i0 + r, 0);
+      for (int t = j0; t < i0; ++t) acc = fma(lrow[t], X[t * LB + c], acc);
+      X[(i0 + r) * LB + c] = acc;
+      __syncthreads();
+      T v = T(0);
+      const T* drow = P + pk(i0 + r, i0);
+      for (int s2 = 0; s2 <= r; ++s2) v = fma(drow[s2], X[(i0 + s2) * LB + c], v);
+      __syncthreads();
+      X[(i0 + r) * LB + c] = -v;
+      out(i0 + r, j0 + c, -v);
+      __syncthreads();
     }
   }
 }

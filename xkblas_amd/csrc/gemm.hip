@@ -50,6 +50,7 @@ struct GemmParams {
   int tri;
   int kmode;       // inner range per tile for a triangular operand (KRANGE_*)
   int vecA, vecB;  // 16-byte vector loads allowed (base and ld aligned)
+  int prio;        // raise the waves' issue priority (latency-bound chain steps)
   T alpha, beta;
   const T* A;
   long lda, sA;
@@ -86,6 +87,7 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmParams<T> p)
 
   __shared__ T As[2][BK][LA];
   __shared__ T Bs[2][BK][LB];
+  if (p.prio) __builtin_amdgcn_s_setprio(3);
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -265,6 +267,11 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmParams<T> p)
 
 static int g_force_tile = 0;
 void gemm_set_tile(int bm) { g_force_tile = bm; }
+
+// Products launched on this stream run their waves at raised issue priority
+// (the latency-bound diagonal chain of POTRF, beside full-chip updates).
+static hipStream_t g_prio_stream = nullptr;
+void gemm_set_priority_stream(hipStream_t s) { g_prio_stream = s; }
 int gemm_get_tile() { return g_force_tile; }
 
 template <typename T, int BM, int BK>
@@ -298,6 +305,7 @@ void gemm(hipStream_t s, int transA, int transB, int m, int n, int k, T alpha, c
   auto aligned = [](const void* q, long ld, long stride) {
     return ((uintptr_t)q % 16 == 0) && ((ld * (long)sizeof(T)) % 16 == 0) && ((stride * (long)sizeof(T)) % 16 == 0);
   };
+  p.prio = s == g_prio_stream && s != nullptr;
   p.vecA = aligned(A, lda, sA);
   p.vecB = aligned(B, ldb, sB);
   const bool ta = transA != NoTrans, tb = transB != NoTrans;

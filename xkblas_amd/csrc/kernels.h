@@ -48,6 +48,10 @@ void zero_triangle(hipStream_t s, int uplo, int n, T* A, long lda);
 
 // Fill an int array with a value (stream-ordered, graph-capturable).
 void fill_int(hipStream_t s, int* p, int n, int v);
+// Products enqueued on stream s (null: none) raise their waves' issue
+// priority: the latency-bound diagonal chain of POTRF beside full-chip
+// trailing updates.
+void gemm_set_priority_stream(hipStream_t s);
 // device address of the allocation-free batched run's fixed array, for the
 // current device (called once per device at initialisation)
 void batch_noalloc_init();
