@@ -29,6 +29,8 @@ sys.path.insert(0, ROOT)
 FP64_PEAK = 78.6  # TFLOP/s, MI355X dense FP64 MFMA (AMD datasheet)
 BF16_PEAK = 2500.0
 FP8_PEAK = 5000.0
+FP32_PEAK = 157.3  # TFLOP/s, dense FP32 MFMA
+HBM_PEAK = 8000.0  # GB/s, HBM3E
 
 
 def profiled_traffic(section="split-BF16"):
@@ -555,8 +557,31 @@ def bench_batched(xk, seed=2027, reps=5, parts=False):
             assert L.xkblas_memcpy_htod(pool, host.ctypes.data_as(p), host.nbytes) == 0
 
         upload()
-        plan = L.xkblas_batch_create(ctypes.cast(arr, p), len(arr))
+        # GEMM item shapes (xkblas_set_batch_tile): 64 x 64 only, per problem
+        # (the default), 32 x 32 only; each timed as a graph-replayed launch
+        mode_ms = {}
+        for mode in (0, 2, 1):
+            L.xkblas_set_batch_tile(mode)
+            mp = L.xkblas_batch_create(ctypes.cast(arr, p), len(arr))
+            assert mp and L.xkblas_batch_use_graph(mp, 1) == 0
+            L.xkblas_batch_run(mp)
+            L.xkblas_sync()
+            tm = []
+            for _ in range(reps):
+                upload()
+                L.xkblas_sync()
+                L.xkblas_profile_reset()
+                L.xkblas_batch_run(mp)
+                L.xkblas_sync()
+                tm.append(xk.profile("batch")[1])
+            mode_ms[{0: "tiles_64", 1: "tiles_per_problem", 2: "tiles_32"}[mode]] = round(min(tm), 3)
+            L.xkblas_batch_use_graph(mp, 0)
+            L.xkblas_batch_destroy(mp)
+        upload()
+        plan = L.xkblas_batch_create(ctypes.cast(arr, p), len(arr))  # default mode 1
         assert plan
+        nit, grid, pfl, pby = ctypes.c_int(), ctypes.c_int(), ctypes.c_double(), ctypes.c_double()
+        L.xkblas_batch_info(plan, ctypes.byref(nit), ctypes.byref(grid), ctypes.byref(pfl), ctypes.byref(pby))
         assert L.xkblas_batch_use_graph(plan, 1) == 0
         L.xkblas_batch_run(plan)
         L.xkblas_sync()
@@ -610,7 +635,27 @@ def bench_batched(xk, seed=2027, reps=5, parts=False):
             L.xkblas_batch_destroy(pl)
         L.xkblas_device_free(pool)
         best = min(ms)
+        # roofline of the batch: its flops at the MFMA peak of the type or its
+        # algorithmic bytes (operands read once, C read and written) at HBM
+        # bandwidth, whichever is longer
+        peak = FP64_PEAK if code == 0 else FP32_PEAK
+        t_flop, t_byte = pfl.value / (peak * 1e9), pby.value / (HBM_PEAK * 1e6)  # ms
+        # GEMM tiles (64 x 64, or 32 x 32 where that covers <= 3/4 of the
+        # area, the default rule) with k in slices of 16: the MFMA work the
+        # GEMM items execute
+        def padded(m, n):
+            a64 = -(-m // 64) * -(-n // 64) * 4096
+            a32 = -(-m // 32) * -(-n // 32) * 1024
+            return a32 if 4 * a32 <= 3 * a64 else a64
+        gemm_exec = float(sum(2.0 * padded(m, n) * ((k + 15) // 16 * 16) for m, n, k in g))
+        gemm_alg = float(sum(2.0 * m * n * k for m, n, k in g))
         out[name] = {"ms": round(best, 3), "gflops": round(flops / best / 1e6, 1), "problems": len(arr),
+                     "work_items": nit.value, "workgroups": grid.value, "gemm_item_shapes_ms": mode_ms,
+                     "roofline": {"bound": "mfma" if t_flop >= t_byte else "hbm",
+                                  "flops": pfl.value, "bytes": pby.value,
+                                  "roofline_ms": round(max(t_flop, t_byte), 4),
+                                  "frac": round(max(t_flop, t_byte) / best, 4),
+                                  "gemm_executed_over_algorithmic_flops": round(gemm_exec / gemm_alg, 3)},
                      "one_launch_per_problem_ms": round(t_each * 1e3, 3),
                      "speedup_vs_one_launch_per_problem": round(t_each * 1e3 / best, 2),
                      "max_rel_err_sampled": err, **part_ms}

@@ -56,7 +56,8 @@ void xkblas_set_cache_limit(size_t bytes);
  * TRSM: Left, Lower, NoTrans, Non-unit: L X = alpha B with L = A (m x m,
  *       lda) and B = C (m x n, ldc) overwritten by X.
  * dtype 0 = FP64, 1 = FP32 (mixed in one batch).  A plan holds the work list
- * (64 x 64 GEMM tiles and 64-column TRSM panels, longest first) on the
+ * (64 x 64 or 32 x 32 GEMM tiles and 64-column TRSM panels, longest
+ * first) on the
  * device; xkblas_batch_run enqueues one counter fill and one persistent
  * kernel on the library's compute stream (xkblas_sync waits for it);
  * xkblas_batch_use_graph(plan, 1) captures that pair into a hipGraph that
@@ -75,6 +76,11 @@ typedef struct xkblas_batch_problem {
 } xkblas_batch_problem_t;
 
 void* xkblas_batch_create(const xkblas_batch_problem_t* problems, int count);
+/* GEMM work-item shape of plans created afterwards: 0 = 64 x 64 tiles only,
+ * 1 (default) = 32 x 32 tiles for a problem whose 32 x 32 tiling covers at
+ * most 3/4 of the area of its 64 x 64 one (small m or n), 2 = 32 x 32 only;
+ * environment XKBLAS_BATCH_TILE.  A knob of the tile-cost model. */
+void  xkblas_set_batch_tile(int mode);
 int   xkblas_batch_run(void* plan);
 int   xkblas_batch_use_graph(void* plan, int on);
 int   xkblas_batch_info(void* plan, int* nitems, int* grid, double* flops, double* bytes);

@@ -111,6 +111,7 @@ SIGNATURES = {
     "xkblas_batch_create": (_p, [_p, _i]),
     "xkblas_batch_run": (_i, [_p]),
     "xkblas_batch_run_noalloc": (_i, [_p, _i]),
+    "xkblas_set_batch_tile": (None, [_i]),
     "xkblas_batch_use_graph": (_i, [_p, _i]),
     "xkblas_batch_info": (_i, [_p, _p, _p, _p, _p]),
     "xkblas_batch_destroy": (None, [_p]),

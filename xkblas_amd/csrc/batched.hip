@@ -49,7 +49,7 @@ struct BatchProblem {
 struct BatchItem {
   int prob;
   int tm, tn;  // GEMM tile indices; TRSM: tn = panel index
-  int pad;
+  int tile;    // GEMM tile order (64 or 32)
 };
 
 __device__ __forceinline__ bd4 bmfma(double a, double b, bd4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
@@ -78,29 +78,32 @@ __device__ __forceinline__ int item_tid()
   return t;
 }
 
-// Main loop of the 64 x 64 tile (tm, tn) of alpha op(A) op(B): 4 waves as
-// 2 x 2 quarters of 32 x 32, each 2 x 2 MFMA blocks of 16 x 16; 16-deep k
-// slices double-buffered in LDS, the next slice prefetched into registers
-// while the current one is multiplied.  Ends with a barrier after the last
-// LDS read, so the caller may overwrite `smem` at once.
-template <typename T>
-__device__ __forceinline__ void tile64_mainloop(int ta, int tb, int M, int N, int K, const T* __restrict__ A, long lda,
-                                                const T* __restrict__ B, long ldb, int m0, int n0, T* smem,
-                                                typename BAcc<T>::type (&acc)[2][2])
+// Main loop of the TBM x TBM tile (TBM = 64 or 32) at (m0, n0) of alpha
+// op(A) op(B): 4 waves as 2 x 2 quarters of TBM/2 x TBM/2, each TQ x TQ MFMA
+// blocks of 16 x 16 (TQ = TBM / 32); 16-deep k slices double-buffered in
+// LDS, the next slice prefetched into registers while the current one is
+// multiplied.  Ends with a barrier after the last LDS read, so the caller
+// may overwrite `smem` at once.
+template <typename T, int TBM>
+__device__ __forceinline__ void tile_mainloop(int ta, int tb, int M, int N, int K, const T* __restrict__ A, long lda,
+                                              const T* __restrict__ B, long ldb, int m0, int n0, T* smem,
+                                              typename BAcc<T>::type (&acc)[TBM / 32][TBM / 32])
 {
+  constexpr int TQ = TBM / 32, WT = TBM / 2;
+  constexpr int V = TBM * TB_BK / 256;  // elements of each slice per thread
+  constexpr int SH = TBM == 64 ? 6 : 5;  // log2(TBM)
   T(*As)[TB_BK][TB_LD] = reinterpret_cast<T(*)[TB_BK][TB_LD]>(smem);
   T(*Bs)[TB_BK][TB_LD] = reinterpret_cast<T(*)[TB_BK][TB_LD]>(smem + 2 * TB_BK * TB_LD);
   typedef typename BAcc<T>::type acc_t;
   const int tid = item_tid(), lane = tid & 63, wave = tid >> 6;
   const int wm = wave & 1, wn = wave >> 1;
-  // each thread moves 4 elements of each 64 x 16 slice of op(A) and op(B)
-  T ra[4], rb[4];
+  T ra[V], rb[V];
   auto load = [&](int k0) {
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const int e = tid + 256 * v;  // 0 .. 1023
-      // op(A) element (i, kk): i = e % 64, kk = e / 64 (coalesced when !ta)
-      int i = e & 63, kk = e >> 6;
+    for (int v = 0; v < V; ++v) {
+      const int e = tid + 256 * v;  // 0 .. TBM * 16 - 1
+      // op(A) element (i, kk): i = e % TBM, kk = e / TBM (coalesced when !ta)
+      int i = e & (TBM - 1), kk = e >> SH;
       if (ta) { kk = e & 15; i = e >> 4; }
       const int gi = m0 + i, gk = k0 + kk;
       T x = T(0);
@@ -108,7 +111,7 @@ __device__ __forceinline__ void tile64_mainloop(int ta, int tb, int M, int N, in
       ra[v] = x;
       // op(B) element (kk, j)
       int j = e >> 4, kb = e & 15;
-      if (tb) { j = e & 63; kb = e >> 6; }
+      if (tb) { j = e & (TBM - 1); kb = e >> SH; }
       const int gj = n0 + j, gkb = k0 + kb;
       T y = T(0);
       if (gj < N && gkb < K) y = tb ? B[gj + (long)gkb * ldb] : B[gkb + (long)gj * ldb];
@@ -117,20 +120,20 @@ __device__ __forceinline__ void tile64_mainloop(int ta, int tb, int M, int N, in
   };
   auto store = [&](int buf) {
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
+    for (int v = 0; v < V; ++v) {
       const int e = tid + 256 * v;
-      int i = e & 63, kk = e >> 6;
+      int i = e & (TBM - 1), kk = e >> SH;
       if (ta) { kk = e & 15; i = e >> 4; }
       As[buf][kk][i] = ra[v];
       int j = e >> 4, kb = e & 15;
-      if (tb) { j = e & 63; kb = e >> 6; }
+      if (tb) { j = e & (TBM - 1); kb = e >> SH; }
       Bs[buf][kb][j] = rb[v];
     }
   };
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < TQ; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
+    for (int j = 0; j < TQ; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
   const int nk = (K + TB_BK - 1) / TB_BK;
   const int lr = lane & 15, lk = lane >> 4;
   if (nk > 0) {
@@ -143,15 +146,15 @@ __device__ __forceinline__ void tile64_mainloop(int ta, int tb, int M, int N, in
       if (more) load((kt + 1) * TB_BK);
 #pragma unroll
       for (int ks = 0; ks < TB_BK / 4; ++ks) {
-        T af[2], bf[2];
+        T af[TQ], bf[TQ];
 #pragma unroll
-        for (int i = 0; i < 2; ++i) af[i] = As[buf][ks * 4 + lk][wm * 32 + i * 16 + lr];
+        for (int i = 0; i < TQ; ++i) af[i] = As[buf][ks * 4 + lk][wm * WT + i * 16 + lr];
 #pragma unroll
-        for (int j = 0; j < 2; ++j) bf[j] = Bs[buf][ks * 4 + lk][wn * 32 + j * 16 + lr];
+        for (int j = 0; j < TQ; ++j) bf[j] = Bs[buf][ks * 4 + lk][wn * WT + j * 16 + lr];
 #pragma unroll
-        for (int i = 0; i < 2; ++i)
+        for (int i = 0; i < TQ; ++i)
 #pragma unroll
-          for (int j = 0; j < 2; ++j) acc[i][j] = bmfma(bf[j], af[i], acc[i][j]);
+          for (int j = 0; j < TQ; ++j) acc[i][j] = bmfma(bf[j], af[i], acc[i][j]);
       }
       if (more) store(buf ^ 1);
       __syncthreads();
@@ -160,34 +163,46 @@ __device__ __forceinline__ void tile64_mainloop(int ta, int tb, int M, int N, in
   }
 }
 
-// Tile element of accumulator register r of block (i, j) held by `lane`:
-// row (of op(A) / C) and column (of op(B) / C) inside the 64 x 64 tile.
 template <typename T>
-__device__ __forceinline__ int tile_row(int wave, int lane, int i) { return (wave & 1) * 32 + i * 16 + (lane & 15); }
-template <typename T>
-__device__ __forceinline__ int tile_col(int wave, int lane, int j, int r)
+__device__ __forceinline__ void tile64_mainloop(int ta, int tb, int M, int N, int K, const T* __restrict__ A, long lda,
+                                                const T* __restrict__ B, long ldb, int m0, int n0, T* smem,
+                                                typename BAcc<T>::type (&acc)[2][2])
 {
-  return (wave >> 1) * 32 + j * 16 + (sizeof(T) == 8 ? (lane >> 4) + 4 * r : (lane >> 4) * 4 + r);
+  tile_mainloop<T, 64>(ta, tb, M, N, K, A, lda, B, ldb, m0, n0, smem, acc);
 }
 
-// C tile (tm, tn) of the 64 x 64 tiling of C <- alpha op(A) op(B) + beta C.
-template <typename T>
-__device__ void gemm_tile64(int ta, int tb, int M, int N, int K, T alpha, const T* __restrict__ A, long lda,
-                            const T* __restrict__ B, long ldb, T beta, T* __restrict__ C, long ldc, int tm, int tn,
-                            T* smem)
+// Tile element of accumulator register r of block (i, j) held by `lane`:
+// row (of op(A) / C) and column (of op(B) / C) inside a TBM x TBM tile.
+template <typename T, int TBM = 64>
+__device__ __forceinline__ int tile_row(int wave, int lane, int i)
 {
-  typename BAcc<T>::type acc[2][2];
-  const int m0 = tm * TB_BM, n0 = tn * TB_BM;
-  tile64_mainloop<T>(ta, tb, M, N, K, A, lda, B, ldb, m0, n0, smem, acc);
+  return (wave & 1) * (TBM / 2) + i * 16 + (lane & 15);
+}
+template <typename T, int TBM = 64>
+__device__ __forceinline__ int tile_col(int wave, int lane, int j, int r)
+{
+  return (wave >> 1) * (TBM / 2) + j * 16 + (sizeof(T) == 8 ? (lane >> 4) + 4 * r : (lane >> 4) * 4 + r);
+}
+
+// C tile (tm, tn) of the TBM x TBM tiling of C <- alpha op(A) op(B) + beta C.
+template <typename T, int TBM>
+__device__ void gemm_tile(int ta, int tb, int M, int N, int K, T alpha, const T* __restrict__ A, long lda,
+                          const T* __restrict__ B, long ldb, T beta, T* __restrict__ C, long ldc, int tm, int tn,
+                          T* smem)
+{
+  constexpr int TQ = TBM / 32;
+  typename BAcc<T>::type acc[TQ][TQ];
+  const int m0 = tm * TBM, n0 = tn * TBM;
+  tile_mainloop<T, TBM>(ta, tb, M, N, K, A, lda, B, ldb, m0, n0, smem, acc);
   const int tid = item_tid(), lane = tid & 63, wave = tid >> 6;
 #pragma unroll
-  for (int i = 0; i < 2; ++i) {
-    const int gm = m0 + tile_row<T>(wave, lane, i);
+  for (int i = 0; i < TQ; ++i) {
+    const int gm = m0 + tile_row<T, TBM>(wave, lane, i);
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
+    for (int j = 0; j < TQ; ++j) {
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int gn = n0 + tile_col<T>(wave, lane, j, r);
+        const int gn = n0 + tile_col<T, TBM>(wave, lane, j, r);
         if (gm < M && gn < N) {
           T* c = C + gm + (long)gn * ldc;
           T v = alpha * acc[i][j][r];
@@ -321,15 +336,29 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void b
     const BatchItem w = items[it];
     const BatchProblem p = probs[w.prob];
     if (p.kind == 0) {
-      if (p.dtype == 0)
-        gemm_tile64<double>(p.ta != NoTrans, p.tb != NoTrans, p.m, p.n, p.k, p.alpha,
-                            reinterpret_cast<const double*>(p.A), p.lda, reinterpret_cast<const double*>(p.B), p.ldb,
-                            p.beta, reinterpret_cast<double*>(p.C), p.ldc, w.tm, w.tn, smem_d);
-      else
-        gemm_tile64<float>(p.ta != NoTrans, p.tb != NoTrans, p.m, p.n, p.k, (float)p.alpha,
-                           reinterpret_cast<const float*>(p.A), p.lda, reinterpret_cast<const float*>(p.B), p.ldb,
-                           (float)p.beta, reinterpret_cast<float*>(p.C), p.ldc, w.tm, w.tn,
-                           reinterpret_cast<float*>(smem_d));
+      const int ta = p.ta != NoTrans, tb = p.tb != NoTrans;
+      if (p.dtype == 0) {
+        const double* A = reinterpret_cast<const double*>(p.A);
+        const double* B = reinterpret_cast<const double*>(p.B);
+        double* C = reinterpret_cast<double*>(p.C);
+        if (w.tile == 32)
+          gemm_tile<double, 32>(ta, tb, p.m, p.n, p.k, p.alpha, A, p.lda, B, p.ldb, p.beta, C, p.ldc, w.tm, w.tn,
+                                smem_d);
+        else
+          gemm_tile<double, 64>(ta, tb, p.m, p.n, p.k, p.alpha, A, p.lda, B, p.ldb, p.beta, C, p.ldc, w.tm, w.tn,
+                                smem_d);
+      } else {
+        const float* A = reinterpret_cast<const float*>(p.A);
+        const float* B = reinterpret_cast<const float*>(p.B);
+        float* C = reinterpret_cast<float*>(p.C);
+        float* sm = reinterpret_cast<float*>(smem_d);
+        if (w.tile == 32)
+          gemm_tile<float, 32>(ta, tb, p.m, p.n, p.k, (float)p.alpha, A, p.lda, B, p.ldb, (float)p.beta, C, p.ldc,
+                               w.tm, w.tn, sm);
+        else
+          gemm_tile<float, 64>(ta, tb, p.m, p.n, p.k, (float)p.alpha, A, p.lda, B, p.ldb, (float)p.beta, C, p.ldc,
+                               w.tm, w.tn, sm);
+      }
     } else {
       if (p.dtype == 0) trsm_panel<double>(p, w.tn, smem_d);
       else trsm_panel<float>(p, w.tn, reinterpret_cast<float*>(smem_d));
@@ -368,13 +397,32 @@ struct BatchCost {
   BatchItem w;
 };
 
+// GEMM item shape: 0 = 64 x 64 tiles only, 1 (default) = 32 x 32 tiles for a
+// problem whose 32 x 32 tiling covers at most 3/4 of the area the 64 x 64
+// one does (a small m or n), 2 = 32 x 32 only (XKBLAS_BATCH_TILE).
+static int g_batch_tile = getenv("XKBLAS_BATCH_TILE") ? atoi(getenv("XKBLAS_BATCH_TILE")) : 1;
+
+static int gemm_item_tile(int m, int n)
+{
+  if (g_batch_tile == 0) return 64;
+  if (g_batch_tile == 2) return 32;
+  const long a64 = (long)((m + 63) / 64) * ((n + 63) / 64) * 4096;
+  const long a32 = (long)((m + 31) / 32) * ((n + 31) / 32) * 1024;
+  return 4 * a32 <= 3 * a64 ? 32 : 64;
+}
+
 static long batch_count_items(const xkblas_batch_problem_t* probs, int count)
 {
   long n = 0;
   for (int i = 0; i < count; ++i) {
     const xkblas_batch_problem_t& q = probs[i];
     if (q.m <= 0 || q.n <= 0) continue;
-    n += q.kind == XKBLAS_BATCH_GEMM ? (long)((q.m + 63) / 64) * ((q.n + 63) / 64) : (long)((q.n + 63) / 64);
+    if (q.kind == XKBLAS_BATCH_GEMM) {
+      const int t = gemm_item_tile(q.m, q.n);
+      n += (long)((q.m + t - 1) / t) * ((q.n + t - 1) / t);
+    } else {
+      n += (q.n + 63) / 64;
+    }
   }
   return n;
 }
@@ -394,11 +442,16 @@ static long batch_build(const xkblas_batch_problem_t* probs, int count, BatchPro
     XK_CHECK(q.dtype == 0 || q.dtype == 1, "batch: dtype must be 0 (FP64) or 1 (FP32)");
     if (q.m <= 0 || q.n <= 0) continue;
     if (q.kind == XKBLAS_BATCH_GEMM) {
-      const int tmx = (q.m + 63) / 64, tnx = (q.n + 63) / 64;
+      const int t = gemm_item_tile(q.m, q.n);
+      const int tmx = (q.m + t - 1) / t, tnx = (q.n + t - 1) / t;
+      // a tile's k-loop work plus a fixed overhead (a 32 x 32 tile runs at a
+      // quarter of the MFMA work per slice, hence the same time per k as half
+      // of a 64 x 64 one)
+      const double c = t == 64 ? 2.0 * 64 * 64 * std::max(q.k, 1) + 64 * 64 * 8.0
+                               : 2.0 * 64 * 32 * std::max(q.k, 1) + 64 * 64 * 8.0;
       for (int a = 0; a < tmx; ++a)
         for (int b = 0; b < tnx; ++b) {
-          // 64 x 64 tile: k-loop work plus a fixed overhead
-          work[nw] = BatchCost{2.0 * 64 * 64 * std::max(q.k, 1) + 64 * 64 * 8.0, nw, BatchItem{i, a, b, 0}};
+          work[nw] = BatchCost{c, nw, BatchItem{i, a, b, t}};
           ++nw;
         }
       *flops += 2.0 * q.m * q.n * (double)q.k;
@@ -410,7 +463,7 @@ static long batch_build(const xkblas_batch_problem_t* probs, int count, BatchPro
       const double nbk = (q.m + 63) / 64;
       const double c = 2.0 * 64 * 64 * 64 * nbk * (nbk - 1) / 2 + nbk * 2.0 * 64 * 64 * 128;
       for (int b = 0; b < np; ++b) {
-        work[nw] = BatchCost{c, nw, BatchItem{i, 0, b, 0}};
+        work[nw] = BatchCost{c, nw, BatchItem{i, 0, b, 64}};
         ++nw;
       }
       *flops += (double)q.m * q.m * q.n;
@@ -559,6 +612,12 @@ int xkblas_batch_run_noalloc(const xkblas_batch_problem_t* problems, int count)
                      d_counter);
   XK_HIP(hipGetLastError());
   return 0;
+}
+
+void xkblas_set_batch_tile(int mode)
+{
+  std::lock_guard<std::mutex> lock(api_mutex());
+  g_batch_tile = mode < 0 || mode > 2 ? 1 : mode;
 }
 
 int xkblas_batch_use_graph(void* plan, int on)
