@@ -56,9 +56,8 @@ void xkblas_set_cache_limit(size_t bytes);
  * TRSM: Left, Lower, NoTrans, Non-unit: L X = alpha B with L = A (m x m,
  *       lda) and B = C (m x n, ldc) overwritten by X.
  * dtype 0 = FP64, 1 = FP32 (mixed in one batch).  A plan holds the work list
- * (64 x 64 or 32 x 32 GEMM tiles and 64-column TRSM panels, longest
- * first) on the
- * device; xkblas_batch_run enqueues one counter fill and one persistent
+ * (64 x 64 or 32 x 32 GEMM tiles and 64-column TRSM panels, longest first)
+ * on the device; xkblas_batch_run enqueues one counter fill and one persistent
  * kernel on the library's compute stream (xkblas_sync waits for it);
  * xkblas_batch_use_graph(plan, 1) captures that pair into a hipGraph that
  * later runs replay. */

@@ -524,6 +524,9 @@ void dev_trmm(const Exec& x, int side, int uplo, int transA, int diag, int m, in
 // GEMM (2m x n x 2k); for a (conjugate) transposed operand the transpose of
 // its block matrix is built and read transposed.
 enum { WS_ZA = 20, WS_ZB = 21, WS_ZP = 22 };
+// the same three for products on the diagonal stream (x.zset = 1), which
+// run beside products on the compute stream
+enum { WS_ZA2 = 29, WS_ZB2 = 30, WS_ZP2 = 31 };
 
 // uplo/herm: write only a triangle of the square C (HERK: real diagonal).
 template <typename T>
@@ -550,8 +553,8 @@ static void cgemm_impl(const Exec& x, int transA, int transB, int m, int n, int 
     const int zi = transB == ConjTrans ? -1 : 1;  // Zi = zi Bi (stored orientation)
     const long ldu = ta ? rup(2L * k) : rup(n), ldv = tb ? rup(2L * k) : rup(n);
     const size_t usz = (size_t)ldu * (ta ? n : 2 * k), vsz = (size_t)ldv * (tb ? n : 2 * k);
-    T* U = (T*)scratch(WS_ZA, sizeof(T) * usz);
-    T* V = (T*)scratch(WS_ZB, sizeof(T) * 2 * vsz);
+    T* U = (T*)scratch(x.zset ? WS_ZA2 : WS_ZA, sizeof(T) * usz);
+    T* V = (T*)scratch(x.zset ? WS_ZB2 : WS_ZB, sizeof(T) * 2 * vsz);
     T* V1 = V;
     T* V2 = V + vsz;
     cplx_pack2<T>(x.s, ta, 0, 1, transA == ConjTrans ? -1 : 1, ar, ac, A, lda, U, ldu);
@@ -562,7 +565,7 @@ static void cgemm_impl(const Exec& x, int transA, int transB, int m, int n, int 
     else cplx_pack2<T>(x.s, tb, 0, 1, -zi, br, bc, B, ldb, V1, ldv);
     cplx_pack2<T>(x.s, tb, 1, 1, zi, br, bc, B, ldb, V2, ldv);
     const long ldp = rup(2L * n);
-    T* P = (T*)scratch(WS_ZP, sizeof(T) * (size_t)ldp * n);
+    T* P = (T*)scratch(x.zset ? WS_ZP2 : WS_ZP, sizeof(T) * (size_t)ldp * n);
     const int opu = ta ? Trans : NoTrans, opv = tb ? NoTrans : Trans;
     dev_gemmt<T>(x, uplo, opu, opv, n, 2 * k, T(1), U, ldu, V1, same ? ldu : ldv, T(0), P, ldp);
     dev_gemmt<T>(x, uplo, opu, opv, n, 2 * k, T(1), U, ldu, V2, ldv, T(0), P + n, ldp);
@@ -574,17 +577,17 @@ static void cgemm_impl(const Exec& x, int transA, int transB, int m, int n, int 
   const bool ta = transA != NoTrans;
   const int ar = ta ? k : m, ac = ta ? m : k;
   const long ldya = rup(2L * ar);
-  T* YA = (T*)scratch(WS_ZA, sizeof(T) * (size_t)ldya * 2 * ac);
+  T* YA = (T*)scratch(x.zset ? WS_ZA2 : WS_ZA, sizeof(T) * (size_t)ldya * 2 * ac);
   cplx_pack_a<T>(x.s, transA == Trans ? -1 : 1, ar, ac, A, lda, YA, ldya);
   // op(B): NoTrans -> [Br; Bi]; Trans (n x k stored) -> [Br, Bi] read
   // transposed; ConjTrans -> [Br, -Bi] read transposed
   const bool tb = transB != NoTrans;
   const int br = tb ? n : k, bc = tb ? k : n;
   const long ldyb = rup(tb ? (long)br : 2L * br);
-  T* YB = (T*)scratch(WS_ZB, sizeof(T) * (size_t)ldyb * (tb ? 2 * bc : bc));
+  T* YB = (T*)scratch(x.zset ? WS_ZB2 : WS_ZB, sizeof(T) * (size_t)ldyb * (tb ? 2 * bc : bc));
   cplx_pack_b<T>(x.s, tb ? 0 : 1, transB == ConjTrans ? -1 : 1, br, bc, B, ldb, YB, ldyb);
   const long ldp = rup(2L * m);
-  T* P = (T*)scratch(WS_ZP, sizeof(T) * (size_t)ldp * n);
+  T* P = (T*)scratch(x.zset ? WS_ZP2 : WS_ZP, sizeof(T) * (size_t)ldp * n);
   dev_gemm<T>(x, ta ? Trans : NoTrans, tb ? Trans : NoTrans, 2 * m, n, 2 * k, T(1), YA, ldya, YB, ldyb, T(0), P, ldp);
   cplx_combine<T>(x.s, m, n, P, ldp, alpha, beta, C, ldc, uplo, herm);
 }
@@ -766,37 +769,56 @@ void dev_ctrsm(const Exec& x, int side, int uplo, int transA, int diag, int m, i
 // algorithm of dev_potrf without lookahead; each diagonal block is factored
 // recursively (leaves of 64) and inverted, the panel is a product with the
 // inverse, the trailing update a Hermitian rank-nb product on the triangle.
-enum { WS_ZDI = 24, WS_ZPAN = 25, WS_ZINV = 26 };
+enum { WS_ZDI = 24, WS_ZPAN = 25, WS_ZINV = 26, WS_ZDW = 32 };
 
+// Cholesky factor and inverse of a diagonal block of order n (A = L L^H or
+// U^H U), recursively: factor A11 and invert it, panel with the inverse,
+// Hermitian update of A22, factor and invert A22, and the off-diagonal block
+// of the inverse from two products (L: I21 = -I22 L21 I11; U: I12 = -I11
+// U12 I22).  Leaves of 64: cplx_potrf_leaf, then cplx_trtri_leaf.  Tinv's
+// block on the other side of the diagonal is never written: the caller
+// zeroes it once.  Temporary: WS_ZDW (one n1 x n2 block).
 template <typename T>
-static void cpotrf_diag(const Exec& x, int uplo, int n, T* A, long lda, int* info, int base)
+static void cpotrf_diag(const Exec& x, int uplo, int n, T* A, long lda, T* Tinv, long ldinv, int* info, int base)
 {
   if (n <= 64) {
     cplx_potrf_leaf<T>(x.s, uplo, n, A, lda, info, base);
+    cplx_trtri_leaf<T>(x.s, uplo, 0, n, A, lda, 0, Tinv, ldinv, 0, 1);
     return;
   }
   auto at = [](auto* b, long off) { return b + 2 * off; };
   const int n1 = half64(n), n2 = n - n1;
   const T one[2] = {T(1), T(0)}, mone[2] = {T(-1), T(0)}, zero[2] = {T(0), T(0)};
-  cpotrf_diag<T>(x, uplo, n1, A, lda, info, base);
-  T* inv = (T*)scratch(WS_ZDI, 2 * sizeof(T) * (size_t)n1 * n1);
-  dev_ctrtri<T>(x, uplo, 0, n1, A, lda, inv, n1);
-  T* W = (T*)scratch(WS_ZPAN, 2 * sizeof(T) * (size_t)n1 * n2);
   T* A22 = at(A, n1 + (long)n1 * lda);
+  T* I22 = at(Tinv, n1 + (long)n1 * ldinv);
+  cpotrf_diag<T>(x, uplo, n1, A, lda, Tinv, ldinv, info, base);
+  T* W = (T*)scratch(WS_ZDW, 2 * sizeof(T) * (size_t)n1 * n2);
   if (uplo == Lower) {
     T* A21 = at(A, n1);
     copy2d<T>(x.s, 2 * n2, n1, A21, 2 * lda, W, 2L * n2);
-    cgemm_impl<T>(x, NoTrans, ConjTrans, n2, n1, n1, one, W, n2, inv, n1, zero, A21, lda, 0, 0);
+    cgemm_impl<T>(x, NoTrans, ConjTrans, n2, n1, n1, one, W, n2, Tinv, ldinv, zero, A21, lda, 0, 0);
     cgemm_impl<T>(x, NoTrans, ConjTrans, n2, n2, n1, mone, A21, lda, A21, lda, one, A22, lda, Lower, 1);
+    cpotrf_diag<T>(x, uplo, n2, A22, lda, I22, ldinv, info, base + n1);
+    cgemm_impl<T>(x, NoTrans, NoTrans, n2, n1, n1, one, A21, lda, Tinv, ldinv, zero, W, n2, 0, 0);
+    cgemm_impl<T>(x, NoTrans, NoTrans, n2, n1, n2, mone, I22, ldinv, W, n2, zero, at(Tinv, n1), ldinv, 0, 0);
   } else {
     T* A12 = at(A, (long)n1 * lda);
     copy2d<T>(x.s, 2 * n1, n2, A12, 2 * lda, W, 2L * n1);
-    cgemm_impl<T>(x, ConjTrans, NoTrans, n1, n2, n1, one, inv, n1, W, n1, zero, A12, lda, 0, 0);
+    cgemm_impl<T>(x, ConjTrans, NoTrans, n1, n2, n1, one, Tinv, ldinv, W, n1, zero, A12, lda, 0, 0);
     cgemm_impl<T>(x, ConjTrans, NoTrans, n2, n2, n1, mone, A12, lda, A12, lda, one, A22, lda, Upper, 1);
+    cpotrf_diag<T>(x, uplo, n2, A22, lda, I22, ldinv, info, base + n1);
+    cgemm_impl<T>(x, NoTrans, NoTrans, n1, n2, n2, one, A12, lda, I22, ldinv, zero, W, n1, 0, 0);
+    cgemm_impl<T>(x, NoTrans, NoTrans, n1, n2, n1, mone, Tinv, ldinv, W, n1, zero, at(Tinv, (long)n1 * ldinv),
+                  ldinv, 0, 0);
   }
-  cpotrf_diag<T>(x, uplo, n2, A22, lda, info, base + n1);
 }
 
+// Right-looking blocked Cholesky (the reference's tile algorithm,
+// blas/zpotrf.c:89-187, in complex arithmetic) with the structure of
+// dev_potrf: each diagonal block is factored and inverted on the side stream
+// (its own scratch set) while the compute stream applies the previous panel
+// to the trailing matrix; the next block column is updated first so that
+// its factorization can start at once.
 template <typename T>
 void dev_cpotrf(const Exec& x, int uplo, int n, T* A, long lda, int nb, int* info)
 {
@@ -805,26 +827,73 @@ void dev_cpotrf(const Exec& x, int uplo, int n, T* A, long lda, int nb, int* inf
   nb = std::max(64, std::min(nb, n));
   const int nt = (n + nb - 1) / nb;
   const T one[2] = {T(1), T(0)}, mone[2] = {T(-1), T(0)}, zero[2] = {T(0), T(0)};
+  // inverses of two consecutive diagonal blocks (ping-pong for lookahead),
+  // zeroed once: nothing writes their blocks across the diagonal
+  T* inv = (T*)scratch(WS_ZINV, 2 * sizeof(T) * (size_t)2 * nb * nb);
+  const bool lookahead = x.s_diag != nullptr && nt > 1;
+  Exec xd = x;
+  if (lookahead) {
+    xd.s = x.s_diag;
+    xd.zset = 1;
+  }
+  hipEvent_t ev_diag = x.ev0, ev_look = x.ev1;
+  const bool own_events = !ev_diag || !ev_look;
+  if (own_events) {
+    XK_HIP(hipEventCreateWithFlags(&ev_diag, hipEventDisableTiming));
+    XK_HIP(hipEventCreateWithFlags(&ev_look, hipEventDisableTiming));
+  }
+  if (lookahead) {
+    XK_HIP(hipEventRecord(ev_look, x.s));
+    XK_HIP(hipStreamWaitEvent(xd.s, ev_look, 0));
+  }
+  fill_zero<T>(xd.s, 2 * nb, 2 * nb, inv, 2L * nb);
+  auto blk = [&](int i, int j) { return at(A, (long)i * nb + (long)j * nb * lda); };
+  auto tsz = [&](int i) { return i == nt - 1 ? n - i * nb : nb; };
+  cpotrf_diag<T>(xd, uplo, tsz(0), blk(0, 0), lda, inv, nb, info, 0);
   for (int k = 0; k < nt; ++k) {
-    const int k0 = k * nb, kk = std::min(nb, n - k0), rest = n - k0 - kk;
-    T* Akk = at(A, (long)k0 * (1 + lda));
-    cpotrf_diag<T>(x, uplo, kk, Akk, lda, info, k0);
-    if (rest == 0) break;
-    T* inv = (T*)scratch(WS_ZINV, 2 * sizeof(T) * (size_t)nb * nb);
-    dev_ctrtri<T>(x, uplo, 0, kk, Akk, lda, inv, nb);
-    T* W = (T*)scratch(WS_ZPAN, 2 * sizeof(T) * (size_t)kk * rest);
-    T* A22 = at(A, (long)(k0 + kk) * (1 + lda));
-    if (uplo == Lower) {
-      T* A21 = at(A, (long)(k0 + kk) + (long)k0 * lda);
-      copy2d<T>(x.s, 2 * rest, kk, A21, 2 * lda, W, 2L * rest);
-      cgemm_impl<T>(x, NoTrans, ConjTrans, rest, kk, kk, one, W, rest, inv, nb, zero, A21, lda, 0, 0);
-      cgemm_impl<T>(x, NoTrans, ConjTrans, rest, rest, kk, mone, A21, lda, A21, lda, one, A22, lda, Lower, 1);
-    } else {
-      T* A12 = at(A, (long)k0 + (long)(k0 + kk) * lda);
-      copy2d<T>(x.s, 2 * kk, rest, A12, 2 * lda, W, 2L * kk);
-      cgemm_impl<T>(x, ConjTrans, NoTrans, kk, rest, kk, one, inv, nb, W, kk, zero, A12, lda, 0, 0);
-      cgemm_impl<T>(x, ConjTrans, NoTrans, rest, rest, kk, mone, A12, lda, A12, lda, one, A22, lda, Upper, 1);
+    const int kk = tsz(k);
+    T* ik = at(inv, (long)(k & 1) * nb * nb);
+    if (lookahead) {
+      XK_HIP(hipEventRecord(ev_diag, xd.s));
+      XK_HIP(hipStreamWaitEvent(x.s, ev_diag, 0));
     }
+    if (k == nt - 1) break;
+    const int rest = n - (k + 1) * nb, k1 = tsz(k + 1), r2 = rest - k1;
+    T* W = (T*)scratch(WS_ZPAN, 2 * sizeof(T) * (size_t)kk * rest);
+    if (uplo == Lower) {
+      T* P = blk(k + 1, k);  // rest x kk panel
+      copy2d<T>(x.s, 2 * rest, kk, P, 2 * lda, W, 2L * rest);
+      cgemm_impl<T>(x, NoTrans, ConjTrans, rest, kk, kk, one, W, rest, ik, nb, zero, P, lda, 0, 0);
+      // next diagonal block first (lookahead), then the block column below
+      // it and the trailing triangle
+      cgemm_impl<T>(x, NoTrans, ConjTrans, k1, k1, kk, mone, P, lda, P, lda, one, blk(k + 1, k + 1), lda, Lower, 1);
+      if (lookahead) XK_HIP(hipEventRecord(ev_look, x.s));
+      if (r2 > 0) {
+        const T* P2 = at(P, k1);
+        cgemm_impl<T>(x, NoTrans, ConjTrans, r2, k1, kk, mone, P2, lda, P, lda, one, blk(k + 2, k + 1), lda, 0, 0);
+        cgemm_impl<T>(x, NoTrans, ConjTrans, r2, r2, kk, mone, P2, lda, P2, lda, one, blk(k + 2, k + 2), lda, Lower,
+                      1);
+      }
+    } else {
+      T* P = blk(k, k + 1);  // kk x rest panel
+      copy2d<T>(x.s, 2 * kk, rest, P, 2 * lda, W, 2L * kk);
+      cgemm_impl<T>(x, ConjTrans, NoTrans, kk, rest, kk, one, ik, nb, W, kk, zero, P, lda, 0, 0);
+      cgemm_impl<T>(x, ConjTrans, NoTrans, k1, k1, kk, mone, P, lda, P, lda, one, blk(k + 1, k + 1), lda, Upper, 1);
+      if (lookahead) XK_HIP(hipEventRecord(ev_look, x.s));
+      if (r2 > 0) {
+        const T* P2 = at(P, (long)k1 * lda);
+        cgemm_impl<T>(x, ConjTrans, NoTrans, k1, r2, kk, mone, P, lda, P2, lda, one, blk(k + 1, k + 2), lda, 0, 0);
+        cgemm_impl<T>(x, ConjTrans, NoTrans, r2, r2, kk, mone, P2, lda, P2, lda, one, blk(k + 2, k + 2), lda, Upper,
+                      1);
+      }
+    }
+    T* inext = at(inv, (long)((k + 1) & 1) * nb * nb);
+    if (lookahead) XK_HIP(hipStreamWaitEvent(xd.s, ev_look, 0));
+    cpotrf_diag<T>(xd, uplo, k1, blk(k + 1, k + 1), lda, inext, nb, info, (k + 1) * nb);
+  }
+  if (own_events) {
+    XK_HIP(hipEventDestroy(ev_diag));
+    XK_HIP(hipEventDestroy(ev_look));
   }
 }
 
