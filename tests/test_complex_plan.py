@@ -95,3 +95,41 @@ def test_triangle_plan_herk_real_part_is_a_syrk(trans):
     assert same and equal
     X = A if trans == N_ else A.conj().T
     np.testing.assert_allclose(Pr + 1j * Pi, X @ X.conj().T, atol=1e-12)
+
+
+def rho_form(A):
+    """The 2n x 2n real form [rho(a_ij)], rho(z) = [re -im; im re]
+    (cplx_realform_kernel, full matrix instead of its lower triangle)."""
+    n = A.shape[0]
+    M = np.empty((2 * n, 2 * n))
+    M[0::2, 0::2] = A.real
+    M[1::2, 1::2] = A.real
+    M[1::2, 0::2] = A.imag
+    M[0::2, 1::2] = -A.imag
+    return M
+
+
+def test_real_form_cholesky_is_rho_of_complex_factor():
+    """The complex POTRF's diagonal blocks: the real Cholesky factor of the
+    real form is rho(L) and its inverse rho(inv(L)), so that cplx_fromreal's
+    l_ij = R[2i][2j] + i R[2i+1][2j] reads L and inv(L).  A sign slip in rho
+    (a transposed imaginary block) makes the real form non-symmetric or the
+    read-back the conjugate factor, which fails here."""
+    r = np.random.default_rng(41)
+    n = 9
+    G = r.standard_normal((n, n)) + 1j * r.standard_normal((n, n))
+    A = G @ G.conj().T + n * np.eye(n)
+    A = (A + A.conj().T) / 2
+    np.fill_diagonal(A, A.diagonal().real)
+    L = np.linalg.cholesky(A)
+    M = rho_form(A)
+    np.testing.assert_allclose(M, M.T, atol=0)
+    R = np.linalg.cholesky(M)
+    Rinv = np.linalg.inv(R)
+    back = lambda X: X[0::2, 0::2] + 1j * X[1::2, 0::2]
+    np.testing.assert_allclose(np.tril(back(R)), L, atol=1e-12)
+    np.testing.assert_allclose(np.tril(back(Rinv)), np.linalg.inv(L), atol=1e-12)
+    np.testing.assert_allclose(R, rho_form(L), atol=1e-12)
+    # the diagonal blocks of the real factor are l_ii I: the odd rows' entry
+    # left of the diagonal is zero
+    np.testing.assert_allclose(np.diag(R, -1)[0::2], 0, atol=1e-12)

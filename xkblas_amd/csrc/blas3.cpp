@@ -106,12 +106,14 @@ void dev_trtri(const Exec& x, hipStream_t s, int uplo, int diag, int n, const T*
 
 // Recursive Cholesky of one diagonal block with its inverse (Tinv = inv(L)
 // for Lower, inv(U) for Upper).
+// pair != 0: A is the real form of a complex block (orders 2i, 2i + 1 stand
+// for complex order i), and a failing pivot reports its complex order.
 template <typename T>
 static void potrf_diag(hipStream_t s, int uplo, int n, T* A, long lda, T* Tinv, long ldinv, int* info, int base,
-                       int wslot, bool zeroed = false)
+                       int wslot, bool zeroed = false, int pair = 0)
 {
   if (n <= 64) {
-    potrf_leaf<T>(s, uplo, n, A, lda, 0, Tinv, ldinv, 0, 1, info, base);
+    potrf_leaf<T>(s, uplo, n, A, lda, 0, Tinv, ldinv, 0, 1, info, base, pair);
     return;
   }
   const int n1 = half64(n), n2 = n - n1;
@@ -119,7 +121,7 @@ static void potrf_diag(hipStream_t s, int uplo, int n, T* A, long lda, T* Tinv, 
   T* A22 = A + n1 + (long)n1 * lda;
   T* I11 = Tinv;
   T* I22 = Tinv + n1 + (long)n1 * ldinv;
-  potrf_diag<T>(s, uplo, n1, A11, lda, I11, ldinv, info, base, wslot, zeroed);
+  potrf_diag<T>(s, uplo, n1, A11, lda, I11, ldinv, info, base, wslot, zeroed, pair);
   T* W = (T*)scratch(wslot, sizeof(T) * (size_t)n1 * n2);
   if (uplo == Lower) {
     T* A21 = A + n1;
@@ -127,7 +129,7 @@ static void potrf_diag(hipStream_t s, int uplo, int n, T* A, long lda, T* Tinv, 
     copy2d<T>(s, n2, n1, A21, lda, W, n2);
     gemm<T>(s, NoTrans, Trans, n2, n1, n1, T(1), W, n2, I11, ldinv, T(0), A21, lda);   // L21 = A21 inv(L11)^T
     gemm<T>(s, NoTrans, Trans, n2, n2, n1, T(-1), A21, lda, A21, lda, T(1), A22, lda, TRI_LOWER);
-    potrf_diag<T>(s, uplo, n2, A22, lda, I22, ldinv, info, base + n1, wslot, zeroed);
+    potrf_diag<T>(s, uplo, n2, A22, lda, I22, ldinv, info, base + n1, wslot, zeroed, pair);
     gemm<T>(s, NoTrans, NoTrans, n2, n1, n1, T(1), A21, lda, I11, ldinv, T(0), W, n2);  // L21 inv(L11)
     gemm<T>(s, NoTrans, NoTrans, n2, n1, n2, T(-1), I22, ldinv, W, n2, T(0), I21, ldinv);
     if (!zeroed) fill_zero<T>(s, n1, n2, Tinv + (long)n1 * ldinv, ldinv);
@@ -137,7 +139,7 @@ static void potrf_diag(hipStream_t s, int uplo, int n, T* A, long lda, T* Tinv, 
     copy2d<T>(s, n1, n2, A12, lda, W, n1);
     gemm<T>(s, Trans, NoTrans, n1, n2, n1, T(1), I11, ldinv, W, n1, T(0), A12, lda);     // U12 = inv(U11)^T A12
     gemm<T>(s, Trans, NoTrans, n2, n2, n1, T(-1), A12, lda, A12, lda, T(1), A22, lda, TRI_UPPER);
-    potrf_diag<T>(s, uplo, n2, A22, lda, I22, ldinv, info, base + n1, wslot, zeroed);
+    potrf_diag<T>(s, uplo, n2, A22, lda, I22, ldinv, info, base + n1, wslot, zeroed, pair);
     gemm<T>(s, NoTrans, NoTrans, n1, n2, n2, T(1), A12, lda, I22, ldinv, T(0), W, n1);   // U12 inv(U22)
     gemm<T>(s, NoTrans, NoTrans, n1, n2, n1, T(-1), I11, ldinv, W, n1, T(0), I12, ldinv);
     if (!zeroed) fill_zero<T>(s, n2, n1, Tinv + n1, ldinv);
@@ -524,9 +526,6 @@ void dev_trmm(const Exec& x, int side, int uplo, int transA, int diag, int m, in
 // GEMM (2m x n x 2k); for a (conjugate) transposed operand the transpose of
 // its block matrix is built and read transposed.
 enum { WS_ZA = 20, WS_ZB = 21, WS_ZP = 22 };
-// the same three for products on the diagonal stream (x.zset = 1), which
-// run beside products on the compute stream
-enum { WS_ZA2 = 29, WS_ZB2 = 30, WS_ZP2 = 31 };
 
 // uplo/herm: write only a triangle of the square C (HERK: real diagonal).
 template <typename T>
@@ -553,8 +552,8 @@ static void cgemm_impl(const Exec& x, int transA, int transB, int m, int n, int 
     const int zi = transB == ConjTrans ? -1 : 1;  // Zi = zi Bi (stored orientation)
     const long ldu = ta ? rup(2L * k) : rup(n), ldv = tb ? rup(2L * k) : rup(n);
     const size_t usz = (size_t)ldu * (ta ? n : 2 * k), vsz = (size_t)ldv * (tb ? n : 2 * k);
-    T* U = (T*)scratch(x.zset ? WS_ZA2 : WS_ZA, sizeof(T) * usz);
-    T* V = (T*)scratch(x.zset ? WS_ZB2 : WS_ZB, sizeof(T) * 2 * vsz);
+    T* U = (T*)scratch(WS_ZA, sizeof(T) * usz);
+    T* V = (T*)scratch(WS_ZB, sizeof(T) * 2 * vsz);
     T* V1 = V;
     T* V2 = V + vsz;
     cplx_pack2<T>(x.s, ta, 0, 1, transA == ConjTrans ? -1 : 1, ar, ac, A, lda, U, ldu);
@@ -565,7 +564,7 @@ static void cgemm_impl(const Exec& x, int transA, int transB, int m, int n, int 
     else cplx_pack2<T>(x.s, tb, 0, 1, -zi, br, bc, B, ldb, V1, ldv);
     cplx_pack2<T>(x.s, tb, 1, 1, zi, br, bc, B, ldb, V2, ldv);
     const long ldp = rup(2L * n);
-    T* P = (T*)scratch(x.zset ? WS_ZP2 : WS_ZP, sizeof(T) * (size_t)ldp * n);
+    T* P = (T*)scratch(WS_ZP, sizeof(T) * (size_t)ldp * n);
     const int opu = ta ? Trans : NoTrans, opv = tb ? NoTrans : Trans;
     dev_gemmt<T>(x, uplo, opu, opv, n, 2 * k, T(1), U, ldu, V1, same ? ldu : ldv, T(0), P, ldp);
     dev_gemmt<T>(x, uplo, opu, opv, n, 2 * k, T(1), U, ldu, V2, ldv, T(0), P + n, ldp);
@@ -577,17 +576,17 @@ static void cgemm_impl(const Exec& x, int transA, int transB, int m, int n, int 
   const bool ta = transA != NoTrans;
   const int ar = ta ? k : m, ac = ta ? m : k;
   const long ldya = rup(2L * ar);
-  T* YA = (T*)scratch(x.zset ? WS_ZA2 : WS_ZA, sizeof(T) * (size_t)ldya * 2 * ac);
+  T* YA = (T*)scratch(WS_ZA, sizeof(T) * (size_t)ldya * 2 * ac);
   cplx_pack_a<T>(x.s, transA == Trans ? -1 : 1, ar, ac, A, lda, YA, ldya);
   // op(B): NoTrans -> [Br; Bi]; Trans (n x k stored) -> [Br, Bi] read
   // transposed; ConjTrans -> [Br, -Bi] read transposed
   const bool tb = transB != NoTrans;
   const int br = tb ? n : k, bc = tb ? k : n;
   const long ldyb = rup(tb ? (long)br : 2L * br);
-  T* YB = (T*)scratch(x.zset ? WS_ZB2 : WS_ZB, sizeof(T) * (size_t)ldyb * (tb ? 2 * bc : bc));
+  T* YB = (T*)scratch(WS_ZB, sizeof(T) * (size_t)ldyb * (tb ? 2 * bc : bc));
   cplx_pack_b<T>(x.s, tb ? 0 : 1, transB == ConjTrans ? -1 : 1, br, bc, B, ldb, YB, ldyb);
   const long ldp = rup(2L * m);
-  T* P = (T*)scratch(x.zset ? WS_ZP2 : WS_ZP, sizeof(T) * (size_t)ldp * n);
+  T* P = (T*)scratch(WS_ZP, sizeof(T) * (size_t)ldp * n);
   dev_gemm<T>(x, ta ? Trans : NoTrans, tb ? Trans : NoTrans, 2 * m, n, 2 * k, T(1), YA, ldya, YB, ldyb, T(0), P, ldp);
   cplx_combine<T>(x.s, m, n, P, ldp, alpha, beta, C, ldc, uplo, herm);
 }
@@ -766,57 +765,37 @@ void dev_ctrsm(const Exec& x, int side, int uplo, int transA, int diag, int m, i
 }
 
 // Complex Cholesky (A = L L^H lower, U^H U upper): the right-looking blocked
-// algorithm of dev_potrf without lookahead; each diagonal block is factored
-// recursively (leaves of 64) and inverted, the panel is a product with the
-// inverse, the trailing update a Hermitian rank-nb product on the triangle.
-enum { WS_ZDI = 24, WS_ZPAN = 25, WS_ZINV = 26, WS_ZDW = 32 };
+// algorithm of dev_potrf; each diagonal block is factored and inverted
+// through its real form, the panel is a product with the inverse, the
+// trailing update a Hermitian rank-nb product on the triangle.
+enum { WS_ZR = 24, WS_ZPAN = 25, WS_ZINV = 26, WS_ZRI = 29, WS_ZDW = 30 };
 
-// Cholesky factor and inverse of a diagonal block of order n (A = L L^H or
-// U^H U), recursively: factor A11 and invert it, panel with the inverse,
-// Hermitian update of A22, factor and invert A22, and the off-diagonal block
-// of the inverse from two products (L: I21 = -I22 L21 I11; U: I12 = -I11
-// U12 I22).  Leaves of 64: cplx_potrf_leaf, then cplx_trtri_leaf.  Tinv's
-// block on the other side of the diagonal is never written: the caller
-// zeroes it once.  Temporary: WS_ZDW (one n1 x n2 block).
+// Cholesky factor and inverse of a complex diagonal block of order n through
+// its real form: with rho(z) = [re -im; im re], the 2n x 2n real matrix
+// M = [rho(a_ij)] is symmetric positive definite when A is Hermitian positive
+// definite, and its Cholesky factor is rho(L) (lower triangular, diagonal
+// blocks diag(l_ii, l_ii)), whose inverse is rho(inv(L)).  The real recursion
+// (64-order multi-wave leaves, one real product per step) replaces a complex
+// one whose products are four launches each; the chain is latency-bound, so
+// the real form's twice-larger flop count costs little.  R and Rinv hold M,
+// then rho(L) and rho(inv(L)) (their lower triangles; Rinv's upper triangle
+// is zeroed once by the caller and only ever receives zeros).  Tinv gets
+// inv(L) (Lower) or inv(U) = inv(L)^H (Upper) in its uplo triangle.  The
+// leading dimension ldr of R and Rinv stays the same for every block, so
+// that Rinv's upper triangle stays zero.
 template <typename T>
-static void cpotrf_diag(const Exec& x, int uplo, int n, T* A, long lda, T* Tinv, long ldinv, int* info, int base)
+static void cpotrf_diag(hipStream_t s, int uplo, int n, T* A, long lda, T* Tinv, long ldinv, T* R, T* Rinv, long ldr,
+                        int* info, int base)
 {
-  if (n <= 64) {
-    cplx_potrf_leaf<T>(x.s, uplo, n, A, lda, info, base);
-    cplx_trtri_leaf<T>(x.s, uplo, 0, n, A, lda, 0, Tinv, ldinv, 0, 1);
-    return;
-  }
-  auto at = [](auto* b, long off) { return b + 2 * off; };
-  const int n1 = half64(n), n2 = n - n1;
-  const T one[2] = {T(1), T(0)}, mone[2] = {T(-1), T(0)}, zero[2] = {T(0), T(0)};
-  T* A22 = at(A, n1 + (long)n1 * lda);
-  T* I22 = at(Tinv, n1 + (long)n1 * ldinv);
-  cpotrf_diag<T>(x, uplo, n1, A, lda, Tinv, ldinv, info, base);
-  T* W = (T*)scratch(WS_ZDW, 2 * sizeof(T) * (size_t)n1 * n2);
-  if (uplo == Lower) {
-    T* A21 = at(A, n1);
-    copy2d<T>(x.s, 2 * n2, n1, A21, 2 * lda, W, 2L * n2);
-    cgemm_impl<T>(x, NoTrans, ConjTrans, n2, n1, n1, one, W, n2, Tinv, ldinv, zero, A21, lda, 0, 0);
-    cgemm_impl<T>(x, NoTrans, ConjTrans, n2, n2, n1, mone, A21, lda, A21, lda, one, A22, lda, Lower, 1);
-    cpotrf_diag<T>(x, uplo, n2, A22, lda, I22, ldinv, info, base + n1);
-    cgemm_impl<T>(x, NoTrans, NoTrans, n2, n1, n1, one, A21, lda, Tinv, ldinv, zero, W, n2, 0, 0);
-    cgemm_impl<T>(x, NoTrans, NoTrans, n2, n1, n2, mone, I22, ldinv, W, n2, zero, at(Tinv, n1), ldinv, 0, 0);
-  } else {
-    T* A12 = at(A, (long)n1 * lda);
-    copy2d<T>(x.s, 2 * n1, n2, A12, 2 * lda, W, 2L * n1);
-    cgemm_impl<T>(x, ConjTrans, NoTrans, n1, n2, n1, one, Tinv, ldinv, W, n1, zero, A12, lda, 0, 0);
-    cgemm_impl<T>(x, ConjTrans, NoTrans, n2, n2, n1, mone, A12, lda, A12, lda, one, A22, lda, Upper, 1);
-    cpotrf_diag<T>(x, uplo, n2, A22, lda, I22, ldinv, info, base + n1);
-    cgemm_impl<T>(x, NoTrans, NoTrans, n1, n2, n2, one, A12, lda, I22, ldinv, zero, W, n1, 0, 0);
-    cgemm_impl<T>(x, NoTrans, NoTrans, n1, n2, n1, mone, Tinv, ldinv, W, n1, zero, at(Tinv, (long)n1 * ldinv),
-                  ldinv, 0, 0);
-  }
+  cplx_realform<T>(s, uplo, n, A, lda, R, ldr);
+  potrf_diag<T>(s, Lower, 2 * n, R, ldr, Rinv, ldr, info, 2 * base, WS_ZDW, true, 1);
+  cplx_fromreal<T>(s, uplo, n, R, ldr, A, lda, Rinv, Tinv, ldinv);
 }
 
 // Right-looking blocked Cholesky (the reference's tile algorithm,
 // blas/zpotrf.c:89-187, in complex arithmetic) with the structure of
 // dev_potrf: each diagonal block is factored and inverted on the side stream
-// (its own scratch set) while the compute stream applies the previous panel
+// (its own scratch buffers) while the compute stream applies the previous panel
 // to the trailing matrix; the next block column is updated first so that
 // its factorization can start at once.
 template <typename T>
@@ -831,11 +810,11 @@ void dev_cpotrf(const Exec& x, int uplo, int n, T* A, long lda, int nb, int* inf
   // zeroed once: nothing writes their blocks across the diagonal
   T* inv = (T*)scratch(WS_ZINV, 2 * sizeof(T) * (size_t)2 * nb * nb);
   const bool lookahead = x.s_diag != nullptr && nt > 1;
-  Exec xd = x;
-  if (lookahead) {
-    xd.s = x.s_diag;
-    xd.zset = 1;
-  }
+  hipStream_t sd = lookahead ? x.s_diag : x.s;
+  // real forms of one diagonal block and of its factor's inverse: only the
+  // diagonal stream uses them, one block after the other
+  T* R = (T*)scratch(WS_ZR, sizeof(T) * (size_t)4 * nb * nb);
+  T* Rinv = (T*)scratch(WS_ZRI, sizeof(T) * (size_t)4 * nb * nb);
   hipEvent_t ev_diag = x.ev0, ev_look = x.ev1;
   const bool own_events = !ev_diag || !ev_look;
   if (own_events) {
@@ -844,17 +823,18 @@ void dev_cpotrf(const Exec& x, int uplo, int n, T* A, long lda, int nb, int* inf
   }
   if (lookahead) {
     XK_HIP(hipEventRecord(ev_look, x.s));
-    XK_HIP(hipStreamWaitEvent(xd.s, ev_look, 0));
+    XK_HIP(hipStreamWaitEvent(sd, ev_look, 0));
   }
-  fill_zero<T>(xd.s, 2 * nb, 2 * nb, inv, 2L * nb);
+  fill_zero<T>(sd, 2 * nb, 2 * nb, inv, 2L * nb);
+  fill_zero<T>(sd, 2 * nb, 2 * nb, Rinv, 2L * nb);
   auto blk = [&](int i, int j) { return at(A, (long)i * nb + (long)j * nb * lda); };
   auto tsz = [&](int i) { return i == nt - 1 ? n - i * nb : nb; };
-  cpotrf_diag<T>(xd, uplo, tsz(0), blk(0, 0), lda, inv, nb, info, 0);
+  cpotrf_diag<T>(sd, uplo, tsz(0), blk(0, 0), lda, inv, nb, R, Rinv, 2L * nb, info, 0);
   for (int k = 0; k < nt; ++k) {
     const int kk = tsz(k);
     T* ik = at(inv, (long)(k & 1) * nb * nb);
     if (lookahead) {
-      XK_HIP(hipEventRecord(ev_diag, xd.s));
+      XK_HIP(hipEventRecord(ev_diag, sd));
       XK_HIP(hipStreamWaitEvent(x.s, ev_diag, 0));
     }
     if (k == nt - 1) break;
@@ -888,8 +868,8 @@ void dev_cpotrf(const Exec& x, int uplo, int n, T* A, long lda, int nb, int* inf
       }
     }
     T* inext = at(inv, (long)((k + 1) & 1) * nb * nb);
-    if (lookahead) XK_HIP(hipStreamWaitEvent(xd.s, ev_look, 0));
-    cpotrf_diag<T>(xd, uplo, k1, blk(k + 1, k + 1), lda, inext, nb, info, (k + 1) * nb);
+    if (lookahead) XK_HIP(hipStreamWaitEvent(sd, ev_look, 0));
+    cpotrf_diag<T>(sd, uplo, k1, blk(k + 1, k + 1), lda, inext, nb, R, Rinv, 2L * nb, info, (k + 1) * nb);
   }
   if (own_events) {
     XK_HIP(hipEventDestroy(ev_diag));

@@ -22,7 +22,6 @@ struct Exec {
   int mode;           // xkblas_mode_math_t
   int nsplit;         // CRT moduli for split modes (0 = automatic)
   hipEvent_t ev0 = nullptr, ev1 = nullptr;  // POTRF's stream-ordering events (created per call when null)
-  int zset = 0;       // complex products' scratch set (1: the diagonal stream's own)
 };
 
 // Scratch slots of dev_potrf, and the bytes each needs for order n, block nb.

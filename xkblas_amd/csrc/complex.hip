@@ -181,57 +181,48 @@ __global__ __launch_bounds__(64) void cplx_trtri_leaf_kernel(int uplo, int unit,
   }
 }
 
-// Cholesky of one Hermitian block of order n <= 64 in place, one wave, in the
-// lower view (Upper: L = U^H, so U = L^H is written back).  Right-looking by
-// columns in LDS; lane i owns row i of the trailing update.  A non-positive
-// (or NaN) pivot at column j stops the factorization and puts base + j + 1
-// into *info (atomic min).
+// Real form of a Hermitian block for its Cholesky factorization: the lower
+// triangle (r >= c) of the 2n x 2n real M with M[2i+p][2j+q] = rho(a_ij)[p][q],
+// rho(z) = [re -im; im re], a_ij (i >= j) from the stored triangle (Upper:
+// conj(A(j, i))); the diagonal's imaginary parts are taken as zero, as LAPACK
+// does.
 template <typename T>
-__global__ __launch_bounds__(64) void cplx_potrf_leaf_kernel(int uplo, int n, Cplx<T>* __restrict__ A, long lda,
-                                                             int* info, int base)
+__global__ __launch_bounds__(256) void cplx_realform_kernel(int uplo, int n, int c0, const Cplx<T>* __restrict__ A,
+                                                            long lda, T* __restrict__ R, long ldr)
 {
-  __shared__ T Lr[64 * 65], Li[64 * 65];
-  const int i = threadIdx.x;
-  for (int k = 0; k < n; ++k)
-    if (i < n && k <= i) {
-      Cplx<T> v = uplo == Lower ? A[i + (long)k * lda] : A[k + (long)i * lda];
-      if (uplo != Lower) v.im = -v.im;
-      Lr[i + k * 65] = v.re;
-      Li[i + k * 65] = v.im;
-    }
-  __syncthreads();
-  for (int j = 0; j < n; ++j) {
-    const T d = Lr[j + j * 65];
-    if (!(d > T(0))) {  // the same value in every lane: a uniform exit
-      if (i == 0) atomicMin(info, base + j + 1);
-      break;
-    }
-    const T sd = sqrt(d);
-    __syncthreads();
-    if (i == j) {
-      Lr[j + j * 65] = sd;
-      Li[j + j * 65] = T(0);
-    } else if (i > j && i < n) {
-      Lr[i + j * 65] /= sd;
-      Li[i + j * 65] /= sd;
-    }
-    __syncthreads();
-    if (i > j && i < n) {
-      // row i of the trailing triangle: A(i,k) -= L(i,j) conj(L(k,j)), j < k <= i
-      const T lr = Lr[i + j * 65], li = Li[i + j * 65];
-      for (int k = j + 1; k <= i; ++k) {
-        const T kr = Lr[k + j * 65], ki = Li[k + j * 65];
-        Lr[i + k * 65] -= lr * kr + li * ki;
-        Li[i + k * 65] -= li * kr - lr * ki;
-      }
-    }
-    __syncthreads();
+  const int r = blockIdx.x * 256 + threadIdx.x;
+  const int c = c0 + (int)blockIdx.y;
+  if (r >= 2 * n || c > r) return;
+  const int i = r >> 1, j = c >> 1, p = r & 1, q = c & 1;
+  Cplx<T> a = uplo == Lower ? A[i + (long)j * lda] : A[j + (long)i * lda];
+  if (uplo != Lower) a.im = -a.im;
+  if (i == j) a.im = T(0);
+  R[r + (long)c * ldr] = p == q ? a.re : (p ? a.im : -a.im);
+}
+
+// The complex factor and inverse back from the real ones: l_ij = R[2i][2j] +
+// i R[2i+1][2j] for i >= j (a real diagonal), to A's stored triangle (Upper:
+// U(j, i) = conj(l_ij)); likewise inv(L) from Rinv to Tinv (Upper: inv(U) =
+// inv(L)^H).  Tinv's other triangle is not written.
+template <typename T>
+__global__ __launch_bounds__(256) void cplx_fromreal_kernel(int uplo, int n, int j0, const T* __restrict__ R, long ldr,
+                                                            Cplx<T>* __restrict__ A, long lda,
+                                                            const T* __restrict__ Rinv, Cplx<T>* __restrict__ Tinv,
+                                                            long ldinv)
+{
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  const int j = j0 + (int)blockIdx.y;
+  if (i >= n || j > i) return;
+  const long e = 2 * i + (long)(2 * j) * ldr;
+  Cplx<T> l{R[e], i == j ? T(0) : R[e + 1]};
+  Cplx<T> v{Rinv[e], i == j ? T(0) : Rinv[e + 1]};
+  if (uplo == Lower) {
+    A[i + (long)j * lda] = l;
+    Tinv[i + (long)j * ldinv] = v;
+  } else {
+    A[j + (long)i * lda] = Cplx<T>{l.re, -l.im};
+    Tinv[j + (long)i * ldinv] = Cplx<T>{v.re, -v.im};
   }
-  for (int k = 0; k < n; ++k)
-    if (i < n && k <= i) {
-      if (uplo == Lower) A[i + (long)k * lda] = Cplx<T>{Lr[i + k * 65], Li[i + k * 65]};
-      else A[k + (long)i * lda] = Cplx<T>{Lr[i + k * 65], -Li[i + k * 65]};
-    }
 }
 
 template <typename F>
@@ -309,15 +300,30 @@ void cplx_trtri_leaf(hipStream_t s, int uplo, int unit, int n, const T* A, long 
 }
 
 template <typename T>
-void cplx_potrf_leaf(hipStream_t s, int uplo, int n, T* A, long lda, int* info, int base)
+void cplx_realform(hipStream_t s, int uplo, int n, const T* A, long lda, T* R, long ldr)
 {
-  XK_CHECK(n >= 1 && n <= 64, "complex potrf leaf order must be in [1,64]");
-  hipLaunchKernelGGL((cplx_potrf_leaf_kernel<T>), dim3(1), dim3(64), 0, s, uplo, n, (Cplx<T>*)A, lda, info, base);
+  if (n <= 0) return;
+  over_columns(2 * n, 2 * n, [&](dim3 g, int j0) {
+    hipLaunchKernelGGL((cplx_realform_kernel<T>), g, dim3(256), 0, s, uplo, n, j0, (const Cplx<T>*)A, lda, R, ldr);
+  });
+  XK_HIP(hipGetLastError());
+}
+
+template <typename T>
+void cplx_fromreal(hipStream_t s, int uplo, int n, const T* R, long ldr, T* A, long lda, const T* Rinv, T* Tinv,
+                   long ldinv)
+{
+  if (n <= 0) return;
+  over_columns(n, n, [&](dim3 g, int j0) {
+    hipLaunchKernelGGL((cplx_fromreal_kernel<T>), g, dim3(256), 0, s, uplo, n, j0, R, ldr, (Cplx<T>*)A, lda, Rinv,
+                       (Cplx<T>*)Tinv, ldinv);
+  });
   XK_HIP(hipGetLastError());
 }
 
 #define XK_INST(T)                                                                                         \
-  template void cplx_potrf_leaf<T>(hipStream_t, int, int, T*, long, int*, int);                            \
+  template void cplx_realform<T>(hipStream_t, int, int, const T*, long, T*, long);                         \
+  template void cplx_fromreal<T>(hipStream_t, int, int, const T*, long, T*, long, const T*, T*, long);      \
   template void cplx_trtri_leaf<T>(hipStream_t, int, int, int, const T*, long, long, T*, long, long, int); \
   template void cplx_pack_a<T>(hipStream_t, int, int, int, const T*, long, T*, long);                      \
   template void cplx_pack_b<T>(hipStream_t, int, int, int, int, const T*, long, T*, long);                 \

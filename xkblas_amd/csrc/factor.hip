@@ -91,7 +91,7 @@ __device__ __forceinline__ int pk(int i, int k) { return (i * (i + 1) >> 1) + k;
 
 template <typename T>
 __global__ __launch_bounds__(256) void potrf_leaf_kernel(int uplo, int n, T* A, long lda, long sA, T* Tinv,
-                                                         long ldinv, long sInv, int* info, int base)
+                                                         long ldinv, long sInv, int* info, int base, int pair)
 {
   __shared__ T P[LEAF * (LEAF + 1) / 2];
   __shared__ T X[LEAF * LB];  // X[i * LB + c]: rows of one block column of inv(L)
@@ -126,7 +126,8 @@ __global__ __launch_bounds__(256) void potrf_leaf_kernel(int uplo, int n, T* A, 
         const T p = bcast(d[j], j);
         if (!(p > T(0)) && !failed && c0 + j < n) {
           failed = true;
-          if (lane == 0) atomicMin(info, base + c0 + j + 1);
+          // pair: rows 2i, 2i+1 stand for order i of a complex block
+          if (lane == 0) atomicMin(info, pair ? (base + c0 + j) / 2 + 1 : base + c0 + j + 1);
         }
         const T sq = sqrt(p);
         d[j] = i > j ? d[j] / sq : (i == j ? sq : T(0));
@@ -246,12 +247,12 @@ __global__ __launch_bounds__(64) void trtri_leaf_kernel(int uplo, int diag, int 
 
 template <typename T>
 void potrf_leaf(hipStream_t s, int uplo, int n, T* A, long lda, long sA, T* Tinv, long ldinv, long sInv,
-                int batch, int* info, int base)
+                int batch, int* info, int base, int pair)
 {
   XK_CHECK(n >= 1 && n <= LEAF, "potrf leaf order must be in [1,64]");
   if (batch <= 0) return;
   hipLaunchKernelGGL((potrf_leaf_kernel<T>), dim3(batch), dim3(256), 0, s, uplo, n, A, lda, sA, Tinv, ldinv, sInv,
-                     info, base);
+                     info, base, pair);
   XK_HIP(hipGetLastError());
 }
 
@@ -321,7 +322,7 @@ void fill_int(hipStream_t s, int* p, int n, int v)
 }
 
 #define XK_INST(T)                                                                                           \
-  template void potrf_leaf<T>(hipStream_t, int, int, T*, long, long, T*, long, long, int, int*, int);         \
+  template void potrf_leaf<T>(hipStream_t, int, int, T*, long, long, T*, long, long, int, int*, int, int);         \
   template void trtri_leaf<T>(hipStream_t, int, int, int, const T*, long, long, T*, long, long, int);         \
   template void copy2d<T>(hipStream_t, int, int, const T*, long, T*, long);                                    \
   template void zero_triangle<T>(hipStream_t, int, int, T*, long);

@@ -26,10 +26,12 @@ int gemm_get_tile();
 // upper when uplo == Upper), in place, plus the inverse of each factor
 // written to Linv (lower: inv(L); upper: inv(U)).  info[0] receives
 // base_order + j + 1 for the first non-positive pivot (atomic min, starts at
-// INT_MAX), nothing on success.
+// INT_MAX), nothing on success.  pair != 0: the block is the real form of a
+// complex block (orders 2i, 2i + 1 stand for complex order i), and a failing
+// pivot j reports (base_order + j) / 2 + 1.
 template <typename T>
 void potrf_leaf(hipStream_t s, int uplo, int n, T* A, long lda, long strideA, T* Linv, long ldinv,
-                long strideInv, int batch, int* info, int base_order);
+                long strideInv, int batch, int* info, int base_order, int pair = 0);
 
 // Inverse of `batch` triangular blocks of order n <= 64 into Tinv (the other
 // triangle of Tinv is zeroed, like the full block).
@@ -99,11 +101,16 @@ enum : int { CPLX_SYM = 0, CPLX_HERM = 1, CPLX_TRI = 2, CPLX_UNIT = 3 };
 template <typename T>
 void cplx_trtri_leaf(hipStream_t s, int uplo, int unit, int n, const T* A, long lda, long sA, T* Tinv, long ldinv,
                      long sInv, int batch);
-// Cholesky (A = L L^H, or U^H U for Upper) of one complex Hermitian block of
-// order n <= 64 in place; a failing pivot j puts base + j + 1 into *info
-// (atomic min).
+// R (2n x 2n real, lower triangle) <- the real form [rho(a_ij)], rho(z) =
+// [re -im; im re], of the Hermitian block whose uplo triangle A stores.
 template <typename T>
-void cplx_potrf_leaf(hipStream_t s, int uplo, int n, T* A, long lda, int* info, int base);
+void cplx_realform(hipStream_t s, int uplo, int n, const T* A, long lda, T* R, long ldr);
+// From the real forms rho(L) (R) and rho(inv(L)) (Rinv, same leading
+// dimension): L to A's uplo triangle (Upper: U = L^H) and inv(L) to Tinv's
+// (Upper: inv(U)).
+template <typename T>
+void cplx_fromreal(hipStream_t s, int uplo, int n, const T* R, long ldr, T* A, long lda, const T* Rinv, T* Tinv,
+                   long ldinv);
 template <typename T>
 void cplx_expand(hipStream_t s, int mode, int uplo, int n, const T* A, long lda, T* Y, long ldy);
 
