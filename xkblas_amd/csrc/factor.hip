@@ -2,12 +2,11 @@
 // Cholesky + inverse of blocks of order <= 64, triangular inverse, and small
 // helpers (2-D copy, triangle zeroing, int fill).
 //
-// A leaf is one wave (64 lanes) per block; the padded rows/columns beyond n
-// are the identity, so every block is handled as a 64 x 64 problem.  The
-// Cholesky leaf keeps the block in registers (one row per lane, cross-lane
-// reads for the broadcasts) and writes the factor and its inverse in one
-// launch; the triangular-inverse leaf holds the block in LDS (column stride
-// 65) and solves one column of inv(L) per lane.
+// The padded rows/columns of a leaf beyond n are the identity, so every
+// block is handled as a 64 x 64 problem.  The Cholesky leaf (four waves)
+// writes the factor and its inverse in one launch; the triangular-inverse
+// leaf (one wave) holds the block in LDS (column stride 65) and solves one
+// column of inv(L) per lane.
 #include "common.h"
 #include "kernels.h"
 #include <climits>
@@ -27,6 +26,37 @@ __device__ __forceinline__ double bcast(double x, int src)
 __device__ __forceinline__ float bcast(float x, int src)
 {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), src));
+}
+
+// sq = sqrt(p) and rs = 1 / sq for a pivot p > 0, without a division:
+// Goldschmidt's iteration from the hardware reciprocal square root, two
+// Newton corrections of the root (which make sq the correctly rounded
+// root in the normal range) and one of the reciprocal (about 1 ulp).
+// Pivots outside [2^-900, 2^900] (and p <= 0, NaN) take sqrt() and a
+// division.
+__device__ __forceinline__ void sqrt_rcp(double p, double& sq, double& rs)
+{
+  if (!(p >= 0x1p-900 && p <= 0x1p+900)) {
+    sq = sqrt(p);
+    rs = 1.0 / sq;
+    return;
+  }
+  const double r = __builtin_amdgcn_rsq(p);
+  double g = p * r, h = 0.5 * r;
+  const double e = fma(-h, g, 0.5);
+  g = fma(g, e, g);
+  h = fma(h, e, h);
+  g = fma(fma(-g, g, p), h, g);
+  g = fma(fma(-g, g, p), h, g);
+  double q = 2.0 * h;
+  q = fma(q, fma(-g, q, 1.0), q);
+  sq = g;
+  rs = q;
+}
+__device__ __forceinline__ void sqrt_rcp(float p, float& sq, float& rs)
+{
+  sq = sqrtf(p);
+  rs = 1.0f / sq;
 }
 
 // Row `lane` of the lower-triangular view L of block A:
@@ -67,24 +97,37 @@ __device__ __forceinline__ void lower_inverse_to(const T* Ls, int lane, bool uni
   }
 }
 
+// Workgroup barrier that orders LDS accesses only: global stores in flight
+// are not waited for (nothing in the kernel reads them back).
+__device__ __forceinline__ void lds_barrier()
+{
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // Cholesky factor and its inverse for one block of order <= 64, blocked by
 // 16 and spread over four waves.
 // LDS: the lower triangle packed by rows, P[i (i + 1) / 2 + k] = L(i, k),
-// and one 64 x 16 block column of the inverse: 24.3 KB in FP64.  A
-// trailing-update GEMM that fills a CU (four 33 KB workgroups) leaves 28 KB
-// of LDS free, so a leaf launched beside it starts at once in that gap
+// and one 64 x 16 block column of the inverse: 24.4 KB in FP64.
+// A trailing-update GEMM that fills a CU (four 33 KB workgroups) leaves
+// 28 KB of LDS free, so a leaf launched beside it starts at once in that gap
 // instead of waiting for a CU to drain (a leaf holding the block and its
-// inverse unpacked, 72 KB, waited up to milliseconds).
+// inverse unpacked, 72 KB, waited up to milliseconds).  About 125 VGPRs: it
+// also fits beside three update-GEMM waves (104-112 VGPRs each) on a SIMD.
 // For each 16-column block jb:
 //   wave 0 factors the 16 x 16 diagonal block in registers (lane i holds row
-//   i, cross-lane reads for the broadcasts: at most 15 per step), writes it
-//   to A and puts its inverse (lane c, column c) in its place in P;
+//   i, cross-lane reads for the broadcasts), with the pivots' roots and
+//   reciprocals from one reciprocal-square-root iteration (no division on
+//   the chain), then inverts it (lane c solves column c), writes L to A and
+//   the inverse in its place in P;
 //   all threads solve the panel below, L21 = A21 inv(L11)^T, and update the
 //   trailing triangle, A22 -= L21 L21^T.
 // Then P holds L below the diagonal blocks and the diagonal blocks'
-// inverses; inv(L) follows by block columns jb: X_jb = inv(L_jb,jb), X_ib =
-// -inv(L_ib,ib) sum_{t=jb}^{ib-1} L[ib][t] X_t.  Rows/columns beyond n are the
-// identity.  info receives the first order whose pivot is not positive.
+// inverses, from which inv(L) follows by block columns.  The barriers order
+// LDS only (no wait for the global stores of L and inv(L)).  Rows/columns
+// beyond n are the identity.  info receives the first order whose pivot is
+// not positive.
 constexpr int LB = 16;  // inner block of the leaf
 
 __device__ __forceinline__ int pk(int i, int k) { return (i * (i + 1) >> 1) + k; }
@@ -94,7 +137,8 @@ __global__ __launch_bounds__(256) void potrf_leaf_kernel(int uplo, int n, T* A, 
                                                          long ldinv, long sInv, int* info, int base, int pair)
 {
   __shared__ T P[LEAF * (LEAF + 1) / 2];
-  __shared__ T X[LEAF * LB];  // X[i * LB + c]: rows of one block column of inv(L)
+  __shared__ T X[LEAF * LB];            // X[i * LB + c]: rows of one block column of inv(L)
+  __shared__ T Lr[LB];                  // reciprocals of the diagonal block's pivots
   // the leaf is the latency-bound step of the diagonal chain and shares its
   // CU with update GEMM waves: its waves issue first
   __builtin_amdgcn_s_setprio(3);
@@ -107,57 +151,73 @@ __global__ __launch_bounds__(256) void potrf_leaf_kernel(int uplo, int n, T* A, 
       else A[k + (long)i * lda] = v;
     }
   };
-  for (int e = tid; e < LEAF * LEAF; e += 256) {
-    const int i = e & 63, k = e >> 6;
-    if (k <= i) P[pk(i, k)] = lower_view(A, lda, uplo, n, i, k);
+  {
+    // all loads in flight before the first LDS store
+    T v[LEAF * LEAF / 256];
+#pragma unroll
+    for (int u = 0; u < LEAF * LEAF / 256; ++u) {
+      const int e = tid + 256 * u, i = e & 63, k = e >> 6;
+      v[u] = k <= i ? lower_view(A, lda, uplo, n, i, k) : T(0);
+    }
+#pragma unroll
+    for (int u = 0; u < LEAF * LEAF / 256; ++u) {
+      const int e = tid + 256 * u, i = e & 63, k = e >> 6;
+      if (k <= i) P[pk(i, k)] = v[u];
+    }
   }
-  __syncthreads();
+  lds_barrier();
 
-  bool failed = false;  // (wave 0)
+  int fail = INT_MAX;  // first order with a non-positive pivot (wave 0)
   for (int jb = 0; jb < LEAF / LB; ++jb) {
     const int c0 = jb * LB;
     if (wave == 0) {
       const int i = lane & (LB - 1);
+      const bool own = lane < LB;
       T d[LB];
 #pragma unroll
       for (int k = 0; k < LB; ++k) d[k] = k <= i ? P[pk(c0 + i, c0 + k)] : T(0);
+      // right-looking: the pivot chain needs no cross-lane data beyond the
+      // pivot itself (lane j + 1 updates its own diagonal entry)
 #pragma unroll
       for (int j = 0; j < LB; ++j) {
         const T p = bcast(d[j], j);
-        if (!(p > T(0)) && !failed && c0 + j < n) {
-          failed = true;
-          // pair: rows 2i, 2i+1 stand for order i of a complex block
-          if (lane == 0) atomicMin(info, pair ? (base + c0 + j) / 2 + 1 : base + c0 + j + 1);
-        }
-        const T sq = sqrt(p);
-        d[j] = i > j ? d[j] / sq : (i == j ? sq : T(0));
+        fail = (!(p > T(0)) && fail == INT_MAX && c0 + j < n) ? c0 + j : fail;
+        T sq, rs;
+        sqrt_rcp(p, sq, rs);
+        if (lane == 0) Lr[j] = rs;
+        d[j] = i > j ? d[j] * rs : (i == j ? sq : T(0));
 #pragma unroll
         for (int k = j + 1; k < LB; ++k) d[k] = fma(-d[j], bcast(d[j], k), d[k]);
       }
-      T y[LB];  // column i of the block's inverse
+      if (own) {
+#pragma unroll
+        for (int k = 0; k < LB; ++k)
+          if (k <= i) put(c0 + i, c0 + k, d[k]);
+      }
+      // column i of inv(L11): y_r = (delta_ri - sum_{k<r} L(r, k) y_k) / L(r, r),
+      // L(r, k) read from lane r
+      T y[LB];
 #pragma unroll
       for (int r = 0; r < LB; ++r) {
         T acc = (r == i) ? T(1) : T(0);
 #pragma unroll
         for (int k = 0; k < r; ++k) acc = fma(-bcast(d[k], r), y[k], acc);
-        y[r] = r < i ? T(0) : acc / bcast(d[r], r);
+        y[r] = r < i ? T(0) : acc * Lr[r];
       }
-      if (lane < LB) {
 #pragma unroll
-        for (int k = 0; k < LB; ++k)
-          if (k <= lane) put(c0 + lane, c0 + k, d[k]);
-        // (the whole block was read into d before any lane writes)
+      for (int r = 0; r < LB; ++r) d[r] = y[r];
+      if (own) {
 #pragma unroll
         for (int r = 0; r < LB; ++r)
-          if (r >= lane) P[pk(c0 + r, c0 + lane)] = y[r];
+          if (r >= i) P[pk(c0 + r, c0 + i)] = d[r];
       }
     }
-    __syncthreads();
+    lds_barrier();
     const int m = LEAF - c0 - LB;  // rows below the diagonal block
     if (m > 0) {
       // panel: L[c0+LB+r][c0+c] = sum_{t <= c} A[c0+LB+r][c0+t] inv(L11)[c][t]
       T pv[3];
-#pragma unroll
+#pragma unroll 1
       for (int u = 0; u < 3; ++u) {
         const int e = tid + 256 * u;
         pv[u] = T(0);
@@ -165,68 +225,95 @@ __global__ __launch_bounds__(256) void potrf_leaf_kernel(int uplo, int n, T* A, 
           const int r = e % m, c = e / m;
           const T* arow = P + pk(c0 + LB + r, c0);
           const T* irow = P + pk(c0 + c, c0);
-          T acc = T(0);
-          for (int t = 0; t <= c; ++t) acc = fma(arow[t], irow[t], acc);
-          pv[u] = acc;
+          T a0 = T(0), a1 = T(0);
+#pragma unroll
+          for (int t = 0; t < LB; t += 2) {
+            a0 = fma(arow[t], t <= c ? irow[t] : T(0), a0);
+            a1 = fma(arow[t + 1], t + 1 <= c ? irow[t + 1] : T(0), a1);
+          }
+          pv[u] = a0 + a1;
         }
       }
-      __syncthreads();
+      lds_barrier();
 #pragma unroll
       for (int u = 0; u < 3; ++u) {
         const int e = tid + 256 * u;
         if (e < m * LB) P[pk(c0 + LB + e % m, c0 + e / m)] = pv[u];
       }
-      __syncthreads();
-      // trailing triangle: L[c0+LB+r][c0+LB+q] -= sum_t L21[r][t] L21[q][t], r >= q
-      for (int e = tid; e < m * m; e += 256) {
-        const int r = e % m, q = e / m;
-        if (r >= q) {
+      lds_barrier();
+      // trailing triangle: L[c0+LB+r][c0+LB+q] -= sum_t L21[r][t] L21[q][t],
+      // q <= r < m, element e = r (r + 1) / 2 + q of the triangle; each
+      // element is read and written by its own thread only
+      const int cnt = m * (m + 1) / 2;
+#pragma unroll 1
+      for (int u = 0; u < 5; ++u) {  // cnt <= 48 * 49 / 2 < 5 * 256
+        const int e = tid + 256 * u;
+        if (e < cnt) {
+          int r = (int)((sqrtf(8.0f * (float)e + 1.0f) - 1.0f) * 0.5f);
+          if ((r + 1) * (r + 2) / 2 <= e) ++r;
+          if (r * (r + 1) / 2 > e) --r;
+          const int q = e - r * (r + 1) / 2;
           const T* rr = P + pk(c0 + LB + r, c0);
           const T* rq = P + pk(c0 + LB + q, c0);
-          T acc = P[pk(c0 + LB + r, c0 + LB + q)];
+          T a0 = P[pk(c0 + LB + r, c0 + LB + q)], a1 = T(0);
 #pragma unroll
-          for (int t = 0; t < LB; ++t) acc = fma(-rr[t], rq[t], acc);
-          P[pk(c0 + LB + r, c0 + LB + q)] = acc;
+          for (int t = 0; t < LB; t += 2) {
+            a0 = fma(-rr[t], rq[t], a0);
+            a1 = fma(-rr[t + 1], rq[t + 1], a1);
+          }
+          P[pk(c0 + LB + r, c0 + LB + q)] = a0 + a1;
         }
       }
-      __syncthreads();
+      lds_barrier();
     }
   }
+  // pair: orders 2i, 2i + 1 stand for order i of a complex block
+  if (tid == 0 && fail != INT_MAX) atomicMin(info, pair ? (base + fail) / 2 + 1 : base + fail + 1);
   // the factor below the diagonal blocks to the stored triangle
   for (int e = tid; e < LEAF * LEAF; e += 256) {
     const int i = e & 63, k = e >> 6;
     if (k < (i & ~(LB - 1))) put(i, k, P[pk(i, k)]);
   }
   if (!Tinv) return;
-  // inv(L) by block columns; thread (r, c) owns entry (r, c) of a 16 x 16 block
-  const int r = tid & (LB - 1), c = tid >> 4;
   auto out = [&](int i, int j, T v) {  // inv(L)(i, j) to Tinv (Upper: inv(U) = inv(L)^T)
     if (i < n && j < n) {
       if (uplo == Lower) Tinv[i + (long)j * ldinv] = v;
       else Tinv[j + (long)i * ldinv] = v;
     }
   };
+  // inv(L) by block columns jb: X_jb = inv(L_jb,jb), X_ib = -inv(L_ib,ib)
+  // sum_{t=jb}^{ib-1} L[ib][t] X_t; thread (r, c) owns entry (r, c) of a
+  // 16 x 16 block
+  const int r = tid & (LB - 1), c = tid >> 4;
   for (int jb = 0; jb < LEAF / LB; ++jb) {
     const int j0 = jb * LB;
     const T xd = c <= r ? P[pk(j0 + r, j0 + c)] : T(0);  // inv(L_jb,jb)
     X[(j0 + r) * LB + c] = xd;
     out(j0 + r, j0 + c, xd);
     for (int ib = 0; ib < jb; ++ib) out(ib * LB + r, j0 + c, T(0));
-    __syncthreads();
+    lds_barrier();
     for (int ib = jb + 1; ib < LEAF / LB; ++ib) {
       const int i0 = ib * LB;
-      T acc = T(0);
       const T* lrow = P + pk(i0 + r, 0);
-      for (int t = j0; t < i0; ++t) acc = fma(lrow[t], X[t * LB + c], acc);
-      X[(i0 + r) * LB + c] = acc;
-      __syncthreads();
-      T v = T(0);
+      T a0 = T(0), a1 = T(0);
+      for (int t = j0; t < i0; t += 2) {  // i0 - j0 is a multiple of 16
+        a0 = fma(lrow[t], X[t * LB + c], a0);
+        a1 = fma(lrow[t + 1], X[(t + 1) * LB + c], a1);
+      }
+      X[(i0 + r) * LB + c] = a0 + a1;
+      lds_barrier();
       const T* drow = P + pk(i0 + r, i0);
-      for (int s2 = 0; s2 <= r; ++s2) v = fma(drow[s2], X[(i0 + s2) * LB + c], v);
-      __syncthreads();
-      X[(i0 + r) * LB + c] = -v;
-      out(i0 + r, j0 + c, -v);
-      __syncthreads();
+      a0 = T(0);
+      a1 = T(0);
+#pragma unroll
+      for (int s2 = 0; s2 < LB; s2 += 2) {
+        a0 = fma(s2 <= r ? drow[s2] : T(0), X[(i0 + s2) * LB + c], a0);
+        a1 = fma(s2 + 1 <= r ? drow[s2 + 1] : T(0), X[(i0 + s2 + 1) * LB + c], a1);
+      }
+      lds_barrier();
+      X[(i0 + r) * LB + c] = -(a0 + a1);
+      out(i0 + r, j0 + c, -(a0 + a1));
+      lds_barrier();
     }
   }
 }
