@@ -370,10 +370,12 @@ def main():
         print("bench: split-%s dgemm" % label, file=sys.stderr, flush=True)
         errs = accuracy(xk, A, B, mode, args.nb, full_ref=full_ref)
         t = time_dgemm(xk, A, B, C, args.steps, args.warmup, mode, args.nb)
-        h2h = host_to_host(xk, A, B, C, mode, args.nb)
+        # kernel statistics of the timed resident steps only (the host-to-host
+        # run below launches per row panel)
         c_mg, ms_mg, fl_mg = kernel_stats(xk, cls)
         c_pr, ms_pr, _ = kernel_stats(xk, "split_prep")
         c_cr, ms_cr, _ = kernel_stats(xk, "split_crt")
+        h2h = host_to_host(xk, A, B, C, mode, args.nb)
         L.xkblas_set_modemath(0)
         nm, npl = ctypes.c_int(), ctypes.c_int()
         bits = L.xkblas_split_info(mode, n, L.xkblas_get_split_count(), ctypes.byref(nm), ctypes.byref(npl))
