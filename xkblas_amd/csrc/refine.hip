@@ -108,6 +108,8 @@ __global__ __launch_bounds__(256) void add2d_kernel(int m, int n, const double* 
 // the same way by the reduction kernel.  The remaining error is that of the
 // 64-term runs and one final rounding: about sqrt(k / 64) times smaller.
 constexpr int RES_SUB = 64, RES_KR = 1024;
+constexpr int RES_WAVES = 8;           // 16 rows each: 128 rows per workgroup
+constexpr int RES_LDK = RES_SUB + 4;   // padded k stride of the staged X run
 typedef double rd4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ void two_sum(double a, double b, double& s, double& e)
@@ -118,17 +120,20 @@ __device__ __forceinline__ void two_sum(double a, double b, double& s, double& e
 }
 
 // Partial sums over the k-range of blockIdx.z: P[z * sp + i + j ldp] =
-// (sum, error) of sum_k A(i, k) X(k, j).  Workgroup: 64 rows x 64 columns,
+// (sum, error) of sum_k A(i, k) X(k, j).  Workgroup: 128 rows x 64 columns,
 // wave w the rows 16 w .. 16 w + 15, four 16 x 16 blocks of
-// v_mfma_f64_16x16x4f64 (operands loaded straight from memory: A is read
-// once, the k-range of X is shared by every row block and stays in L2).
-__global__ __launch_bounds__(256) void resid_partial_kernel(int n, int nrhs, const double* __restrict__ A, long lda,
+// v_mfma_f64_16x16x4f64.  Each 64-k run of X is staged once in LDS for the
+// eight waves (Xs[j * RES_LDK + k]); A is read once, straight from memory;
+// the loads of the next run (A and X, in registers) are issued before the
+// products of the current one.
+__global__ __launch_bounds__(512) void resid_partial_kernel(int n, int nrhs, const double* __restrict__ A, long lda,
                                                             const double* __restrict__ X, long ldx,
                                                             double2* __restrict__ P, long ldp, long sp)
 {
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  __shared__ double Xs[64 * RES_LDK];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int li = lane & 15, lk = lane >> 4;
-  const int i0 = blockIdx.x * 64 + wave * 16, j0 = blockIdx.y * 64;
+  const int i0 = blockIdx.x * (16 * RES_WAVES) + wave * 16, j0 = blockIdx.y * 64;
   const int k0 = blockIdx.z * RES_KR, k1 = min(n, k0 + RES_KR);
   const int ia = i0 + li;
   const bool rowok = ia < n;
@@ -137,25 +142,45 @@ __global__ __launch_bounds__(256) void resid_partial_kernel(int n, int nrhs, con
   for (int b = 0; b < 4; ++b)
 #pragma unroll
     for (int r = 0; r < 4; ++r) hi[b][r] = lo[b][r] = 0.0;
+  constexpr int NX = 64 * RES_SUB / (64 * RES_WAVES), NA = RES_SUB / 4;
+  double xv[NX], av[NA], an[NA];
+  // the loads of run ks (X into xv, A into the given registers)
+  auto load_run = [&](int ks, double* a) {
+    const int ke = min(k1, ks + RES_SUB);
+#pragma unroll
+    for (int u = 0; u < NX; ++u) {
+      const int e = tid + 64 * RES_WAVES * u, j = e >> 6, kk = ks + (e & 63);
+      xv[u] = (kk < ke && j0 + j < nrhs) ? X[kk + (long)(j0 + j) * ldx] : 0.0;
+    }
+#pragma unroll
+    for (int t = 0; t < NA; ++t) {
+      const int kk = ks + 4 * t + lk;
+      a[t] = (rowok && kk < ke) ? A[ia + (long)kk * lda] : 0.0;
+    }
+  };
+  load_run(k0, an);
   for (int ks = k0; ks < k1; ks += RES_SUB) {
+#pragma unroll
+    for (int t = 0; t < NA; ++t) av[t] = an[t];
+    __syncthreads();  // the previous run's X reads are done
+#pragma unroll
+    for (int u = 0; u < NX; ++u) {
+      const int e = tid + 64 * RES_WAVES * u;
+      Xs[(e >> 6) * RES_LDK + (e & 63)] = xv[u];
+    }
+    __syncthreads();
+    // the next run's loads fly under this run's products
+    if (ks + RES_SUB < k1) load_run(ks + RES_SUB, an);
     rd4 acc[4];
 #pragma unroll
     for (int b = 0; b < 4; ++b) acc[b] = rd4{0.0, 0.0, 0.0, 0.0};
-    const int ke = min(k1, ks + RES_SUB);
-#pragma unroll 4
-    for (int k = ks; k < ke; k += 4) {
-      const int kk = k + lk;
-      const bool kok = kk < ke;
-      const double a = (rowok && kok) ? A[ia + (long)kk * lda] : 0.0;
-      double x[4];
 #pragma unroll
-      for (int b = 0; b < 4; ++b) {
-        const int j = j0 + 16 * b + li;
-        x[b] = (kok && j < nrhs) ? X[kk + (long)j * ldx] : 0.0;
-      }
+    for (int t = 0; t < RES_SUB / 4; ++t) {
       // (x first: lane holds element (row li, column lk + 4 r) of the block)
 #pragma unroll
-      for (int b = 0; b < 4; ++b) acc[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(x[b], a, acc[b], 0, 0, 0);
+      for (int b = 0; b < 4; ++b)
+        acc[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(Xs[(16 * b + li) * RES_LDK + 4 * t + lk], av[t], acc[b], 0, 0,
+                                                      0);
     }
 #pragma unroll
     for (int b = 0; b < 4; ++b)
@@ -214,8 +239,8 @@ void resid_accurate(hipStream_t s, int n, int nrhs, const double* A, long lda, c
   const long sp = ldp * nrhs;
   double2* P = static_cast<double2*>(work);
   XK_CHECK((nrhs + 63) / 64 <= 65535 && nz <= 65535 && nrhs <= 65535, "residual grid");
-  hipLaunchKernelGGL(resid_partial_kernel, dim3((n + 63) / 64, (nrhs + 63) / 64, nz), dim3(256), 0, s, n, nrhs, A,
-                     lda, X, ldx, P, ldp, sp);
+  hipLaunchKernelGGL(resid_partial_kernel, dim3((n + 16 * RES_WAVES - 1) / (16 * RES_WAVES), (nrhs + 63) / 64, nz),
+                     dim3(64 * RES_WAVES), 0, s, n, nrhs, A, lda, X, ldx, P, ldp, sp);
   hipLaunchKernelGGL(resid_reduce_kernel, dim3((n + 255) / 256, nrhs), dim3(256), 0, s, n, nz, B, ldb, P, ldp, sp,
                      R, ldr);
   XK_HIP(hipGetLastError());
