@@ -557,17 +557,27 @@ static void cgemm_impl(const Exec& x, int transA, int transB, int m, int n, int 
     T* V1 = V;
     T* V2 = V + vsz;
     cplx_pack2<T>(x.s, ta, 0, 1, transA == ConjTrans ? -1 : 1, ar, ac, A, lda, U, ldu);
-    // [Zr -Zi] equals U when B is A with the conjugate operation (HERK):
-    // the real part is then a SYRK of U
-    const bool same = A == B && lda == ldb && ta == tb && (transA == ConjTrans ? -1 : 1) == -zi;
+    // MFMA modes: both products in one launch (batch of two: U shared, V1
+    // and V2 one stride apart, Pr and Pi n rows apart), so the second fills
+    // the partial last round of the first's tiles.  Split modes: two
+    // products; [Zr -Zi] equals U when B is A with the conjugate operation
+    // (HERK), and the real part is then a SYRK of U
+    const bool one_launch = !split_enabled<T>(x);
+    const bool same =
+        !one_launch && A == B && lda == ldb && ta == tb && (transA == ConjTrans ? -1 : 1) == -zi;
     if (same) V1 = U;
     else cplx_pack2<T>(x.s, tb, 0, 1, -zi, br, bc, B, ldb, V1, ldv);
     cplx_pack2<T>(x.s, tb, 1, 1, zi, br, bc, B, ldb, V2, ldv);
     const long ldp = rup(2L * n);
     T* P = (T*)scratch(WS_ZP, sizeof(T) * (size_t)ldp * n);
     const int opu = ta ? Trans : NoTrans, opv = tb ? NoTrans : Trans;
-    dev_gemmt<T>(x, uplo, opu, opv, n, 2 * k, T(1), U, ldu, V1, same ? ldu : ldv, T(0), P, ldp);
-    dev_gemmt<T>(x, uplo, opu, opv, n, 2 * k, T(1), U, ldu, V2, ldv, T(0), P + n, ldp);
+    if (one_launch) {
+      gemm<T>(x.s, opu, opv, n, n, 2 * k, T(1), U, ldu, V1, ldv, T(0), P, ldp, uplo == Lower ? TRI_LOWER : TRI_UPPER,
+              2, 0, (long)vsz, (long)n);
+    } else {
+      dev_gemmt<T>(x, uplo, opu, opv, n, 2 * k, T(1), U, ldu, V1, same ? ldu : ldv, T(0), P, ldp);
+      dev_gemmt<T>(x, uplo, opu, opv, n, 2 * k, T(1), U, ldu, V2, ldv, T(0), P + n, ldp);
+    }
     cplx_combine<T>(x.s, m, n, P, ldp, alpha, beta, C, ldc, uplo, herm);
     return;
   }
