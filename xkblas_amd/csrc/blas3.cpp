@@ -709,9 +709,47 @@ static void dev_ctrtri(const Exec& x, int uplo, int unit, int n, const T* A, lon
   }
 }
 
+// dev_ctrtri over `cnt` triangular blocks of order n at once (block b at
+// A + b sA, its inverse at Tinv + b sInv, strides in complex elements): the
+// leaves of every level are one batched launch for all blocks; the
+// products run block by block.
+template <typename T>
+static void ctrtri_batched(const Exec& x, int uplo, int unit, int n, const T* A, long lda, long sA, T* Tinv,
+                           long ldinv, long sInv, int cnt)
+{
+  if (cnt <= 0) return;
+  if (n <= 64) {
+    cplx_trtri_leaf<T>(x.s, uplo, unit, n, A, lda, sA, Tinv, ldinv, sInv, cnt);
+    return;
+  }
+  const int n1 = half64(n), n2 = n - n1;
+  auto at = [](auto* base, long off) { return base + 2 * off; };
+  ctrtri_batched<T>(x, uplo, unit, n1, A, lda, sA, Tinv, ldinv, sInv, cnt);
+  ctrtri_batched<T>(x, uplo, unit, n2, at(A, n1 + (long)n1 * lda), lda, sA, at(Tinv, n1 + (long)n1 * ldinv), ldinv,
+                    sInv, cnt);
+  T* W = (T*)scratch(WS_ZW, 2 * sizeof(T) * (size_t)n1 * n2);
+  const T one[2] = {T(1), T(0)}, mone[2] = {T(-1), T(0)}, zero[2] = {T(0), T(0)};
+  for (int b = 0; b < cnt; ++b) {
+    const T* Ab = at(A, (long)b * sA);
+    T* Ib = at(Tinv, (long)b * sInv);
+    T* I22 = at(Ib, n1 + (long)n1 * ldinv);
+    if (uplo == Lower) {
+      cgemm_impl<T>(x, NoTrans, NoTrans, n2, n1, n1, one, at(Ab, n1), lda, Ib, ldinv, zero, W, n2, 0, 0);
+      cgemm_impl<T>(x, NoTrans, NoTrans, n2, n1, n2, mone, I22, ldinv, W, n2, zero, at(Ib, n1), ldinv, 0, 0);
+      cplx_combine<T>(x.s, n1, n2, nullptr, 0, zero, zero, at(Ib, (long)n1 * ldinv), ldinv);
+    } else {
+      cgemm_impl<T>(x, NoTrans, NoTrans, n1, n2, n2, one, at(Ab, (long)n1 * lda), lda, I22, ldinv, zero, W, n1, 0, 0);
+      cgemm_impl<T>(x, NoTrans, NoTrans, n1, n2, n1, mone, Ib, ldinv, W, n1, zero, at(Ib, (long)n1 * ldinv), ldinv, 0,
+                    0);
+      cplx_combine<T>(x.s, n2, n1, nullptr, 0, zero, zero, at(Ib, n1), ldinv);
+    }
+  }
+}
+
 // Complex TRSM: the block sweep of dev_trsm (inverse products for the
 // diagonal blocks, updates by -1 with alpha folded into the first step) with
-// complex products.
+// complex products; the inverses of all full diagonal blocks come from one
+// batched recursion.
 template <typename T>
 void dev_ctrsm(const Exec& x, int side, int uplo, int transA, int diag, int m, int n, const T* alpha, const T* A,
                long lda, T* B, long ldb, int nb)
@@ -730,8 +768,11 @@ void dev_ctrsm(const Exec& x, int side, int uplo, int transA, int diag, int m, i
   const int nblk = (na + nb - 1) / nb;
   auto bsz = [&](int i) { return i == nblk - 1 ? na - i * nb : nb; };
   T* inv = (T*)scratch(WS_TRSM, 2 * sizeof(T) * (size_t)nblk * nb * nb);
-  for (int d = 0; d < nblk; ++d)
-    dev_ctrtri<T>(x, uplo, diag == Unit, bsz(d), at(A, (long)d * nb * (1 + lda)), lda, at(inv, (long)d * nb * nb), nb);
+  const int nfull = bsz(nblk - 1) == nb ? nblk : nblk - 1;
+  ctrtri_batched<T>(x, uplo, diag == Unit, nb, A, lda, (long)nb * (1 + lda), inv, nb, (long)nb * nb, nfull);
+  if (nfull < nblk)
+    dev_ctrtri<T>(x, uplo, diag == Unit, bsz(nblk - 1), at(A, (long)(nblk - 1) * nb * (1 + lda)), lda,
+                  at(inv, (long)(nblk - 1) * nb * nb), nb);
   auto opblk = [&](int i, int j) -> const T* {
     return tr ? at(A, (long)j * nb + (long)i * nb * lda) : at(A, (long)i * nb + (long)j * nb * lda);
   };
