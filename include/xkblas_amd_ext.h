@@ -121,7 +121,7 @@ void  xkblas_mlp_destroy(void* mlp);
 double xkblas_tune_time(int op, int dtype, int mode, int transA, int transB, int m, int n, int k, int tile, int nb,
                         int split_count, int split_kernel, int iters);
 void xkblas_tune_release(void);
-void xkblas_set_gemm_tile(int tile);        /* 0 = automatic, 64, 128 */
+void xkblas_set_gemm_tile(int tile);        /* 0 = automatic, 32 (FP64), 64, 128 */
 void xkblas_set_split_kernel(int variant);  /* BF16 residue GEMM: 0 = 8 waves of 128x64,
                                              1 = 4 waves of 128x128 */
 int  xkblas_split_side_bits(int k, int nsplit);  /* split-BF16 bits per side t */

@@ -18,7 +18,8 @@ void gemm(hipStream_t s, int transA, int transB, int m, int n, int k, T alpha,
           const T* A, long lda, const T* B, long ldb, T beta, T* C, long ldc,
           int tri = TRI_FULL, int batch = 1, long sA = 0, long sB = 0, long sC = 0, int kmode = 0);
 
-// Forces the macro-tile shape of gemm(): 0 = automatic, 64 or 128.
+// Forces the macro-tile shape of gemm(): 0 = automatic, 32 (FP64; FP32 runs
+// 64), 64 or 128.
 void gemm_set_tile(int bm);
 int gemm_get_tile();
 

@@ -128,14 +128,14 @@ extern "C" {
 // inner k), 2 TRSM (Left, Lower, NoTrans, Non-unit, B m x n, block order nb),
 // 3 POTRF (lower, order m, block order nb).  dtype 0 FP64, 1 FP32; mode is
 // the math mode (split modes apply to FP64).  tile: GEMM macro tile (0 auto,
-// 64, 128); split_count: 0 auto; split_kernel: 0 / 1.  Returns the best of
+// 32 (FP64; FP32 runs 64), 64, 128); split_count: 0 auto; split_kernel: 0 / 1.  Returns the best of
 // `iters` kernel times in ms, or a negative value on bad arguments.
 double xkblas_tune_time(int op, int dtype, int mode, int transA, int transB, int m, int n, int k, int tile, int nb,
                         int split_count, int split_kernel, int iters)
 {
   if (op < 0 || op > 3 || m < 1 || n < 1 || k < 0 || iters < 1) return -1.0;
   if (dtype != 0 && dtype != 1) return -1.0;
-  if (tile != 0 && tile != 64 && tile != 128) return -1.0;
+  if (tile != 0 && tile != 32 && tile != 64 && tile != 128) return -1.0;
   xkblas_init();
   std::lock_guard<std::mutex> lock(api_mutex());
   Runtime& r = rt();
@@ -158,7 +158,7 @@ double xkblas_tune_time(int op, int dtype, int mode, int transA, int transB, int
 void xkblas_set_gemm_tile(int tile)
 {
   std::lock_guard<std::mutex> lock(api_mutex());
-  gemm_set_tile(tile == 64 || tile == 128 ? tile : 0);
+  gemm_set_tile(tile == 32 || tile == 64 || tile == 128 ? tile : 0);
 }
 
 void xkblas_set_split_kernel(int variant)

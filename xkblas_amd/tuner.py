@@ -37,17 +37,17 @@ KERN = {"gemm": 1, "syrk": 6, "trsm": 4, "potrf": 4}
 
 
 SPLIT_MODES = (2, 3)  # XKBLAS_SPLIT_BF16_MATH, XKBLAS_SPLIT_FP8_MATH
-MAXCOUNT = {2: 13, 3: 20}
 
 
 def split_counts(mode, k):
-    """Split counts the model may choose for inner dimension k: the
-    library's default (the smallest count keeping >= 46 bits of every row's
-    largest entry, the accuracy contract of the split modes) and one more."""
+    """The split count for inner dimension k: the library's default, the
+    smallest count keeping >= 46 bits of every row's largest entry (the
+    accuracy contract of the split modes).  More moduli only add planes, so
+    the count is an accuracy setting, not a speed choice."""
     L = xk.lib()
     nm = ctypes.c_int()
     L.xkblas_split_info(mode, max(k, 2), 0, ctypes.byref(nm), None)
-    return [c for c in (nm.value, nm.value + 1) if c <= MAXCOUNT[mode]]
+    return [nm.value]
 
 
 def candidates(op, dtype, mode, m, n, k):
@@ -56,9 +56,10 @@ def candidates(op, dtype, mode, m, n, k):
         if mode in SPLIT_MODES and dtype == 0:
             kernels = (0, 1) if mode == 2 else (0,)
             return [{"tile": 0, "nb": 0, "split": c, "kernel": kv} for c in split_counts(mode, k) for kv in kernels]
-        return [{"tile": t, "nb": 0, "split": 0, "kernel": 0} for t in (64, 128)]
+        tiles = (32, 64, 128) if dtype == 0 else (64, 128)
+        return [{"tile": t, "nb": 0, "split": 0, "kernel": 0} for t in tiles]
     dim = m
-    nbs = [b for b in (64, 128, 256, 512, 1024, 2048) if b <= max(64, dim)]
+    nbs = [b for b in (64, 128, 256, 512, 1024, 2048, 4096) if b <= max(64, dim) and (b < 4096 or dim >= 16384)]
     return [{"tile": 0, "nb": b, "split": 0, "kernel": 0} for b in nbs]
 
 
@@ -82,7 +83,7 @@ def features(op, dtype, mode, m, n, k, cfg):
     f[3 + OPS[op]] = 1.0
     f[7] = float(dtype)
     f[8] = 1.0 if mode == 2 else 0.0
-    f[9] = {0: 0.5, 64: 0.0, 128: 1.0}[cfg["tile"]]
+    f[9] = {0: 0.5, 32: -0.5, 64: 0.0, 128: 1.0}[cfg["tile"]]
     f[10] = (math.log2(cfg["nb"]) - 6.0) / 5.0 if cfg["nb"] else 0.0
     f[11] = cfg["split"] / 20.0
     f[12] = float(cfg["kernel"])
