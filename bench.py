@@ -559,10 +559,10 @@ def bench_batched(xk, seed=2027, reps=5, parts=False):
             assert L.xkblas_memcpy_htod(pool, host.ctypes.data_as(p), host.nbytes) == 0
 
         upload()
-        # GEMM item shapes (xkblas_set_batch_tile): 64 x 64 only, per problem
-        # (the default), 32 x 32 only; each timed as a graph-replayed launch
+        # GEMM item shapes (xkblas_set_batch_tile): 64 x 64 only (the default),
+        # per problem, 32 x 32 only; each timed as a graph-replayed launch
         mode_ms = {}
-        for mode in (0, 2, 1):
+        for mode in (1, 2, 0):
             L.xkblas_set_batch_tile(mode)
             mp = L.xkblas_batch_create(ctypes.cast(arr, p), len(arr))
             assert mp and L.xkblas_batch_use_graph(mp, 1) == 0
@@ -580,7 +580,7 @@ def bench_batched(xk, seed=2027, reps=5, parts=False):
             L.xkblas_batch_use_graph(mp, 0)
             L.xkblas_batch_destroy(mp)
         upload()
-        plan = L.xkblas_batch_create(ctypes.cast(arr, p), len(arr))  # default mode 1
+        plan = L.xkblas_batch_create(ctypes.cast(arr, p), len(arr))  # default mode 0
         assert plan
         nit, grid, pfl, pby = ctypes.c_int(), ctypes.c_int(), ctypes.c_double(), ctypes.c_double()
         L.xkblas_batch_info(plan, ctypes.byref(nit), ctypes.byref(grid), ctypes.byref(pfl), ctypes.byref(pby))
