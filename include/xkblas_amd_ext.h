@@ -48,6 +48,11 @@ void xkblas_set_pipeline(int on);
  * XKBLAS_CACHE_LIMIT, kaapi/kaapi_impl.c:311-317). */
 long xkblas_cache_bytes(void);
 long xkblas_cache_evictions(void);
+/* Bytes copied host-to-device and device-to-host by the tile cache since
+ * initialisation (no reference counterpart: the reference reports copies in
+ * its trace, cpy.csv).  Eviction drops clean mirrors (every tile also valid
+ * on the host) before dirty ones, which need a write-back. */
+void xkblas_cache_traffic(double* h2d_bytes, double* d2h_bytes);
 void xkblas_set_cache_limit(size_t bytes);
 
 /* ---- batched uneven problems in one persistent launch -------------------

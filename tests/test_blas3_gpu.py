@@ -298,6 +298,39 @@ def test_tile_cache_eviction(xk):
     xk.invalidate()
 
 
+def test_tile_cache_evicts_clean_mirrors_first(xk):
+    """With a dirty output less recently used than a clean operand, eviction
+    drops the clean one (no write-back, no device-to-host bytes) and the
+    results stay right."""
+    import ctypes
+    L = xk.lib()
+    r = rng(44)
+    n = 512
+    L.xkblas_set_param(256, 0)
+    xk.invalidate()  # no mirrors of earlier tests; their scratch stays counted
+    L.xkblas_set_cache_limit(L.xkblas_cache_bytes() + 4 * n * n * 8)  # room for 4 matrices
+    A, B = uniform(r, n, n), uniform(r, n, n)
+    Y, Z, W = (np.zeros((n, n), order="F") for _ in range(3))
+    h, d0, d1 = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+    try:
+        xk.gemm(111, 111, 1.0, A, A, 0.0, Y, sync=False)  # A clean, Y dirty
+        xk.gemm(111, 111, 1.0, B, B, 0.0, Z, sync=False)  # B clean, Z dirty: the cache is full
+        L.xkblas_sync()
+        L.xkblas_cache_traffic(ctypes.byref(h), ctypes.byref(d0))
+        ev0 = L.xkblas_cache_evictions()
+        xk.gemm(111, 111, 1.0, A, A, 0.0, W, sync=False)  # W needs room: B (clean) goes, not Y
+        L.xkblas_sync()
+        L.xkblas_cache_traffic(ctypes.byref(h), ctypes.byref(d1))
+        assert L.xkblas_cache_evictions() > ev0
+        assert d1.value == d0.value  # no write-back: the dirty Y and Z stayed
+        for X, out in ((A, Y), (B, Z), (A, W)):
+            xk.coherent(out)
+            np.testing.assert_allclose(out, X @ X, atol=1e-11)
+    finally:
+        L.xkblas_set_cache_limit(1 << 40)
+        xk.invalidate()
+
+
 def _pinned(L, shape):
     """A Fortran-ordered float64 array in xkblas_malloc (pinned) memory."""
     import ctypes

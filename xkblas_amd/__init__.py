@@ -107,6 +107,7 @@ SIGNATURES = {
     "xkblas_set_pipeline": (None, [_i]),
     "xkblas_cache_bytes": (_l, []),
     "xkblas_cache_evictions": (_l, []),
+    "xkblas_cache_traffic": (None, [ctypes.c_void_p, ctypes.c_void_p]),
     "xkblas_set_cache_limit": (None, [_sz]),
     "xkblas_batch_create": (_p, [_p, _i]),
     "xkblas_batch_run": (_i, [_p]),

@@ -131,14 +131,30 @@ std::vector<Replica*>& Runtime::begin_call()
 // Evict least-recently-used mirrors outside `keep` (dirty tiles written back
 // first) until `need` more bytes fit the limit.  strict: an operand set that
 // cannot fit is fatal; otherwise returns false.
+// A mirror whose every tile is also valid on the host: dropping it moves no
+// data.
+static bool mirror_clean(const Replica* r)
+{
+  for (size_t t = 0; t < r->state.size(); ++t)
+    if (!(r->state[t] & HOST_VALID)) return false;
+  return true;
+}
+
+// Victims: clean mirrors first (no write-back, no wait), least recently used
+// first among them; then the least recently used dirty one.
 bool Runtime::evict_for(size_t need, const std::vector<Replica*>& keep, bool strict)
 {
   while (cache_used + need > cache_limit) {
     Replica* victim = nullptr;
+    bool vclean = false;
     for (auto& kv : map) {
       Replica* r = kv.second;
       if (std::find(keep.begin(), keep.end(), r) != keep.end()) continue;
-      if (!victim || r->lru < victim->lru) victim = r;
+      const bool clean = mirror_clean(r);
+      if (!victim || (clean && !vclean) || (clean == vclean && r->lru < victim->lru)) {
+        victim = r;
+        vclean = clean;
+      }
     }
     if (!victim) {
       // scratch counts against the limit, but only the operands themselves
@@ -151,9 +167,11 @@ bool Runtime::evict_for(size_t need, const std::vector<Replica*>& keep, bool str
                cache_limit);
       fatal("tile cache", __FILE__, __LINE__, msg);
     }
-    writeback(victim, 0, 0, victim->M, victim->N, 0);
-    XK_HIP(hipStreamSynchronize(s_d2h));
-    drain_stages();
+    if (!vclean) {
+      writeback(victim, 0, 0, victim->M, victim->N, 0);
+      XK_HIP(hipStreamSynchronize(s_d2h));
+      drain_stages();
+    }
     drop(victim);
     stats.evictions++;
   }
@@ -2139,6 +2157,13 @@ long xkblas_cache_evictions(void)
 {
   Lock l(g_mu);
   return rt().stats.evictions;
+}
+
+void xkblas_cache_traffic(double* h2d_bytes, double* d2h_bytes)
+{
+  Lock l(g_mu);
+  if (h2d_bytes) *h2d_bytes = rt().stats.h2d_bytes;
+  if (d2h_bytes) *d2h_bytes = rt().stats.d2h_bytes;
 }
 
 void xkblas_set_cache_limit(size_t bytes)
