@@ -98,10 +98,9 @@ void* scratch(int slot, size_t bytes)
   if (r.scratch_size[slot] < bytes) {
     size_t sz = std::max(bytes + bytes / 4, (size_t)16 << 20);
     if (r.scratch_ptr[slot]) {
-      // kernels already enqueued may still use the old buffer
-      XK_HIP(hipDeviceSynchronize());
-      XK_HIP(hipFree(r.scratch_ptr[slot]));
-      r.cache_used -= r.scratch_size[slot];
+      // kernels already enqueued may still use the old buffer: it is freed
+      // at the next synchronisation point
+      r.retired_scratch.emplace_back(r.scratch_ptr[slot], r.scratch_size[slot]);
       r.scratch_ptr[slot] = nullptr;
       r.scratch_size[slot] = 0;
     }
@@ -161,6 +160,7 @@ bool Runtime::evict_for(size_t need, const std::vector<Replica*>& keep, bool str
       // beyond it are fatal: scratch may overshoot the limit
       size_t scratch_bytes = 0;
       for (int i = 0; i < NSCRATCH; ++i) scratch_bytes += scratch_size[i];
+      for (const auto& b : retired_scratch) scratch_bytes += b.second;
       if (!strict || cache_used - std::min(cache_used, scratch_bytes) + need <= cache_limit) return false;
       char msg[256];
       snprintf(msg, sizeof msg, "operands need %zu bytes of HBM beyond the cache limit of %zu bytes", need,
@@ -187,6 +187,16 @@ void* Runtime::device_alloc(size_t bytes, const std::vector<Replica*>& keep)
   if (e == hipErrorOutOfMemory) {
     (void)hipGetLastError();
     sync_all();
+    // every stream has drained: outgrown scratch buffers can go first
+    for (auto& b : retired_scratch) {
+      XK_HIP(hipFree(b.first));
+      cache_used -= b.second;
+    }
+    retired_scratch.clear();
+    e = hipMalloc(&p, bytes);
+  }
+  if (e == hipErrorOutOfMemory) {
+    (void)hipGetLastError();
     std::vector<Replica*> victims;
     for (auto& kv : map)
       if (std::find(keep.begin(), keep.end(), kv.second) == keep.end()) victims.push_back(kv.second);
@@ -224,6 +234,11 @@ void Runtime::drop(Replica* r)
 // every stream has drained).
 void Runtime::free_retired()
 {
+  for (auto& b : retired_scratch) {
+    XK_HIP(hipFree(b.first));
+    cache_used -= b.second;
+  }
+  retired_scratch.clear();
   for (Replica* r : retired) {
     clear_panels(r);
     if (r->dev) XK_HIP(hipFree(r->dev));

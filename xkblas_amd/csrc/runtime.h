@@ -100,6 +100,11 @@ struct Runtime {
   // their pointer key (an aliased operand of other dimensions): freed once
   // every stream has drained (sync_all).
   std::vector<Replica*> retired;
+  // Scratch buffers outgrown during a call: kernels already enqueued may
+  // still use them, so they are freed at the next point where every stream
+  // has drained instead of synchronising the device in the middle of a call
+  // (their bytes stay counted against the cache limit until then).
+  std::vector<std::pair<void*, size_t>> retired_scratch;
 
   // Pageable host memory goes through a ring of pinned staging buffers: a
   // host-to-device chunk is packed by the copy threads, then sent; a
