@@ -146,7 +146,42 @@ static void potrf_diag(hipStream_t s, int uplo, int n, T* A, long lda, T* Tinv, 
   }
 }
 
-enum { WS_TRSM = 4, WS_TRSMW = 5 };
+enum { WS_TRSM = 4, WS_TRSMW = 5, WS_SPLITK = 31 };
+
+// Products with few output tiles and a long inner dimension (the inverse
+// products and late updates of a TRSM sweep with few right-hand sides) are
+// latency-bound: each tile walks its k slices one after the other.  Split
+// the inner dimension into 2, 4 or 8 chunks computed by one batched launch
+// into partial products, summed by a reduction (XKBLAS_SPLITK=0 turns it
+// off).  Returns false when the shape does not qualify.
+static bool splitk_enabled()
+{
+  static const int v = getenv("XKBLAS_SPLITK") ? atoi(getenv("XKBLAS_SPLITK")) : 1;
+  return v != 0;
+}
+
+template <typename T>
+static bool gemm_splitk(hipStream_t s, int ta, int tb, int m, int n, int k, T alpha, const T* A, long lda,
+                        const T* B, long ldb, T beta, T* C, long ldc)
+{
+  if (!splitk_enabled() || m <= 0 || n <= 0 || k < 512) return false;
+  const long t32 = (long)((m + 31) / 32) * ((n + 31) / 32);
+  if (t32 >= 256) return false;
+  int parts = 1;
+  for (int p = 8; p >= 2; p /= 2)
+    if (k % (16 * p) == 0 && t32 * p <= 1024) {
+      parts = p;
+      break;
+    }
+  if (parts == 1) return false;
+  const int kc = k / parts;
+  T* W = (T*)scratch(WS_SPLITK, sizeof(T) * (size_t)parts * m * n);
+  const long sA = ta != NoTrans ? (long)kc : (long)kc * lda;
+  const long sB = tb != NoTrans ? (long)kc * ldb : (long)kc;
+  gemm<T>(s, ta, tb, m, n, kc, alpha, A, lda, B, ldb, T(0), W, m, TRI_FULL, parts, sA, sB, (long)m * n);
+  splitk_reduce<T>(s, m, n, parts, W, beta, C, ldc);
+  return true;
+}
 
 void potrf_scratch_caps(int n, int nb, size_t elt, size_t cap[4])
 {
@@ -386,15 +421,23 @@ static void trsm_sweep(const Exec& x, int side, int uplo, int transA, int m, int
       const int kk = bsz(k);
       const T la = s == 0 ? alpha : T(1);
       T* Bk = B + (long)k * nb;
-      copy2d<T>(x.s, kk, n, Bk, ldb, W, kk);
-      gemm<T>(x.s, ta, NoTrans, kk, n, kk, la, inv + (size_t)k * nb * nb, nb, W, kk, T(0), Bk, ldb);
+      // the inverse product reads Bk and writes it: through W, or through
+      // the split-k partials
+      if (!gemm_splitk<T>(x.s, ta, NoTrans, kk, n, kk, la, inv + (size_t)k * nb * nb, nb, Bk, ldb, T(0), Bk, ldb)) {
+        copy2d<T>(x.s, kk, n, Bk, ldb, W, kk);
+        gemm<T>(x.s, ta, NoTrans, kk, n, kk, la, inv + (size_t)k * nb * nb, nb, W, kk, T(0), Bk, ldb);
+      }
       if (s == nblk - 1) break;
+      const bool plain = !split_enabled<T>(x);
       if (lower_eff) {  // rows below block k
         const int r0 = (k + 1) * nb, rows = m - r0;
-        dev_gemm<T>(x, ta, NoTrans, rows, n, kk, T(-1), opblk(k + 1, k), lda, Bk, ldb, la, B + r0, ldb);
+        if (!(plain && gemm_splitk<T>(x.s, ta, NoTrans, rows, n, kk, T(-1), opblk(k + 1, k), lda, Bk, ldb, la,
+                                      B + r0, ldb)))
+          dev_gemm<T>(x, ta, NoTrans, rows, n, kk, T(-1), opblk(k + 1, k), lda, Bk, ldb, la, B + r0, ldb);
       } else {  // rows above block k
         const int rows = k * nb;
-        dev_gemm<T>(x, ta, NoTrans, rows, n, kk, T(-1), opblk(0, k), lda, Bk, ldb, la, B, ldb);
+        if (!(plain && gemm_splitk<T>(x.s, ta, NoTrans, rows, n, kk, T(-1), opblk(0, k), lda, Bk, ldb, la, B, ldb)))
+          dev_gemm<T>(x, ta, NoTrans, rows, n, kk, T(-1), opblk(0, k), lda, Bk, ldb, la, B, ldb);
       }
     }
   } else {
@@ -405,15 +448,22 @@ static void trsm_sweep(const Exec& x, int side, int uplo, int transA, int m, int
       const int kk = bsz(k);
       const T la = s == 0 ? alpha : T(1);
       T* Bk = B + (long)k * nb * ldb;
-      copy2d<T>(x.s, m, kk, Bk, ldb, W, m);
-      gemm<T>(x.s, NoTrans, ta, m, kk, kk, la, W, m, inv + (size_t)k * nb * nb, nb, T(0), Bk, ldb);
+      if (!gemm_splitk<T>(x.s, NoTrans, ta, m, kk, kk, la, Bk, ldb, inv + (size_t)k * nb * nb, nb, T(0), Bk, ldb)) {
+        copy2d<T>(x.s, m, kk, Bk, ldb, W, m);
+        gemm<T>(x.s, NoTrans, ta, m, kk, kk, la, W, m, inv + (size_t)k * nb * nb, nb, T(0), Bk, ldb);
+      }
       if (s == nblk - 1) break;
+      const bool plain = !split_enabled<T>(x);
       if (forward) {  // columns right of block k
         const int c0 = (k + 1) * nb, cols = n - c0;
-        dev_gemm<T>(x, NoTrans, ta, m, cols, kk, T(-1), Bk, ldb, opblk(k, k + 1), lda, la, B + (long)c0 * ldb, ldb);
+        if (!(plain && gemm_splitk<T>(x.s, NoTrans, ta, m, cols, kk, T(-1), Bk, ldb, opblk(k, k + 1), lda, la,
+                                      B + (long)c0 * ldb, ldb)))
+          dev_gemm<T>(x, NoTrans, ta, m, cols, kk, T(-1), Bk, ldb, opblk(k, k + 1), lda, la, B + (long)c0 * ldb,
+                      ldb);
       } else {  // columns left of block k
         const int cols = k * nb;
-        dev_gemm<T>(x, NoTrans, ta, m, cols, kk, T(-1), Bk, ldb, opblk(k, 0), lda, la, B, ldb);
+        if (!(plain && gemm_splitk<T>(x.s, NoTrans, ta, m, cols, kk, T(-1), Bk, ldb, opblk(k, 0), lda, la, B, ldb)))
+          dev_gemm<T>(x, NoTrans, ta, m, cols, kk, T(-1), Bk, ldb, opblk(k, 0), lda, la, B, ldb);
       }
     }
   }

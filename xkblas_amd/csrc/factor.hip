@@ -395,6 +395,32 @@ void zero_triangle(hipStream_t s, int uplo, int n, T* A, long lda)
   XK_HIP(hipGetLastError());
 }
 
+// C <- beta C + sum_p W_p (W_p: m x n, leading dimension m, p-th at p m n);
+// beta == 0 never reads C.
+template <typename T>
+__global__ void splitk_reduce_kernel(int m, int n, int parts, const T* __restrict__ W, T beta, T* __restrict__ C,
+                                     long ldc)
+{
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  const int j = blockIdx.y;
+  if (i >= m || j >= n) return;
+  const long mn = (long)m * n, e = i + (long)j * m;
+  T v = W[e];
+  for (int p = 1; p < parts; ++p) v += W[p * mn + e];
+  T* c = C + i + (long)j * ldc;
+  *c = beta != T(0) ? beta * *c + v : v;
+}
+
+template <typename T>
+void splitk_reduce(hipStream_t s, int m, int n, int parts, const T* W, T beta, T* C, long ldc)
+{
+  if (m <= 0 || n <= 0) return;
+  XK_CHECK(n <= 65535, "splitk_reduce: too many columns");
+  hipLaunchKernelGGL((splitk_reduce_kernel<T>), dim3((m + 255) / 256, n), dim3(256), 0, s, m, n, parts, W, beta, C,
+                     ldc);
+  XK_HIP(hipGetLastError());
+}
+
 __global__ void fill_int_kernel(int* p, int n, int v)
 {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -412,7 +438,8 @@ void fill_int(hipStream_t s, int* p, int n, int v)
   template void potrf_leaf<T>(hipStream_t, int, int, T*, long, long, T*, long, long, int, int*, int, int);         \
   template void trtri_leaf<T>(hipStream_t, int, int, int, const T*, long, long, T*, long, long, int);         \
   template void copy2d<T>(hipStream_t, int, int, const T*, long, T*, long);                                    \
-  template void zero_triangle<T>(hipStream_t, int, int, T*, long);
+  template void zero_triangle<T>(hipStream_t, int, int, T*, long);                                            \
+  template void splitk_reduce<T>(hipStream_t, int, int, int, const T*, T, T*, long);
 XK_INST(double)
 XK_INST(float)
 #undef XK_INST

@@ -49,6 +49,11 @@ void copy2d(hipStream_t s, int m, int n, const T* X, long ldx, T* Y, long ldy);
 template <typename T>
 void zero_triangle(hipStream_t s, int uplo, int n, T* A, long lda);
 
+// C <- beta C + sum over `parts` of the m x n partial products W (leading
+// dimension m, one every m n elements); beta == 0 never reads C.
+template <typename T>
+void splitk_reduce(hipStream_t s, int m, int n, int parts, const T* W, T beta, T* C, long ldc);
+
 // Fill an int array with a value (stream-ordered, graph-capturable).
 void fill_int(hipStream_t s, int* p, int n, int v);
 // Products enqueued on stream s (null: none) raise their waves' issue
