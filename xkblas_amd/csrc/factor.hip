@@ -28,30 +28,28 @@ __device__ __forceinline__ float bcast(float x, int src)
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), src));
 }
 
-// sq = sqrt(p) and rs = 1 / sq for a pivot p > 0, without a division:
-// Goldschmidt's iteration from the hardware reciprocal square root, two
-// Newton corrections of the root (which make sq the correctly rounded
-// root in the normal range) and one of the reciprocal (about 1 ulp).
-// Pivots outside [2^-900, 2^900] (and p <= 0, NaN) take sqrt() and a
-// division.
+// sq = sqrt(p) and rs = 1 / sq for a finite pivot p > 0, without a division
+// and without a branch: p = 2^(2h) p' with p' in [0.5, 2) (exact scaling by
+// the exponent, denormal p included), Goldschmidt's iteration on p' from the
+// hardware reciprocal square root, two Newton corrections of the root
+// (correctly rounded) and one of the reciprocal (about 1 ulp), then
+// sq = 2^h sqrt(p'), rs = 2^-h / sqrt(p').  For p <= 0, infinite or NaN the
+// results are unspecified (the caller reports such a pivot from p itself).
 __device__ __forceinline__ void sqrt_rcp(double p, double& sq, double& rs)
 {
-  if (!(p >= 0x1p-900 && p <= 0x1p+900)) {
-    sq = sqrt(p);
-    rs = 1.0 / sq;
-    return;
-  }
-  const double r = __builtin_amdgcn_rsq(p);
-  double g = p * r, h = 0.5 * r;
-  const double e = fma(-h, g, 0.5);
+  const int h = __builtin_amdgcn_frexp_exp(p) >> 1;
+  const double ps = __builtin_ldexp(p, -2 * h);
+  const double r = __builtin_amdgcn_rsq(ps);
+  double g = ps * r, hh = 0.5 * r;
+  const double e = fma(-hh, g, 0.5);
   g = fma(g, e, g);
-  h = fma(h, e, h);
-  g = fma(fma(-g, g, p), h, g);
-  g = fma(fma(-g, g, p), h, g);
-  double q = 2.0 * h;
+  hh = fma(hh, e, hh);
+  g = fma(fma(-g, g, ps), hh, g);
+  g = fma(fma(-g, g, ps), hh, g);
+  double q = 2.0 * hh;
   q = fma(q, fma(-g, q, 1.0), q);
-  sq = g;
-  rs = q;
+  sq = __builtin_ldexp(g, h);
+  rs = __builtin_ldexp(q, -h);
 }
 __device__ __forceinline__ void sqrt_rcp(float p, float& sq, float& rs)
 {
@@ -177,18 +175,22 @@ __global__ __launch_bounds__(256) void potrf_leaf_kernel(int uplo, int n, T* A, 
 #pragma unroll
       for (int k = 0; k < LB; ++k) d[k] = k <= i ? P[pk(c0 + i, c0 + k)] : T(0);
       // right-looking: the pivot chain needs no cross-lane data beyond the
-      // pivot itself (lane j + 1 updates its own diagonal entry)
+      // pivot itself (lane j + 1 updates its own diagonal entry).  The chain
+      // has no branch (lane j keeps reciprocal j, stored after the loop), so
+      // the updates of one step are scheduled under the next pivot's root.
+      T myrs = T(0);
 #pragma unroll
       for (int j = 0; j < LB; ++j) {
         const T p = bcast(d[j], j);
         fail = (!(p > T(0)) && fail == INT_MAX && c0 + j < n) ? c0 + j : fail;
         T sq, rs;
         sqrt_rcp(p, sq, rs);
-        if (lane == 0) Lr[j] = rs;
+        myrs = lane == j ? rs : myrs;
         d[j] = i > j ? d[j] * rs : (i == j ? sq : T(0));
 #pragma unroll
         for (int k = j + 1; k < LB; ++k) d[k] = fma(-d[j], bcast(d[j], k), d[k]);
       }
+      if (lane < LB) Lr[lane] = myrs;
       if (own) {
 #pragma unroll
         for (int k = 0; k < LB; ++k)
