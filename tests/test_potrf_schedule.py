@@ -2,7 +2,8 @@
 blas3.cpp): one rank-NB update per panel, or paired updates (an even step
 updates only the next block column; the odd step after it applies both
 panels as one rank-2NB product, the first ~29 % of the trailing columns at
-once and the rest at the next step).  Each step's products are applied in the
+once and the rest at the next step).  The panel is formed in two parts: its
+rows in the next block row first (all the lookahead reads), then the rest.  Each step's products are applied in the
 same order and on the same blocks as the library's launches, with NumPy's
 Cholesky for the diagonal blocks; the result must be the Cholesky factor of A
 for both triangles, ragged last blocks and odd / even block counts.  A part
@@ -40,9 +41,12 @@ def sim(n, nb, agg, uplo):
         c0 = k - 1 if pair else k
         r0 = (k+1)*nb
         if uplo == "L":
-            A[r0:, k*nb:k*nb+kk] = A[r0:, k*nb:k*nb+kk] @ inv[k].T
+            # the panel's rows in block row k+1, the lookahead, then the rest
+            A[r0:r0+k1, k*nb:k*nb+kk] = A[r0:r0+k1, k*nb:k*nb+kk] @ inv[k].T
+            U1 = A[r0:r0+k1, c0*nb:(k+1)*nb].copy()
+            A[r0:r0+k1, r0:r0+k1] -= np.tril(U1 @ U1.T)
+            A[r0+k1:, k*nb:k*nb+kk] = A[r0+k1:, k*nb:k*nb+kk] @ inv[k].T
             U = A[r0:, c0*nb:(k+1)*nb].copy()
-            A[r0:r0+k1, r0:r0+k1] -= np.tril(U[:k1] @ U[:k1].T)
             if r2 > 0:
                 A[r0+k1:, r0:r0+k1] -= U[k1:] @ U[:k1].T
                 c = r0 + k1
@@ -58,9 +62,11 @@ def sim(n, nb, agg, uplo):
                 Up, cp = prev
                 A[cp:, cp:] -= np.tril(Up @ Up.T)
         else:
-            A[k*nb:k*nb+kk, r0:] = inv[k].T @ A[k*nb:k*nb+kk, r0:]
+            A[k*nb:k*nb+kk, r0:r0+k1] = inv[k].T @ A[k*nb:k*nb+kk, r0:r0+k1]
+            U1 = A[c0*nb:(k+1)*nb, r0:r0+k1].copy()
+            A[r0:r0+k1, r0:r0+k1] -= np.triu(U1.T @ U1)
+            A[k*nb:k*nb+kk, r0+k1:] = inv[k].T @ A[k*nb:k*nb+kk, r0+k1:]
             U = A[c0*nb:(k+1)*nb, r0:].copy()
-            A[r0:r0+k1, r0:r0+k1] -= np.triu(U[:, :k1].T @ U[:, :k1])
             if r2 > 0:
                 A[r0:r0+k1, r0+k1:] -= U[:, :k1].T @ U[:, k1:]
                 c = r0 + k1

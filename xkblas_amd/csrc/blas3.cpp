@@ -277,13 +277,20 @@ void dev_potrf(const Exec& x, int uplo, int n, T* A, long lda, int nb, int* info
     T* W = (T*)scratch(WS_PANEL, sizeof(T) * (size_t)rest * kk);
     if (uplo == Lower) {
       T* P = blk(k + 1, k);  // rest x kk panel
-      copy2d<T>(x.s, rest, kk, P, lda, W, rest);
-      dev_gemm<T>(x, NoTrans, Trans, rest, kk, kk, T(1), W, rest, ik, nb, T(0), P, lda);
+      // the panel's rows in block row k+1 first: the lookahead reads only
+      // them, so the next diagonal factorization waits for k1 rows of the
+      // panel product instead of all of them
+      copy2d<T>(x.s, k1, kk, P, lda, W, rest);
+      dev_gemm<T>(x, NoTrans, Trans, k1, kk, kk, T(1), W, rest, ik, nb, T(0), P, lda);
       const T* U = A + (long)(k + 1) * nb + (long)c0 * nb * lda;  // rest x kw
       if (split) split_cache_register((const double*)U, lda, rest, kw, 1);
       // lookahead: update the next diagonal block and factor it on the side
       dev_syrk<T>(x, Lower, NoTrans, k1, kw, T(-1), U, lda, T(1), blk(k + 1, k + 1), lda);
       if (lookahead) XK_HIP(hipEventRecord(ev_look, x.s));
+      if (r2 > 0) {
+        copy2d<T>(x.s, r2, kk, P + k1, lda, W + k1, rest);
+        dev_gemm<T>(x, NoTrans, Trans, r2, kk, kk, T(1), W + k1, rest, ik, nb, T(0), P + k1, lda);
+      }
       // the rest is enqueued before the diagonal factorization's many small
       // launches, so the compute stream has work while the host issues them:
       // block column k+1 below its diagonal block, then the trailing triangle
@@ -304,12 +311,18 @@ void dev_potrf(const Exec& x, int uplo, int n, T* A, long lda, int nb, int* info
       if (prev.n > 0) dev_syrk<T>(x, Lower, NoTrans, prev.n, prev.kw, T(-1), prev.U, lda, T(1), prev.C, lda);
     } else {
       T* P = blk(k, k + 1);  // kk x rest panel
-      copy2d<T>(x.s, kk, rest, P, lda, W, kk);
-      dev_gemm<T>(x, Trans, NoTrans, kk, rest, kk, T(1), ik, nb, W, kk, T(0), P, lda);
+      // the panel's columns in block column k+1 first (as above)
+      copy2d<T>(x.s, kk, k1, P, lda, W, kk);
+      dev_gemm<T>(x, Trans, NoTrans, kk, k1, kk, T(1), ik, nb, W, kk, T(0), P, lda);
       const T* U = A + (long)c0 * nb + (long)(k + 1) * nb * lda;  // kw x rest
       if (split) split_cache_register((const double*)U, lda, rest, kw, 0);
       dev_syrk<T>(x, Upper, Trans, k1, kw, T(-1), U, lda, T(1), blk(k + 1, k + 1), lda);
       if (lookahead) XK_HIP(hipEventRecord(ev_look, x.s));
+      if (r2 > 0) {
+        copy2d<T>(x.s, kk, r2, P + (long)k1 * lda, lda, W + (long)k1 * kk, kk);
+        dev_gemm<T>(x, Trans, NoTrans, kk, r2, kk, T(1), ik, nb, W + (long)k1 * kk, kk, T(0), P + (long)k1 * lda,
+                    lda);
+      }
       if (r2 > 0) {
         dev_gemm<T>(x, Trans, NoTrans, k1, r2, kw, T(-1), U, lda, U + (long)k1 * lda, lda, T(1), blk(k + 1, k + 2),
                     lda);
