@@ -956,13 +956,17 @@ void dev_cpotrf(const Exec& x, int uplo, int n, T* A, long lda, int nb, int* inf
     T* W = (T*)scratch(WS_ZPAN, 2 * sizeof(T) * (size_t)kk * rest);
     if (uplo == Lower) {
       T* P = blk(k + 1, k);  // rest x kk panel
-      copy2d<T>(x.s, 2 * rest, kk, P, 2 * lda, W, 2L * rest);
-      cgemm_impl<T>(x, NoTrans, ConjTrans, rest, kk, kk, one, W, rest, ik, nb, zero, P, lda, 0, 0);
-      // next diagonal block first (lookahead), then the block column below
-      // it and the trailing triangle
+      // the panel's rows in block row k+1 first (all that the lookahead
+      // reads), then the next diagonal block (lookahead), the rest of the
+      // panel, the block column below the next diagonal block and the
+      // trailing triangle
+      copy2d<T>(x.s, 2 * k1, kk, P, 2 * lda, W, 2L * rest);
+      cgemm_impl<T>(x, NoTrans, ConjTrans, k1, kk, kk, one, W, rest, ik, nb, zero, P, lda, 0, 0);
       cgemm_impl<T>(x, NoTrans, ConjTrans, k1, k1, kk, mone, P, lda, P, lda, one, blk(k + 1, k + 1), lda, Lower, 1);
       if (lookahead) XK_HIP(hipEventRecord(ev_look, x.s));
       if (r2 > 0) {
+        copy2d<T>(x.s, 2 * r2, kk, at(P, k1), 2 * lda, at(W, k1), 2L * rest);
+        cgemm_impl<T>(x, NoTrans, ConjTrans, r2, kk, kk, one, at(W, k1), rest, ik, nb, zero, at(P, k1), lda, 0, 0);
         const T* P2 = at(P, k1);
         cgemm_impl<T>(x, NoTrans, ConjTrans, r2, k1, kk, mone, P2, lda, P, lda, one, blk(k + 2, k + 1), lda, 0, 0);
         cgemm_impl<T>(x, NoTrans, ConjTrans, r2, r2, kk, mone, P2, lda, P2, lda, one, blk(k + 2, k + 2), lda, Lower,
@@ -970,11 +974,14 @@ void dev_cpotrf(const Exec& x, int uplo, int n, T* A, long lda, int nb, int* inf
       }
     } else {
       T* P = blk(k, k + 1);  // kk x rest panel
-      copy2d<T>(x.s, 2 * kk, rest, P, 2 * lda, W, 2L * kk);
-      cgemm_impl<T>(x, ConjTrans, NoTrans, kk, rest, kk, one, ik, nb, W, kk, zero, P, lda, 0, 0);
+      copy2d<T>(x.s, 2 * kk, k1, P, 2 * lda, W, 2L * kk);
+      cgemm_impl<T>(x, ConjTrans, NoTrans, kk, k1, kk, one, ik, nb, W, kk, zero, P, lda, 0, 0);
       cgemm_impl<T>(x, ConjTrans, NoTrans, k1, k1, kk, mone, P, lda, P, lda, one, blk(k + 1, k + 1), lda, Upper, 1);
       if (lookahead) XK_HIP(hipEventRecord(ev_look, x.s));
       if (r2 > 0) {
+        copy2d<T>(x.s, 2 * kk, r2, at(P, (long)k1 * lda), 2 * lda, at(W, (long)k1 * kk), 2L * kk);
+        cgemm_impl<T>(x, ConjTrans, NoTrans, kk, r2, kk, one, ik, nb, at(W, (long)k1 * kk), kk, zero,
+                      at(P, (long)k1 * lda), lda, 0, 0);
         const T* P2 = at(P, (long)k1 * lda);
         cgemm_impl<T>(x, ConjTrans, NoTrans, k1, r2, kk, mone, P, lda, P2, lda, one, blk(k + 1, k + 2), lda, 0, 0);
         cgemm_impl<T>(x, ConjTrans, NoTrans, r2, r2, kk, mone, P2, lda, P2, lda, one, blk(k + 2, k + 2), lda, Upper,
