@@ -201,7 +201,8 @@ static int potrf_pairing() { return g_potrf_pair; }
 void potrf_set_pairing(int v) { g_potrf_pair = v < 0 ? 1 : (v ? 1 : 0); }
 
 template <typename T>
-void dev_potrf(const Exec& x, int uplo, int n, T* A, long lda, int nb, int* info)
+void dev_potrf(const Exec& x, int uplo, int n, T* A, long lda, int nb, int* info,
+               const std::function<void()>& beside_first)
 {
   if (n <= 0) return;
   nb = std::max(64, std::min(nb, n));
@@ -251,6 +252,7 @@ void dev_potrf(const Exec& x, int uplo, int n, T* A, long lda, int nb, int* info
   // one fill here replaces a fill per recursion node
   fill_zero<T>(sd, nb, 2 * nb, inv, nb);
   potrf_diag<T>(sd, uplo, tsz(0), blk(0, 0), lda, inv, nb, info, 0, WS_DIAG0, true);
+  if (beside_first) beside_first();
   for (int k = 0; k < nt; ++k) {
     const int kk = tsz(k);
     T* ik = inv + (size_t)(k & 1) * nb * nb;
@@ -1058,9 +1060,10 @@ int dev_posv_ir(const Exec& x, const Exec& xl, const Exec& xf, int uplo, int n, 
     trsm_sweep<double>(x, Left, uplo, tA, n, nrhs, 1.0, F, ldf, Y, ldy, tnb, inv);
     trsm_sweep<double>(x, Left, uplo, tA == NoTrans ? Trans : NoTrans, n, nrhs, 1.0, F, ldf, Y, ldy, tnb, inv);
   };
-  auto factor = [&](const Exec& e) {  // F <- Cholesky factor of A's image under e; returns info
+  // F <- Cholesky factor of A's image under e; returns info
+  auto factor = [&](const Exec& e, const std::function<void()>& beside_first) {
     fill_int(x.s, dinfo, 1, INT_MAX);
-    dev_potrf<double>(e, uplo, n, F, ldf, nb, dinfo);
+    dev_potrf<double>(e, uplo, n, F, ldf, nb, dinfo, beside_first);
     XK_HIP(hipMemcpyAsync(&hinfo, dinfo, sizeof(int), hipMemcpyDeviceToHost, x.s));
     XK_HIP(hipStreamSynchronize(x.s));
     return hinfo == INT_MAX ? 0 : hinfo;
@@ -1081,18 +1084,31 @@ int dev_posv_ir(const Exec& x, const Exec& xl, const Exec& xf, int uplo, int n, 
     XK_HIP(hipStreamSynchronize(x.s));
   };
 
-  symm_expand(x.s, uplo, n, A, lda, Af, ldf, F, ldf);
-  col_reduce(x.s, 1, n, n, Af, ldf, V);  // row sums of |A| = column sums (symmetric)
-  XK_HIP(hipMemcpyAsync(hv.data(), V, sizeof(double) * n, hipMemcpyDeviceToHost, x.s));
-  XK_HIP(hipStreamSynchronize(x.s));
-  double anrm = 0.0;
-  for (int j = 0; j < n; ++j) anrm = (hv[j] != hv[j] || hv[j] > anrm) ? hv[j] : anrm;
-  const double cte = anrm * 1.1102230246251565e-16 * std::sqrt((double)n);  // eps = 2^-53
+  // A's full image Af and the factor's copy F: the first block column before
+  // the factorization, the rest (and the row sums of |A|, needed only by the
+  // stopping test) on the compute stream beside the first diagonal block's
+  // factorization, which would otherwise leave it idle
+  const int c1 = std::min(n, (std::max(64, std::min(nb, n)) + 63) / 64 * 64);
+  symm_expand(x.s, uplo, n, A, lda, Af, ldf, F, ldf, 0, c1);
+  auto expand_rest = [&]() {
+    symm_expand(x.s, uplo, n, A, lda, Af, ldf, F, ldf, c1, n);
+    col_reduce(x.s, 1, n, n, Af, ldf, V);  // row sums of |A| = column sums (symmetric)
+  };
+  double cte = 0.0;
+  auto norm_a = [&]() {  // after the factorization's synchronisation
+    XK_HIP(hipMemcpyAsync(hv.data(), V, sizeof(double) * n, hipMemcpyDeviceToHost, x.s));
+    XK_HIP(hipStreamSynchronize(x.s));
+    double anrm = 0.0;
+    for (int j = 0; j < n; ++j) anrm = (hv[j] != hv[j] || hv[j] > anrm) ? hv[j] : anrm;
+    cte = anrm * 1.1102230246251565e-16 * std::sqrt((double)n);  // eps = 2^-53
+  };
   // dsposv's test (continue while rnrm > xnrm cte; NaN never passes)
   auto res_ok = [&](int j) { return hv[n + j] <= hv[n + nrhs + j] * cte; };
 
   int fallback = 0;
-  if (factor(xl) != 0) {
+  const int finfo = factor(xl, expand_rest);
+  norm_a();
+  if (finfo != 0) {
     fallback = -3;
   } else {
     trsm_inverses<double>(x, uplo, NonUnit, n, F, ldf, tnb, inv);
@@ -1130,7 +1146,7 @@ int dev_posv_ir(const Exec& x, const Exec& xl, const Exec& xf, int uplo, int n, 
   // the accurate factor, as LAPACK dsposv falls back to dpotrf
   *iter = fallback;
   symm_expand<double>(x.s, uplo, n, A, lda, F, ldf, nullptr, 0);
-  const int info = factor(xf);
+  const int info = factor(xf, std::function<void()>());
   if (info != 0) return info;
   trsm_inverses<double>(x, uplo, NonUnit, n, F, ldf, tnb, inv);
   copy2d<double>(x.s, n, nrhs, B, ldb, X, ldx);
@@ -1143,7 +1159,7 @@ int dev_posv_ir(const Exec& x, const Exec& xl, const Exec& xf, int uplo, int n, 
   template void dev_gemm<T>(const Exec&, int, int, int, int, int, T, const T*, long, const T*, long, T, T*, long); \
   template void dev_syrk<T>(const Exec&, int, int, int, int, T, const T*, long, T, T*, long);                     \
   template void dev_trsm<T>(const Exec&, int, int, int, int, int, int, T, const T*, long, T*, long, int);        \
-  template void dev_potrf<T>(const Exec&, int, int, T*, long, int, int*);                                         \
+  template void dev_potrf<T>(const Exec&, int, int, T*, long, int, int*, const std::function<void()>&);          \
   template void dev_trtri<T>(const Exec&, hipStream_t, int, int, int, const T*, long, T*, long, int);        \
   template void dev_gemmt<T>(const Exec&, int, int, int, int, int, T, const T*, long, const T*, long, T, T*,     \
                              long);                                                                             \

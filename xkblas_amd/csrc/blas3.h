@@ -4,6 +4,7 @@
 // each step of the algorithm is one launch over all tiles it touches.
 #pragma once
 #include "common.h"
+#include <functional>
 
 namespace xk {
 
@@ -44,9 +45,13 @@ void dev_trsm(const Exec& x, int side, int uplo, int transA, int diag, int m, in
 
 // In-place blocked Cholesky with block order nb.  info (device int) receives
 // the LAPACK info as an atomic min over failing orders (INT_MAX = success);
-// the caller initialises it.
+// the caller initialises it.  beside_first, when set, is called once to
+// enqueue work on x.s right after the first diagonal block's factorization
+// has gone to the side stream (it runs beside it); it must not touch the
+// first block column of A, which must be in place before the call.
 template <typename T>
-void dev_potrf(const Exec& x, int uplo, int n, T* A, long lda, int nb, int* info);
+void dev_potrf(const Exec& x, int uplo, int n, T* A, long lda, int nb, int* info,
+               const std::function<void()>& beside_first = std::function<void()>());
 // trailing updates of dev_potrf: 1 paired (default, also for -1), 0 one per panel
 void potrf_set_pairing(int v);
 

@@ -65,9 +65,11 @@ void gemm_set_priority_stream(hipStream_t s);
 void batch_noalloc_init();
 
 // Y (and Z when non-null) <- the full symmetric n x n matrix whose uplo
-// triangle is stored in A.
+// triangle is stored in A, columns c0 .. c1 - 1 (c0 a multiple of 64; c1 < 0:
+// up to n; whole tiles of 64 columns are written).
 template <typename T>
-void symm_expand(hipStream_t s, int uplo, int n, const T* A, long lda, T* Y, long ldy, T* Z, long ldz);
+void symm_expand(hipStream_t s, int uplo, int n, const T* A, long lda, T* Y, long ldy, T* Z, long ldz, int c0 = 0,
+                 int c1 = -1);
 
 // Y <- the n x n triangular matrix whose uplo triangle is stored in A (zeros
 // outside it, a unit diagonal when unit != 0).

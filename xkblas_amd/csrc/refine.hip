@@ -8,15 +8,17 @@
 namespace xk {
 
 // Y (and Z when non-null) <- the symmetric matrix whose uplo triangle is
-// stored in A.  One 64 x 64 tile per workgroup (256 threads as 64 rows x 4
-// column phases); a tile on the mirrored side reads its transposed source
-// tile coalesced and transposes it through LDS.
+// stored in A, tile columns jt0 .. jt0 + gridDim.y - 1.  One 64 x 64 tile per
+// workgroup (256 threads as 64 rows x 4 column phases); a tile on the
+// mirrored side reads its transposed source tile coalesced and transposes it
+// through LDS.
 template <typename T>
-__global__ __launch_bounds__(256) void symm_expand_kernel(int uplo, int n, const T* __restrict__ A, long lda,
-                                                          T* __restrict__ Y, long ldy, T* __restrict__ Z, long ldz)
+__global__ __launch_bounds__(256) void symm_expand_kernel(int uplo, int n, int jt0, const T* __restrict__ A,
+                                                          long lda, T* __restrict__ Y, long ldy, T* __restrict__ Z,
+                                                          long ldz)
 {
   __shared__ T t[64][65];
-  const int bi = blockIdx.x, bj = blockIdx.y;
+  const int bi = blockIdx.x, bj = jt0 + blockIdx.y;
   const int i0 = bi * 64, j0 = bj * 64;
   const bool lower = uplo == Lower;
   const bool mirror = lower ? bi < bj : bi > bj;
@@ -247,16 +249,21 @@ void resid_accurate(hipStream_t s, int n, int nrhs, const double* A, long lda, c
 }
 
 template <typename T>
-void symm_expand(hipStream_t s, int uplo, int n, const T* A, long lda, T* Y, long ldy, T* Z, long ldz)
+void symm_expand(hipStream_t s, int uplo, int n, const T* A, long lda, T* Y, long ldy, T* Z, long ldz, int c0,
+                 int c1)
 {
   if (n <= 0) return;
-  const int nt = (n + 63) / 64;
+  if (c1 < 0 || c1 > n) c1 = n;
+  if (c0 >= c1) return;  // empty range (c0 may then be n, of any value)
+  XK_CHECK(c0 >= 0 && c0 % 64 == 0, "symm_expand: first column must be a multiple of 64");
+  const int nt = (n + 63) / 64, jt0 = c0 / 64, ntc = (c1 + 63) / 64 - jt0;
   XK_CHECK(nt < 65536, "symm_expand: order too large");
-  hipLaunchKernelGGL((symm_expand_kernel<T>), dim3(nt, nt), dim3(256), 0, s, uplo, n, A, lda, Y, ldy, Z, ldz);
+  hipLaunchKernelGGL((symm_expand_kernel<T>), dim3(nt, ntc), dim3(256), 0, s, uplo, n, jt0, A, lda, Y, ldy, Z, ldz);
   XK_HIP(hipGetLastError());
 }
-template void symm_expand<double>(hipStream_t, int, int, const double*, long, double*, long, double*, long);
-template void symm_expand<float>(hipStream_t, int, int, const float*, long, float*, long, float*, long);
+template void symm_expand<double>(hipStream_t, int, int, const double*, long, double*, long, double*, long, int,
+                                  int);
+template void symm_expand<float>(hipStream_t, int, int, const float*, long, float*, long, float*, long, int, int);
 
 template <typename T>
 void tri_expand(hipStream_t s, int uplo, int unit, int n, const T* A, long lda, T* Y, long ldy)
