@@ -202,11 +202,13 @@ void potrf_set_pairing(int v) { g_potrf_pair = v < 0 ? 1 : (v ? 1 : 0); }
 
 template <typename T>
 void dev_potrf(const Exec& x, int uplo, int n, T* A, long lda, int nb, int* info,
-               const std::function<void()>& beside_first)
+               const std::function<void()>& beside_first, T* keep_inv, int keep_nb)
 {
   if (n <= 0) return;
   nb = std::max(64, std::min(nb, n));
   const int nt = (n + nb - 1) / nb;
+  XK_CHECK(!keep_inv || (keep_nb > 0 && (nt == 1 || nb % keep_nb == 0)),
+           "dev_potrf: kept inverses need a block order that divides nb");
   // inverses of two consecutive diagonal blocks (ping-pong for lookahead)
   T* inv = (T*)scratch(WS_INV, sizeof(T) * (size_t)2 * nb * nb);
   hipEvent_t ev_diag = x.ev0, ev_look = x.ev1;
@@ -251,7 +253,19 @@ void dev_potrf(const Exec& x, int uplo, int n, T* A, long lda, int nb, int* info
   // the inverses' blocks above the diagonal are never written with nonzeros:
   // one fill here replaces a fill per recursion node
   fill_zero<T>(sd, nb, 2 * nb, inv, nb);
+  // the diagonal keep_nb sub-blocks of block k's inverse are the inverses of
+  // the factor's diagonal blocks of order keep_nb
+  auto keep = [&](int k, const T* ik) {
+    if (!keep_inv) return;
+    const int kk = tsz(k);
+    for (int j = 0; j * keep_nb < kk; ++j) {
+      const int sz = std::min(keep_nb, kk - j * keep_nb);
+      const size_t g = (size_t)k * nb / keep_nb + j;
+      copy2d<T>(sd, sz, sz, ik + (long)j * keep_nb * (1 + nb), nb, keep_inv + g * keep_nb * keep_nb, keep_nb);
+    }
+  };
   potrf_diag<T>(sd, uplo, tsz(0), blk(0, 0), lda, inv, nb, info, 0, WS_DIAG0, true);
+  keep(0, inv);
   if (beside_first) beside_first();
   for (int k = 0; k < nt; ++k) {
     const int kk = tsz(k);
@@ -351,6 +365,7 @@ void dev_potrf(const Exec& x, int uplo, int n, T* A, long lda, int nb, int* info
     if (!lookahead)
       potrf_diag<T>(x.s, uplo, tsz(k + 1), blk(k + 1, k + 1), lda, inv + (size_t)((k + 1) & 1) * nb * nb, nb,
                     info, (k + 1) * nb, WS_DIAG0, true);
+    keep(k + 1, inv + (size_t)((k + 1) & 1) * nb * nb);
   }
   if (split) split_cache_clear();
   if (lookahead) gemm_set_priority_stream(nullptr);
@@ -1060,10 +1075,14 @@ int dev_posv_ir(const Exec& x, const Exec& xl, const Exec& xf, int uplo, int n, 
     trsm_sweep<double>(x, Left, uplo, tA, n, nrhs, 1.0, F, ldf, Y, ldy, tnb, inv);
     trsm_sweep<double>(x, Left, uplo, tA == NoTrans ? Trans : NoTrans, n, nrhs, 1.0, F, ldf, Y, ldy, tnb, inv);
   };
+  // the factorization hands over its diagonal blocks' inverses when the
+  // solves' block order divides its own (else they are formed afterwards)
+  const int nbf = std::max(64, std::min(nb, n));
+  const bool kept = nbf >= n || nbf % tnb == 0;
   // F <- Cholesky factor of A's image under e; returns info
   auto factor = [&](const Exec& e, const std::function<void()>& beside_first) {
     fill_int(x.s, dinfo, 1, INT_MAX);
-    dev_potrf<double>(e, uplo, n, F, ldf, nb, dinfo, beside_first);
+    dev_potrf<double>(e, uplo, n, F, ldf, nb, dinfo, beside_first, kept ? inv : nullptr, tnb);
     XK_HIP(hipMemcpyAsync(&hinfo, dinfo, sizeof(int), hipMemcpyDeviceToHost, x.s));
     XK_HIP(hipStreamSynchronize(x.s));
     return hinfo == INT_MAX ? 0 : hinfo;
@@ -1111,7 +1130,7 @@ int dev_posv_ir(const Exec& x, const Exec& xl, const Exec& xf, int uplo, int n, 
   if (finfo != 0) {
     fallback = -3;
   } else {
-    trsm_inverses<double>(x, uplo, NonUnit, n, F, ldf, tnb, inv);
+    if (!kept) trsm_inverses<double>(x, uplo, NonUnit, n, F, ldf, tnb, inv);
     copy2d<double>(x.s, n, nrhs, B, ldb, X, ldx);
     solve(X, ldx);
     for (int it = 0;; ++it) {
@@ -1148,7 +1167,7 @@ int dev_posv_ir(const Exec& x, const Exec& xl, const Exec& xf, int uplo, int n, 
   symm_expand<double>(x.s, uplo, n, A, lda, F, ldf, nullptr, 0);
   const int info = factor(xf, std::function<void()>());
   if (info != 0) return info;
-  trsm_inverses<double>(x, uplo, NonUnit, n, F, ldf, tnb, inv);
+  if (!kept) trsm_inverses<double>(x, uplo, NonUnit, n, F, ldf, tnb, inv);
   copy2d<double>(x.s, n, nrhs, B, ldb, X, ldx);
   solve(X, ldx);
   XK_HIP(hipStreamSynchronize(x.s));
@@ -1159,7 +1178,7 @@ int dev_posv_ir(const Exec& x, const Exec& xl, const Exec& xf, int uplo, int n, 
   template void dev_gemm<T>(const Exec&, int, int, int, int, int, T, const T*, long, const T*, long, T, T*, long); \
   template void dev_syrk<T>(const Exec&, int, int, int, int, T, const T*, long, T, T*, long);                     \
   template void dev_trsm<T>(const Exec&, int, int, int, int, int, int, T, const T*, long, T*, long, int);        \
-  template void dev_potrf<T>(const Exec&, int, int, T*, long, int, int*, const std::function<void()>&);          \
+  template void dev_potrf<T>(const Exec&, int, int, T*, long, int, int*, const std::function<void()>&, T*, int); \
   template void dev_trtri<T>(const Exec&, hipStream_t, int, int, int, const T*, long, T*, long, int);        \
   template void dev_gemmt<T>(const Exec&, int, int, int, int, int, T, const T*, long, const T*, long, T, T*,     \
                              long);                                                                             \

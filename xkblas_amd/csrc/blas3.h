@@ -49,9 +49,14 @@ void dev_trsm(const Exec& x, int side, int uplo, int transA, int diag, int m, in
 // enqueue work on x.s right after the first diagonal block's factorization
 // has gone to the side stream (it runs beside it); it must not touch the
 // first block column of A, which must be in place before the call.
+// keep_inv, when set, receives the inverses of the factor's diagonal blocks of
+// order keep_nb, one after another (keep_nb x keep_nb each, a shorter last
+// one included), zeros outside their triangle: the layout of a TRSM sweep's
+// inverses.  keep_nb must divide the block order nb (or nb >= n).
 template <typename T>
 void dev_potrf(const Exec& x, int uplo, int n, T* A, long lda, int nb, int* info,
-               const std::function<void()>& beside_first = std::function<void()>());
+               const std::function<void()>& beside_first = std::function<void()>(), T* keep_inv = nullptr,
+               int keep_nb = 0);
 // trailing updates of dev_potrf: 1 paired (default, also for -1), 0 one per panel
 void potrf_set_pairing(int v);
 
