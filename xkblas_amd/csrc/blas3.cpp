@@ -202,11 +202,13 @@ void potrf_set_pairing(int v) { g_potrf_pair = v < 0 ? 1 : (v ? 1 : 0); }
 
 template <typename T>
 void dev_potrf(const Exec& x, int uplo, int n, T* A, long lda, int nb, int* info,
-               const std::function<void()>& beside_first, T* keep_inv, int keep_nb)
+               const PotrfHooks<T>* hooks)
 {
   if (n <= 0) return;
   nb = std::max(64, std::min(nb, n));
   const int nt = (n + nb - 1) / nb;
+  T* const keep_inv = hooks ? hooks->keep_inv : nullptr;
+  const int keep_nb = hooks ? hooks->keep_nb : 0;
   XK_CHECK(!keep_inv || (keep_nb > 0 && (nt == 1 || nb % keep_nb == 0)),
            "dev_potrf: kept inverses need a block order that divides nb");
   // inverses of two consecutive diagonal blocks (ping-pong for lookahead)
@@ -266,7 +268,7 @@ void dev_potrf(const Exec& x, int uplo, int n, T* A, long lda, int nb, int* info
   };
   potrf_diag<T>(sd, uplo, tsz(0), blk(0, 0), lda, inv, nb, info, 0, WS_DIAG0, true);
   keep(0, inv);
-  if (beside_first) beside_first();
+  if (hooks && hooks->beside_first) hooks->beside_first();
   for (int k = 0; k < nt; ++k) {
     const int kk = tsz(k);
     T* ik = inv + (size_t)(k & 1) * nb * nb;
@@ -357,6 +359,8 @@ void dev_potrf(const Exec& x, int uplo, int n, T* A, long lda, int nb, int* info
       }
       if (prev.n > 0) dev_syrk<T>(x, Upper, Trans, prev.n, prev.kw, T(-1), prev.U, lda, T(1), prev.C, lda);
     }
+    // block column k of the factor is final (its panel and diagonal block)
+    if (hooks && hooks->after_panel) hooks->after_panel(k, ik);
     if (lookahead) {
       XK_HIP(hipStreamWaitEvent(sd, ev_look, 0));
       potrf_diag<T>(sd, uplo, k1, blk(k + 1, k + 1), lda, inv + (size_t)((k + 1) & 1) * nb * nb, nb, info,
@@ -367,6 +371,7 @@ void dev_potrf(const Exec& x, int uplo, int n, T* A, long lda, int nb, int* info
                     info, (k + 1) * nb, WS_DIAG0, true);
     keep(k + 1, inv + (size_t)((k + 1) & 1) * nb * nb);
   }
+  if (hooks && hooks->after_panel) hooks->after_panel(nt - 1, inv + (size_t)((nt - 1) & 1) * nb * nb);
   if (split) split_cache_clear();
   if (lookahead) gemm_set_priority_stream(nullptr);
   if (own_events) {
@@ -1075,14 +1080,57 @@ int dev_posv_ir(const Exec& x, const Exec& xl, const Exec& xf, int uplo, int n, 
     trsm_sweep<double>(x, Left, uplo, tA, n, nrhs, 1.0, F, ldf, Y, ldy, tnb, inv);
     trsm_sweep<double>(x, Left, uplo, tA == NoTrans ? Trans : NoTrans, n, nrhs, 1.0, F, ldf, Y, ldy, tnb, inv);
   };
+  // A's full image Af and the factor's copy F: the first block column before
+  // the factorization, the rest (and the row sums of |A|, needed only by the
+  // stopping test) on the compute stream beside the first diagonal block's
+  // factorization, which would otherwise leave it idle
+  const int c1 = std::min(n, (std::max(64, std::min(nb, n)) + 63) / 64 * 64);
+  symm_expand(x.s, uplo, n, A, lda, Af, ldf, F, ldf, 0, c1);
+  auto expand_rest = [&]() {
+    symm_expand(x.s, uplo, n, A, lda, Af, ldf, F, ldf, c1, n);
+    col_reduce(x.s, 1, n, n, Af, ldf, V);  // row sums of |A| = column sums (symmetric)
+  };
+  // X <- B: the first solve runs in place, its forward sweep inside the
+  // factorization
+  copy2d<double>(x.s, n, nrhs, B, ldb, X, ldx);
   // the factorization hands over its diagonal blocks' inverses when the
   // solves' block order divides its own (else they are formed afterwards)
   const int nbf = std::max(64, std::min(nb, n));
   const bool kept = nbf >= n || nbf % tnb == 0;
-  // F <- Cholesky factor of A's image under e; returns info
-  auto factor = [&](const Exec& e, const std::function<void()>& beside_first) {
+  // the first solve's forward sweep (L y = b, or U^T y = b), one block of the
+  // factorization's order at a time, enqueued on the compute stream as soon
+  // as the factor's block column k is final: it runs inside the
+  // factorization, whose last steps leave the compute stream waiting for the
+  // diagonal chain
+  auto forward_step = [&](int k, const double* ik) {
+    const int r0 = k * nbf, kk = std::min(nbf, n - r0), rest = n - r0 - kk;
+    double* Xk = X + r0;
+    if (!gemm_splitk<double>(x.s, tA, NoTrans, kk, nrhs, kk, 1.0, ik, nbf, Xk, ldx, 0.0, Xk, ldx)) {
+      double* W = (double*)scratch(WS_TRSMW, sizeof(double) * (size_t)kk * nrhs);
+      copy2d<double>(x.s, kk, nrhs, Xk, ldx, W, kk);
+      gemm<double>(x.s, tA, NoTrans, kk, nrhs, kk, 1.0, ik, nbf, W, kk, 0.0, Xk, ldx);
+    }
+    if (rest > 0) {
+      // block column k of L below its diagonal block, or block row k of U
+      // right of it (transposed)
+      const double* Pk = uplo == Lower ? F + (r0 + kk) + (long)r0 * ldf : F + r0 + (long)(r0 + kk) * ldf;
+      if (!gemm_splitk<double>(x.s, tA, NoTrans, rest, nrhs, kk, -1.0, Pk, ldf, Xk, ldx, 1.0, Xk + kk, ldx))
+        dev_gemm<double>(x, tA, NoTrans, rest, nrhs, kk, -1.0, Pk, ldf, Xk, ldx, 1.0, Xk + kk, ldx);
+    }
+  };
+  // F <- Cholesky factor of A's image under e; returns info.  first: the
+  // reduced factor, with A's remaining expansion beside its first diagonal
+  // block and the first solve's forward sweep woven in
+  auto factor = [&](const Exec& e, bool first) {
     fill_int(x.s, dinfo, 1, INT_MAX);
-    dev_potrf<double>(e, uplo, n, F, ldf, nb, dinfo, beside_first, kept ? inv : nullptr, tnb);
+    PotrfHooks<double> hk;
+    hk.keep_inv = kept ? inv : nullptr;
+    hk.keep_nb = tnb;
+    if (first) {
+      hk.beside_first = expand_rest;
+      hk.after_panel = forward_step;
+    }
+    dev_potrf<double>(e, uplo, n, F, ldf, nb, dinfo, &hk);
     XK_HIP(hipMemcpyAsync(&hinfo, dinfo, sizeof(int), hipMemcpyDeviceToHost, x.s));
     XK_HIP(hipStreamSynchronize(x.s));
     return hinfo == INT_MAX ? 0 : hinfo;
@@ -1103,16 +1151,6 @@ int dev_posv_ir(const Exec& x, const Exec& xl, const Exec& xf, int uplo, int n, 
     XK_HIP(hipStreamSynchronize(x.s));
   };
 
-  // A's full image Af and the factor's copy F: the first block column before
-  // the factorization, the rest (and the row sums of |A|, needed only by the
-  // stopping test) on the compute stream beside the first diagonal block's
-  // factorization, which would otherwise leave it idle
-  const int c1 = std::min(n, (std::max(64, std::min(nb, n)) + 63) / 64 * 64);
-  symm_expand(x.s, uplo, n, A, lda, Af, ldf, F, ldf, 0, c1);
-  auto expand_rest = [&]() {
-    symm_expand(x.s, uplo, n, A, lda, Af, ldf, F, ldf, c1, n);
-    col_reduce(x.s, 1, n, n, Af, ldf, V);  // row sums of |A| = column sums (symmetric)
-  };
   double cte = 0.0;
   auto norm_a = [&]() {  // after the factorization's synchronisation
     XK_HIP(hipMemcpyAsync(hv.data(), V, sizeof(double) * n, hipMemcpyDeviceToHost, x.s));
@@ -1125,14 +1163,15 @@ int dev_posv_ir(const Exec& x, const Exec& xl, const Exec& xf, int uplo, int n, 
   auto res_ok = [&](int j) { return hv[n + j] <= hv[n + nrhs + j] * cte; };
 
   int fallback = 0;
-  const int finfo = factor(xl, expand_rest);
+  const int finfo = factor(xl, true);
   norm_a();
   if (finfo != 0) {
     fallback = -3;
   } else {
     if (!kept) trsm_inverses<double>(x, uplo, NonUnit, n, F, ldf, tnb, inv);
-    copy2d<double>(x.s, n, nrhs, B, ldb, X, ldx);
-    solve(X, ldx);
+    // the first solve's backward sweep (its forward sweep ran inside the
+    // factorization)
+    trsm_sweep<double>(x, Left, uplo, tA == NoTrans ? Trans : NoTrans, n, nrhs, 1.0, F, ldf, X, ldx, tnb, inv);
     for (int it = 0;; ++it) {
       residual();
       if (it == itermax) {
@@ -1165,7 +1204,7 @@ int dev_posv_ir(const Exec& x, const Exec& xl, const Exec& xf, int uplo, int n, 
   // the accurate factor, as LAPACK dsposv falls back to dpotrf
   *iter = fallback;
   symm_expand<double>(x.s, uplo, n, A, lda, F, ldf, nullptr, 0);
-  const int info = factor(xf, std::function<void()>());
+  const int info = factor(xf, false);
   if (info != 0) return info;
   if (!kept) trsm_inverses<double>(x, uplo, NonUnit, n, F, ldf, tnb, inv);
   copy2d<double>(x.s, n, nrhs, B, ldb, X, ldx);
@@ -1178,7 +1217,7 @@ int dev_posv_ir(const Exec& x, const Exec& xl, const Exec& xf, int uplo, int n, 
   template void dev_gemm<T>(const Exec&, int, int, int, int, int, T, const T*, long, const T*, long, T, T*, long); \
   template void dev_syrk<T>(const Exec&, int, int, int, int, T, const T*, long, T, T*, long);                     \
   template void dev_trsm<T>(const Exec&, int, int, int, int, int, int, T, const T*, long, T*, long, int);        \
-  template void dev_potrf<T>(const Exec&, int, int, T*, long, int, int*, const std::function<void()>&, T*, int); \
+  template void dev_potrf<T>(const Exec&, int, int, T*, long, int, int*, const PotrfHooks<T>*);                  \
   template void dev_trtri<T>(const Exec&, hipStream_t, int, int, int, const T*, long, T*, long, int);        \
   template void dev_gemmt<T>(const Exec&, int, int, int, int, int, T, const T*, long, const T*, long, T, T*,     \
                              long);                                                                             \

@@ -43,20 +43,33 @@ template <typename T>
 void dev_trsm(const Exec& x, int side, int uplo, int transA, int diag, int m, int n, T alpha, const T* A, long lda,
               T* B, long ldb, int nb);
 
+// Optional work of a caller woven into dev_potrf (all members may be empty):
+//  * beside_first() is called once to enqueue work on x.s right after the
+//    first diagonal block's factorization has gone to the side stream (it
+//    runs beside it); it must not touch the first block column of A, which
+//    must be in place before the call;
+//  * after_panel(k, ik) is called on the host once per block k, in order, to
+//    enqueue work on x.s once block column k of the factor is final (L below
+//    the diagonal block: Lower; U right of it: Upper); ik = the inverse of
+//    diagonal block k (leading dimension nb), valid for work so enqueued;
+//  * keep_inv receives the inverses of the factor's diagonal blocks of order
+//    keep_nb, one after another (keep_nb x keep_nb each, a shorter last one
+//    included), zeros outside their triangle: the layout of a TRSM sweep's
+//    inverses.  keep_nb must divide the block order nb (or nb >= n).
+template <typename T>
+struct PotrfHooks {
+  std::function<void()> beside_first;
+  std::function<void(int, const T*)> after_panel;
+  T* keep_inv = nullptr;
+  int keep_nb = 0;
+};
+
 // In-place blocked Cholesky with block order nb.  info (device int) receives
 // the LAPACK info as an atomic min over failing orders (INT_MAX = success);
-// the caller initialises it.  beside_first, when set, is called once to
-// enqueue work on x.s right after the first diagonal block's factorization
-// has gone to the side stream (it runs beside it); it must not touch the
-// first block column of A, which must be in place before the call.
-// keep_inv, when set, receives the inverses of the factor's diagonal blocks of
-// order keep_nb, one after another (keep_nb x keep_nb each, a shorter last
-// one included), zeros outside their triangle: the layout of a TRSM sweep's
-// inverses.  keep_nb must divide the block order nb (or nb >= n).
+// the caller initialises it.
 template <typename T>
 void dev_potrf(const Exec& x, int uplo, int n, T* A, long lda, int nb, int* info,
-               const std::function<void()>& beside_first = std::function<void()>(), T* keep_inv = nullptr,
-               int keep_nb = 0);
+               const PotrfHooks<T>* hooks = nullptr);
 // trailing updates of dev_potrf: 1 paired (default, also for -1), 0 one per panel
 void potrf_set_pairing(int v);
 
