@@ -106,10 +106,13 @@ __global__ __launch_bounds__(256) void add2d_kernel(int m, int n, const double* 
 // that level.  Here the k products of each element are summed by MFMA in
 // runs of RES_SUB = 64 only; the run sums are added without error
 // (TwoSum, the rounding errors kept in a second word) per workgroup over a
-// k-range of RES_KR, and the ranges' (sum, error) pairs are added to B in
-// the same way by the reduction kernel.  The remaining error is that of the
-// 64-term runs and one final rounding: about sqrt(k / 64) times smaller.
-constexpr int RES_SUB = 64, RES_KR = 1024;
+// k-range of RES_KR (XKBLAS_RES_KR, a multiple of 64), and the ranges'
+// (sum, error) pairs are added to B in the same way by the reduction kernel.
+// The remaining error is that of the 64-term runs and one final rounding:
+// about sqrt(k / 64) times smaller.  RES_KR = 4096 rather than 1024: a
+// quarter of the partial pairs written and read back (N = 32768, 64 right-hand
+// sides: 3.65 + 0.19 -> 3.53 + 0.05 ms per residual).
+constexpr int RES_SUB = 64, RES_KR = 4096;
 constexpr int RES_WAVES = 8;           // 16 rows each: 128 rows per workgroup
 constexpr int RES_LDK = RES_SUB + 4;   // padded k stride of the staged X run
 typedef double rd4 __attribute__((ext_vector_type(4)));
@@ -130,13 +133,13 @@ __device__ __forceinline__ void two_sum(double a, double b, double& s, double& e
 // products of the current one.
 __global__ __launch_bounds__(512) void resid_partial_kernel(int n, int nrhs, const double* __restrict__ A, long lda,
                                                             const double* __restrict__ X, long ldx,
-                                                            double2* __restrict__ P, long ldp, long sp)
+                                                            double2* __restrict__ P, long ldp, long sp, int kr)
 {
   __shared__ double Xs[64 * RES_LDK];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int li = lane & 15, lk = lane >> 4;
   const int i0 = blockIdx.x * (16 * RES_WAVES) + wave * 16, j0 = blockIdx.y * 64;
-  const int k0 = blockIdx.z * RES_KR, k1 = min(n, k0 + RES_KR);
+  const int k0 = blockIdx.z * kr, k1 = min(n, k0 + kr);
   const int ia = i0 + li;
   const bool rowok = ia < n;
   double hi[4][4], lo[4][4];
@@ -225,10 +228,16 @@ __global__ __launch_bounds__(256) void resid_reduce_kernel(int n, int nz, const 
   R[i + (long)j * ldr] = s + c;
 }
 
+static int res_kr()
+{
+  static const int v = getenv("XKBLAS_RES_KR") ? atoi(getenv("XKBLAS_RES_KR")) / RES_SUB * RES_SUB : RES_KR;
+  return v > 0 ? v : RES_KR;
+}
+
 size_t resid_scratch_bytes(int n, int nrhs)
 {
   const long ldp = (n + 63) / 64 * 64;
-  const long nz = (n + RES_KR - 1) / RES_KR;
+  const long nz = (n + res_kr() - 1) / res_kr();
   return sizeof(double2) * (size_t)nz * ldp * nrhs;
 }
 
@@ -237,12 +246,12 @@ void resid_accurate(hipStream_t s, int n, int nrhs, const double* A, long lda, c
 {
   if (n <= 0 || nrhs <= 0) return;
   const long ldp = (n + 63) / 64 * 64;
-  const int nz = (n + RES_KR - 1) / RES_KR;
+  const int nz = (n + res_kr() - 1) / res_kr();
   const long sp = ldp * nrhs;
   double2* P = static_cast<double2*>(work);
   XK_CHECK((nrhs + 63) / 64 <= 65535 && nz <= 65535 && nrhs <= 65535, "residual grid");
   hipLaunchKernelGGL(resid_partial_kernel, dim3((n + 16 * RES_WAVES - 1) / (16 * RES_WAVES), (nrhs + 63) / 64, nz),
-                     dim3(64 * RES_WAVES), 0, s, n, nrhs, A, lda, X, ldx, P, ldp, sp);
+                     dim3(64 * RES_WAVES), 0, s, n, nrhs, A, lda, X, ldx, P, ldp, sp, res_kr());
   hipLaunchKernelGGL(resid_reduce_kernel, dim3((n + 255) / 256, nrhs), dim3(256), 0, s, n, nz, B, ldb, P, ldp, sp,
                      R, ldr);
   XK_HIP(hipGetLastError());
