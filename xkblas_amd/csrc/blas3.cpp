@@ -1050,7 +1050,7 @@ static bool ir_column_done(bool res_ok, int corr, double dn, double dprev, doubl
   return rho / (1.0 - rho) * dn <= IR_TAU * xn;
 }
 
-enum { WS_IR_A = 16, WS_IR_F = 17, WS_IR_R = 18, WS_IR_V = 19, WS_IR_P = 28 };
+enum { WS_IR_A = 16, WS_IR_F = 17, WS_IR_R = 18, WS_IR_V = 19, WS_IR_P = 28, WS_IR_F32 = 32, WS_IR_S32 = 33 };
 
 int dev_posv_ir(const Exec& x, const Exec& xl, const Exec& xf, int uplo, int n, int nrhs, const double* A, long lda,
                 const double* B, long ldb, double* X, long ldx, int nb, int itermax, int* iter)
@@ -1069,8 +1069,8 @@ int dev_posv_ir(const Exec& x, const Exec& xl, const Exec& xf, int uplo, int n, 
   double* R = (double*)scratch(WS_IR_R, sizeof(double) * ((size_t)ldf * nrhs + (size_t)nblk * tnb * tnb));
   double* inv = R + (size_t)ldf * nrhs;
   // n column sums of |A|, then per column: residual, solution and correction
-  // norms; then the info word
-  double* V = (double*)scratch(WS_IR_V, sizeof(double) * ((size_t)n + 3 * nrhs + 1));
+  // norms; then the info word; then nrhs column maxima (FP32 solves)
+  double* V = (double*)scratch(WS_IR_V, sizeof(double) * ((size_t)n + 4 * nrhs + 1));
   int* dinfo = (int*)(V + n + 3 * nrhs);
   std::vector<double> hv((size_t)n + 3 * nrhs), dprev(nrhs, 0.0);
   int hinfo = 0;
@@ -1079,6 +1079,30 @@ int dev_posv_ir(const Exec& x, const Exec& xl, const Exec& xf, int uplo, int n, 
   auto solve = [&](double* Y, long ldy) {          // Y <- F^-1 Y (F = L L^T or U^T U)
     trsm_sweep<double>(x, Left, uplo, tA, n, nrhs, 1.0, F, ldf, Y, ldy, tnb, inv);
     trsm_sweep<double>(x, Left, uplo, tA == NoTrans ? Trans : NoTrans, n, nrhs, 1.0, F, ldf, Y, ldy, tnb, inv);
+  };
+  // The solves with the reduced factor need only its own accuracy (about
+  // 2^-25 with the default moduli): like LAPACK dsposv's single-precision
+  // solves they run in FP32 (XKBLAS_IR_SOLVE32=0: FP64), on an FP32 image of
+  // the factor formed block column by block column inside the factorization,
+  // with each right-hand side scaled by a power of two (no underflow of small
+  // residuals).  Only while |A| keeps the factor and the inverses of its
+  // diagonal blocks inside FP32's range, and when the factor has more than
+  // one diagonal block (one block is factored entirely in FP64: FP32 solves
+  // would then cost a correction); the residuals stay accurate FP64.
+  static const bool solve32_env = !(getenv("XKBLAS_IR_SOLVE32") && atoi(getenv("XKBLAS_IR_SOLVE32")) == 0);
+  const bool want32 = solve32_env && std::max(64, std::min(nb, n)) < n;
+  float* F32 = want32 ? (float*)scratch(WS_IR_F32, sizeof(float) * (size_t)ldf * n) : nullptr;
+  float* inv32 = want32 ? (float*)scratch(WS_IR_S32, sizeof(float) * ((size_t)nblk * tnb * tnb +
+                                                                      (size_t)ldf * nrhs)) : nullptr;
+  float* Y32 = want32 ? inv32 + (size_t)nblk * tnb * tnb : nullptr;
+  double* cmax = V + n + 3 * nrhs + 1;  // column maxima of the right-hand side being solved
+  // Y <- F^-1 Y in FP32 (back: the backward sweep only)
+  auto solve32 = [&](double* Y, long ldy, bool back) {
+    col_reduce(x.s, 0, n, nrhs, Y, ldy, cmax);
+    to_f32(x.s, n, nrhs, Y, ldy, cmax, Y32, ldf);
+    if (!back) trsm_sweep<float>(x, Left, uplo, tA, n, nrhs, 1.0f, F32, ldf, Y32, ldf, tnb, inv32);
+    trsm_sweep<float>(x, Left, uplo, tA == NoTrans ? Trans : NoTrans, n, nrhs, 1.0f, F32, ldf, Y32, ldf, tnb, inv32);
+    from_f32(x.s, n, nrhs, Y32, ldf, cmax, Y, ldy, false);
   };
   // A's full image Af and the factor's copy F: the first block column before
   // the factorization, the rest (and the row sums of |A|, needed only by the
@@ -1105,6 +1129,11 @@ int dev_posv_ir(const Exec& x, const Exec& xl, const Exec& xf, int uplo, int n, 
   auto forward_step = [&](int k, const double* ik) {
     const int r0 = k * nbf, kk = std::min(nbf, n - r0), rest = n - r0 - kk;
     double* Xk = X + r0;
+    // the FP32 image of the factor's block column (Lower) or row (Upper) k
+    if (F32) {
+      if (uplo == Lower) to_f32(x.s, n - r0, kk, F + r0 + (long)r0 * ldf, ldf, nullptr, F32 + r0 + (long)r0 * ldf, ldf);
+      else to_f32(x.s, kk, n - r0, F + r0 + (long)r0 * ldf, ldf, nullptr, F32 + r0 + (long)r0 * ldf, ldf);
+    }
     if (!gemm_splitk<double>(x.s, tA, NoTrans, kk, nrhs, kk, 1.0, ik, nbf, Xk, ldx, 0.0, Xk, ldx)) {
       double* W = (double*)scratch(WS_TRSMW, sizeof(double) * (size_t)kk * nrhs);
       copy2d<double>(x.s, kk, nrhs, Xk, ldx, W, kk);
@@ -1151,11 +1180,11 @@ int dev_posv_ir(const Exec& x, const Exec& xl, const Exec& xf, int uplo, int n, 
     XK_HIP(hipStreamSynchronize(x.s));
   };
 
-  double cte = 0.0;
+  double cte = 0.0, anrm = 0.0;
   auto norm_a = [&]() {  // after the factorization's synchronisation
     XK_HIP(hipMemcpyAsync(hv.data(), V, sizeof(double) * n, hipMemcpyDeviceToHost, x.s));
     XK_HIP(hipStreamSynchronize(x.s));
-    double anrm = 0.0;
+    anrm = 0.0;
     for (int j = 0; j < n; ++j) anrm = (hv[j] != hv[j] || hv[j] > anrm) ? hv[j] : anrm;
     cte = anrm * 1.1102230246251565e-16 * std::sqrt((double)n);  // eps = 2^-53
   };
@@ -1169,9 +1198,12 @@ int dev_posv_ir(const Exec& x, const Exec& xl, const Exec& xf, int uplo, int n, 
     fallback = -3;
   } else {
     if (!kept) trsm_inverses<double>(x, uplo, NonUnit, n, F, ldf, tnb, inv);
+    const bool s32 = F32 && anrm > 0x1p-100 && anrm < 0x1p+100;
+    if (s32) to_f32(x.s, tnb, nblk * tnb, inv, tnb, nullptr, inv32, tnb);
     // the first solve's backward sweep (its forward sweep ran inside the
     // factorization)
-    trsm_sweep<double>(x, Left, uplo, tA == NoTrans ? Trans : NoTrans, n, nrhs, 1.0, F, ldf, X, ldx, tnb, inv);
+    if (s32) solve32(X, ldx, true);
+    else trsm_sweep<double>(x, Left, uplo, tA == NoTrans ? Trans : NoTrans, n, nrhs, 1.0, F, ldf, X, ldx, tnb, inv);
     for (int it = 0;; ++it) {
       residual();
       if (it == itermax) {
@@ -1184,7 +1216,8 @@ int dev_posv_ir(const Exec& x, const Exec& xl, const Exec& xf, int uplo, int n, 
         }
         break;
       }
-      solve(R, ldf);  // the correction d
+      if (s32) solve32(R, ldf, false);  // the correction d
+      else solve(R, ldf);
       col_reduce(x.s, 0, n, nrhs, R, ldf, dn);
       add2d(x.s, n, nrhs, R, ldf, X, ldx);
       to_host(3);

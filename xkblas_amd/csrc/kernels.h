@@ -124,6 +124,11 @@ void cplx_expand(hipStream_t s, int mode, int uplo, int n, const T* A, long lda,
 
 // X <- X + D (m x n blocks).
 void add2d(hipStream_t s, int m, int n, const double* D, long ldd, double* X, long ldx);
+// Z <- float(Y 2^-e_j), e_j the exponent of colmax[j] (colmax null: no
+// scaling); from_f32 undoes it: Y <- double(Z) 2^e_j, or Y += that (acc).
+void to_f32(hipStream_t s, int m, int n, const double* Y, long ldy, const double* colmax, float* Z, long ldz);
+void from_f32(hipStream_t s, int m, int n, const float* Z, long ldz, const double* colmax, double* Y, long ldy,
+              bool acc);
 
 // R <- B - A X (A n x n, X and B n x nrhs) with the k products summed in
 // short MFMA runs and the run sums added without rounding error (TwoSum):

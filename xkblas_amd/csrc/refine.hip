@@ -295,6 +295,58 @@ void col_reduce(hipStream_t s, int op, int m, int n, const double* X, long ldx, 
   XK_HIP(hipGetLastError());
 }
 
+// FP32 images for the refinement's solves.  Z <- float(Y 2^-e_j) per column j,
+// with e_j the exponent of colmax[j] (0 when it is 0 or not finite): the
+// scaling is exact and keeps small residuals away from FP32's underflow.
+__global__ __launch_bounds__(256) void to_f32_kernel(int m, int n, const double* __restrict__ Y, long ldy,
+                                                     const double* __restrict__ colmax, float* __restrict__ Z, long ldz)
+{
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  const int j = blockIdx.y;
+  if (i >= m || j >= n) return;
+  const double c = colmax ? colmax[j] : 0.0;
+  const int e = (c > 0.0 && c <= 1.7976931348623157e308) ? __builtin_amdgcn_frexp_exp(c) : 0;
+  Z[i + (long)j * ldz] = (float)__builtin_ldexp(Y[i + (long)j * ldy], -e);
+}
+
+// Y <- double(Z) 2^e_j (acc: Y += ...), the inverse of to_f32_kernel's scaling.
+__global__ __launch_bounds__(256) void from_f32_kernel(int m, int n, const float* __restrict__ Z, long ldz,
+                                                       const double* __restrict__ colmax, double* __restrict__ Y,
+                                                       long ldy, int acc)
+{
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  const int j = blockIdx.y;
+  if (i >= m || j >= n) return;
+  const double c = colmax ? colmax[j] : 0.0;
+  const int e = (c > 0.0 && c <= 1.7976931348623157e308) ? __builtin_amdgcn_frexp_exp(c) : 0;
+  const double v = __builtin_ldexp((double)Z[i + (long)j * ldz], e);
+  double* y = Y + i + (long)j * ldy;
+  *y = acc ? *y + v : v;
+}
+
+void to_f32(hipStream_t s, int m, int n, const double* Y, long ldy, const double* colmax, float* Z, long ldz)
+{
+  if (m <= 0 || n <= 0) return;
+  for (int j0 = 0; j0 < n; j0 += 65535) {
+    const int nn = std::min(65535, n - j0);
+    hipLaunchKernelGGL(to_f32_kernel, dim3((m + 255) / 256, nn), dim3(256), 0, s, m, nn, Y + (long)j0 * ldy, ldy,
+                       colmax ? colmax + j0 : nullptr, Z + (long)j0 * ldz, ldz);
+  }
+  XK_HIP(hipGetLastError());
+}
+
+void from_f32(hipStream_t s, int m, int n, const float* Z, long ldz, const double* colmax, double* Y, long ldy,
+              bool acc)
+{
+  if (m <= 0 || n <= 0) return;
+  for (int j0 = 0; j0 < n; j0 += 65535) {
+    const int nn = std::min(65535, n - j0);
+    hipLaunchKernelGGL(from_f32_kernel, dim3((m + 255) / 256, nn), dim3(256), 0, s, m, nn, Z + (long)j0 * ldz, ldz,
+                       colmax ? colmax + j0 : nullptr, Y + (long)j0 * ldy, ldy, acc ? 1 : 0);
+  }
+  XK_HIP(hipGetLastError());
+}
+
 void add2d(hipStream_t s, int m, int n, const double* D, long ldd, double* X, long ldx)
 {
   if (m <= 0 || n <= 0) return;

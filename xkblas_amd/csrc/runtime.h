@@ -63,7 +63,7 @@ struct ProfStat {
 };
 
 struct Runtime {
-  static constexpr int NSCRATCH = 32;
+  static constexpr int NSCRATCH = 34;
   static constexpr int INFO_SLOTS = 1024;
 
   bool device_ready = false;
