@@ -13,6 +13,8 @@ n = 2048 with the oracle's tiled Cholesky + two tiled triangular solves,
 and with LAPACK's FP64 solve for the fallbacks, check that only the uplo triangle of
 A is read (the other one holds NaN), and force both fallbacks of dsposv: the
 correction limit (itermax 0) and a matrix that is not positive definite.
+The solves with the reduced factor run in FP32 within a range of ||A||: a
+scaled family checks both sides of it.
 """
 import numpy as np
 import pytest
@@ -142,6 +144,35 @@ def test_dposv_ir_more_moduli_fewer_corrections(xk):
         L.xkblas_set_posv_ir(0, -1)
         L.xkblas_set_param(nb0, 0)
     assert its[0] >= its[1] >= its[2] and its[0] >= 1, its
+
+
+@pytest.mark.parametrize("s", [-120, -60, 80, 95])
+def test_dposv_ir_scaled_matrix(xk, s):
+    """The solves with the reduced factor run in FP32 while ||A||_inf stays in
+    [2^-100, 2^100] (||A0||_inf ~ 2^13 here: s = -60 and 80 inside, -120 and
+    95 outside, where they run in FP64), the right-hand sides scaled per
+    column by a power of two.  A = 2^s A0, B = 2^(s // 2) B0: X is A0^-1 B0
+    scaled exactly, at the 1e-14 bar either way."""
+    L = xk.lib()
+    r = rng(101)
+    n, nrhs = 1000, 6
+    A0 = spd(r, n)
+    B0 = uniform(r, n, nrhs)
+    A = np.asfortranarray(np.ldexp(A0, s))
+    B = np.asfortranarray(np.ldexp(B0, s // 2))
+    nb0 = L.xkblas_get_param()
+    L.xkblas_set_param(256, 0)
+    try:
+        X, info, it = xk.posv_ir(LOWER, _stored(A, LOWER), B)
+    finally:
+        L.xkblas_set_param(nb0, 0)
+        xk.invalidate()
+    assert info == 0 and 2 <= it <= 10, (info, it)
+    cols = np.arange(4)
+    Lc = np.linalg.cholesky(A0)
+    Xs = refined_solution(A0, Lc, B0[:, cols], np.linalg.solve(A0, B0[:, cols]))
+    Xu = np.ldexp(X[:, cols], s - s // 2)
+    assert np.abs(Xu - Xs).max() / np.abs(Xs).max() <= 1e-14
 
 
 @pytest.mark.parametrize("bad", [0, 333, 999])
