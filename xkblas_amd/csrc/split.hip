@@ -188,9 +188,10 @@ int split_plane_count(int k, int requested, bool fp8)
 //  row_contig: a block of 256 threads owns 256 consecutive rows and a chunk
 //  of AM_K k; thread t walks row i0 + t (consecutive lanes read consecutive
 //  addresses);
-//  else: each wave owns one row and a chunk of AM_K k (64 lanes x 8 bytes
-//  contiguous per load), reduced across the wave at the end.
-constexpr int AM_K = 256;
+//  else: each wave owns four rows and a chunk of AM_K k (64 lanes x 8 bytes
+//  contiguous per load, all 16 loads of a lane in flight at once), each row
+//  reduced across the wave at the end.
+constexpr int AM_K = 256, AM_RW = 4;
 __global__ __launch_bounds__(256) void rowstat_kernel(const double* __restrict__ X, long ld, int rows, int kdim,
                                                       int row_contig, unsigned long long* __restrict__ bits,
                                                       double* __restrict__ part, long pld)
@@ -210,20 +211,35 @@ __global__ __launch_bounds__(256) void rowstat_kernel(const double* __restrict__
       q = fma(v, v, q);
     }
   } else {
-    row = blockIdx.y * 4 + (t >> 6);
-    if (row >= rows) return;
-    const double* x = X + (long)row * ld;
-    for (int k = k0 + (t & 63); k < k1; k += 64) {
-      const double v = x[k];
-      m = fmax(m, fabs(v));
-      q = fma(v, v, q);
-    }
+    const int row0 = (blockIdx.y * 4 + (t >> 6)) * AM_RW;
+    double v[AM_RW][AM_K / 64];
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-      m = fmax(m, __shfl_xor(m, o));
-      q += __shfl_xor(q, o);
+    for (int r = 0; r < AM_RW; ++r)
+#pragma unroll
+      for (int c = 0; c < AM_K / 64; ++c) {
+        const int k = k0 + (t & 63) + 64 * c;
+        v[r][c] = (row0 + r < rows && k < k1) ? X[(long)(row0 + r) * ld + k] : 0.0;
+      }
+#pragma unroll
+    for (int r = 0; r < AM_RW; ++r) {
+      double mr = 0.0, qr = 0.0;
+      // the chunk's terms in increasing k per lane, as one row per wave did
+#pragma unroll
+      for (int c = 0; c < AM_K / 64; ++c) {
+        mr = fmax(mr, fabs(v[r][c]));
+        qr = fma(v[r][c], v[r][c], qr);
+      }
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+        mr = fmax(mr, __shfl_xor(mr, o));
+        qr += __shfl_xor(qr, o);
+      }
+      if ((t & 63) == 0 && row0 + r < rows) {
+        part[(long)blockIdx.x * pld + row0 + r] = qr;
+        if (mr > 0.0) atomicMax(bits + row0 + r, (unsigned long long)__double_as_longlong(mr));
+      }
     }
-    if ((t & 63) != 0) return;
+    return;
   }
   part[(long)blockIdx.x * pld + row] = q;
   if (m > 0.0) atomicMax(bits + row, (unsigned long long)__double_as_longlong(m));
@@ -320,18 +336,25 @@ __global__ __launch_bounds__(256) void split_kernel(SplitParams sp)
   __shared__ double T[64][65];
   const int t = threadIdx.x;
   const int i0 = blockIdx.y * 64, k0 = blockIdx.x * 64;
-#pragma unroll 4
+  // all 16 loads of the thread in flight before the first LDS store (the
+  // kernel streams the operand once: it is bound by the bytes in flight)
+  double xs[16];
+#pragma unroll
   for (int s = 0; s < 16; ++s) {
     int i, k;
     if (sp.row_contig) { i = t & 63; k = (t >> 6) + 4 * s; }
     else { k = t & 63; i = (t >> 6) + 4 * s; }
     const int gi = i0 + i, gk = k0 + k;
-    double v = 0.0;
-    if (gi < sp.rows && gk < sp.kdim) {
-      v = sp.row_contig ? sp.X[gi + (long)gk * sp.ld] : sp.X[gk + (long)gi * sp.ld];
-      v = rint(ldexp(v, sp.shift[gi]));
-    }
-    T[i][k] = v;
+    xs[s] = (gi < sp.rows && gk < sp.kdim) ? (sp.row_contig ? sp.X[gi + (long)gk * sp.ld] : sp.X[gk + (long)gi * sp.ld])
+                                           : 0.0;
+  }
+#pragma unroll
+  for (int s = 0; s < 16; ++s) {
+    int i, k;
+    if (sp.row_contig) { i = t & 63; k = (t >> 6) + 4 * s; }
+    else { k = t & 63; i = (t >> 6) + 4 * s; }
+    const int gi = i0 + i, gk = k0 + k;
+    T[i][k] = (gi < sp.rows && gk < sp.kdim) ? rint(ldexp(xs[s], sp.shift[gi])) : 0.0;
   }
   __syncthreads();
   // thread -> rows (t/8, t/8 + 32), 8 consecutive k: per plane two stores,
@@ -1243,13 +1266,23 @@ __global__ __launch_bounds__(256) void reconstruct_kernel(RecParams rp)
   Word w[NPL];
 #pragma unroll
   for (int q = 0; q < NPL; ++q) w[q] = *reinterpret_cast<const Word*>(r + (long)min(q, rp.nplane - 1) * rp.sR);
-  // the old C values (beta != 0) are requested with the planes as well
+  // the old C values (beta != 0) are requested with the planes as well; four
+  // whole rows at a 16-byte aligned address move as two 16-byte words
   double* c = rp.C + (long)j * rp.ldc;
+  const bool vec = i0 + 3 < rp.m && ((reinterpret_cast<uintptr_t>(c + i0) & 15) == 0);
   double cold[4] = {0.0, 0.0, 0.0, 0.0};
   if (rp.beta != 0.0) {
+    if (vec) {
+      const double2 a = reinterpret_cast<const double2*>(c + i0)[0], b = reinterpret_cast<const double2*>(c + i0)[1];
+      cold[0] = a.x;
+      cold[1] = a.y;
+      cold[2] = b.x;
+      cold[3] = b.y;
+    } else {
 #pragma unroll
-    for (int e = 0; e < 4; ++e)
-      if (i0 + e < rp.m) cold[e] = c[i0 + e];
+      for (int e = 0; e < 4; ++e)
+        if (i0 + e < rp.m) cold[e] = c[i0 + e];
+    }
   }
   double S[4][3] = {};
 #pragma unroll
@@ -1284,6 +1317,18 @@ __global__ __launch_bounds__(256) void reconstruct_kernel(RecParams rp)
     }
   }
   const int sb = rp.shB[j];
+  if (vec && (rp.tri == TRI_FULL || (rp.tri == TRI_LOWER && i0 >= j) || (rp.tri == TRI_UPPER && i0 + 3 <= j))) {
+    double out[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const double x = crt_one(rp, S[e][0], S[e][1], S[e][2]);
+      out[e] = rp.alpha * ldexp(x, -(rp.shA[i0 + e] + sb));
+      if (rp.beta != 0.0) out[e] = fma(rp.beta, cold[e], out[e]);
+    }
+    reinterpret_cast<double2*>(c + i0)[0] = make_double2(out[0], out[1]);
+    reinterpret_cast<double2*>(c + i0)[1] = make_double2(out[2], out[3]);
+    return;
+  }
 #pragma unroll
   for (int e = 0; e < 4; ++e) {
     const int i = i0 + e;
@@ -1468,7 +1513,7 @@ static void split_gemm_chunk(const Exec& x, int transA, int transB, int m, int n
     XK_CHECK((rows + 3) / 4 <= 65535, "row-statistics grid");
     double* part = (double*)scratch(WS_SN, sizeof(double) * (size_t)nchunk * rpad);
     hipLaunchKernelGGL(fill_u64_kernel, dim3((int)((rpad + 255) / 256)), dim3(256), 0, s, bits, (int)rpad);
-    const dim3 sgrid((unsigned)nchunk, (unsigned)(rc ? (rows + 255) / 256 : (rows + 3) / 4));
+    const dim3 sgrid((unsigned)nchunk, (unsigned)(rc ? (rows + 255) / 256 : (rows + 4 * AM_RW - 1) / (4 * AM_RW)));
     hipLaunchKernelGGL(rowstat_kernel, sgrid, dim3(256), 0, s, X, ld, rows, k, rc, bits, part, rpad);
     hipLaunchKernelGGL(shift_kernel, dim3((rows + 255) / 256), dim3(256), 0, s, bits, part, rpad, nchunk, rows,
                        teff, tsafe, grow, shift);
