@@ -360,8 +360,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void b
                                w.tm, w.tn, sm);
       }
     } else {
+      // a TRSM panel is a chain of dependent blocks and the longest ones set
+      // the batch's time: its waves issue ahead of the GEMM items beside them
+      __builtin_amdgcn_s_setprio(2);
       if (p.dtype == 0) trsm_panel<double>(p, w.tn, smem_d);
       else trsm_panel<float>(p, w.tn, reinterpret_cast<float*>(smem_d));
+      __builtin_amdgcn_s_setprio(0);
     }
     __syncthreads();  // smem free for the next item index
   }
