@@ -283,7 +283,7 @@ def main(argv=None):
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--budget", type=float, default=None, help="seconds of collection")
     ap.add_argument("--data", default=None, help="start from the timings in this .npz")
-    ap.add_argument("--out", default="gpurun_out/tuner")
+    ap.add_argument("--out", default="tuner_out", help="directory for the timings and the trained model")
     ap.add_argument("--collect-only", action="store_true")
     ap.add_argument("--eval", type=int, default=24, help="held-out problems to judge")
     args = ap.parse_args(argv)
