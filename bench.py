@@ -559,10 +559,11 @@ def bench_batched(xk, seed=2027, reps=5, parts=False):
             assert L.xkblas_memcpy_htod(pool, host.ctypes.data_as(p), host.nbytes) == 0
 
         upload()
-        # GEMM item shapes (xkblas_set_batch_tile): 64 x 64 only (the default),
-        # per problem, 32 x 32 only; each timed as a graph-replayed launch
+        # GEMM item shapes (xkblas_set_batch_tile): per problem, 32 x 32 only,
+        # 64 x 64 only (the default plan below picks per precision: per problem
+        # for FP64, 64 x 64 for FP32); each timed as a graph-replayed launch
         mode_ms = {}
-        for mode in (1, 2, 0):
+        for mode in (1, 2, 3):
             L.xkblas_set_batch_tile(mode)
             mp = L.xkblas_batch_create(ctypes.cast(arr, p), len(arr))
             assert mp and L.xkblas_batch_use_graph(mp, 1) == 0
@@ -576,9 +577,10 @@ def bench_batched(xk, seed=2027, reps=5, parts=False):
                 L.xkblas_batch_run(mp)
                 L.xkblas_sync()
                 tm.append(xk.profile("batch")[1])
-            mode_ms[{0: "tiles_64", 1: "tiles_per_problem", 2: "tiles_32"}[mode]] = round(min(tm), 3)
+            mode_ms[{3: "tiles_64", 1: "tiles_per_problem", 2: "tiles_32"}[mode]] = round(min(tm), 3)
             L.xkblas_batch_use_graph(mp, 0)
             L.xkblas_batch_destroy(mp)
+        L.xkblas_set_batch_tile(0)
         upload()
         plan = L.xkblas_batch_create(ctypes.cast(arr, p), len(arr))  # default mode 0
         assert plan
@@ -642,13 +644,13 @@ def bench_batched(xk, seed=2027, reps=5, parts=False):
         # bandwidth, whichever is longer
         peak = FP64_PEAK if code == 0 else FP32_PEAK
         t_flop, t_byte = pfl.value / (peak * 1e9), pby.value / (HBM_PEAK * 1e6)  # ms
-        # GEMM tiles (64 x 64, or 32 x 32 where that covers <= 3/4 of the
-        # area, the default rule) with k in slices of 16: the MFMA work the
-        # GEMM items execute
+        # GEMM tiles of the default plan (FP64: 64 x 64, or 32 x 32 where that
+        # covers <= 3/4 of the area; FP32: 64 x 64) with k in slices of 16:
+        # the MFMA work the GEMM items execute
         def padded(m, n):
             a64 = -(-m // 64) * -(-n // 64) * 4096
             a32 = -(-m // 32) * -(-n // 32) * 1024
-            return a32 if 4 * a32 <= 3 * a64 else a64
+            return a32 if code == 0 and 4 * a32 <= 3 * a64 else a64
         gemm_exec = float(sum(2.0 * padded(m, n) * ((k + 15) // 16 * 16) for m, n, k in g))
         gemm_alg = float(sum(2.0 * m * n * k for m, n, k in g))
         out[name] = {"ms": round(best, 3), "gflops": round(flops / best / 1e6, 1), "problems": len(arr),

@@ -80,10 +80,11 @@ typedef struct xkblas_batch_problem {
 } xkblas_batch_problem_t;
 
 void* xkblas_batch_create(const xkblas_batch_problem_t* problems, int count);
-/* GEMM work-item shape of plans created afterwards: 0 (default) = 64 x 64
- * tiles only, 1 = 32 x 32 tiles for a problem whose 32 x 32 tiling covers at
- * most 3/4 of the area of its 64 x 64 one (small m or n), 2 = 32 x 32 only;
- * environment XKBLAS_BATCH_TILE.  Results are the same bit for bit. */
+/* GEMM work-item shape of plans created afterwards: 1 = 32 x 32 tiles for a
+ * problem whose 32 x 32 tiling covers at most 3/4 of the area of its 64 x 64
+ * one (small m or n), 2 = 32 x 32 only, 3 = 64 x 64 only, 0 (default) = rule
+ * 1 for FP64 problems and 64 x 64 for FP32 ones; environment
+ * XKBLAS_BATCH_TILE.  Results are the same bit for bit. */
 void  xkblas_set_batch_tile(int mode);
 int   xkblas_batch_run(void* plan);
 int   xkblas_batch_use_graph(void* plan, int on);

@@ -401,18 +401,20 @@ struct BatchCost {
   BatchItem w;
 };
 
-// GEMM item shape: 0 (default) = 64 x 64 tiles only, 1 = 32 x 32 tiles for a
-// problem whose 32 x 32 tiling covers at most 3/4 of the area the 64 x 64
-// one does (a small m or n), 2 = 32 x 32 only (XKBLAS_BATCH_TILE).  Less
-// padding with 32 x 32 items does not pay for their number on the uneven
-// batch of 4096 GEMMs + 1024 TRSMs (measured on MI355X over five runs:
-// 64 x 64 only 1.62 ms, per problem 1.68 ms on average).
+// GEMM item shape: 1 = 32 x 32 tiles for a problem whose 32 x 32 tiling
+// covers at most 3/4 of the area the 64 x 64 one does (a small m or n), 2 =
+// 32 x 32 only, 3 = 64 x 64 only, 0 (default) = rule 1 for FP64 problems and
+// 64 x 64 for FP32 ones (XKBLAS_BATCH_TILE).  On the uneven batch of 4096
+// GEMMs + 1024 TRSMs (MI355X, best of five per run, four runs), FP64: per
+// problem 1.54-1.70 ms (mean 1.60) vs 64 x 64 1.56-1.80 (mean 1.70); FP32:
+// 64 x 64 1.06-1.10 vs per problem 1.13-1.14 ms.
 static int g_batch_tile = getenv("XKBLAS_BATCH_TILE") ? atoi(getenv("XKBLAS_BATCH_TILE")) : 0;
 
-static int gemm_item_tile(int m, int n)
+static int gemm_item_tile(int m, int n, int dtype)
 {
-  if (g_batch_tile == 0) return 64;
-  if (g_batch_tile == 2) return 32;
+  const int mode = g_batch_tile == 0 ? (dtype == 0 ? 1 : 3) : g_batch_tile;
+  if (mode == 3) return 64;
+  if (mode == 2) return 32;
   const long a64 = (long)((m + 63) / 64) * ((n + 63) / 64) * 4096;
   const long a32 = (long)((m + 31) / 32) * ((n + 31) / 32) * 1024;
   return 4 * a32 <= 3 * a64 ? 32 : 64;
@@ -425,7 +427,7 @@ static long batch_count_items(const xkblas_batch_problem_t* probs, int count)
     const xkblas_batch_problem_t& q = probs[i];
     if (q.m <= 0 || q.n <= 0) continue;
     if (q.kind == XKBLAS_BATCH_GEMM) {
-      const int t = gemm_item_tile(q.m, q.n);
+      const int t = gemm_item_tile(q.m, q.n, q.dtype);
       n += (long)((q.m + t - 1) / t) * ((q.n + t - 1) / t);
     } else {
       n += (q.n + 63) / 64;
@@ -449,7 +451,7 @@ static long batch_build(const xkblas_batch_problem_t* probs, int count, BatchPro
     XK_CHECK(q.dtype == 0 || q.dtype == 1, "batch: dtype must be 0 (FP64) or 1 (FP32)");
     if (q.m <= 0 || q.n <= 0) continue;
     if (q.kind == XKBLAS_BATCH_GEMM) {
-      const int t = gemm_item_tile(q.m, q.n);
+      const int t = gemm_item_tile(q.m, q.n, q.dtype);
       const int tmx = (q.m + t - 1) / t, tnx = (q.n + t - 1) / t;
       // a tile's k-loop work plus a fixed overhead (a 32 x 32 tile runs at a
       // quarter of the MFMA work per slice, hence the same time per k as half
@@ -624,7 +626,7 @@ int xkblas_batch_run_noalloc(const xkblas_batch_problem_t* problems, int count)
 void xkblas_set_batch_tile(int mode)
 {
   std::lock_guard<std::mutex> lock(api_mutex());
-  g_batch_tile = mode < 0 || mode > 2 ? 0 : mode;
+  g_batch_tile = mode < 0 || mode > 3 ? 0 : mode;
 }
 
 int xkblas_batch_use_graph(void* plan, int on)
