@@ -146,6 +146,33 @@ def test_dposv_ir_more_moduli_fewer_corrections(xk):
     assert its[0] >= its[1] >= its[2] and its[0] >= 1, its
 
 
+@pytest.mark.parametrize("moduli", [5, 7])
+def test_dposv_ir_coarse_factor(xk, moduli):
+    """A coarser reduced factor (5 CRT moduli, about 2^-16 per product) still
+    converges on a well-conditioned A, with more corrections than the default
+    7, to the same 1e-14 bar against the refined solution."""
+    L = xk.lib()
+    r = rng(103)
+    n, nrhs = 2048, 8
+    A = spd(r, n)
+    B = uniform(r, n, nrhs)
+    nb0 = L.xkblas_get_param()
+    L.xkblas_set_param(512, 0)
+    L.xkblas_set_posv_ir(moduli, 30)
+    try:
+        X, info, it = xk.posv_ir(LOWER, _stored(A, LOWER), B)
+    finally:
+        L.xkblas_set_posv_ir(0, -1)
+        L.xkblas_set_param(nb0, 0)
+        xk.invalidate()
+    assert info == 0 and 2 <= it <= 6, (info, it)
+    cols = np.arange(4)
+    Lc = np.linalg.cholesky(A)
+    Xs = refined_solution(A, Lc, B[:, cols], np.linalg.solve(A, B[:, cols]))
+    assert np.abs(X[:, cols] - Xs).max() / np.abs(Xs).max() <= 1e-14
+    assert np.all(_criterion(A, B, X) < 2.0)
+
+
 @pytest.mark.parametrize("s", [-120, -60, 80, 95])
 def test_dposv_ir_scaled_matrix(xk, s):
     """The solves with the reduced factor run in FP32 while ||A||_inf stays in
